@@ -1,0 +1,18 @@
+"""MI355X-native dense ZNCC tracker with AutoRIFT.jl's interface.
+
+Python host over the C ABI of libautorift_hip.so (include/autorift_hip.h). Names follow the
+reference (src/api.jl, src/points.jl, src/track.jl, src/multichip.jl); Julia's mutating
+`reinit!` / `autorift!` are spelled `reinit_` / `autorift_`.
+"""
+from ._lib import (AR_NCC, AR_ZNCC, ArgumentError, AutoRIFTError, DimensionMismatch,
+                   NoDeviceError, Params, default_params, lib)
+from .core import Context, DisplacementField, MultichipResult
+from .points import (PointSet, chip_bounds, gridpoints, gridpoints_xy, pointset, sanitize_,
+                     search_bounds)
+
+__all__ = [
+    "AR_NCC", "AR_ZNCC", "ArgumentError", "AutoRIFTError", "DimensionMismatch", "NoDeviceError",
+    "Params", "default_params", "lib", "Context", "DisplacementField", "MultichipResult",
+    "PointSet", "chip_bounds", "gridpoints", "gridpoints_xy", "pointset", "sanitize_",
+    "search_bounds",
+]

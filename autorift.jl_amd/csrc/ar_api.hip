@@ -1,0 +1,532 @@
+// Host side of the C ABI: context lifecycle, device-resident image pairs, pass geometry and the
+// dispatch of the per-point tracker. See include/autorift_hip.h for the contract of each entry.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/autorift_hip.h"
+#include "ar_internal.h"
+#include "layout.h"
+#include "refine.h"
+#include "track_common.h"
+
+namespace ar {
+
+// Defined in track.hip.
+__global__ void track_u8_kernel(TrackArgs a);
+template <int IS_U8>
+__global__ void track_generic_kernel(TrackArgs a);
+
+}  // namespace ar
+
+using namespace ar;
+
+static std::mutex g_err_mu;
+static std::string g_err;   // errors with no context (create)
+
+static const size_t kLdsLimit = 160 * 1024 - 2048;   // leave room for static shared arrays
+
+static ar_status fail(ar_context* ctx, ar_status st, const std::string& msg) {
+    if (ctx) {
+        ctx->err = msg;
+    } else {
+        std::lock_guard<std::mutex> lk(g_err_mu);
+        g_err = msg;
+    }
+    return st;
+}
+
+#define HIPCHK(ctx, expr)                                                                    \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess)                                                                \
+            return fail((ctx), _e == hipErrorOutOfMemory ? AR_ERR_OUT_OF_MEMORY : AR_ERR_DEVICE, \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+    } while (0)
+
+// ---------------------------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------------------------
+
+ar_status DevBuf::reserve(ar_context* ctx, size_t n) {
+    if (n <= cap) return AR_OK;
+    if (ptr) hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    size_t want = n < 256 ? 256 : n;
+    HIPCHK(ctx, hipMalloc(&ptr, want));
+    cap = want;
+    return AR_OK;
+}
+
+void DevBuf::release() {
+    if (ptr) hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------------------------
+
+// Combined validity (src/preprocess.jl:106) and an "every pixel valid" flag.
+__global__ void combine_masks_kernel(const uint8_t* a, const uint8_t* b, uint8_t* out, int64_t n,
+                                     int* all_valid) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int bad = 0;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint8_t v = (a[i] && b[i]) ? 1 : 0;
+        out[i] = v;
+        bad |= !v;
+    }
+    if (__syncthreads_or(bad) && threadIdx.x == 0) atomicAnd(all_valid, 0);
+}
+
+// isfinite mask of a float image (src/preprocess.jl:74-79).
+__global__ void finite_mask_kernel(const float* img, uint8_t* out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) out[i] = isfinite(img[i]) ? 1 : 0;
+}
+
+__global__ void fill_u8_kernel(uint8_t* out, uint8_t v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) out[i] = v;
+}
+
+// _pass_geometry (src/track.jl:176-197) plus the facts the dispatcher needs: every searchable
+// chip area a power of two. g[0..5] = cx, cy, rx, ry, px, py (max); g[6] = fits; g[7] = pow2.
+__global__ void pass_geometry_kernel(ar_points pts, int64_t n, int64_t ny, int64_t nx,
+                                     unsigned long long* g, int* flags) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long cx = 0, cy = 0, rx = 0, ry = 0, px = 0, py = 0;
+    int fits = 1, pow2 = 1;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t prx = pts.radius_x[i], pry = pts.radius_y[i];
+        if (!(prx > 0 && pry > 0)) continue;
+        int64_t csx = pts.chip_size_x[i], csy = pts.chip_size_y[i];
+        cx = max(cx, (unsigned long long)(csx < 0 ? 0 : csx));
+        cy = max(cy, (unsigned long long)(csy < 0 ? 0 : csy));
+        rx = max(rx, (unsigned long long)prx);
+        ry = max(ry, (unsigned long long)pry);
+        int64_t qx = csx / 2 + prx + (int64_t)ceil(fabs(pts.dx_prior[i]));
+        int64_t qy = csy / 2 + pry + (int64_t)ceil(fabs(pts.dy_prior[i]));
+        px = max(px, (unsigned long long)(qx < 0 ? 0 : qx));
+        py = max(py, (unsigned long long)(qy < 0 ? 0 : qy));
+        int64_t hx = csx / 2, hy = csy / 2;
+        int64_t fx = (int64_t)floor(pts.x[i]), fy = (int64_t)floor(pts.y[i]);
+        int64_t r0 = fy - hy - pry, r1 = fy + hy + pry - 2;
+        int64_t c0 = fx - hx - prx, c1 = fx + hx + prx - 2;
+        fits &= (r0 >= 1 && r1 <= ny && c0 >= 1 && c1 <= nx) ? 1 : 0;
+        int64_t area = (2 * hy) * (2 * hx);
+        pow2 &= (area > 0 && (area & (area - 1)) == 0) ? 1 : 0;
+    }
+    atomicMax(&g[0], cx);
+    atomicMax(&g[1], cy);
+    atomicMax(&g[2], rx);
+    atomicMax(&g[3], ry);
+    atomicMax(&g[4], px);
+    atomicMax(&g[5], py);
+    if (!fits) atomicAnd(&flags[0], 0);
+    if (!pow2) atomicAnd(&flags[1], 0);
+}
+
+__global__ void skip_all_kernel(float* dx, float* dy, float* corr, uint8_t* s, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        dx[i] = __int_as_float(0x7fc00000);
+        dy[i] = __int_as_float(0x7fc00000);
+        corr[i] = __int_as_float(0x7fc00000);
+        s[i] = 0;
+    }
+}
+
+static int grid_for(int64_t n, int per_block = 256) {
+    int64_t b = (n + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > 4096) b = 4096;
+    return (int)b;
+}
+
+// ---------------------------------------------------------------------------------------------
+// ABI
+// ---------------------------------------------------------------------------------------------
+
+extern "C" {
+
+void ar_params_default(ar_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->preprocess = AR_PRE_HIGHPASS;
+    p->filter_width = 5;
+    p->quantize = AR_QUANT_UINT8;
+    p->measure = AR_ZNCC;
+    p->upsampling = 64;
+    p->outliers = 1;
+    p->gardner_window = 5;
+    p->gardner_iterations = 3;
+    p->min_agree_fraction = 8.0 / 25.0;
+    p->agree_tolerance = 0.2;
+    p->mad_scale = 4.0;
+    p->chip_size_base = 32;
+    p->chip_size_min = 32;
+    p->chip_size_max = 128;
+    p->chip_aspect = 1.0;
+    p->grid_spacing = 32;
+    p->search_radius_x = 25;
+    p->search_radius_y = 25;
+    p->min_search_radius = 6;
+    p->coarse_stride = 4;
+    p->coarse_buffer = 8;
+    p->min_coarse_valid_fraction = 0.01;
+    p->dx_prior = 0.0;
+    p->dy_prior = 0.0;
+    p->fill_window = 3;
+}
+
+const char* ar_version(void) { return "autorift_hip 0.1.0 (gfx950)"; }
+
+const char* ar_last_error(const ar_context* ctx) {
+    if (ctx) return ctx->err.c_str();
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_err.c_str();
+}
+
+ar_status ar_context_create(const ar_params* p, int32_t device, ar_context** out) {
+    if (!out) return fail(nullptr, AR_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(nullptr, AR_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= ndev)
+        return fail(nullptr, AR_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        return fail(nullptr, AR_ERR_DEVICE, "hipGetDeviceProperties failed");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(nullptr, AR_ERR_NO_DEVICE,
+                    std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
+    ar_context* c = new ar_context();
+    if (p) c->params = *p;
+    else ar_params_default(&c->params);
+    c->device = device;
+    c->ncu = prop.multiProcessorCount;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return fail(nullptr, AR_ERR_DEVICE, "stream/event creation failed");
+    }
+    *out = c;
+    return AR_OK;
+}
+
+void ar_context_destroy(ar_context* ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (DevBuf* b : ctx->all_bufs()) b->release();
+    hipEventDestroy(ctx->ev0);
+    hipEventDestroy(ctx->ev1);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+}  // extern "C"
+
+// Upload masks (or derive them) and combine; sets ctx->all_valid.
+ar_status ar_upload_masks(ar_context* ctx, const uint8_t* ref_valid, const uint8_t* sec_valid,
+                          int64_t N) {
+    ar_status st;
+    if ((st = ctx->rmask.reserve(ctx, N)) || (st = ctx->smask.reserve(ctx, N)) ||
+        (st = ctx->okmask.reserve(ctx, N)) || (st = ctx->flags.reserve(ctx, 64)))
+        return st;
+    // ref/sec validity come from the caller, else from the prepared images (already written to
+    // rmask/smask by the preparation step when the caller passed NULL).
+    if (ref_valid) HIPCHK(ctx, hipMemcpyAsync(ctx->rmask.ptr, ref_valid, N, hipMemcpyHostToDevice, ctx->stream));
+    if (sec_valid) HIPCHK(ctx, hipMemcpyAsync(ctx->smask.ptr, sec_valid, N, hipMemcpyHostToDevice, ctx->stream));
+    int one = 1;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->flags.ptr, &one, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    combine_masks_kernel<<<grid_for(N), 256, 0, ctx->stream>>>(
+        (const uint8_t*)ctx->rmask.ptr, (const uint8_t*)ctx->smask.ptr, (uint8_t*)ctx->okmask.ptr, N,
+        (int*)ctx->flags.ptr);
+    HIPCHK(ctx, hipGetLastError());
+    int all = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&all, ctx->flags.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->all_valid = all != 0;
+    return AR_OK;
+}
+
+extern "C" ar_status ar_set_pair(ar_context* ctx, const void* ref, const void* sec,
+                                 const uint8_t* ref_valid, const uint8_t* sec_valid, int64_t ny,
+                                 int64_t nx, int32_t dtype) {
+    if (!ctx) return fail(nullptr, AR_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (!ref || !sec) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "both images are required");
+    if (ny <= 0 || nx <= 0) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "image size must be positive");
+    if (dtype != AR_U8 && dtype != AR_F32)
+        return fail(ctx, AR_ERR_UNSUPPORTED, "prepared pairs must be UInt8 or Float32");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int64_t N = ny * nx;
+    size_t es = dtype == AR_U8 ? 1 : 4;
+    ar_status st;
+    if ((st = ctx->pref.reserve(ctx, N * es)) || (st = ctx->psec.reserve(ctx, N * es)) ||
+        (st = ctx->rmask.reserve(ctx, N)) || (st = ctx->smask.reserve(ctx, N)))
+        return st;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->pref.ptr, ref, N * es, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->psec.ptr, sec, N * es, hipMemcpyHostToDevice, ctx->stream));
+    ctx->ny = ny;
+    ctx->nx = nx;
+    ctx->pdtype = dtype;
+    ctx->has_pair = true;
+    // default masks: isfinite (all true for integer images)
+    if (!ref_valid) {
+        if (dtype == AR_U8) fill_u8_kernel<<<grid_for(N), 256, 0, ctx->stream>>>((uint8_t*)ctx->rmask.ptr, 1, N);
+        else finite_mask_kernel<<<grid_for(N), 256, 0, ctx->stream>>>((const float*)ctx->pref.ptr, (uint8_t*)ctx->rmask.ptr, N);
+    }
+    if (!sec_valid) {
+        if (dtype == AR_U8) fill_u8_kernel<<<grid_for(N), 256, 0, ctx->stream>>>((uint8_t*)ctx->smask.ptr, 1, N);
+        else finite_mask_kernel<<<grid_for(N), 256, 0, ctx->stream>>>((const float*)ctx->psec.ptr, (uint8_t*)ctx->smask.ptr, N);
+    }
+    HIPCHK(ctx, hipGetLastError());
+    return ar_upload_masks(ctx, ref_valid, sec_valid, N);
+}
+
+extern "C" ar_status ar_get_pair(ar_context* ctx, void* ref, void* sec, uint8_t* ref_valid,
+                                 uint8_t* sec_valid, int32_t* dtype) {
+    if (!ctx) return fail(nullptr, AR_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (!ctx->has_pair) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "no image pair set");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int64_t N = ctx->ny * ctx->nx;
+    size_t es = ctx->pdtype == AR_U8 ? 1 : 4;
+    if (ref) HIPCHK(ctx, hipMemcpyAsync(ref, ctx->pref.ptr, N * es, hipMemcpyDeviceToHost, ctx->stream));
+    if (sec) HIPCHK(ctx, hipMemcpyAsync(sec, ctx->psec.ptr, N * es, hipMemcpyDeviceToHost, ctx->stream));
+    if (ref_valid) HIPCHK(ctx, hipMemcpyAsync(ref_valid, ctx->rmask.ptr, N, hipMemcpyDeviceToHost, ctx->stream));
+    if (sec_valid) HIPCHK(ctx, hipMemcpyAsync(sec_valid, ctx->smask.ptr, N, hipMemcpyDeviceToHost, ctx->stream));
+    if (dtype) *dtype = ctx->pdtype;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return AR_OK;
+}
+
+// The pass on device-resident points. All pointers device pointers.
+ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t up,
+                        int32_t measure, float* dx, float* dy, float* corr, uint8_t* srch) {
+    if (!ctx->has_pair) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "no image pair set");
+    if (n < 0) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "negative point count");
+    if (up < 1 || up > 64 || (up & (up - 1)) != 0)
+        return fail(ctx, AR_ERR_UNSUPPORTED,
+                    "upsampling must be a power of two in 1..64 on the device path");
+    if (measure != AR_ZNCC && measure != AR_NCC)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "measure must be ZNCC or NCC");
+    if (n == 0) return AR_OK;
+    ar_status st;
+    if ((st = ctx->geom.reserve(ctx, 128))) return st;
+    unsigned long long g0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int f0[2] = {1, 1};
+    HIPCHK(ctx, hipMemcpyAsync(ctx->geom.ptr, g0, sizeof(g0), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync((char*)ctx->geom.ptr + 64, f0, sizeof(f0), hipMemcpyHostToDevice, ctx->stream));
+    pass_geometry_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(
+        dp, n, ctx->ny, ctx->nx, (unsigned long long*)ctx->geom.ptr, (int*)((char*)ctx->geom.ptr + 64));
+    HIPCHK(ctx, hipGetLastError());
+    unsigned long long g[8];
+    int f[2];
+    HIPCHK(ctx, hipMemcpyAsync(g, ctx->geom.ptr, sizeof(g), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(f, (char*)ctx->geom.ptr + 64, sizeof(f), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    int64_t cx = g[0], cy = g[1], rx = g[2], ry = g[3], px = g[4], py = g[5];
+    bool fits = f[0] != 0, pow2 = f[1] != 0;
+    if (cx == 0) {
+        skip_all_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dx, dy, corr, srch, n);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        return AR_OK;
+    }
+    if (cx > 512 || cy > 512 || rx > 256 || ry > 256)
+        return fail(ctx, AR_ERR_UNSUPPORTED, "chip size above 512 or search radius above 256");
+
+    TrackArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.ref = ctx->pref.ptr;
+    a.sec = ctx->psec.ptr;
+    a.ok = ctx->all_valid ? nullptr : (const uint8_t*)ctx->okmask.ptr;
+    a.ny = ctx->ny;
+    a.nx = ctx->nx;
+    a.padx = fits ? 0 : px + 2;
+    a.pady = fits ? 0 : py + 2;
+    a.x = dp.x; a.y = dp.y; a.dx_prior = dp.dx_prior; a.dy_prior = dp.dy_prior;
+    a.radius_x = dp.radius_x; a.radius_y = dp.radius_y;
+    a.chip_size_x = dp.chip_size_x; a.chip_size_y = dp.chip_size_y;
+    a.n = n;
+    a.up = (int32_t)up;
+    a.measure = measure;
+    a.dx = dx; a.dy = dy; a.corr = corr; a.searched = srch;
+    // even chip extents actually used (chip_bounds), src/points.jl:358-366
+    a.max_ch = (int32_t)(2 * (cy / 2));
+    a.max_cw = (int32_t)(2 * (cx / 2));
+    a.max_rx = (int32_t)rx;
+    a.max_ry = (int32_t)ry;
+
+    bool use_u8 = ctx->pdtype == AR_U8 && pow2;
+    if (use_u8) {
+        U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, (int)up);
+        if (L.total > kLdsLimit) use_u8 = false;
+        else a.lds_bytes = L.total;
+    }
+    void (*kern)(TrackArgs);
+    if (use_u8) {
+        kern = track_u8_kernel;
+    } else {
+        a.lds_bytes = (size_t)cascade_floats((int)up) * 4 + 64;
+        kern = ctx->pdtype == AR_U8 ? track_generic_kernel<1> : track_generic_kernel<0>;
+    }
+    HIPCHK(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)a.lds_bytes));
+    int per_cu = 0;
+    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, AR_WG, a.lds_bytes));
+    if (per_cu < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "tracker kernel does not fit on a CU");
+    int64_t nblocks = (int64_t)ctx->ncu * per_cu;
+    if (nblocks > n) nblocks = n;
+    if (!use_u8) {
+        GenScratch G = gen_scratch(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
+        a.scratch_per_block = G.total;
+        if ((st = ctx->scratch.reserve(ctx, G.total * nblocks))) return st;
+        a.scratch = (unsigned char*)ctx->scratch.ptr;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(AR_WG), a.lds_bytes, ctx->stream, a);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    ctx->last_ms = ms;
+    ctx->last_path = use_u8 ? 0 : 1;
+    return AR_OK;
+}
+
+extern "C" ar_status ar_track_device(ar_context* ctx, const ar_points* dev_pts, int64_t n,
+                                     int64_t upsampling, int32_t measure, float* dev_dx,
+                                     float* dev_dy, float* dev_corr, uint8_t* dev_searched) {
+    if (!ctx) return fail(nullptr, AR_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (!dev_pts || !dev_dx || !dev_dy || !dev_corr || !dev_searched)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    return ar_track_pass(ctx, *dev_pts, n, upsampling, measure, dev_dx, dev_dy, dev_corr,
+                         dev_searched);
+}
+
+// Uploads host point fields into the context's point buffers; returns device views.
+ar_status ar_upload_points(ar_context* ctx, const ar_points* pts, int64_t n, ar_points* dp,
+                           int64_t csx_fill, int64_t csy_fill) {
+    ar_status st;
+    size_t d = sizeof(double) * n, l = sizeof(int64_t) * n;
+    if ((st = ctx->pts.reserve(ctx, 4 * d + 4 * l))) return st;
+    char* b = (char*)ctx->pts.ptr;
+    double* x = (double*)b; double* y = (double*)(b + d);
+    double* pdx = (double*)(b + 2 * d); double* pdy = (double*)(b + 3 * d);
+    int64_t* rx = (int64_t*)(b + 4 * d); int64_t* ry = (int64_t*)(b + 4 * d + l);
+    int64_t* cx = (int64_t*)(b + 4 * d + 2 * l); int64_t* cy = (int64_t*)(b + 4 * d + 3 * l);
+    HIPCHK(ctx, hipMemcpyAsync(x, pts->x, d, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(y, pts->y, d, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(pdx, pts->dx_prior, d, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(pdy, pts->dy_prior, d, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(rx, pts->radius_x, l, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ry, pts->radius_y, l, hipMemcpyHostToDevice, ctx->stream));
+    if (pts->chip_size_x && pts->chip_size_y) {
+        HIPCHK(ctx, hipMemcpyAsync(cx, pts->chip_size_x, l, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(cy, pts->chip_size_y, l, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        std::vector<int64_t> fx(n, csx_fill), fy(n, csy_fill);
+        HIPCHK(ctx, hipMemcpyAsync(cx, fx.data(), l, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(cy, fy.data(), l, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    dp->x = x; dp->y = y; dp->dx_prior = pdx; dp->dy_prior = pdy;
+    dp->radius_x = rx; dp->radius_y = ry; dp->chip_size_x = cx; dp->chip_size_y = cy;
+    return AR_OK;
+}
+
+extern "C" ar_status ar_track(ar_context* ctx, const ar_points* pts, int64_t n,
+                              int64_t upsampling, int32_t measure, float* dx, float* dy,
+                              float* corr, uint8_t* searched) {
+    if (!ctx) return fail(nullptr, AR_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (!pts || !dx || !dy || !corr || !searched)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    if (!pts->x || !pts->y || !pts->radius_x || !pts->radius_y || !pts->dx_prior ||
+        !pts->dy_prior || !pts->chip_size_x || !pts->chip_size_y)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "NULL point field");
+    if (n == 0) return AR_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    ar_status st;
+    ar_points dp;
+    if ((st = ar_upload_points(ctx, pts, n, &dp, 0, 0))) return st;
+    if ((st = ctx->out.reserve(ctx, 13 * n))) return st;
+    float* ddx = (float*)ctx->out.ptr;
+    float* ddy = ddx + n;
+    float* dc = ddy + n;
+    uint8_t* ds = (uint8_t*)(dc + n);
+    if ((st = ar_track_pass(ctx, dp, n, upsampling, measure, ddx, ddy, dc, ds))) return st;
+    HIPCHK(ctx, hipMemcpyAsync(dx, ddx, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(dy, ddy, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(corr, dc, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(searched, ds, n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return AR_OK;
+}
+
+extern "C" ar_status ar_last_kernel(const ar_context* ctx, double* ms, int32_t* path) {
+    if (!ctx) return AR_ERR_INVALID_ARGUMENT;
+    if (ms) *ms = ctx->last_ms;
+    if (path) *path = ctx->last_path;
+    return AR_OK;
+}
+
+extern "C" ar_status ar_correlate_point(ar_context* ctx, const void* search, const void* chip,
+                                        int32_t dtype, int64_t ch, int64_t cw, int64_t rx,
+                                        int64_t ry, int64_t upsampling, int32_t measure,
+                                        double* dx, double* dy, float* corr) {
+    if (!ctx) return fail(nullptr, AR_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (!search || !chip || !dx || !dy || !corr)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    if (ch <= 0 || cw <= 0 || (ch & 1) || (cw & 1) || rx <= 0 || ry <= 0)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "chip extents must be even and radii positive");
+    if (dtype != AR_U8 && dtype != AR_F32)
+        return fail(ctx, AR_ERR_UNSUPPORTED, "dtype must be UInt8 or Float32");
+    // Lay the window out as the reference image and the chip at offset (ry, rx) of an
+    // equally sized secondary, so one grid point with zero prior reproduces correlate!.
+    int64_t wh = ch + 2 * ry - 1, ww = cw + 2 * rx - 1;
+    size_t es = dtype == AR_U8 ? 1 : 4;
+    std::vector<unsigned char> sec(wh * ww * es, 0);
+    for (int64_t j = 0; j < cw; j++)
+        std::memcpy(&sec[((ry) + (j + rx) * wh) * es], (const unsigned char*)chip + j * ch * es,
+                    ch * es);
+    std::vector<uint8_t> ones(wh * ww, 1);
+    // keep the caller's pair, if any, intact
+    ar_context* c = ctx;
+    bool had = c->has_pair;
+    DevBuf sref = c->pref, ssec = c->psec, srm = c->rmask, ssm = c->smask, sok = c->okmask;
+    int64_t sny = c->ny, snx = c->nx;
+    int32_t sdt = c->pdtype;
+    bool sall = c->all_valid;
+    c->pref = DevBuf(); c->psec = DevBuf(); c->rmask = DevBuf(); c->smask = DevBuf();
+    c->okmask = DevBuf();
+    ar_status st = ar_set_pair(c, search, sec.data(), ones.data(), ones.data(), wh, ww, dtype);
+    if (st == AR_OK) {
+        double x = (double)(1 + cw / 2 + rx), y = (double)(1 + ch / 2 + ry), z = 0.0;
+        int64_t r_x = rx, r_y = ry, c_x = cw, c_y = ch;
+        ar_points p = {&x, &y, &r_x, &r_y, &z, &z, &c_x, &c_y};
+        float fdx, fdy, fc;
+        uint8_t s;
+        st = ar_track(c, &p, 1, upsampling, measure, &fdx, &fdy, &fc, &s);
+        *dx = fdx;
+        *dy = fdy;
+        *corr = fc;
+    }
+    c->pref.release(); c->psec.release(); c->rmask.release(); c->smask.release();
+    c->okmask.release();
+    c->pref = sref; c->psec = ssec; c->rmask = srm; c->smask = ssm; c->okmask = sok;
+    c->ny = sny; c->nx = snx; c->pdtype = sdt; c->all_valid = sall; c->has_pair = had;
+    return st;
+}

@@ -1,0 +1,58 @@
+// Host-side state behind the opaque ar_context handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/autorift_hip.h"
+
+struct ar_context;
+
+// A growable device allocation owned by a context.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    ar_status reserve(ar_context* ctx, size_t n);
+    void release();
+};
+
+struct ar_context {
+    ar_params params;
+    int device = 0;
+    int ncu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+
+    // prepared pair (what the tracker correlates), column-major ny x nx
+    bool has_pair = false;
+    int64_t ny = 0, nx = 0;
+    int32_t pdtype = AR_U8;
+    DevBuf pref, psec;          // prepared images
+    DevBuf rmask, smask;        // prepared validity masks
+    DevBuf okmask;              // rmask & smask
+    bool all_valid = true;
+
+    // raw images of the current pair (preparation inputs)
+    DevBuf raw, rawmask, work_f32, work_f64;
+
+    // scratch
+    DevBuf flags, geom, pts, out, scratch, grid_a, grid_b;
+
+    double last_ms = 0;
+    int32_t last_path = -1;
+
+    std::vector<DevBuf*> all_bufs() {
+        return {&pref, &psec, &rmask, &smask, &okmask, &raw, &rawmask, &work_f32, &work_f64,
+                &flags, &geom, &pts, &out, &scratch, &grid_a, &grid_b};
+    }
+};
+
+// internal entry points shared between translation units
+ar_status ar_upload_masks(ar_context* ctx, const uint8_t* ref_valid, const uint8_t* sec_valid,
+                          int64_t N);
+ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t up,
+                        int32_t measure, float* dx, float* dy, float* corr, uint8_t* srch);
+ar_status ar_upload_points(ar_context* ctx, const ar_points* pts, int64_t n, ar_points* dp,
+                           int64_t csx_fill, int64_t csy_fill);

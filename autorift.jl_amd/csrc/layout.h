@@ -1,0 +1,60 @@
+// LDS and scratch layouts of the tracker kernels, shared by the kernels and the host dispatcher.
+#pragma once
+#include <stddef.h>
+
+#include "refine.h"
+
+namespace ar {
+
+struct U8Layout {
+    int ldw, rows, cols;   // window: cols x ldw bytes, column-major, int8 (value ^ 0x80)
+    int cpad;              // chip: max_cw x cpad bytes; chip column v at v*cpad, row u at 16+u
+    size_t off_w, off_t, off_s1, off_s2, off_tmp, total;
+};
+
+__host__ __device__ inline int rup(int x, int m) { return (x + m - 1) / m * m; }
+
+__host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx, int max_ry,
+                                              int up) {
+    U8Layout L;
+    int nti = (2 * max_ry + 15) / 16, ntj = (2 * max_rx + 15) / 16;
+    int nrb = (max_ch + 15 + 15) / 16;   // 16-row blocks of the band, ceil((ch + 15) / 16)
+    L.rows = 16 * (nti + nrb);
+    // an odd number of 16-byte units per column keeps the 16 lanes of a b128 read on
+    // distinct LDS banks
+    L.ldw = ((L.rows / 16) & 1) ? L.rows : L.rows + 16;
+    L.cols = 16 * ntj + max_cw;
+    L.cpad = rup(16 * nrb + 32, 16);
+    int nr = 2 * max_ry, nc = 2 * max_rx;
+    int winw = max_cw + 2 * max_rx - 1;
+    size_t hdr = 1024;   // cascade header alias: patch and scalars live at the front
+    L.off_w = hdr;
+    L.off_t = L.off_w + (size_t)L.cols * L.ldw;
+    L.off_s1 = rup((int)(L.off_t + (size_t)max_cw * L.cpad), 16);
+    L.off_s2 = L.off_s1 + (size_t)nr * nc * 4;
+    L.off_tmp = L.off_s2 + (size_t)nr * nc * 4;
+    size_t colsum = (size_t)nr * winw * 4;
+    size_t accsurf = (size_t)nr * nc * 8;   // int32 accumulators + float approximations
+    L.total = L.off_tmp + (colsum > accsurf ? colsum : accsurf);
+    size_t casc = (size_t)cascade_floats(up) * 4 + 64;
+    if (casc > L.total) L.total = casc;
+    return L;
+}
+
+struct GenScratch {
+    size_t off_chip, off_win, off_s, off_s2, off_surf, total;
+};
+
+__host__ __device__ inline GenScratch gen_scratch(int max_ch, int max_cw, int max_rx, int max_ry) {
+    GenScratch G;
+    int winh = max_ch + 2 * max_ry - 1, winw = max_cw + 2 * max_rx - 1;
+    G.off_chip = 0;
+    G.off_win = rup(max_ch * max_cw * 4, 256);
+    G.off_s = G.off_win + rup(winh * winw * 4, 256);
+    G.off_s2 = G.off_s + rup((winh + 1) * (winw + 1) * 8, 256);
+    G.off_surf = G.off_s2 + rup((winh + 1) * (winw + 1) * 8, 256);
+    G.total = G.off_surf + rup(4 * max_rx * max_ry * 4, 256);
+    return G;
+}
+
+}  // namespace ar

@@ -1,0 +1,478 @@
+// Image preparation on the device: the mask-aware high-pass pre-filter with mask erosion
+// (src/preprocess.jl:275-312, 371-399, 447-460), the NaN-aware banded running-sum box mean it is
+// built on (src/window.jl:289-439), and the UInt8 quantiser (src/preprocess.jl:546-592).
+//
+// The box mean reproduces the reference's summation order exactly: 16-row bands, a Float64
+// running sum down each column of a band (one thread per band column), then a Float64 running
+// sum along each row (one thread per row). Erosion is an exact boolean reduction. The quantiser's
+// global mean and standard deviation are Float64 sums in a fixed (deterministic) parallel order
+// rather than the reference's left-to-right order; they agree to ~1e-16 relative, which can move
+// a pixel only if (v - lo) * scale lies within ~1e-12 of a rounding boundary.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/autorift_hip.h"
+#include "ar_internal.h"
+
+namespace ar {
+
+template <typename T>
+__device__ inline double as_f64(const T* a, int64_t k) { return (double)a[k]; }
+template <typename T>
+__device__ inline float as_f32(const T* a, int64_t k) { return (float)a[k]; }
+
+// Value the box mean sees at k. MASKED: the reference's `masked` Float32 copy (NaN where the mask
+// is false); otherwise Float64 of the raw element.
+template <typename T, bool MASKED>
+__device__ inline double boxval(const T* img, const uint8_t* mask, int64_t k) {
+    if (MASKED) return mask[k] ? (double)as_f32(img, k) : (double)NAN;
+    return as_f64(img, k);
+}
+
+// Column pass of one 16-row band (src/window.jl:299-325 masked, 397-409 dense).
+// Thread per (band, column). colsum/colcnt hold, for every row i, the band's running state at i.
+template <typename T, bool MASKED, bool NANAWARE>
+__global__ void box_colpass_kernel(const T* img, const uint8_t* mask, int64_t nr, int64_t nc,
+                                   int ly, int ry, double* colsum, int32_t* colcnt) {
+    int64_t nb = nr < 16 ? nr : 16;
+    int64_t nbands = (nr + nb - 1) / nb;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbands * nc) return;
+    int64_t j = t / nbands, b = t - j * nbands;   // adjacent threads: adjacent bands, same column
+    int64_t i0 = 1 + b * nb;
+    int64_t i1 = i0 + nb - 1 < nr ? i0 + nb - 1 : nr;
+    const int64_t cj = (j)*nr;
+    double s = 0.0;
+    int32_t c = 0;
+    int64_t a0 = i0 - ly > 1 ? i0 - ly : 1, a1 = i0 + ry < nr ? i0 + ry : nr;
+    for (int64_t i = a0; i <= a1; i++) {
+        double v = boxval<T, MASKED>(img, mask, (i - 1) + cj);
+        if (NANAWARE) {
+            if (!isnan(v)) { s += v; c += 1; }
+        } else {
+            s += v;
+        }
+    }
+    for (int64_t i = i0; i <= i1; i++) {
+        colsum[(i - 1) + cj] = s;
+        if (NANAWARE) colcnt[(i - 1) + cj] = c;
+        int64_t oi = i - ly;
+        if (1 <= oi && oi <= nr) {
+            double v = boxval<T, MASKED>(img, mask, (oi - 1) + cj);
+            if (NANAWARE) {
+                if (!isnan(v)) { s -= v; c -= 1; }
+            } else {
+                s -= v;
+            }
+        }
+        int64_t ii = i + ry + 1;
+        if (ii <= nr) {
+            double v = boxval<T, MASKED>(img, mask, (ii - 1) + cj);
+            if (NANAWARE) {
+                if (!isnan(v)) { s += v; c += 1; }
+            } else {
+                s += v;
+            }
+        }
+    }
+}
+
+// Row pass (src/window.jl:327-361 masked, 411-435 dense) fused with highpass
+// (src/preprocess.jl:304-312): out = mask & isfinite(mean) ? Float32(img) - mean : NaN.
+// Thread per row; adjacent threads read adjacent rows of one column (column-major), coalesced.
+template <typename T, bool NANAWARE>
+__global__ void box_rowpass_highpass_kernel(const T* img, const uint8_t* mask, int64_t nr,
+                                            int64_t nc, int lx, int rx, int ly, int ry,
+                                            const double* colsum, const int32_t* colcnt,
+                                            float* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // 0-based row
+    if (i >= nr) return;
+    int64_t I = i + 1;
+    int64_t ci = (I + ry < nr ? I + ry : nr) - (I - ly > 1 ? I - ly : 1) + 1;
+    double s = 0.0;
+    int32_t c = 0;
+    int64_t b1 = rx + 1 < nc ? rx + 1 : nc;
+    for (int64_t j = 1; j <= b1; j++) {
+        s += colsum[i + (j - 1) * nr];
+        if (NANAWARE) c += colcnt[i + (j - 1) * nr];
+    }
+    for (int64_t j = 1; j <= nc; j++) {
+        float mean;
+        if (NANAWARE) {
+            mean = c > 0 ? (float)(s / (double)c) : NAN;
+        } else {
+            int64_t cj = (j + rx < nc ? j + rx : nc) - (j - lx > 1 ? j - lx : 1) + 1;
+            mean = (float)(s / (double)(ci * cj));
+        }
+        int64_t k = i + (j - 1) * nr;
+        out[k] = (mask[k] && isfinite(mean)) ? as_f32(img, k) - mean : NAN;
+        int64_t oj = j - lx;
+        if (1 <= oj && oj <= nc) {
+            s -= colsum[i + (oj - 1) * nr];
+            if (NANAWARE) c -= colcnt[i + (oj - 1) * nr];
+        }
+        int64_t jj = j + rx + 1;
+        if (jj <= nc) {
+            s += colsum[i + (jj - 1) * nr];
+            if (NANAWARE) c += colcnt[i + (jj - 1) * nr];
+        }
+    }
+}
+
+// _erode_mask (src/preprocess.jl:447-460) and _filtered (275-288): a pixel stays valid only if
+// every in-bounds pixel of its (clamped) w-window is valid; non-finite filtered values become 0
+// and invalid.
+__global__ void erode_finalize_kernel(float* out, const uint8_t* mask, uint8_t* vmask, int64_t nr,
+                                      int64_t nc, int lx, int rx, int ly, int ry, int allmask) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nr * nc) return;
+    int64_t j = k / nr, i = k - j * nr;
+    uint8_t v = mask[k];
+    if (!allmask) {
+        v = 1;
+        int64_t j0 = j - lx > 0 ? j - lx : 0, j1 = j + rx < nc - 1 ? j + rx : nc - 1;
+        int64_t i0 = i - ly > 0 ? i - ly : 0, i1 = i + ry < nr - 1 ? i + ry : nr - 1;
+        for (int64_t jj = j0; jj <= j1 && v; jj++)
+            for (int64_t ii = i0; ii <= i1; ii++)
+                if (!mask[ii + jj * nr]) { v = 0; break; }
+    }
+    float o = out[k];
+    if (!isfinite(o)) {
+        out[k] = 0.0f;
+        v = 0;
+    }
+    vmask[k] = v;
+}
+
+// Mask statistics: every pixel valid? any NaN in the image?
+template <typename T>
+__global__ void mask_stats_kernel(const T* img, const uint8_t* mask, int64_t n, int* flags) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int bad = 0, nan = 0;
+    for (; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        bad |= !mask[k];
+        nan |= isnan(as_f64(img, k)) ? 1 : 0;
+    }
+    if (__syncthreads_or(bad) && threadIdx.x == 0) atomicAnd(&flags[0], 0);
+    if (__syncthreads_or(nan) && threadIdx.x == 0) atomicOr(&flags[1], 1);
+}
+
+template <typename T>
+__global__ void finite_mask_t_kernel(const T* img, uint8_t* out, int64_t n) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; k < n; k += (int64_t)gridDim.x * blockDim.x) out[k] = isfinite(as_f64(img, k)) ? 1 : 0;
+}
+
+// Quantiser statistics, deterministic two-level reduction: per-block partials over a fixed
+// block-cyclic assignment, then one block combining them in index order.
+#define QBLOCKS 1024
+template <typename T>
+__global__ void q_partial_kernel(const T* img, const uint8_t* mask, int64_t n, int pass,
+                                 double mean, double* part_s, long long* part_n) {
+    __shared__ double ss[256];
+    __shared__ long long sn[256];
+    double s = 0.0;
+    long long c = 0;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        double v = as_f64(img, k);
+        if (mask[k] && isfinite(v)) {
+            if (pass == 0) s += v;
+            else { double d = v - mean; s += d * d; }
+            c += 1;
+        }
+    }
+    ss[threadIdx.x] = s;
+    sn[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            ss[threadIdx.x] += ss[threadIdx.x + o];
+            sn[threadIdx.x] += sn[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part_s[blockIdx.x] = ss[0];
+        part_n[blockIdx.x] = sn[0];
+    }
+}
+
+__global__ void q_final_kernel(const double* part_s, const long long* part_n, int nparts,
+                               double* out_s, long long* out_n) {
+    __shared__ double ss[256];
+    __shared__ long long sn[256];
+    double s = 0.0;
+    long long c = 0;
+    for (int k = threadIdx.x; k < nparts; k += 256) { s += part_s[k]; c += part_n[k]; }
+    ss[threadIdx.x] = s;
+    sn[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            ss[threadIdx.x] += ss[threadIdx.x + o];
+            sn[threadIdx.x] += sn[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { *out_s = ss[0]; *out_n = sn[0]; }
+}
+
+// round(UInt8, clamp((v - lo) * scale, 0, 255)), ties to even; invalid -> 0; flat -> 0x80.
+template <typename T>
+__global__ void q_map_kernel(const T* img, const uint8_t* mask, int64_t n, int mode, double lo,
+                             double scale, uint8_t* out) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        if (mode == 0) { out[k] = 0; continue; }                       // nothing valid
+        if (mode == 1) { out[k] = mask[k] ? 0x80 : 0x00; continue; }   // sd == 0
+        double v = as_f64(img, k);
+        if (!mask[k] || !isfinite(v)) { out[k] = 0; continue; }
+        double t = (v - lo) * scale;
+        t = t < 0.0 ? 0.0 : (t > 255.0 ? 255.0 : t);
+        out[k] = (uint8_t)rint(t);
+    }
+}
+
+// quantize(::NoQuantize) (src/preprocess.jl:492-507): non-finite floats -> 0; integers unchanged.
+template <typename T>
+__global__ void noquant_kernel(const T* img, int64_t n, float* out) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        float v = as_f32(img, k);
+        out[k] = isfinite(v) ? v : 0.0f;
+    }
+}
+
+__global__ void copy_u8_kernel(const uint8_t* a, uint8_t* b, int64_t n) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; k < n; k += (int64_t)gridDim.x * blockDim.x) b[k] = a[k];
+}
+
+}  // namespace ar
+
+using namespace ar;
+
+static ar_status pfail(ar_context* ctx, ar_status st, const std::string& m) {
+    ctx->err = m;
+    return st;
+}
+
+#define PCHK(expr)                                                                            \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            return pfail(ctx, _e == hipErrorOutOfMemory ? AR_ERR_OUT_OF_MEMORY : AR_ERR_DEVICE, \
+                         std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+    } while (0)
+
+static int gsz(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return (int)(b < 1 ? 1 : (b > 65535 ? 65535 : b));
+}
+
+// Quantise a float (or raw) image on the device into out (u8).
+template <typename T>
+static ar_status quantize_u8(ar_context* ctx, const T* img, const uint8_t* mask, int64_t N,
+                             uint8_t* out) {
+    ar_status st;
+    if ((st = ctx->grid_a.reserve(ctx, QBLOCKS * 16 + 64))) return st;
+    double* ps = (double*)ctx->grid_a.ptr;
+    long long* pn = (long long*)(ps + QBLOCKS);
+    double* rs = (double*)(pn + QBLOCKS);
+    long long* rn = (long long*)(rs + 1);
+    q_partial_kernel<T><<<QBLOCKS, 256, 0, ctx->stream>>>(img, mask, N, 0, 0.0, ps, pn);
+    q_final_kernel<<<1, 256, 0, ctx->stream>>>(ps, pn, QBLOCKS, rs, rn);
+    double s;
+    long long n;
+    PCHK(hipGetLastError());
+    PCHK(hipMemcpyAsync(&s, rs, 8, hipMemcpyDeviceToHost, ctx->stream));
+    PCHK(hipMemcpyAsync(&n, rn, 8, hipMemcpyDeviceToHost, ctx->stream));
+    PCHK(hipStreamSynchronize(ctx->stream));
+    int mode = 2;
+    double lo = 0, scale = 0;
+    if (n == 0) {
+        mode = 0;
+    } else {
+        double mean = s / (double)n;
+        q_partial_kernel<T><<<QBLOCKS, 256, 0, ctx->stream>>>(img, mask, N, 1, mean, ps, pn);
+        q_final_kernel<<<1, 256, 0, ctx->stream>>>(ps, pn, QBLOCKS, rs, rn);
+        double ss;
+        PCHK(hipGetLastError());
+        PCHK(hipMemcpyAsync(&ss, rs, 8, hipMemcpyDeviceToHost, ctx->stream));
+        PCHK(hipStreamSynchronize(ctx->stream));
+        double sd = n > 1 ? std::sqrt(ss / (double)(n - 1)) : 0.0;
+        if (sd == 0) mode = 1;
+        else {
+            lo = mean - 3 * sd;
+            scale = 255.0 / (6 * sd);
+        }
+    }
+    q_map_kernel<T><<<gsz(N), 256, 0, ctx->stream>>>(img, mask, N, mode, lo, scale, out);
+    PCHK(hipGetLastError());
+    return AR_OK;
+}
+
+// preprocess + quantize of one raw image already on the device (raw, mask) into the prepared
+// slot (dst image, dst mask).
+template <typename T>
+static ar_status prepare_one(ar_context* ctx, const T* raw, const uint8_t* mask, int64_t ny,
+                             int64_t nx, void* dst, uint8_t* dmask) {
+    const ar_params& p = ctx->params;
+    int64_t N = ny * nx;
+    ar_status st;
+    const uint8_t* curmask = mask;
+    float* hp = nullptr;
+    if (p.preprocess == AR_PRE_HIGHPASS) {
+        int w = p.filter_width;
+        if (w < 1) return pfail(ctx, AR_ERR_INVALID_ARGUMENT, "filter width must be >= 1");
+        if ((st = ctx->flags.reserve(ctx, 64))) return st;
+        int f0[2] = {1, 0};
+        PCHK(hipMemcpyAsync(ctx->flags.ptr, f0, 8, hipMemcpyHostToDevice, ctx->stream));
+        mask_stats_kernel<T><<<gsz(N) > 4096 ? 4096 : gsz(N), 256, 0, ctx->stream>>>(
+            raw, mask, N, (int*)ctx->flags.ptr);
+        int f[2];
+        PCHK(hipGetLastError());
+        PCHK(hipMemcpyAsync(f, ctx->flags.ptr, 8, hipMemcpyDeviceToHost, ctx->stream));
+        PCHK(hipStreamSynchronize(ctx->stream));
+        bool allmask = f[0] != 0, anynan = f[1] != 0;
+        int wx = (int)(w < nx ? w : nx), wy = (int)(w < ny ? w : ny);
+        int lx = wx / 2, ly = wy / 2, rx = wx - 1 - lx, ry = wy - 1 - ly;
+        if ((st = ctx->work_f64.reserve(ctx, N * 8 + N * 4))) return st;
+        if ((st = ctx->work_f32.reserve(ctx, N * 4))) return st;
+        double* colsum = (double*)ctx->work_f64.ptr;
+        int32_t* colcnt = (int32_t*)(colsum + N);
+        hp = (float*)ctx->work_f32.ptr;
+        int64_t nb = ny < 16 ? ny : 16;
+        int64_t nthr = ((ny + nb - 1) / nb) * nx;
+        int gb = (int)((nthr + 255) / 256);
+        int gr = (int)((ny + 63) / 64);
+        if (allmask && !anynan) {
+            box_colpass_kernel<T, false, false><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
+            box_rowpass_highpass_kernel<T, false><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, hp);
+        } else if (allmask) {
+            box_colpass_kernel<T, false, true><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
+            box_rowpass_highpass_kernel<T, true><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, hp);
+        } else {
+            box_colpass_kernel<T, true, true><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
+            box_rowpass_highpass_kernel<T, true><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, hp);
+        }
+        PCHK(hipGetLastError());
+        // erosion uses the full filter width w (clamped to the image) like windowmin(f, w)
+        erode_finalize_kernel<<<(unsigned)((N + 255) / 256), 256, 0, ctx->stream>>>(
+            hp, mask, dmask, ny, nx, lx, rx, ly, ry, allmask ? 1 : 0);
+        PCHK(hipGetLastError());
+        curmask = dmask;
+    } else {
+        copy_u8_kernel<<<gsz(N), 256, 0, ctx->stream>>>(mask, dmask, N);
+        PCHK(hipGetLastError());
+        curmask = dmask;
+    }
+    if (p.quantize == AR_QUANT_UINT8) {
+        if (hp) return quantize_u8<float>(ctx, hp, curmask, N, (uint8_t*)dst);
+        return quantize_u8<T>(ctx, raw, curmask, N, (uint8_t*)dst);
+    }
+    // NoQuantize: Float32 output
+    if (hp) {
+        PCHK(hipMemcpyAsync(dst, hp, N * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        noquant_kernel<T><<<gsz(N), 256, 0, ctx->stream>>>(raw, N, (float*)dst);
+        PCHK(hipGetLastError());
+    }
+    return AR_OK;
+}
+
+static size_t dt_size(int32_t dt) {
+    switch (dt) {
+        case AR_U8: return 1;
+        case AR_I16: return 2;
+        case AR_F32: return 4;
+        case AR_F64: return 8;
+    }
+    return 0;
+}
+
+template <typename T>
+static ar_status prepare_slot(ar_context* ctx, const void* host_img, const uint8_t* host_mask,
+                              int64_t ny, int64_t nx, void* dst, uint8_t* dmask) {
+    int64_t N = ny * nx;
+    ar_status st;
+    if ((st = ctx->raw.reserve(ctx, N * sizeof(T))) || (st = ctx->rawmask.reserve(ctx, N)))
+        return st;
+    PCHK(hipMemcpyAsync(ctx->raw.ptr, host_img, N * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    if (host_mask) {
+        PCHK(hipMemcpyAsync(ctx->rawmask.ptr, host_mask, N, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        finite_mask_t_kernel<T><<<gsz(N), 256, 0, ctx->stream>>>((const T*)ctx->raw.ptr,
+                                                                 (uint8_t*)ctx->rawmask.ptr, N);
+        PCHK(hipGetLastError());
+    }
+    return prepare_one<T>(ctx, (const T*)ctx->raw.ptr, (const uint8_t*)ctx->rawmask.ptr, ny, nx,
+                          dst, dmask);
+}
+
+extern "C" ar_status ar_set_images(ar_context* ctx, const void* ref, const void* sec,
+                                   const uint8_t* ref_valid, const uint8_t* sec_valid, int64_t ny,
+                                   int64_t nx, int32_t dtype) {
+    if (!ctx) return AR_ERR_INVALID_ARGUMENT;
+    if (!ref && !sec) return AR_OK;
+    if (ny <= 0 || nx <= 0) return pfail(ctx, AR_ERR_INVALID_ARGUMENT, "image size must be positive");
+    if (dt_size(dtype) == 0) return pfail(ctx, AR_ERR_UNSUPPORTED, "unsupported element type");
+    if ((!ref || !sec) && (!ctx->has_pair || ctx->ny != ny || ctx->nx != nx))
+        return pfail(ctx, AR_ERR_DIMENSION_MISMATCH,
+                     "new images must match the size the context was built for; build a new "
+                     "context (init) for a different image size");
+    const ar_params& p = ctx->params;
+    if (p.preprocess != AR_PRE_NONE && p.preprocess != AR_PRE_HIGHPASS)
+        return pfail(ctx, AR_ERR_UNSUPPORTED, "pre-filter not available on the device path");
+    PCHK(hipSetDevice(ctx->device));
+    int32_t pdt = p.quantize == AR_QUANT_UINT8 ? AR_U8 : AR_F32;
+    if (p.preprocess == AR_PRE_NONE && p.quantize == AR_QUANT_NONE && dtype == AR_U8) pdt = AR_U8;
+    if (ctx->has_pair && (!ref || !sec) && pdt != ctx->pdtype)
+        return pfail(ctx, AR_ERR_INVALID_ARGUMENT, "prepared element type changed");
+    int64_t N = ny * nx;
+    size_t es = pdt == AR_U8 ? 1 : 4;
+    ar_status st;
+    if (!ctx->has_pair || ctx->ny != ny || ctx->nx != nx || ctx->pdtype != pdt) {
+        if ((st = ctx->pref.reserve(ctx, N * es)) || (st = ctx->psec.reserve(ctx, N * es)) ||
+            (st = ctx->rmask.reserve(ctx, N)) || (st = ctx->smask.reserve(ctx, N)))
+            return st;
+    }
+    for (int slot = 0; slot < 2; slot++) {
+        const void* img = slot == 0 ? ref : sec;
+        const uint8_t* m = slot == 0 ? ref_valid : sec_valid;
+        if (!img) continue;
+        void* dst = slot == 0 ? ctx->pref.ptr : ctx->psec.ptr;
+        uint8_t* dm = (uint8_t*)(slot == 0 ? ctx->rmask.ptr : ctx->smask.ptr);
+        if (pdt == AR_U8 && p.preprocess == AR_PRE_NONE && p.quantize == AR_QUANT_NONE) {
+            // an integer image passes through unchanged
+            PCHK(hipMemcpyAsync(dst, img, N, hipMemcpyHostToDevice, ctx->stream));
+            if (m) PCHK(hipMemcpyAsync(dm, m, N, hipMemcpyHostToDevice, ctx->stream));
+            else {
+                if ((st = ctx->rawmask.reserve(ctx, N))) return st;
+                std::vector<uint8_t> ones(N, 1);
+                PCHK(hipMemcpyAsync(dm, ones.data(), N, hipMemcpyHostToDevice, ctx->stream));
+                PCHK(hipStreamSynchronize(ctx->stream));
+            }
+            continue;
+        }
+        switch (dtype) {
+            case AR_U8: st = prepare_slot<uint8_t>(ctx, img, m, ny, nx, dst, dm); break;
+            case AR_I16: st = prepare_slot<int16_t>(ctx, img, m, ny, nx, dst, dm); break;
+            case AR_F32: st = prepare_slot<float>(ctx, img, m, ny, nx, dst, dm); break;
+            default: st = prepare_slot<double>(ctx, img, m, ny, nx, dst, dm); break;
+        }
+        if (st) return st;
+        PCHK(hipStreamSynchronize(ctx->stream));
+    }
+    ctx->ny = ny;
+    ctx->nx = nx;
+    ctx->pdtype = pdt;
+    ctx->has_pair = true;
+    // masks are already in rmask/smask: combine without re-uploading
+    return ar_upload_masks(ctx, nullptr, nullptr, N);
+}

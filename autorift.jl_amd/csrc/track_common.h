@@ -1,0 +1,213 @@
+// Shared device-side pieces of the per-point tracker: pass arguments, point geometry,
+// block reductions, the OpenCV-exact argmax rule and the pyrUp refinement cascade.
+//
+// Arithmetic that must match the CPU path bit for bit is written without FMA contraction
+// (the library is compiled with -ffp-contract=off) and in the reference's operation order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define AR_WG 256
+#define AR_NWAVES (AR_WG / 64)
+
+namespace ar {
+
+struct TrackArgs {
+    const void* ref;          // prepared reference image, column-major ny x nx
+    const void* sec;          // prepared secondary image
+    const uint8_t* ok;        // valid(ref) & valid(sec); nullptr when every pixel is valid
+    int64_t ny, nx;           // image size
+    int64_t padx, pady;       // virtual zero padding of the pass (0 when every window fits)
+    const double* x;          // point fields (src/points.jl:50-74), 1-based coordinates
+    const double* y;
+    const double* dx_prior;
+    const double* dy_prior;
+    const int64_t* radius_x;
+    const int64_t* radius_y;
+    const int64_t* chip_size_x;
+    const int64_t* chip_size_y;
+    int64_t n;
+    int32_t up;               // upsampling; 1 = integer peak only
+    int32_t measure;          // 0 ZNCC, 1 NCC
+    float* dx;                // outputs, one per point
+    float* dy;
+    float* corr;
+    uint8_t* searched;
+    // per-pass maxima the LDS layout is carved for
+    int32_t max_ch, max_cw, max_rx, max_ry;
+    size_t lds_bytes;
+    // generic path: per-block global scratch
+    unsigned char* scratch;
+    size_t scratch_per_block;
+};
+
+// Geometry of one point in padded coordinates (1-based, inclusive), src/points.jl:358-395 and
+// the pass shift of src/track.jl:207-210.
+struct Geom {
+    int64_t cr0, cc0;   // chip top-left
+    int64_t wr0, wc0;   // window top-left
+    int ch, cw, rx, ry; // chip extent (even) and radii
+    double pdx, pdy;
+    int status;         // 0 skip (not searchable / out of bounds), 1 go
+};
+
+__device__ inline int64_t jfloor(double v) { return (int64_t)floor(v); }
+
+__device__ inline Geom point_geom(const TrackArgs& a, int64_t p) {
+    Geom g;
+    g.status = 0;
+    int64_t prx = a.radius_x[p], pry = a.radius_y[p];
+    g.rx = (int)prx;
+    g.ry = (int)pry;
+    if (!(prx > 0 && pry > 0)) return g;
+    double X = a.x[p] + ((double)a.padx + 0.5);
+    double Y = a.y[p] + ((double)a.pady + 0.5);
+    g.pdx = a.dx_prior[p];
+    g.pdy = a.dy_prior[p];
+    int64_t hx = a.chip_size_x[p] / 2, hy = a.chip_size_y[p] / 2;
+    int64_t ccx = jfloor(X - g.pdx), ccy = jfloor(Y - g.pdy);
+    g.cr0 = ccy - hy;
+    g.cc0 = ccx - hx;
+    int64_t cr1 = ccy + hy - 1, cc1 = ccx + hx - 1;
+    int64_t wcx = jfloor(X), wcy = jfloor(Y);
+    g.wr0 = wcy - hy - pry;
+    g.wc0 = wcx - hx - prx;
+    int64_t wr1 = wcy + hy + pry - 2, wc1 = wcx + hx + prx - 2;
+    g.ch = (int)(2 * hy);
+    g.cw = (int)(2 * hx);
+    int64_t NY = a.ny + 2 * a.pady, NX = a.nx + 2 * a.padx;
+    bool chip_in = (cr1 < g.cr0 || cc1 < g.cc0) ||
+                   (g.cr0 >= 1 && cr1 <= NY && g.cc0 >= 1 && cc1 <= NX);
+    bool win_in = (wr1 < g.wr0 || wc1 < g.wc0) ||
+                  (g.wr0 >= 1 && wr1 <= NY && g.wc0 >= 1 && wc1 <= NX);
+    if (!chip_in || !win_in) return g;
+    if (g.ch == 0 || g.cw == 0) return g;   // empty chip: _any_valid is false
+    g.status = 1;
+    return g;
+}
+
+// Padded (1-based) pixel -> linear index into the unpadded image, or -1 in the padding.
+__device__ inline int64_t pix(const TrackArgs& a, int64_t r, int64_t c) {
+    int64_t orow = r - a.pady - 1, ocol = c - a.padx - 1;
+    if (orow < 0 || orow >= a.ny || ocol < 0 || ocol >= a.nx) return -1;
+    return orow + ocol * a.ny;
+}
+
+// _any_valid (src/track.jl:416-421) over the chip rectangle. Block-uniform result.
+__device__ inline bool any_valid(const TrackArgs& a, const Geom& g) {
+    if (a.ok == nullptr) {
+        // every real pixel valid: the chip has a valid pixel iff it overlaps the image
+        int64_t r0 = g.cr0 - a.pady - 1, c0 = g.cc0 - a.padx - 1;
+        return r0 + g.ch - 1 >= 0 && r0 < a.ny && c0 + g.cw - 1 >= 0 && c0 < a.nx;
+    }
+    int found = 0;
+    int tot = g.ch * g.cw;
+    for (int t = threadIdx.x; t < tot; t += AR_WG) {
+        int u = t % g.ch, v = t / g.ch;
+        int64_t k = pix(a, g.cr0 + u, g.cc0 + v);
+        if (k >= 0 && a.ok[k]) found = 1;
+    }
+    return __syncthreads_or(found) != 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------------------------
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Block sum; every thread gets the total. `red` holds AR_NWAVES elements.
+template <typename T>
+__device__ inline T block_sum(T v, T* red) {
+    v = wave_sum(v);
+    int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    T s = red[0];
+    for (int k = 1; k < AR_NWAVES; k++) s += red[k];
+    return s;
+}
+
+// Argmax key under peak_index's rule (src/peak.jl:32-61): larger value wins; equal values go to
+// the smaller row-major index; NaN never wins.
+struct Best {
+    float v;
+    int idx;   // row-major linear index, INT32_MAX when empty
+};
+
+__device__ inline bool better(float v, int idx, const Best& b) {
+    return v > b.v || (v == b.v && idx < b.idx);
+}
+
+__device__ inline Best best_merge(Best a, Best b) { return better(b.v, b.idx, a) ? b : a; }
+
+__device__ inline Best wave_best(Best b) {
+    for (int o = 32; o > 0; o >>= 1) {
+        Best c;
+        c.v = __shfl_xor(b.v, o);
+        c.idx = __shfl_xor(b.idx, o);
+        b = best_merge(b, c);
+    }
+    return b;
+}
+
+__device__ inline Best block_best(Best b, Best* red) {
+    b = wave_best(b);
+    int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = b;
+    __syncthreads();
+    Best r = red[0];
+    for (int k = 1; k < AR_NWAVES; k++) r = best_merge(r, red[k]);
+    return r;
+}
+
+__device__ inline Best best_empty() {
+    Best b;
+    b.v = -INFINITY;
+    b.idx = 0x7fffffff;
+    return b;
+}
+
+// ---------------------------------------------------------------------------------------------
+// pyrUp (src/peak.jl:196-337): taps in upsampled coordinates with BORDER_REFLECT_101
+// ---------------------------------------------------------------------------------------------
+
+__device__ inline int reflect101_0(int p, int n) {
+    if (n == 1) return 0;
+    int m = n - 1;
+    while (p < 0 || p > m) {
+        if (p < 0) p = -p;
+        if (p > m) p = 2 * m - p;
+    }
+    return p;
+}
+
+// 0-based output position k on an axis with n source samples -> 0-based source taps.
+__device__ inline void taps(int k, int n, int& m, int& c, int& q, int& nt) {
+    if ((k & 1) == 0) {
+        c = k / 2;
+        m = reflect101_0(k - 2, 2 * n) / 2;
+        q = reflect101_0(k + 2, 2 * n) / 2;
+        nt = 3;
+    } else {
+        c = reflect101_0(k - 1, 2 * n) / 2;
+        q = reflect101_0(k + 1, 2 * n) / 2;
+        m = c;
+        nt = 2;
+    }
+}
+
+#define AR_WC (6.0f / 8)
+#define AR_WS (1.0f / 8)
+#define AR_WH (4.0f / 8)
+
+__device__ inline float tap3(float a, float b, float c) { return AR_WC * b + AR_WS * (a + c); }
+__device__ inline float tap2(float b, float c) { return AR_WH * (b + c); }
+
+}  // namespace ar

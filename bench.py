@@ -1,0 +1,183 @@
+"""Benchmark: NCC search points/s at chip 32, search radius 25, 64x subpixel refinement.
+
+Workload: a 10000x10000 seeded band-limited texture (Gaussian blur sigma 1.5 px), the secondary
+warped by a smooth field (sum of 8 Gaussians, peak 20 px, bilinear) plus N(0, 2^2) noise; both
+quantised to UInt8 as the default pipeline does; grid spacing 24 px with the margin of the
+largest chip level (128), i.e. 410 x 410 = 168 100 search points, each correlated with a 32 x 32
+chip over a 50 x 50 shift surface and refined 64x. One step = one tracking pass over all points
+with images and points already resident in HBM.
+
+Prints one JSON line (see the driver contract): value = searched points per second.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "autorift.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+INT8_PEAK_TOPS = 5000.0          # MI355X dense int8 MFMA, TOP/s
+FLOP_PER_POINT = 2 * 50 * 50 * 32 * 32   # useful multiply-adds of the direct contraction, x2
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def synthetic_pair(n, seed=3):
+    """Seeded texture and a smoothly warped, noisy copy (float32)."""
+    from scipy import ndimage
+    rng = np.random.default_rng(seed)
+    tex = rng.standard_normal((n, n), dtype=np.float32)
+    tex = ndimage.gaussian_filter(tex, 1.5, truncate=4.0, mode="mirror")
+    tex -= tex.min()
+    tex *= 255.0 / tex.max()
+    # smooth displacement field: 8 Gaussians, scaled so max |d| = 20 px
+    frng = np.random.default_rng(seed + 1)
+    m = 256                       # evaluate coarsely, then upsample (field is smooth)
+    yy, xx = np.meshgrid(np.linspace(0, n, m), np.linspace(0, n, m), indexing="ij")
+    fx = np.zeros((m, m))
+    fy = np.zeros((m, m))
+    for _ in range(8):
+        cy, cx = frng.uniform(0, n, 2)
+        s = frng.uniform(500, 2000)
+        ax, ay = frng.normal(0, 1, 2)
+        g = np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * s * s))
+        fx += ax * g
+        fy += ay * g
+    scale = 20.0 / max(np.abs(fx).max(), np.abs(fy).max())
+    fx = ndimage.zoom(fx * scale, n / m, order=1).astype(np.float32)
+    fy = ndimage.zoom(fy * scale, n / m, order=1).astype(np.float32)
+    sec = np.empty_like(tex)
+    step = 1000
+    for r0 in range(0, n, step):
+        r1 = min(n, r0 + step)
+        I, J = np.meshgrid(np.arange(r0, r1, dtype=np.float32), np.arange(n, dtype=np.float32),
+                           indexing="ij")
+        coords = np.stack([I - fy[r0:r1], J - fx[r0:r1]])
+        sec[r0:r1] = ndimage.map_coordinates(tex, coords, order=1, mode="nearest")
+    sec += np.random.default_rng(seed + 2).normal(0, 2, sec.shape).astype(np.float32)
+    return tex, sec
+
+
+def to_u8(img):
+    """Synthetic-input quantisation (mean +- 3 sd onto 0..255); stands in for the pipeline's
+    pre-filter when only the tracking pass is timed."""
+    m = float(img.mean(dtype=np.float64))
+    s = float(img.std(dtype=np.float64))
+    return np.clip(np.rint((img - (m - 3 * s)) * (255.0 / (6 * s))), 0, 255).astype(np.uint8)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=10000)
+    ap.add_argument("--spacing", type=int, default=24)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="points in the CPU baseline sample (0: sized to ~15 s)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        sys.exit(2)
+
+    import torch
+
+    import autorift
+
+    n = args.size
+    t0 = time.time()
+    tex, sec = synthetic_pair(n)
+    ref8, sec8 = to_u8(tex), to_u8(sec)
+    del tex, sec
+    log(f"synthetic pair {n}x{n} in {time.time() - t0:.1f}s")
+    grid = autorift.gridpoints((n, n), args.spacing, chip_size=128, search_radius=25)
+    pts = grid.rebuild(chip_size_x=np.full(grid.shape, 32, np.int64),
+                       chip_size_y=np.full(grid.shape, 32, np.int64))
+    npts = len(pts)
+    ctx = autorift.Context()
+    ctx.set_pair(ref8, sec8)
+
+    dev = torch.device("cuda:0")
+    fields = [torch.from_numpy(np.ascontiguousarray(np.ravel(getattr(pts, f), order="F"))).to(dev)
+              for f in ("x", "y", "radius_x", "radius_y", "dx_prior", "dy_prior", "chip_size_x",
+                        "chip_size_y")]
+    out = [torch.empty(npts, dtype=torch.float32, device=dev) for _ in range(3)]
+    out.append(torch.empty(npts, dtype=torch.uint8, device=dev))
+    fptr = [t.data_ptr() for t in fields]
+    optr = [t.data_ptr() for t in out]
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        ctx.track_device(fptr, npts, 64, 0, optr)
+    torch.cuda.synchronize()
+    kms = []
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.track_device(fptr, npts, 64, 0, optr)
+        kms.append(ctx.last_kernel()[0])
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    ms_step = (t2 - t1) * 1000 / args.steps
+    searched = int(out[3].sum().item())
+    value = searched / (ms_step / 1000)
+    kavg = float(np.mean(kms))
+    achieved = searched * FLOP_PER_POINT / (kavg / 1000) / 1e12
+    dx = out[0].cpu().numpy()
+    log(f"{npts} points, searched {searched}, {ms_step:.2f} ms/step, kernel {kavg:.2f} ms, "
+        f"measured {np.count_nonzero(~np.isnan(dx))}, path {ctx.last_kernel()[1]}")
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        k = args.cpu_sample or 4000
+        sel = np.linspace(0, npts - 1, k).astype(np.int64)
+        flat = {f: np.ravel(getattr(pts, f), order="F")[sel]
+                for f in ("x", "y", "radius_x", "radius_y", "dx_prior", "dy_prior",
+                          "chip_size_x", "chip_size_y")}
+        ok = np.ones((n, n), bool)
+        tc = time.perf_counter()
+        oracle.track(ref8, sec8, ok, flat["x"], flat["y"], flat["radius_x"], flat["radius_y"],
+                     flat["chip_size_x"], flat["chip_size_y"], up=64,
+                     num_mode=oracle.NUM_FFT, nthreads=threads)
+        el = time.perf_counter() - tc
+        cpu = {"value": k / el, "unit": "points/s", "cores": threads, "kind": "port",
+               "sample": f"{k} of the {npts} grid points (evenly spaced), same images, FFT "
+                         f"numerator, OpenMP dynamic schedule, {el:.1f} s"}
+        log(f"cpu baseline {k / el:.0f} points/s on {threads} threads")
+
+    line = {
+        "metric": "NCC search points/sec (chip 32, r=25), %roofline; p50/p99 pair latency",
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": f"track pass, {n}x{n} UInt8 pair, {npts} grid points "
+                               f"(spacing {args.spacing}), chip 32, r 25, PyramidRefine(64)",
+                   "points": npts},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
+                     "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

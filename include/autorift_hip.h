@@ -1,0 +1,156 @@
+/*
+ * autorift_hip -- C ABI of the MI355X (gfx950) dense ZNCC tracker.
+ *
+ * Drop-in boundary for AutoRIFT.jl's hot path. A Julia host keeps `autorift`, `init`,
+ * `reinit!`, `autorift!`, the `Cache` lifecycle and `PointSet` (src/api.jl:30-295,
+ * src/points.jl:50-74) and calls these entry points through `ccall`, the way src/plans.jl
+ * calls FFTW (opaque handle, caller-owned buffers, status returns).
+ *
+ * Conventions
+ *   - Images and gridded point fields are column-major (Julia layout); pixel coordinates are
+ *     1-based Float64 as in src/points.jl.
+ *   - The caller owns every host buffer for the duration of a call; the library never keeps a
+ *     host pointer. The context owns its device memory.
+ *   - No exception crosses the ABI: every entry point returns an ar_status and
+ *     ar_last_error() gives the message.
+ *   - A context is not thread-safe (one per task, like Cache, src/api.jl:27-28); distinct
+ *     contexts may run concurrently, each on its own HIP stream.
+ */
+#ifndef AUTORIFT_HIP_H
+#define AUTORIFT_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    AR_OK = 0,
+    AR_ERR_INVALID_ARGUMENT = 1,   /* ArgumentError in the reference */
+    AR_ERR_DIMENSION_MISMATCH = 2, /* DimensionMismatch, e.g. src/api.jl:120-123 */
+    AR_ERR_OUT_OF_MEMORY = 3,      /* device allocation failed */
+    AR_ERR_DEVICE = 4,             /* HIP runtime error */
+    AR_ERR_UNSUPPORTED = 5,        /* element type or geometry the device path does not take */
+    AR_ERR_NO_DEVICE = 6           /* no gfx950 device visible */
+} ar_status;
+
+typedef enum { AR_U8 = 0, AR_F32 = 1, AR_F64 = 2, AR_I16 = 3 } ar_dtype;
+typedef enum { AR_ZNCC = 0, AR_NCC = 1 } ar_measure;
+typedef enum { AR_PRE_NONE = 0, AR_PRE_HIGHPASS = 1 } ar_preprocess;
+typedef enum { AR_QUANT_NONE = 0, AR_QUANT_UINT8 = 1 } ar_quantize;
+
+/* Flattened Params (src/types.jl:781-829) with the defaults of params() (src/params.jl:306-392).
+ * Built by the host from its resolved parameter set; the library never parses strings. */
+typedef struct {
+    int32_t preprocess;           /* ar_preprocess, default AR_PRE_HIGHPASS */
+    int32_t filter_width;         /* Highpass width, default 5 */
+    int32_t quantize;             /* ar_quantize, default AR_QUANT_UINT8 */
+    int32_t measure;              /* ar_measure, default AR_ZNCC */
+    int64_t upsampling;           /* PyramidRefine(64); 1 means NoRefine */
+    int32_t outliers;             /* 1 GardnerFilter, 0 NoOutlierFilter */
+    int32_t _pad0;
+    int64_t gardner_window;       /* 5 */
+    int64_t gardner_iterations;   /* 3 */
+    double min_agree_fraction;    /* 8/25 */
+    double agree_tolerance;       /* 0.2 */
+    double mad_scale;             /* 4.0 */
+    int64_t chip_size_base;       /* 32 */
+    int64_t chip_size_min;        /* = base */
+    int64_t chip_size_max;        /* = 4 base */
+    double chip_aspect;           /* 1.0 */
+    int64_t grid_spacing;         /* = base */
+    int64_t search_radius_x;      /* 25 */
+    int64_t search_radius_y;      /* 25 */
+    int64_t min_search_radius;    /* 6 */
+    int64_t coarse_stride;        /* 4 */
+    int64_t coarse_buffer;        /* 8 */
+    double min_coarse_valid_fraction; /* 0.01 */
+    double dx_prior;              /* 0.0 */
+    double dy_prior;              /* 0.0 */
+    int64_t fill_window;          /* 3 */
+} ar_params;
+
+/* PointSet fields (src/points.jl:50-74), structure of arrays of equal length. */
+typedef struct {
+    const double* x;
+    const double* y;
+    const int64_t* radius_x;
+    const int64_t* radius_y;
+    const double* dx_prior;
+    const double* dy_prior;
+    const int64_t* chip_size_x;
+    const int64_t* chip_size_y;
+} ar_points;
+
+typedef struct ar_context ar_context;
+
+/* Fill `p` with the defaults of params() (src/params.jl:306-392). */
+void ar_params_default(ar_params* p);
+
+/* init (src/api.jl:66-76): opaque per-Cache state on `device` (HIP ordinal). */
+ar_status ar_context_create(const ar_params* p, int32_t device, ar_context** out);
+void ar_context_destroy(ar_context* ctx);
+
+/* init / reinit! image preparation (src/api.jl:104-134, 305-311): uploads the raw pair, runs
+ * preprocess (src/preprocess.jl:126-159, 304-312, 447-460) and quantize
+ * (src/preprocess.jl:473-531, 546-592) on the device and keeps the prepared pair resident.
+ * A NULL mask is derived from isfinite (src/preprocess.jl:74-79). A NULL image keeps the
+ * current one of that slot; sizes must match the context's (DimensionMismatch otherwise). */
+ar_status ar_set_images(ar_context* ctx, const void* ref, const void* sec,
+                        const uint8_t* ref_valid, const uint8_t* sec_valid, int64_t ny,
+                        int64_t nx, int32_t dtype);
+
+/* An already prepared ImagePair (src/preprocess.jl:41-79), used as is: what track! sees.
+ * dtype AR_U8 or AR_F32. */
+ar_status ar_set_pair(ar_context* ctx, const void* ref, const void* sec,
+                      const uint8_t* ref_valid, const uint8_t* sec_valid, int64_t ny, int64_t nx,
+                      int32_t dtype);
+
+/* Copies the prepared pair back (for inspection and tests). Element type is the prepared one:
+ * u8 after AR_QUANT_UINT8, f32 otherwise. Any pointer may be NULL. */
+ar_status ar_get_pair(ar_context* ctx, void* ref, void* sec, uint8_t* ref_valid,
+                      uint8_t* sec_valid, int32_t* dtype);
+
+/* track (src/track.jl:103-160) over n points with host buffers. upsampling 1 = integer peak
+ * (NoRefine), else PyramidRefine(upsampling), a power of two <= 64. Writes dx, dy,
+ * correlation (NaN where there is no measurement) and searched (0/1). */
+ar_status ar_track(ar_context* ctx, const ar_points* pts, int64_t n, int64_t upsampling,
+                   int32_t measure, float* dx, float* dy, float* corr, uint8_t* searched);
+
+/* As ar_track, with every pointer (point fields and outputs) in device memory of the context's
+ * device. Runs on the context's stream and returns after the stream has drained. */
+ar_status ar_track_device(ar_context* ctx, const ar_points* dev_pts, int64_t n,
+                          int64_t upsampling, int32_t measure, float* dev_dx, float* dev_dy,
+                          float* dev_corr, uint8_t* dev_searched);
+
+/* correlate_multichip (src/multichip.jl:106-137) over a gridded point set (nr x nc,
+ * column-major fields; chip sizes are set per level, so chip_size_x/y may be NULL).
+ * interpolated is returned as bytes (the host packs a BitMatrix). */
+ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, int64_t nr, int64_t nc,
+                           float* dx, float* dy, float* corr, uint16_t* chip_size,
+                           uint8_t* interpolated);
+
+/* Debug path: one chip against one window, as correlate! (src/correlate.jl:528-557) followed by
+ * peak_offset / subpixel_peak (src/peak.jl:102-106, 456-522). search is
+ * (ch + 2ry - 1) x (cw + 2rx - 1) and chip ch x cw, both column-major; ch and cw even.
+ * dx, dy are NaN for a degenerate (constant) chip. Uses the context's device only. */
+ar_status ar_correlate_point(ar_context* ctx, const void* search, const void* chip,
+                             int32_t dtype, int64_t ch, int64_t cw, int64_t rx, int64_t ry,
+                             int64_t upsampling, int32_t measure, double* dx, double* dy,
+                             float* corr);
+
+/* Device timing of the last ar_track / ar_track_device kernel, in milliseconds (HIP events on
+ * the context's stream), and the kernel path it took: 0 u8 matrix-core path, 1 generic. */
+ar_status ar_last_kernel(const ar_context* ctx, double* ms, int32_t* path);
+
+/* Message of the last failed call on ctx (or of the last failed ar_context_create when ctx is
+ * NULL). Valid until the next call on the same context. */
+const char* ar_last_error(const ar_context* ctx);
+
+/* Library version string. */
+const char* ar_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

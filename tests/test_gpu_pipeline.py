@@ -1,0 +1,134 @@
+"""GPU parity of image preparation and the multi-chip-size search against the CPU oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from _util import shifted_pair, synthetic_texture
+
+pytestmark = pytest.mark.gpu
+
+autorift = pytest.importorskip("autorift")
+
+
+def _params(**kw):
+    p = autorift.default_params()
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _oracle_prepare(img, mask, quantize=True, w=5):
+    hp, vm = oracle.highpass_filtered(img, mask, w)
+    if quantize:
+        return oracle.to_uint8(hp, vm), vm
+    return hp, vm   # already finite: _filtered zeroes non-finite values
+
+
+def _images(kind, n=(96, 130), seed=0):
+    rng = np.random.default_rng(seed)
+    t = synthetic_texture(n, seed=seed)
+    if kind == "u8":
+        return np.rint(t * 255).astype(np.uint8), None
+    if kind == "i16":
+        return np.rint(t * 4000 - 2000).astype(np.int16), None
+    if kind == "f64":
+        return (t * 100).astype(np.float64), None
+    img = (t * 255).astype(np.float32)
+    if kind == "f32_nan":
+        img[rng.random(n) < 0.01] = np.nan
+        img[10:20, 30:45] = np.nan
+        return img, None
+    if kind == "f32_mask":
+        m = np.ones(n, bool)
+        m[:, :7] = False
+        m[50:58, 60:90] = False
+        return img, m
+    return img, None
+
+
+@pytest.mark.parametrize("kind", ["u8", "i16", "f32", "f64", "f32_nan", "f32_mask"])
+@pytest.mark.parametrize("quant", [True, False])
+def test_prepare_matches_oracle(kind, quant):
+    img, mask = _images(kind)
+    img2, _ = _images(kind, seed=1)
+    p = _params(quantize=1 if quant else 0)
+    print(f"prepare {kind} quantize={quant}", flush=True)
+    with autorift.Context(p) as ctx:
+        ctx.set_images(img, img2, mask, mask)
+        r, s, rv, sv = ctx.get_pair()
+    m = np.isfinite(img.astype(np.float64)) if mask is None else mask
+    er, evm = _oracle_prepare(img, m, quant)
+    np.testing.assert_array_equal(rv, evm)
+    if quant:
+        assert r.dtype == np.uint8
+        np.testing.assert_array_equal(r, er)
+    else:
+        assert r.dtype == np.float32
+        np.testing.assert_array_equal(r.view(np.uint32), np.asarray(er, np.float32).view(np.uint32))
+
+
+def test_prepare_reinit_keeps_other_slot():
+    a, _ = _images("f32", seed=2)
+    b, _ = _images("f32", seed=3)
+    c, _ = _images("f32", seed=4)
+    with autorift.Context() as ctx:
+        ctx.set_images(a, b)
+        r0, s0, _, _ = ctx.get_pair()
+        ctx.set_images(secondary=c)
+        r1, s1, _, _ = ctx.get_pair()
+        np.testing.assert_array_equal(r0, r1)
+        ec, _ = _oracle_prepare(c, np.ones(c.shape, bool))
+        np.testing.assert_array_equal(s1, ec)
+        with pytest.raises(autorift.DimensionMismatch):
+            ctx.set_images(secondary=c[:50])
+
+
+def _grid(n, spacing, margin):
+    xs = np.arange(margin + 1, n - margin + 1, spacing, dtype=np.float64)
+    return autorift.gridpoints_xy(xs, xs, chip_size=32, search_radius=25)
+
+
+@pytest.mark.parametrize("case", ["shift_u8", "shift_f32_masked", "warp_u8_levels"])
+def test_multichip_matches_oracle(case):
+    if case == "shift_u8":
+        ref, sec = shifted_pair(400, (4.4, -3.2), seed=5, dtype=np.uint8)
+        ok = np.ones(ref.shape, bool)
+        grid = _grid(400, 8, 90)
+        p = _params(preprocess=0, quantize=0)
+        op = oracle.params(chip_size=32)
+    elif case == "shift_f32_masked":
+        ref, sec = shifted_pair(300, (-2.6, 1.9), seed=6, dtype=np.float32)
+        ok = np.ones(ref.shape, bool)
+        ok[100:140, 60:120] = False
+        grid = _grid(300, 6, 60)
+        p = _params(preprocess=0, quantize=0, chip_size_base=16, chip_size_min=16,
+                    chip_size_max=64)
+        op = oracle.params(chip_size=16, chip_size_max=64)
+    else:
+        # low texture in one half forces coarser levels to take over
+        ref, sec = shifted_pair(360, (1.5, 2.5), seed=7, dtype=np.float32)
+        ref = ref * 255
+        sec = sec * 255
+        ref[:, 180:] = ref[:, 180:] * 0.02 + 100
+        sec[:, 180:] = sec[:, 180:] * 0.02 + 100
+        rng = np.random.default_rng(1)
+        sec = (sec + rng.normal(0, 1, sec.shape)).astype(np.float32)
+        ref = np.clip(np.rint(ref), 0, 255).astype(np.uint8)
+        sec = np.clip(np.rint(sec), 0, 255).astype(np.uint8)
+        ok = np.ones(ref.shape, bool)
+        grid = _grid(360, 7, 60)
+        p = _params(preprocess=0, quantize=0, chip_size_base=16, chip_size_min=16,
+                    chip_size_max=64)
+        op = oracle.params(chip_size=16, chip_size_max=64)
+    print(f"multichip {case} grid {grid.shape}", flush=True)
+    with autorift.Context(p) as ctx:
+        ctx.set_pair(ref, sec, ok, ok)
+        g = ctx.run_multichip(grid)
+    e = oracle.multichip(ref, sec, ok, grid.x, grid.y, grid.radius_x, grid.radius_y,
+                         grid.dx_prior, grid.dy_prior, p=op, nthreads=8)
+    for a, b in zip((g.dx, g.dy, g.correlation), e[:3]):
+        np.testing.assert_array_equal(np.isnan(a), np.isnan(b))
+        np.testing.assert_array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
+    np.testing.assert_array_equal(g.chip_size, e[3])
+    np.testing.assert_array_equal(g.interpolated, e[4])
+    assert g.nmeasured() > 0.5 * g.dx.size
