@@ -6,13 +6,16 @@ reference (src/api.jl, src/points.jl, src/track.jl, src/multichip.jl); Julia's m
 """
 from ._lib import (AR_NCC, AR_ZNCC, ArgumentError, AutoRIFTError, DimensionMismatch,
                    NoDeviceError, Params, default_params, lib)
+from .api import (Cache, ImagePair, autorift, autorift_, autorift_with_grid, chip_sizes, init,
+                  params, reinit_, track)
 from .core import Context, DisplacementField, MultichipResult
 from .points import (PointSet, chip_bounds, gridpoints, gridpoints_xy, pointset, sanitize_,
                      search_bounds)
 
 __all__ = [
     "AR_NCC", "AR_ZNCC", "ArgumentError", "AutoRIFTError", "DimensionMismatch", "NoDeviceError",
-    "Params", "default_params", "lib", "Context", "DisplacementField", "MultichipResult",
-    "PointSet", "chip_bounds", "gridpoints", "gridpoints_xy", "pointset", "sanitize_",
-    "search_bounds",
+    "Params", "default_params", "lib", "Cache", "ImagePair", "autorift", "autorift_",
+    "autorift_with_grid", "chip_sizes", "init", "params", "reinit_", "track", "Context",
+    "DisplacementField", "MultichipResult", "PointSet", "chip_bounds", "gridpoints",
+    "gridpoints_xy", "pointset", "sanitize_", "search_bounds",
 ]

@@ -1,0 +1,105 @@
+"""CPU tests of the host side: the C ABI library loads and exports what include/*.h declares,
+parameter resolution, and point-set geometry (test/params.jl, test/points.jl)."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import autorift
+from autorift import ArgumentError, DimensionMismatch
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names |= set(re.findall(r"\b(ar_[a-z_0-9]+)\s*\(", src))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = autorift.lib()
+    declared = _declared()
+    assert {"ar_context_create", "ar_track", "ar_run_multichip", "ar_set_images"} <= declared
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    bound = {s[0] for s in autorift._lib.SIGNATURES}
+    assert declared == bound
+
+
+def test_version_and_defaults_without_device():
+    assert autorift.lib().ar_version().startswith(b"autorift_hip")
+    p = autorift.default_params()
+    assert (p.chip_size_base, p.chip_size_max, p.upsampling, p.search_radius_x) == (32, 128, 64, 25)
+    assert (p.gardner_window, p.gardner_iterations, p.fill_window) == (5, 3, 3)
+    assert abs(p.min_agree_fraction - 8 / 25) < 1e-15
+
+
+def test_no_device_is_a_status_not_a_crash():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is visible")
+    with pytest.raises(autorift.NoDeviceError):
+        autorift.Context()
+
+
+def test_params_defaults_and_validation():
+    p = autorift.params()
+    assert (p.chip_size_base, p.chip_size_min, p.chip_size_max, p.grid_spacing) == (32, 32, 128, 32)
+    assert p.preprocess == 1 and p.quantize == 1 and p.upsampling == 64 and p.outliers == 1
+    assert autorift.chip_sizes(p) == [32, 64, 128]
+    assert autorift.chip_sizes(autorift.params(chip_size=16, chip_size_max=128)) == [16, 32, 64, 128]
+    assert autorift.params(subpixel="none").upsampling == 1
+    assert autorift.params(search_radius=(10, 20)).search_radius_y == 20
+    for bad in (dict(chip_size=30), dict(chip_size=32, chip_size_max=96),
+                dict(chip_size=32, chip_size_min=16), dict(upsampling=48),
+                dict(similarity="cosine"), dict(outlier_window=4), dict(fill_window=2),
+                dict(search_radius=0), dict(outliers="none", mad_scale=3.0),
+                dict(min_coarse_valid_fraction=1.5)):
+        with pytest.raises(ArgumentError):
+            autorift.params(**bad)
+
+
+def test_pointset_and_grid_geometry():
+    pts = autorift.pointset([10.0, 50.5, 200.0], [20.0, 60.0, 30.25], search_radius=25,
+                            chip_size=32)
+    assert len(pts) == 3 and pts.radius_x[1] == 25 and pts.chip_size_x[1] == 32
+    g = autorift.gridpoints((512, 512), 32, chip_size=32, search_radius=25)
+    # margin cld(32, 2) + 25 + 1 = 42 and centres 43:32:470 (src/points.jl:244-249)
+    assert g.shape == (14, 14)
+    assert g.x[0, 0] == 43 and g.y[0, 0] == 43 and g.x[0, 1] == 75 and g.y[1, 0] == 75
+    for i in range(len(g)):
+        (r0, r1), (c0, c1) = autorift.search_bounds(g, i)
+        assert r0 >= 1 and c0 >= 1 and r1 <= 512 and c1 <= 512
+    assert autorift.gridpoints_xy(np.arange(100.0, 201.0, 10.0), np.arange(50.0, 101.0, 10.0),
+                                  chip_size=16).shape == (6, 11)
+    with pytest.raises(ArgumentError, match="no search center fits"):
+        autorift.gridpoints((32, 32), 8, chip_size=64, search_radius=25)
+    with pytest.raises(ArgumentError, match="must be positive"):
+        autorift.gridpoints((512, 512), 0)
+    with pytest.raises(ArgumentError):
+        autorift.gridpoints((40, 40), 8, chip_size=32, search_radius=25)
+    with pytest.raises(DimensionMismatch):
+        autorift.pointset([1.0, 2.0], [1.0])
+    with pytest.raises(ArgumentError):
+        autorift.pointset([1.0], [1.0], chip_size=32.5)
+    # chip_aspect rounds the height to even
+    assert autorift.pointset([1.0], [1.0], chip_size=32, chip_aspect=1.3).chip_size_y[0] == 42
+    # bounds: even chip centred half a pixel off, asymmetric window (src/points.jl:358-395)
+    p1 = autorift.pointset([100.0], [50.0], search_radius=10, chip_size=16, dx_prior=3.0)
+    assert autorift.chip_bounds(p1, 0) == ((42, 57), (89, 104))
+    assert autorift.search_bounds(p1, 0) == ((32, 66), (82, 116))
+    r, c = autorift.search_bounds(p1, 0)
+    assert (r[1] - r[0] + 1) == 16 + 2 * 10 - 1
+
+
+def test_sanitize():
+    pts = autorift.pointset(np.zeros(4), np.zeros(4), search_radius_x=[0, 3, 10, -1],
+                            search_radius_y=[5, 3, 2, 4])
+    assert autorift.sanitize_(pts, 6) == 2
+    assert list(pts.radius_x) == [0, 6, 10, 0] and list(pts.radius_y) == [0, 6, 6, 0]
