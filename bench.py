@@ -2,7 +2,7 @@
 
 Workload: a 10000x10000 seeded band-limited texture (Gaussian blur sigma 1.5 px), the secondary
 warped by a smooth field (sum of 8 Gaussians, peak 20 px, bilinear) plus N(0, 2^2) noise; both
-quantised to UInt8 as the default pipeline does; grid spacing 24 px with the margin of the
+prepared on the device by the default pipeline (high-pass width 5, UInt8 quantisation); grid spacing 24 px with the margin of the
 largest chip level (128), i.e. 410 x 410 = 168 100 search points, each correlated with a 32 x 32
 chip over a 50 x 50 shift surface and refined 64x. One step = one tracking pass over all points
 with images and points already resident in HBM.
@@ -66,12 +66,56 @@ def synthetic_pair(n, seed=3):
     return tex, sec
 
 
-def to_u8(img):
-    """Synthetic-input quantisation (mean +- 3 sd onto 0..255); stands in for the pipeline's
-    pre-filter when only the tracking pass is timed."""
-    m = float(img.mean(dtype=np.float64))
-    s = float(img.std(dtype=np.float64))
-    return np.clip(np.rint((img - (m - 3 * s)) * (255.0 / (6 * s))), 0, 255).astype(np.uint8)
+def cfg1_pair(seed, n=512, shift=(3.30, -1.70)):
+    """512^2 blurred white noise scaled to 0..255; the secondary is the reference Fourier-shifted
+    by (dx, dy) plus N(0, 2^2) noise (Float32)."""
+    from scipy import ndimage
+    rng = np.random.default_rng(seed)
+    a = ndimage.gaussian_filter(rng.standard_normal((n, n)), 1.5, truncate=4.0, mode="wrap")
+    a = (a - a.min()) * (255.0 / (a.max() - a.min()))
+    ky = np.fft.fftfreq(n)[:, None]
+    kx = np.fft.fftfreq(n)[None, :]
+    dx, dy = shift
+    b = np.real(np.fft.ifft2(np.fft.fft2(a) * np.exp(-2j * np.pi * (kx * dx + ky * dy))))
+    b = b + rng.normal(0, 2, b.shape)
+    return a.astype(np.float32), b.astype(np.float32)
+
+
+def pair_latency(autorift, npairs, warm=2):
+    """Default pipeline on 512^2 pairs (high-pass, UInt8, multi-chip search over the 784-point
+    grid 40:16:472), one context reused across pairs as init/reinit!/autorift! do."""
+    xs = np.arange(40.0, 473.0, 16.0)
+    grid = autorift.gridpoints_xy(xs, xs, chip_size=32, search_radius=25)
+    pairs = [cfg1_pair(100 + k) for k in range(npairs + warm)]
+    ctx = autorift.Context(autorift.params())
+    lat = []
+    meas = []
+    for k, (a, b) in enumerate(pairs):
+        t = time.perf_counter()
+        ctx.set_images(a, b)
+        r = ctx.run_multichip(grid)
+        el = time.perf_counter() - t
+        if k >= warm:
+            lat.append(el * 1000)
+            meas.append((np.nanmedian(r.dx), np.nanmedian(r.dy), r.nmeasured()))
+    ctx.close()
+    lat = np.array(lat)
+    return {"pairs": npairs, "grid_points": len(grid), "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "pairs_per_s": float(1000 / lat.mean()),
+            "median_dx": float(np.median([m[0] for m in meas])),
+            "median_dy": float(np.median([m[1] for m in meas])),
+            "expected_dx_dy": [-3.30, 1.70]}
+
+
+def copy_calibration(nbytes):
+    """A plain device copy of known size, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    x = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes}), flush=True)
 
 
 def main():
@@ -84,7 +128,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="points in the CPU baseline sample (0: sized to ~15 s)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--latency-pairs", type=int, default=20)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run a device copy of this many bytes (profiling calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md, not run)"}))
@@ -97,15 +147,18 @@ def main():
     n = args.size
     t0 = time.time()
     tex, sec = synthetic_pair(n)
-    ref8, sec8 = to_u8(tex), to_u8(sec)
-    del tex, sec
     log(f"synthetic pair {n}x{n} in {time.time() - t0:.1f}s")
     grid = autorift.gridpoints((n, n), args.spacing, chip_size=128, search_radius=25)
     pts = grid.rebuild(chip_size_x=np.full(grid.shape, 32, np.int64),
                        chip_size_y=np.full(grid.shape, 32, np.int64))
     npts = len(pts)
-    ctx = autorift.Context()
-    ctx.set_pair(ref8, sec8)
+    ctx = autorift.Context(autorift.params())
+    t0 = time.perf_counter()
+    ctx.set_images(tex, sec)          # default pipeline on the device: highpass(5) + UInt8
+    prep_s = time.perf_counter() - t0
+    del tex, sec
+    ref8, sec8, _, _ = ctx.get_pair()
+    log(f"device preparation (upload + highpass + quantize, both images) {prep_s:.2f}s")
 
     dev = torch.device("cuda:0")
     fields = [torch.from_numpy(np.ascontiguousarray(np.ravel(getattr(pts, f), order="F"))).to(dev)
@@ -135,6 +188,13 @@ def main():
     dx = out[0].cpu().numpy()
     log(f"{npts} points, searched {searched}, {ms_step:.2f} ms/step, kernel {kavg:.2f} ms, "
         f"measured {np.count_nonzero(~np.isnan(dx))}, path {ctx.last_kernel()[1]}")
+
+    lat = None
+    if args.latency_pairs > 0:
+        lat = pair_latency(autorift, args.latency_pairs)
+        log(f"pair latency p50 {lat['p50_ms']:.2f} ms p99 {lat['p99_ms']:.2f} ms "
+            f"({lat['pairs_per_s']:.1f} pairs/s), median dx {lat['median_dx']:.3f} "
+            f"dy {lat['median_dy']:.3f}")
 
     cpu = None
     if not args.no_cpu:
@@ -175,6 +235,8 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
                      "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": None},
         "cpu_baseline": cpu,
+        "pair_latency": lat,
+        "prepare_s": prep_s,
     }
     print(json.dumps(line), flush=True)
 
