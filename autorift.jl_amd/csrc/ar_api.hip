@@ -58,9 +58,11 @@ ar_status DevBuf::reserve(ar_context* ctx, size_t n) {
     if (ptr) hipFree(ptr);
     ptr = nullptr;
     cap = 0;
-    size_t want = n < 256 ? 256 : n;
+    // 256 bytes of slack: kernels may read a few bytes past the last element with aligned
+    // dword loads (never used, never faulting)
+    size_t want = (n < 256 ? 256 : n) + 256;
     HIPCHK(ctx, hipMalloc(&ptr, want));
-    cap = want;
+    cap = want - 256;
     return AR_OK;
 }
 

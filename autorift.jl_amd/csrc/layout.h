@@ -33,7 +33,7 @@ __host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx
     L.off_s1 = rup((int)(L.off_t + (size_t)max_cw * L.cpad), 16);
     L.off_s2 = L.off_s1 + (size_t)nr * nc * 4;
     L.off_tmp = L.off_s2 + (size_t)nr * nc * 4;
-    size_t colsum = (size_t)nr * winw * 4;
+    size_t colsum = (size_t)nr * winw * 8;   // W and W^2 column sums
     size_t accsurf = (size_t)nr * nc * 8;   // int32 accumulators + float approximations
     L.total = L.off_tmp + (colsum > accsurf ? colsum : accsurf);
     size_t casc = (size_t)cascade_floats(up) * 4 + 64;

@@ -84,6 +84,9 @@ __global__ void box_colpass_kernel(const T* img, const uint8_t* mask, int64_t nr
 // Row pass (src/window.jl:327-361 masked, 411-435 dense) fused with highpass
 // (src/preprocess.jl:304-312): out = mask & isfinite(mean) ? Float32(img) - mean : NaN.
 // Thread per row; adjacent threads read adjacent rows of one column (column-major), coalesced.
+// The running sum is sequential along the row, as in the reference; the loads it consumes do not
+// depend on it, so they are fetched RU columns ahead.
+#define RU 8
 template <typename T, bool NANAWARE>
 __global__ void box_rowpass_highpass_kernel(const T* img, const uint8_t* mask, int64_t nr,
                                             int64_t nc, int lx, int rx, int ly, int ry,
@@ -100,25 +103,48 @@ __global__ void box_rowpass_highpass_kernel(const T* img, const uint8_t* mask, i
         s += colsum[i + (j - 1) * nr];
         if (NANAWARE) c += colcnt[i + (j - 1) * nr];
     }
-    for (int64_t j = 1; j <= nc; j++) {
-        float mean;
-        if (NANAWARE) {
-            mean = c > 0 ? (float)(s / (double)c) : NAN;
-        } else {
-            int64_t cj = (j + rx < nc ? j + rx : nc) - (j - lx > 1 ? j - lx : 1) + 1;
-            mean = (float)(s / (double)(ci * cj));
+    for (int64_t j0 = 1; j0 <= nc; j0 += RU) {
+        double xo[RU], xi[RU];
+        int32_t co[RU], cn[RU];
+        uint8_t mk[RU];
+        float iv[RU];
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            int64_t j = j0 + u;
+            int64_t oj = j - lx, jj = j + rx + 1;
+            bool in = j <= nc;
+            bool vo = in && 1 <= oj && oj <= nc, vi = in && jj <= nc;
+            xo[u] = vo ? colsum[i + (oj - 1) * nr] : 0.0;
+            xi[u] = vi ? colsum[i + (jj - 1) * nr] : 0.0;
+            if (NANAWARE) {
+                co[u] = vo ? colcnt[i + (oj - 1) * nr] : 0;
+                cn[u] = vi ? colcnt[i + (jj - 1) * nr] : 0;
+            }
+            int64_t k = i + (in ? j - 1 : 0) * nr;
+            mk[u] = mask[k];
+            iv[u] = as_f32(img, k);
         }
-        int64_t k = i + (j - 1) * nr;
-        out[k] = (mask[k] && isfinite(mean)) ? as_f32(img, k) - mean : NAN;
-        int64_t oj = j - lx;
-        if (1 <= oj && oj <= nc) {
-            s -= colsum[i + (oj - 1) * nr];
-            if (NANAWARE) c -= colcnt[i + (oj - 1) * nr];
-        }
-        int64_t jj = j + rx + 1;
-        if (jj <= nc) {
-            s += colsum[i + (jj - 1) * nr];
-            if (NANAWARE) c += colcnt[i + (jj - 1) * nr];
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            int64_t j = j0 + u;
+            if (j > nc) break;
+            float mean;
+            if (NANAWARE) {
+                mean = c > 0 ? (float)(s / (double)c) : NAN;
+            } else {
+                int64_t cj = (j + rx < nc ? j + rx : nc) - (j - lx > 1 ? j - lx : 1) + 1;
+                mean = (float)(s / (double)(ci * cj));
+            }
+            out[i + (j - 1) * nr] = (mk[u] && isfinite(mean)) ? iv[u] - mean : NAN;
+            int64_t oj = j - lx, jj = j + rx + 1;
+            if (1 <= oj && oj <= nc) {
+                s -= xo[u];
+                if (NANAWARE) c -= co[u];
+            }
+            if (jj <= nc) {
+                s += xi[u];
+                if (NANAWARE) c += cn[u];
+            }
         }
     }
 }

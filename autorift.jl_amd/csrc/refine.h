@@ -41,25 +41,120 @@ struct CascadeOut {
     float val;
 };
 
-// One full pyrUp step in LDS: src (n x n, row-major) -> dst (2n x 2n), via tmp (2n x n).
-__device__ inline void pyrup_full(const float* src, float* tmp, float* dst, int n) {
-    int th = 2 * n;
-    for (int t = threadIdx.x; t < th * n; t += AR_WG) {
-        int i = t / n, c = t - i * n;
-        int m, c0, q, nt;
-        taps(i, n, m, c0, q, nt);
-        tmp[t] = nt == 3 ? tap3(src[m * n + c], src[c0 * n + c], src[q * n + c])
-                         : tap2(src[c0 * n + c], src[q * n + c]);
+// Pair form of the taps: source sample k of an axis of n samples feeds outputs 2k and 2k+1,
+//   out[2k]   = tap3(src[km], src[k], src[kq]),   out[2k+1] = tap2(src[k], src[kq]),
+// with km = k == 0 ? 1 : k - 1 and kq = min(k + 1, n - 1) -- exactly taps(2k) and taps(2k+1).
+// Three loads give two outputs.
+__device__ inline void pair_src(int k, int n, int& km, int& kq) {
+    km = k == 0 ? 1 : k - 1;
+    kq = k + 1 < n - 1 ? k + 1 : n - 1;
+}
+
+// The passes below give each thread U tasks at a time (t, t + 256, ...), issuing all loads
+// before any arithmetic so that several LDS reads are in flight per thread.
+#define AR_U 4
+
+// Vertical pass: rows 2k, 2k+1 for k in [k0, k1) of an axis of n source rows, W columns.
+// src holds source rows from s_off (pitch W); dst holds output rows from d_off (pitch W).
+template <int W>
+__device__ inline void vert_pairs(const float* src, int n, int s_off, float* dst, int d_off,
+                                  int k0, int k1) {
+    const int total = (k1 - k0) * W;
+    for (int t0 = threadIdx.x; t0 < total; t0 += AR_U * AR_WG) {
+        float a[AR_U], b[AR_U], q[AR_U];
+        int o[AR_U];
+#pragma unroll
+        for (int u = 0; u < AR_U; u++) {
+            int t = t0 + u * AR_WG;
+            t = t < total ? t : total - 1;
+            int kk = t / W, c = t - kk * W, k = k0 + kk;
+            int km, kq;
+            pair_src(k, n, km, kq);
+            a[u] = src[(km - s_off) * W + c];
+            b[u] = src[(k - s_off) * W + c];
+            q[u] = src[(kq - s_off) * W + c];
+            o[u] = (2 * k - d_off) * W + c;
+        }
+#pragma unroll
+        for (int u = 0; u < AR_U; u++) {
+            if (t0 + u * AR_WG < total) {
+                dst[o[u]] = tap3(a[u], b[u], q[u]);
+                dst[o[u] + W] = tap2(b[u], q[u]);
+            }
+        }
     }
+}
+
+// Horizontal pass: `rows` rows of WS source columns -> 2 WS output columns.
+template <int WS>
+__device__ inline void horiz_pairs(const float* src, float* dst, int rows) {
+    const int total = rows * WS;
+    for (int t0 = threadIdx.x; t0 < total; t0 += AR_U * AR_WG) {
+        float a[AR_U], b[AR_U], q[AR_U];
+        int o[AR_U];
+#pragma unroll
+        for (int u = 0; u < AR_U; u++) {
+            int t = t0 + u * AR_WG;
+            t = t < total ? t : total - 1;
+            int r = t / WS, k = t - r * WS;
+            int km, kq;
+            pair_src(k, WS, km, kq);
+            const float* row = src + r * WS;
+            a[u] = row[km];
+            b[u] = row[k];
+            q[u] = row[kq];
+            o[u] = r * 2 * WS + 2 * k;
+        }
+#pragma unroll
+        for (int u = 0; u < AR_U; u++) {
+            if (t0 + u * AR_WG < total) {
+                dst[o[u]] = tap3(a[u], b[u], q[u]);
+                dst[o[u] + 1] = tap2(b[u], q[u]);
+            }
+        }
+    }
+}
+
+// Horizontal pass of the final level straight into the argmax: row r of the strip is output row
+// row0 + r; sample (row, col) has row-major index row * 2WS + col.
+template <int WS>
+__device__ inline Best horiz_pairs_argmax(const float* src, int rows, int row0) {
+    Best best = best_empty();
+    const int total = rows * WS;
+    for (int t0 = threadIdx.x; t0 < total; t0 += AR_U * AR_WG) {
+        float a[AR_U], b[AR_U], q[AR_U];
+        int idx[AR_U];
+#pragma unroll
+        for (int u = 0; u < AR_U; u++) {
+            int t = t0 + u * AR_WG;
+            t = t < total ? t : total - 1;
+            int r = t / WS, k = t - r * WS;
+            int km, kq;
+            pair_src(k, WS, km, kq);
+            const float* row = src + r * WS;
+            a[u] = row[km];
+            b[u] = row[k];
+            q[u] = row[kq];
+            idx[u] = (row0 + r) * 2 * WS + 2 * k;
+        }
+#pragma unroll
+        for (int u = 0; u < AR_U; u++) {
+            if (t0 + u * AR_WG < total) {
+                float e = tap3(a[u], b[u], q[u]), o = tap2(b[u], q[u]);
+                if (better(e, idx[u], best)) { best.v = e; best.idx = idx[u]; }
+                if (better(o, idx[u] + 1, best)) { best.v = o; best.idx = idx[u] + 1; }
+            }
+        }
+    }
+    return best;
+}
+
+// One full pyrUp step in LDS: src (N x N, row-major) -> dst (2N x 2N), via tmp (2N x N).
+template <int N>
+__device__ inline void pyrup_full(const float* src, float* tmp, float* dst) {
+    vert_pairs<N>(src, N, 0, tmp, 0, 0, N);
     __syncthreads();
-    int tw = 2 * n;
-    for (int t = threadIdx.x; t < th * tw; t += AR_WG) {
-        int i = t / tw, j = t - i * tw;
-        int m, c0, q, nt;
-        taps(j, n, m, c0, q, nt);
-        const float* row = tmp + i * n;
-        dst[t] = nt == 3 ? tap3(row[m], row[c0], row[q]) : tap2(row[c0], row[q]);
-    }
+    horiz_pairs<N>(tmp, dst, 2 * N);
     __syncthreads();
 }
 
@@ -105,13 +200,11 @@ __device__ inline CascadeOut cascade(float* reg, int up, Best* red) {
 
     // Levels 1..LM, ping-ponging so that level LM lands in Y.
     const float* cur = patch;
-    int n = 5;
-    for (int k = 1; k <= LM; k++) {
-        float* dst = ((LM - k) & 1) ? X : Y;
-        pyrup_full(cur, T, dst, n);
-        cur = dst;
-        n *= 2;
-    }
+    if (LM >= 1) { float* d = ((LM - 1) & 1) ? X : Y; pyrup_full<5>(cur, T, d); cur = d; }
+    if (LM >= 2) { float* d = ((LM - 2) & 1) ? X : Y; pyrup_full<10>(cur, T, d); cur = d; }
+    if (LM >= 3) { float* d = ((LM - 3) & 1) ? X : Y; pyrup_full<20>(cur, T, d); cur = d; }
+    if (LM >= 4) { float* d = ((LM - 4) & 1) ? X : Y; pyrup_full<40>(cur, T, d); cur = d; }
+    int n = 5 << LM;
 
     CascadeOut out;
     if (L <= 4) {
@@ -187,66 +280,22 @@ __device__ inline CascadeOut cascade(float* reg, int up, Best* red) {
         int a = 16 * s;
         Best b = best_empty();
         if (L == 5) {
-            // tmp5 strip: vertical pass of level 4 for level-5 rows [a, a+16)
-            for (int t = threadIdx.x; t < 16 * n4; t += AR_WG) {
-                int ii = t / n4, c = t - ii * n4;
-                int m, c0, q, nt;
-                taps(a + ii, n4, m, c0, q, nt);
-                t5[t] = nt == 3 ? tap3(Y[m * n4 + c], Y[c0 * n4 + c], Y[q * n4 + c])
-                                : tap2(Y[c0 * n4 + c], Y[q * n4 + c]);
-            }
+            // level-5 rows [a, a+16) from level 4, then their samples into the argmax
+            vert_pairs<80>(Y, n4, 0, t5, a, a / 2, a / 2 + 8);
             __syncthreads();
-            for (int t = threadIdx.x; t < 16 * 160; t += AR_WG) {
-                int ii = t / 160, j = t - ii * 160;
-                int m, c0, q, nt;
-                taps(j, n4, m, c0, q, nt);
-                const float* row = t5 + ii * n4;
-                float v = nt == 3 ? tap3(row[m], row[c0], row[q]) : tap2(row[c0], row[q]);
-                int idx = (a + ii) * 160 + j;
-                if (better(v, idx, b)) { b.v = v; b.idx = idx; }
-            }
+            b = horiz_pairs_argmax<80>(t5, 16, a);
         } else {
-            int lo5, hi5;
-            feed_rows(a, a + 16, 160, lo5, hi5);
-            int nr5 = hi5 - lo5 + 1;
-            // tmp5 rows lo5..hi5 from level 4
-            for (int t = threadIdx.x; t < nr5 * n4; t += AR_WG) {
-                int ii = t / n4, c = t - ii * n4;
-                int m, c0, q, nt;
-                taps(lo5 + ii, n4, m, c0, q, nt);
-                t5[t] = nt == 3 ? tap3(Y[m * n4 + c], Y[c0 * n4 + c], Y[q * n4 + c])
-                                : tap2(Y[c0 * n4 + c], Y[q * n4 + c]);
-            }
+            // level-6 rows [a, a+16) need level-5 rows [a/2 - 1, a/2 + 8] (clamped), produced
+            // as whole pairs from level 4
+            int lo5 = a / 2 - 1 < 0 ? 0 : a / 2 - 1, hi5 = a / 2 + 8 > 159 ? 159 : a / 2 + 8;
+            int plo = lo5 / 2, phi = hi5 / 2, base5 = 2 * plo, nr5 = 2 * (phi - plo + 1);
+            vert_pairs<80>(Y, n4, 0, t5, base5, plo, phi + 1);
             __syncthreads();
-            // level-5 rows lo5..hi5 (160 wide)
-            for (int t = threadIdx.x; t < nr5 * 160; t += AR_WG) {
-                int ii = t / 160, j = t - ii * 160;
-                int m, c0, q, nt;
-                taps(j, n4, m, c0, q, nt);
-                const float* row = t5 + ii * n4;
-                lv5[t] = nt == 3 ? tap3(row[m], row[c0], row[q]) : tap2(row[c0], row[q]);
-            }
+            horiz_pairs<80>(t5, lv5, nr5);
             __syncthreads();
-            // tmp6 rows a..a+15 (160 wide) from level 5
-            for (int t = threadIdx.x; t < 16 * 160; t += AR_WG) {
-                int ii = t / 160, c = t - ii * 160;
-                int m, c0, q, nt;
-                taps(a + ii, 160, m, c0, q, nt);
-                m -= lo5; c0 -= lo5; q -= lo5;
-                t6[t] = nt == 3 ? tap3(lv5[m * 160 + c], lv5[c0 * 160 + c], lv5[q * 160 + c])
-                                : tap2(lv5[c0 * 160 + c], lv5[q * 160 + c]);
-            }
+            vert_pairs<160>(lv5, 160, base5, t6, a, a / 2, a / 2 + 8);
             __syncthreads();
-            // final level-6 samples of the strip, straight into the argmax
-            for (int t = threadIdx.x; t < 16 * 320; t += AR_WG) {
-                int ii = t / 320, j = t - ii * 320;
-                int m, c0, q, nt;
-                taps(j, 160, m, c0, q, nt);
-                const float* row = t6 + ii * 160;
-                float v = nt == 3 ? tap3(row[m], row[c0], row[q]) : tap2(row[c0], row[q]);
-                int idx = (a + ii) * 320 + j;
-                if (better(v, idx, b)) { b.v = v; b.idx = idx; }
-            }
+            b = horiz_pairs_argmax<160>(t6, 16, a);
         }
         b = block_best(b, red);
         best = best_merge(best, b);

@@ -146,39 +146,75 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         const int cols_used = 16 * ntj + cw;
 
         // ---- stage window (int8, value ^ 0x80; virtual zero padding -> 0x80) and chip
-        for (int t = tid; t < rows_used * cols_used; t += AR_WG) {
-            int c = t / rows_used, r = t - c * rows_used;
-            int8_t v = 0;
-            if (r < winh && c < winw) {
-                int64_t k = pix(a, g.wr0 + r, g.wc0 + c);
-                v = (int8_t)((k >= 0 ? ref[k] : 0) ^ 0x80);
+        {
+            // window origin in unpadded 0-based coordinates; interior windows skip the
+            // per-pixel padding test and keep several independent loads in flight
+            int64_t orow = g.wr0 - a.pady - 1, ocol = g.wc0 - a.padx - 1;
+            bool inside = orow >= 0 && ocol >= 0 && orow + winh <= a.ny && ocol + winw <= a.nx;
+            if (inside) {
+                // Each lane moves 4 rows of one column as a dword: two aligned dword loads and a
+                // byte realignment (the column's start address has any alignment). Half-waves take
+                // alternate columns; rows past the window are tile padding (int8 0).
+                const uint8_t* gb = ref + orow + ocol * a.ny;
+                const int nr4 = rows_used >> 2;
+                const int half = lane >> 5, l32 = lane & 31;
+                for (int c0 = 2 * wave; c0 < cols_used; c0 += 2 * AR_NWAVES) {
+                    const int c = c0 + half;
+                    const bool colok = c < cols_used;
+                    const uintptr_t cadr = (uintptr_t)(gb + (int64_t)c * a.ny);
+                    for (int r4 = l32; r4 < nr4; r4 += 32) {
+                        int r = 4 * r4;
+                        uint32_t v = 0;
+                        if (colok && c < winw && r < winh) {
+                            uintptr_t ad = cadr + (uintptr_t)r;
+                            const uint32_t* al = (const uint32_t*)(ad & ~(uintptr_t)3);
+                            uint32_t d0 = al[0], d1 = al[1];
+                            uint32_t sh = (uint32_t)(ad & 3) * 8;
+                            v = sh ? (d0 >> sh) | (d1 << (32 - sh)) : d0;
+                            v ^= 0x80808080u;
+                            int keep = winh - r;   // rows of this dword inside the window
+                            if (keep < 4) v &= (1u << (8 * keep)) - 1u;
+                        }
+                        if (colok) *(uint32_t*)(Wl + c * L.ldw + r) = v;
+                    }
+                }
+            } else {
+                for (int c = wave; c < cols_used; c += AR_NWAVES)
+                    for (int r = lane; r < rows_used; r += 64) {
+                        int8_t v = 0;
+                        if (r < winh && c < winw) {
+                            int64_t k = pix(a, g.wr0 + r, g.wc0 + c);
+                            v = (int8_t)((k >= 0 ? ref[k] : 0) ^ 0x80);
+                        }
+                        Wl[c * L.ldw + r] = v;
+                    }
             }
-            Wl[c * L.ldw + r] = v;
         }
         int64_t st = 0;
-        for (int t = tid; t < cw * L.cpad; t += AR_WG) {
-            int v = t / L.cpad, x = t - v * L.cpad, u = x - 16;
-            int8_t b = 0;
-            if (u >= 0 && u < ch) {
-                int64_t k = pix(a, g.cr0 + u, g.cc0 + v);
-                uint8_t raw = k >= 0 ? sec[k] : 0;
-                st += raw;
-                b = (int8_t)(raw ^ 0x80);
+        for (int v = wave; v < cw; v += AR_NWAVES)
+            for (int x = lane; x < L.cpad; x += 64) {
+                int u = x - 16;
+                int8_t b = 0;
+                if (u >= 0 && u < ch) {
+                    int64_t k = pix(a, g.cr0 + u, g.cc0 + v);
+                    uint8_t raw = k >= 0 ? sec[k] : 0;
+                    st += raw;
+                    b = (int8_t)(raw ^ 0x80);
+                }
+                Tp[v * L.cpad + x] = b;
             }
-            Tp[t] = b;
-        }
         U8Point P;
         P.n = (int64_t)ch * cw;
         P.sumT = block_sum<int64_t>(st, red64);
         __syncthreads();
         // ---- chip energy, exact: sum (n x - sumT)^2 over n^2
         int64_t e = 0;
-        for (int t = tid; t < ch * cw; t += AR_WG) {
-            int v = t / ch, u = t - v * ch;
-            int64_t x = (int64_t)(uint8_t)(Tp[v * L.cpad + 16 + u] ^ 0x80);
-            int64_t k = P.n * x - P.sumT;
-            e += k * k;
-        }
+        for (int v = wave; v < cw; v += AR_NWAVES)
+            for (int u = lane; u < ch; u += 64) {
+                int64_t x = (int64_t)(uint8_t)(Tp[v * L.cpad + 16 + u] ^ 0x80);
+                int64_t k = P.n * x - P.sumT;
+                e += k * k;
+            }
         int64_t sumk2 = block_sum<int64_t>(e, red64);
         double sq = (double)sumk2 / ((double)P.n * (double)P.n);
         bool ok = sq > 2.220446049250313e-16 * (double)ch * (double)cw;
@@ -186,29 +222,49 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         P.cnorm = sqrt(sq);
         P.inv_cnorm_f = (float)(1.0 / P.cnorm);
 
-        // ---- integer box sums S1 (W) and S2 (W^2) over every shift, sliding
-        for (int pass = 0; pass < 2; pass++) {
-            int32_t* S = pass == 0 ? S1 : S2;
-            for (int c = tid; c < winw; c += AR_WG) {
-                const int8_t* col = Wl + c * L.ldw;
-                int32_t s = 0;
+        // ---- integer box sums S1 (W) and S2 (W^2) over every shift: sliding sums, with each
+        // window column split into KC row chunks and each shift row into KR column chunks
+        {
+            constexpr int KC = 3, KR = 5;
+            int32_t* c1 = colS;
+            int32_t* c2 = colS + NR * winw;
+            const int lc = (NR + KC - 1) / KC, lr = (NC + KR - 1) / KR;
+            for (int t = tid; t < winw * KC; t += AR_WG) {
+                int c = t / KC, k = t - c * KC;
+                int r0 = k * lc, r1 = r0 + lc < NR ? r0 + lc : NR;
+                if (r0 >= r1) continue;
+                const uint8_t* col = (const uint8_t*)(Wl + c * L.ldw);
+                int32_t a1 = 0, a2 = 0;
                 for (int u = 0; u < ch; u++) {
-                    int32_t w = (uint8_t)(col[u] ^ 0x80);
-                    s += pass == 0 ? w : w * w;
+                    int32_t w = col[r0 + u] ^ 0x80;
+                    a1 += w;
+                    a2 += w * w;
                 }
-                for (int r0 = 0; r0 < NR; r0++) {
-                    colS[c * NR + r0] = s;
-                    int32_t wo = (uint8_t)(col[r0] ^ 0x80), wi = (uint8_t)(col[r0 + ch] ^ 0x80);
-                    s += pass == 0 ? wi - wo : wi * wi - wo * wo;
+                for (int r = r0; r < r1; r++) {
+                    c1[c * NR + r] = a1;
+                    c2[c * NR + r] = a2;
+                    int32_t wo = col[r] ^ 0x80, wi = col[r + ch] ^ 0x80;
+                    a1 += wi - wo;
+                    a2 += wi * wi - wo * wo;
                 }
             }
             __syncthreads();
-            for (int i = tid; i < NR; i += AR_WG) {
-                int32_t s = 0;
-                for (int v = 0; v < cw; v++) s += colS[v * NR + i];
-                for (int j = 0; j < NC; j++) {
-                    S[j * NR + i] = s;
-                    if (j + 1 < NC) s += colS[(j + cw) * NR + i] - colS[j * NR + i];
+            for (int t = tid; t < NR * KR; t += AR_WG) {
+                int i = t / KR, k = t - i * KR;
+                int j0 = k * lr, j1 = j0 + lr < NC ? j0 + lr : NC;
+                if (j0 >= j1) continue;
+                int32_t a1 = 0, a2 = 0;
+                for (int v = 0; v < cw; v++) {
+                    a1 += c1[(j0 + v) * NR + i];
+                    a2 += c2[(j0 + v) * NR + i];
+                }
+                for (int j = j0; j < j1; j++) {
+                    S1[j * NR + i] = a1;
+                    S2[j * NR + i] = a2;
+                    if (j + 1 < j1) {
+                        a1 += c1[(j + cw) * NR + i] - c1[j * NR + i];
+                        a2 += c2[(j + cw) * NR + i] - c2[j * NR + i];
+                    }
                 }
             }
             __syncthreads();
@@ -221,47 +277,57 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         const int nq = cw * nrb;                       // 16-wide k-blocks
         const int nm = (nq + 3) / 4;                   // MFMAs per tile
         const int g16 = lane >> 4, l16 = lane & 15;
-        for (int tb = wave; tb < ntiles; tb += 4 * 8) {
-            v4i acc[8];
-            int nt_here = 0;
-            for (int k = 0; k < 8; k++) {
-                acc[k] = (v4i){0, 0, 0, 0};
-                if (tb + 4 * k < ntiles) nt_here = k + 1;
+        // B fragment (banded chip) for k-block q = 4m + g16: 16 bytes of chip column v starting
+        // at row 16 rb - l16, read from the zero-padded column as three aligned qwords.
+        auto load_b = [&](int m) -> v4i {
+            int q = 4 * m + g16;
+            if (q >= nq) return (v4i){0, 0, 0, 0};
+            int v = q / nrb, rb = q - v * nrb;
+            int off = v * L.cpad + 16 + 16 * rb - l16;
+            int base = off & ~7, sh = (off & 7) * 8;
+            const uint64_t* qp = (const uint64_t*)(Tp + base);
+            uint64_t q0 = qp[0], q1 = qp[1], q2 = qp[2];
+            uint64_t lo = sh ? (q0 >> sh) | (q1 << (64 - sh)) : q0;
+            uint64_t hi = sh ? (q1 >> sh) | (q2 << (64 - sh)) : q1;
+            return (v4i){(int)(uint32_t)lo, (int)(uint32_t)(lo >> 32), (int)(uint32_t)hi,
+                         (int)(uint32_t)(hi >> 32)};
+        };
+        // A fragment (window) of tile (ti, tj) for k-block q: 16 window rows of one column.
+        auto load_a = [&](int ti, int tj, int m) -> v4i {
+            int q = 4 * m + g16;
+            if (q >= nq) return (v4i){0, 0, 0, 0};
+            int v = q / nrb, rb = q - v * nrb;
+            const int8_t* ap = Wl + (16 * tj + l16 + v) * L.ldw + 16 * ti + 16 * rb;
+            return *(const v4i*)ap;
+        };
+        // Each wave takes batches of 4 tiles (t = tb + 4k); a batch past the last tile repeats
+        // the last tile and its results are dropped, so every load and MFMA is unconditional.
+        for (int tb = wave; tb < ntiles; tb += 16) {
+            int tt[4], ti[4], tj[4];
+            for (int k = 0; k < 4; k++) {
+                tt[k] = tb + 4 * k;
+                int t = tt[k] < ntiles ? tt[k] : ntiles - 1;
+                ti[k] = t % nti;
+                tj[k] = t / nti;
             }
+            v4i acc[4];
+            for (int k = 0; k < 4; k++) acc[k] = (v4i){0, 0, 0, 0};
+            v4i bf = load_b(0);
             for (int m = 0; m < nm; m++) {
-                int q = 4 * m + g16;
-                v4i bf = (v4i){0, 0, 0, 0};
-                int v = 0, rb = 0;
-                if (q < nq) {
-                    v = q / nrb;
-                    rb = q - v * nrb;
-                    int off = v * L.cpad + 16 + 16 * rb - l16;
-                    int base = off & ~7, sh = (off & 7) * 8;
-                    const uint64_t* qp = (const uint64_t*)(Tp + base);
-                    uint64_t q0 = qp[0], q1 = qp[1], q2 = qp[2];
-                    uint64_t lo = sh ? (q0 >> sh) | (q1 << (64 - sh)) : q0;
-                    uint64_t hi = sh ? (q1 >> sh) | (q2 << (64 - sh)) : q1;
-                    bf = (v4i){(int)(uint32_t)lo, (int)(uint32_t)(lo >> 32), (int)(uint32_t)hi,
-                               (int)(uint32_t)(hi >> 32)};
-                }
-                for (int k = 0; k < 8; k++) {
-                    if (k >= nt_here) break;
-                    int t = tb + 4 * k;
-                    int ti = t % nti, tj = t / nti;
-                    v4i af = (v4i){0, 0, 0, 0};
-                    if (q < nq) {
-                        const int8_t* ap = Wl + (16 * tj + l16 + v) * L.ldw + 16 * ti + 16 * rb;
-                        af = *(const v4i*)ap;
-                    }
-                    acc[k] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf, acc[k], 0, 0, 0);
-                }
+                v4i af[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) af[k] = load_a(ti[k], tj[k], m);
+                v4i bn = m + 1 < nm ? load_b(m + 1) : bf;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    acc[k] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[k], bf, acc[k], 0, 0, 0);
+                bf = bn;
             }
-            for (int k = 0; k < nt_here; k++) {
-                int t = tb + 4 * k;
-                int ti = t % nti, tj = t / nti;
-                int i = 16 * ti + l16;
+            for (int k = 0; k < 4; k++) {
+                if (tt[k] >= ntiles) break;
+                int i = 16 * ti[k] + l16;
                 for (int r = 0; r < 4; r++) {
-                    int j = 16 * tj + 4 * g16 + r;
+                    int j = 16 * tj[k] + 4 * g16 + r;
                     if (i < NR && j < NC) {
                         int o = j * NR + i;
                         int32_t cv = acc[k][r];
@@ -275,22 +341,23 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
 
         // ---- integer peak: screen, then exact values for the contenders
         Best b = best_empty();
-        for (int o = tid; o < NR * NC; o += AR_WG) {
-            int j = o / NR, i = o - j * NR;
-            float v = apx[o];
-            int idx = i * NC + j;
-            if (better(v, idx, b)) { b.v = v; b.idx = idx; }
-        }
+        for (int j = wave; j < NC; j += AR_NWAVES)
+            for (int i = lane; i < NR; i += 64) {
+                float v = apx[j * NR + i];
+                int idx = i * NC + j;
+                if (better(v, idx, b)) { b.v = v; b.idx = idx; }
+            }
         b = block_best(b, red);
         float thr = b.v - 2.0f * AR_SCREEN_EPS;
         Best e2 = best_empty();
-        for (int o = tid; o < NR * NC; o += AR_WG) {
-            if (!(apx[o] >= thr)) continue;
-            int j = o / NR, i = o - j * NR;
-            float v = exact_u8(P, accL[o], S1[o], S2[o], a.measure);
-            int idx = i * NC + j;
-            if (better(v, idx, e2)) { e2.v = v; e2.idx = idx; }
-        }
+        for (int j = wave; j < NC; j += AR_NWAVES)
+            for (int i = lane; i < NR; i += 64) {
+                int o = j * NR + i;
+                if (!(apx[o] >= thr)) continue;
+                float v = exact_u8(P, accL[o], S1[o], S2[o], a.measure);
+                int idx = i * NC + j;
+                if (better(v, idx, e2)) { e2.v = v; e2.idx = idx; }
+            }
         e2 = block_best(e2, red);
         int pi = e2.idx / NC + 1, pj = e2.idx % NC + 1;
         float pval = e2.v;
@@ -327,7 +394,7 @@ __global__ __launch_bounds__(AR_WG) void track_generic_kernel(TrackArgs a) {
     float* reg = (float*)smem;
     __shared__ Best red[AR_NWAVES];
     __shared__ double shd[4];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const GenScratch G = gen_scratch(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
     unsigned char* base = a.scratch + (size_t)blockIdx.x * a.scratch_per_block;
     float* chip = (float*)(base + G.off_chip);
@@ -342,14 +409,12 @@ __global__ __launch_bounds__(AR_WG) void track_generic_kernel(TrackArgs a) {
         if (!any_valid(a, g)) { write_skip(a, p, false); continue; }
         const int ch = g.ch, cw = g.cw, NR = 2 * g.ry, NC = 2 * g.rx;
         const int winh = ch + NR - 1, winw = cw + NC - 1, ls = winh + 1;
-        for (int t = tid; t < winh * winw; t += AR_WG) {
-            int c = t / winh, r = t - c * winh;
-            win[t] = load_pix_f(a, a.ref, IS_U8, g.wr0 + r, g.wc0 + c);
-        }
-        for (int t = tid; t < ch * cw; t += AR_WG) {
-            int v = t / ch, u = t - v * ch;
-            chip[t] = load_pix_f(a, a.sec, IS_U8, g.cr0 + u, g.cc0 + v);
-        }
+        for (int c = wave; c < winw; c += AR_NWAVES)
+            for (int r = lane; r < winh; r += 64)
+                win[c * winh + r] = load_pix_f(a, a.ref, IS_U8, g.wr0 + r, g.wc0 + c);
+        for (int v = wave; v < cw; v += AR_NWAVES)
+            for (int u = lane; u < ch; u += 64)
+                chip[v * ch + u] = load_pix_f(a, a.sec, IS_U8, g.cr0 + u, g.cc0 + v);
         __syncthreads();
         // prepare_chip!: sequential Float64 mean and energy in column-major order
         if (tid == 0) {
@@ -394,8 +459,9 @@ __global__ __launch_bounds__(AR_WG) void track_generic_kernel(TrackArgs a) {
         __syncthreads();
         // direct numerator (src/correlate.jl:799-814) and the normalisation, per shift
         double n = (double)(ch * cw);
-        for (int o = tid; o < NR * NC; o += AR_WG) {
-            int j = o / NR, i = o - j * NR;
+        for (int j = wave; j < NC; j += AR_NWAVES)
+        for (int i = lane; i < NR; i += 64) {
+            int o = j * NR + i;
             double acc = 0.0;
             for (int jj = 0; jj < cw; jj++) {
                 const float* wc = win + (j + jj) * winh + i;
@@ -419,12 +485,12 @@ __global__ __launch_bounds__(AR_WG) void track_generic_kernel(TrackArgs a) {
         }
         __syncthreads();
         Best b = best_empty();
-        for (int o = tid; o < NR * NC; o += AR_WG) {
-            int j = o / NR, i = o - j * NR;
-            float v = surf[o];
-            int idx = i * NC + j;
-            if (better(v, idx, b)) { b.v = v; b.idx = idx; }
-        }
+        for (int j = wave; j < NC; j += AR_NWAVES)
+            for (int i = lane; i < NR; i += 64) {
+                float v = surf[j * NR + i];
+                int idx = i * NC + j;
+                if (better(v, idx, b)) { b.v = v; b.idx = idx; }
+            }
         b = block_best(b, red);
         if (b.idx == 0x7fffffff) b.idx = 0;   // all NaN: (1, 1)
         int pi = b.idx / NC + 1, pj = b.idx % NC + 1;

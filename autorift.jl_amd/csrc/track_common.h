@@ -101,12 +101,11 @@ __device__ inline bool any_valid(const TrackArgs& a, const Geom& g) {
         return r0 + g.ch - 1 >= 0 && r0 < a.ny && c0 + g.cw - 1 >= 0 && c0 < a.nx;
     }
     int found = 0;
-    int tot = g.ch * g.cw;
-    for (int t = threadIdx.x; t < tot; t += AR_WG) {
-        int u = t % g.ch, v = t / g.ch;
-        int64_t k = pix(a, g.cr0 + u, g.cc0 + v);
-        if (k >= 0 && a.ok[k]) found = 1;
-    }
+    for (int v = threadIdx.x >> 6; v < g.cw; v += AR_NWAVES)
+        for (int u = threadIdx.x & 63; u < g.ch; u += 64) {
+            int64_t k = pix(a, g.cr0 + u, g.cc0 + v);
+            if (k >= 0 && a.ok[k]) found = 1;
+        }
     return __syncthreads_or(found) != 0;
 }
 
@@ -178,26 +177,17 @@ __device__ inline Best best_empty() {
 // pyrUp (src/peak.jl:196-337): taps in upsampled coordinates with BORDER_REFLECT_101
 // ---------------------------------------------------------------------------------------------
 
-__device__ inline int reflect101_0(int p, int n) {
-    if (n == 1) return 0;
-    int m = n - 1;
-    while (p < 0 || p > m) {
-        if (p < 0) p = -p;
-        if (p > m) p = 2 * m - p;
-    }
-    return p;
-}
-
-// 0-based output position k on an axis with n source samples -> 0-based source taps.
-__device__ inline void taps(int k, int n, int& m, int& c, int& q, int& nt) {
+// 0-based output position k on an axis with n >= 2 source samples -> 0-based source taps.
+// Closed form of src/peak.jl:312-324 (reflection in upsampled coordinates, BORDER_REFLECT_101):
+// only three positions reflect -- k = 0 (minus tap of sample 0 reflects onto sample 1) and the
+// last two positions (plus tap clamps onto sample n-1).
+__host__ __device__ inline void taps(int k, int n, int& m, int& c, int& q, int& nt) {
+    c = k >> 1;
+    q = c + 1 < n - 1 ? c + 1 : n - 1;
     if ((k & 1) == 0) {
-        c = k / 2;
-        m = reflect101_0(k - 2, 2 * n) / 2;
-        q = reflect101_0(k + 2, 2 * n) / 2;
+        m = c == 0 ? 1 : c - 1;
         nt = 3;
     } else {
-        c = reflect101_0(k - 1, 2 * n) / 2;
-        q = reflect101_0(k + 1, 2 * n) / 2;
         m = c;
         nt = 2;
     }

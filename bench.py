@@ -126,7 +126,7 @@ def main():
     ap.add_argument("--size", type=int, default=10000)
     ap.add_argument("--spacing", type=int, default=24)
     ap.add_argument("--cpu-sample", type=int, default=0,
-                    help="points in the CPU baseline sample (0: sized to ~15 s)")
+                    help="points in the CPU baseline sample (0: every grid point, ~20 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-pairs", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -200,7 +200,7 @@ def main():
     if not args.no_cpu:
         import oracle
         threads = min(16, os.cpu_count() or 1)
-        k = args.cpu_sample or 4000
+        k = min(args.cpu_sample or npts, npts)
         sel = np.linspace(0, npts - 1, k).astype(np.int64)
         flat = {f: np.ravel(getattr(pts, f), order="F")[sel]
                 for f in ("x", "y", "radius_x", "radius_y", "dx_prior", "dy_prior",

@@ -103,3 +103,40 @@ def test_sanitize():
                             search_radius_y=[5, 3, 2, 4])
     assert autorift.sanitize_(pts, 6) == 2
     assert list(pts.radius_x) == [0, 6, 10, 0] and list(pts.radius_y) == [0, 6, 6, 0]
+
+
+def _reflect101_0(p, n):
+    """src/peak.jl:374-382, literally."""
+    if n == 1:
+        return 0
+    m = n - 1
+    while p < 0 or p > m:
+        if p < 0:
+            p = -p
+        if p > m:
+            p = 2 * m - p
+    return p
+
+
+def _taps_reference(k, n):
+    """src/peak.jl:312-324 with 0-based k and 0-based source indices."""
+    p = k
+    if p % 2 == 0:
+        return (_reflect101_0(p - 2, 2 * n) // 2, p // 2, _reflect101_0(p + 2, 2 * n) // 2, 3)
+    c = _reflect101_0(p - 1, 2 * n) // 2
+    return (c, c, _reflect101_0(p + 1, 2 * n) // 2, 2)
+
+
+def _taps_closed(k, n):
+    """The closed form the device cascade uses (csrc/track_common.h: taps)."""
+    c = k >> 1
+    q = min(c + 1, n - 1)
+    if k % 2 == 0:
+        return (1 if c == 0 else c - 1, c, q, 3)
+    return (c, c, q, 2)
+
+
+def test_pyrup_taps_closed_form():
+    for n in list(range(2, 64)) + [80, 160, 320, 640]:
+        for k in range(2 * n):
+            assert _taps_closed(k, n) == _taps_reference(k, n), (k, n)
