@@ -370,7 +370,8 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     a.max_rx = (int32_t)rx;
     a.max_ry = (int32_t)ry;
 
-    bool use_u8 = ctx->pdtype == AR_U8 && pow2;
+    // the u8 kernel's integer identities hold in int32 for chip areas up to 128 x 128
+    bool use_u8 = ctx->pdtype == AR_U8 && pow2 && (int64_t)a.max_ch * a.max_cw <= 16384;
     if (use_u8) {
         U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, (int)up);
         if (L.total > kLdsLimit) use_u8 = false;

@@ -23,14 +23,14 @@ __host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx
     // an odd number of 16-byte units per column keeps the 16 lanes of a b128 read on
     // distinct LDS banks
     L.ldw = ((L.rows / 16) & 1) ? L.rows : L.rows + 16;
-    L.cols = 16 * ntj + max_cw;
+    L.cols = 16 * ntj + max_cw + 2;   // + 2 zero columns read by the padded last k-blocks
     L.cpad = rup(16 * nrb + 32, 16);
     int nr = 2 * max_ry, nc = 2 * max_rx;
     int winw = max_cw + 2 * max_rx - 1;
     size_t hdr = 1024;   // cascade header alias: patch and scalars live at the front
     L.off_w = hdr;
     L.off_t = L.off_w + (size_t)L.cols * L.ldw;
-    L.off_s1 = rup((int)(L.off_t + (size_t)max_cw * L.cpad), 16);
+    L.off_s1 = rup((int)(L.off_t + (size_t)(max_cw + 2) * L.cpad), 16);
     L.off_s2 = L.off_s1 + (size_t)nr * nc * 4;
     L.off_tmp = L.off_s2 + (size_t)nr * nc * 4;
     size_t colsum = (size_t)nr * winw * 8;   // W and W^2 column sums

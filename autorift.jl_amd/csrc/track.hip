@@ -88,34 +88,42 @@ struct U8Point {
     float inv_cnorm_f;
 };
 
+// Every quantity below is an integer below 2^53, so the Float64 forms are exact:
+//   sum W*T = acc + 128 (sumT + s1) - 16384 n      (|.| < 2^31)
+//   n * num = n * sum W*T - sumT * s1
+//   n * var = n * s2 - s1^2                         (ZNCC; NCC uses n * s2)
+__device__ inline double n_times_num(const U8Point& P, int32_t acc, int32_t s1) {
+    int32_t swt = acc + 128 * ((int32_t)P.sumT + s1) - 16384 * (int32_t)P.n;
+    return (double)swt * (double)P.n - (double)P.sumT * (double)s1;
+}
+
+__device__ inline double n_times_var(const U8Point& P, int32_t s1, int32_t s2, int measure) {
+    double ns2 = (double)P.n * (double)s2;
+    return measure == 1 ? ns2 : ns2 - (double)s1 * (double)s1;
+}
+
+// The reference's normalisation (src/correlate.jl:591-600) on the exact sums.
 __device__ inline float exact_u8(const U8Point& P, int32_t acc, int32_t s1, int32_t s2,
                                  int measure) {
-    int64_t n = P.n;
-    int64_t swt = (int64_t)acc + 128 * (P.sumT + (int64_t)s1) - 16384 * n;  // sum W*T
-    int64_t nn = n * swt - P.sumT * (int64_t)s1;                              // n * numerator
-    double invn = 1.0 / (double)n;                                            // exact: n = 2^k
-    double num = (double)nn * invn;
+    double invn = 1.0 / (double)P.n;                // exact: n = 2^k
+    double num = n_times_num(P, acc, s1) * invn;
     double den;
     if (measure == 1) {
         den = P.cnorm * sqrt((double)s2);
     } else {
-        int64_t wint = n * (int64_t)s2 - (int64_t)s1 * (int64_t)s1;          // n * variance
-        double wvar = (double)wint * invn;
+        double wvar = n_times_var(P, s1, s2, 0) * invn;
         den = P.cnorm * sqrt(wvar);
     }
     return den > 0 ? (float)(num / den) : 0.0f;
 }
 
+// FP32 screening score, relative error below ~5e-7: (n num) / (cnorm sqrt(n * n var)).
 __device__ inline float approx_u8(const U8Point& P, int32_t acc, int32_t s1, int32_t s2,
                                   int measure) {
-    int64_t n = P.n;
-    int64_t swt = (int64_t)acc + 128 * (P.sumT + (int64_t)s1) - 16384 * n;
-    int64_t nn = n * swt - P.sumT * (int64_t)s1;
-    int64_t wint = measure == 1 ? n * (int64_t)s2 : n * (int64_t)s2 - (int64_t)s1 * (int64_t)s1;
-    if (wint == 0) return 0.0f;
-    // (nn / n) / (cnorm * sqrt(wint / n)) = nn / (cnorm * sqrt(wint * n))
-    float d = (float)wint * (float)n;
-    return (float)nn * P.inv_cnorm_f * __builtin_amdgcn_rsqf(d);
+    double nv = n_times_var(P, s1, s2, measure);
+    if (nv == 0.0) return 0.0f;
+    float d = (float)nv * (float)P.n;
+    return (float)n_times_num(P, acc, s1) * P.inv_cnorm_f * __builtin_amdgcn_rsqf(d);
 }
 
 __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
@@ -134,7 +142,8 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
     const uint8_t* ref = (const uint8_t*)a.ref;
     const uint8_t* sec = (const uint8_t*)a.sec;
 
-    for (int64_t p = blockIdx.x; p < a.n; p += gridDim.x) {
+    const PointRange pr = point_range(a.n);
+    for (int64_t p = pr.first; p < pr.end; p += pr.step) {
         Geom g = point_geom(a, p);
         if (!g.status) { write_skip(a, p, false); continue; }
         if (!any_valid(a, g)) { write_skip(a, p, false); continue; }
@@ -143,7 +152,7 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         const int nti = (NR + 15) / 16, ntj = (NC + 15) / 16;
         const int nrb = (ch + 15 + 15) / 16;          // band rows r - i in [0, ch), i < 16
         const int rows_used = 16 * (nti + nrb);
-        const int cols_used = 16 * ntj + cw;
+        const int cols_used = 16 * ntj + cw + 2;
 
         // ---- stage window (int8, value ^ 0x80; virtual zero padding -> 0x80) and chip
         {
@@ -168,9 +177,7 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
                         if (colok && c < winw && r < winh) {
                             uintptr_t ad = cadr + (uintptr_t)r;
                             const uint32_t* al = (const uint32_t*)(ad & ~(uintptr_t)3);
-                            uint32_t d0 = al[0], d1 = al[1];
-                            uint32_t sh = (uint32_t)(ad & 3) * 8;
-                            v = sh ? (d0 >> sh) | (d1 << (32 - sh)) : d0;
+                            v = __builtin_amdgcn_alignbyte(al[1], al[0], (uint32_t)(ad & 3));
                             v ^= 0x80808080u;
                             int keep = winh - r;   // rows of this dword inside the window
                             if (keep < 4) v &= (1u << (8 * keep)) - 1u;
@@ -191,18 +198,44 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
             }
         }
         int64_t st = 0;
-        for (int v = wave; v < cw; v += AR_NWAVES)
-            for (int x = lane; x < L.cpad; x += 64) {
-                int u = x - 16;
-                int8_t b = 0;
-                if (u >= 0 && u < ch) {
-                    int64_t k = pix(a, g.cr0 + u, g.cc0 + v);
-                    uint8_t raw = k >= 0 ? sec[k] : 0;
-                    st += raw;
-                    b = (int8_t)(raw ^ 0x80);
-                }
-                Tp[v * L.cpad + x] = b;
+        {
+            int64_t orow = g.cr0 - a.pady - 1, ocol = g.cc0 - a.padx - 1;
+            bool inside = orow >= 0 && ocol >= 0 && orow + ch <= a.ny && ocol + cw <= a.nx;
+            if (inside && (ch & 3) == 0) {
+                // dword path: chip rows u = x - 16 for x in [16, 16 + ch), realigned as above;
+                // the zero band rows and the two trailing zero columns are written as zeros
+                const uint8_t* gb = sec + orow + ocol * a.ny;
+                const int nx4 = L.cpad >> 2;
+                uint32_t acc8 = 0;
+                for (int v = wave; v < cw + 2; v += AR_NWAVES)
+                    for (int x4 = lane; x4 < nx4; x4 += 64) {
+                        int u = 4 * x4 - 16;
+                        uint32_t w = 0;
+                        if (u >= 0 && u < ch && v < cw) {
+                            uintptr_t ad = (uintptr_t)(gb + (int64_t)v * a.ny + u);
+                            const uint32_t* al = (const uint32_t*)(ad & ~(uintptr_t)3);
+                            uint32_t raw = __builtin_amdgcn_alignbyte(al[1], al[0], (uint32_t)(ad & 3));
+                            acc8 = __builtin_amdgcn_sad_u8(raw, 0u, acc8);
+                            w = raw ^ 0x80808080u;
+                        }
+                        *(uint32_t*)(Tp + v * L.cpad + 4 * x4) = w;
+                    }
+                st = acc8;
+            } else {
+                for (int v = wave; v < cw + 2; v += AR_NWAVES)   // two trailing zero columns
+                    for (int x = lane; x < L.cpad; x += 64) {
+                        int u = x - 16;
+                        int8_t b = 0;
+                        if (u >= 0 && u < ch && v < cw) {
+                            int64_t k = pix(a, g.cr0 + u, g.cc0 + v);
+                            uint8_t raw = k >= 0 ? sec[k] : 0;
+                            st += raw;
+                            b = (int8_t)(raw ^ 0x80);
+                        }
+                        Tp[v * L.cpad + x] = b;
+                    }
             }
+        }
         U8Point P;
         P.n = (int64_t)ch * cw;
         P.sumT = block_sum<int64_t>(st, red64);
@@ -277,51 +310,45 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         const int nq = cw * nrb;                       // 16-wide k-blocks
         const int nm = (nq + 3) / 4;                   // MFMAs per tile
         const int g16 = lane >> 4, l16 = lane & 15;
-        // B fragment (banded chip) for k-block q = 4m + g16: 16 bytes of chip column v starting
-        // at row 16 rb - l16, read from the zero-padded column as three aligned qwords.
-        auto load_b = [&](int m) -> v4i {
-            int q = 4 * m + g16;
-            if (q >= nq) return (v4i){0, 0, 0, 0};
-            int v = q / nrb, rb = q - v * nrb;
-            int off = v * L.cpad + 16 + 16 * rb - l16;
-            int base = off & ~7, sh = (off & 7) * 8;
-            const uint64_t* qp = (const uint64_t*)(Tp + base);
-            uint64_t q0 = qp[0], q1 = qp[1], q2 = qp[2];
-            uint64_t lo = sh ? (q0 >> sh) | (q1 << (64 - sh)) : q0;
-            uint64_t hi = sh ? (q1 >> sh) | (q2 << (64 - sh)) : q1;
-            return (v4i){(int)(uint32_t)lo, (int)(uint32_t)(lo >> 32), (int)(uint32_t)hi,
-                         (int)(uint32_t)(hi >> 32)};
-        };
-        // A fragment (window) of tile (ti, tj) for k-block q: 16 window rows of one column.
-        auto load_a = [&](int ti, int tj, int m) -> v4i {
-            int q = 4 * m + g16;
-            if (q >= nq) return (v4i){0, 0, 0, 0};
-            int v = q / nrb, rb = q - v * nrb;
-            const int8_t* ap = Wl + (16 * tj + l16 + v) * L.ldw + 16 * ti + 16 * rb;
-            return *(const v4i*)ap;
-        };
+        // K is walked in 16-wide blocks q = 4m + g16 = (chip column v, band row block rb). Blocks
+        // past the last (nq) read the zero chip columns cw, cw + 1, so the loop has no branches.
         // Each wave takes batches of 4 tiles (t = tb + 4k); a batch past the last tile repeats
-        // the last tile and its results are dropped, so every load and MFMA is unconditional.
+        // the last tile and its results are dropped.
         for (int tb = wave; tb < ntiles; tb += 16) {
-            int tt[4], ti[4], tj[4];
+            int tt[4], ti[4], tj[4], abase[4];
+#pragma unroll
             for (int k = 0; k < 4; k++) {
                 tt[k] = tb + 4 * k;
                 int t = tt[k] < ntiles ? tt[k] : ntiles - 1;
                 ti[k] = t % nti;
                 tj[k] = t / nti;
+                abase[k] = (16 * tj[k] + l16) * L.ldw + 16 * ti[k];
             }
             v4i acc[4];
+#pragma unroll
             for (int k = 0; k < 4; k++) acc[k] = (v4i){0, 0, 0, 0};
-            v4i bf = load_b(0);
+            int v = g16 / nrb, rb = g16 - v * nrb;
             for (int m = 0; m < nm; m++) {
+                // B: 16 bytes of the zero-padded chip column v from row 16 rb - l16, realigned
+                // from five dwords
+                int off = v * L.cpad + 16 + 16 * rb - l16;
+                const uint32_t* bp = (const uint32_t*)(Tp + (off & ~3));
+                uint32_t d0 = bp[0], d1 = bp[1], d2 = bp[2], d3 = bp[3], d4 = bp[4];
+                uint32_t sb = (uint32_t)(off & 3);
+                v4i bf = (v4i){(int)__builtin_amdgcn_alignbyte(d1, d0, sb),
+                               (int)__builtin_amdgcn_alignbyte(d2, d1, sb),
+                               (int)__builtin_amdgcn_alignbyte(d3, d2, sb),
+                               (int)__builtin_amdgcn_alignbyte(d4, d3, sb)};
+                const int aoff = v * L.ldw + 16 * rb;
                 v4i af[4];
 #pragma unroll
-                for (int k = 0; k < 4; k++) af[k] = load_a(ti[k], tj[k], m);
-                v4i bn = m + 1 < nm ? load_b(m + 1) : bf;
+                for (int k = 0; k < 4; k++) af[k] = *(const v4i*)(Wl + abase[k] + aoff);
 #pragma unroll
                 for (int k = 0; k < 4; k++)
                     acc[k] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[k], bf, acc[k], 0, 0, 0);
-                bf = bn;
+                rb += 4;
+                if (rb >= nrb) { rb -= nrb; v++; }
+                if (rb >= nrb) { rb -= nrb; v++; }
             }
             for (int k = 0; k < 4; k++) {
                 if (tt[k] >= ntiles) break;
@@ -403,7 +430,8 @@ __global__ __launch_bounds__(AR_WG) void track_generic_kernel(TrackArgs a) {
     double* S2 = (double*)(base + G.off_s2);
     float* surf = (float*)(base + G.off_surf);
 
-    for (int64_t p = blockIdx.x; p < a.n; p += gridDim.x) {
+    const PointRange pr = point_range(a.n);
+    for (int64_t p = pr.first; p < pr.end; p += pr.step) {
         Geom g = point_geom(a, p);
         if (!g.status) { write_skip(a, p, false); continue; }
         if (!any_valid(a, g)) { write_skip(a, p, false); continue; }

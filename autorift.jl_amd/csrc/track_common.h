@@ -86,6 +86,32 @@ __device__ inline Geom point_geom(const TrackArgs& a, int64_t p) {
     return g;
 }
 
+// Persistent point schedule. Workgroups are dispatched round-robin over the 8 XCDs, each with
+// its own L2; the workgroups of one XCD take one contiguous eighth of the points, so the
+// overlapping windows of neighbouring points are fetched into the same L2. Every point is
+// visited exactly once (for fewer than 8 workgroups, a plain stride).
+struct PointRange {
+    int64_t first, end, step;
+};
+
+__device__ inline PointRange point_range(int64_t n) {
+    PointRange r;
+    const int64_t g = gridDim.x, b = blockIdx.x;
+    if (g < 8) {
+        r.first = b;
+        r.end = n;
+        r.step = g;
+        return r;
+    }
+    const int64_t x = b % 8, slot = b / 8, cnt = (g - x + 7) / 8;
+    const int64_t chunk = (n + 7) / 8;
+    int64_t start = x * chunk, end = start + chunk < n ? start + chunk : n;
+    r.first = start + slot;
+    r.end = end;
+    r.step = cnt;
+    return r;
+}
+
 // Padded (1-based) pixel -> linear index into the unpadded image, or -1 in the padding.
 __device__ inline int64_t pix(const TrackArgs& a, int64_t r, int64_t c) {
     int64_t orow = r - a.pady - 1, ocol = c - a.padx - 1;
