@@ -24,6 +24,8 @@ for p in (ROOT, os.path.join(ROOT, "autorift.jl_amd")):
 
 INT8_PEAK_TOPS = 5000.0          # MI355X dense int8 MFMA, TOP/s
 FLOP_PER_POINT = 2 * 50 * 50 * 32 * 32   # useful multiply-adds of the direct contraction, x2
+# rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the default workload, scaled by a 1 GiB copy
+TRAFFIC_PROFILE = os.path.join("profiles", "track_u8_kernel_10000sq_c32r25.md")
 
 
 def log(*a):
@@ -105,6 +107,18 @@ def pair_latency(autorift, npairs, warm=2):
             "median_dx": float(np.median([m[0] for m in meas])),
             "median_dy": float(np.median([m[1] for m in meas])),
             "expected_dx_dy": [-3.30, 1.70]}
+
+
+def profiled_traffic():
+    """HBM bytes per tracker launch from the committed counter summary (None if absent)."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+            for line in f:
+                if line.startswith("traffic:"):
+                    return int(line.split()[1])
+    except OSError:
+        pass
+    return None
 
 
 def copy_calibration(nbytes):
@@ -216,6 +230,8 @@ def main():
                          f"numerator, OpenMP dynamic schedule, {el:.1f} s"}
         log(f"cpu baseline {k / el:.0f} points/s on {threads} threads")
 
+    # the counter summary covers the default workload only
+    traffic = profiled_traffic() if (n, args.spacing) == (10000, 24) else None
     line = {
         "metric": "NCC search points/sec (chip 32, r=25), %roofline; p50/p99 pair latency",
         "value": value,
@@ -233,7 +249,8 @@ def main():
                                f"(spacing {args.spacing}), chip 32, r 25, PyramidRefine(64)",
                    "points": npts},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
-                     "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": traffic,
+                     "traffic_source": TRAFFIC_PROFILE if traffic is not None else None},
         "cpu_baseline": cpu,
         "pair_latency": lat,
         "prepare_s": prep_s,
