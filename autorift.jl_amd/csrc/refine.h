@@ -2,38 +2,52 @@
 // point. Follows src/peak.jl:32-106 (argmax rule, offsets) and src/peak.jl:196-522 (pyrUp steps
 // and subpixel_peak) element for element: every upsampled sample is computed by the reference's
 // formula in FP32 from the same inputs, so the result is bit-identical whatever the order in which
-// samples are produced.
+// samples are produced, and whichever of them are produced more than once.
 //
-// Cost control without approximation: levels up to 80x80 are materialised in LDS; the last one or
-// two levels are produced in 16-row strips, strongest first, and a strip is skipped only when a
-// proven upper bound on every sample it could hold is strictly below the best sample already
-// found. Each pyrUp pass forms convex combinations (weights 3/4, 1/8, 1/8 or 1/2, 1/2), so a
-// pass raises the maximum by at most two units in the last place; the bound adds a margin of
-// 1e-6 relative, far above that.
+// Cost control without approximation. Levels up to 40x40 (8x) are materialised in LDS. Beyond,
+// the final level is cut into 16x16 tiles, each computed by one wave from the 40x40 level
+// through its own small blocks of the intermediate levels. A tile is skipped only when a
+// proven upper bound on every sample it could hold is strictly below a sample already found:
+// first the 2x2 tiles around the image of the 8x maximum are computed (that best sample is the
+// threshold), then every other tile fed by an 8x sample whose bound reaches the threshold, each
+// re-checked against the bound of its own intermediate blocks before going deeper.
+//
+// The bound. A pass forms tap3 = 0.75 b + 0.125 (a + c) or tap2 = 0.5 (b + c) in FP32. With
+// M = max(a, b, c): fl(a + c) <= 2M and fl(b + c) <= 2M (rounding is monotone and 2M is
+// representable), the power-of-two scalings are exact, fl(0.75 b) <= 0.75 M + 0.75 |M| u, so a
+// pass exceeds M by at most 1.75 |M| u (u = 2^-24) plus subnormal rounding (< 2^-149 per
+// operation). Six passes (8x -> 64x) stay below 6.3e-7 |M|; the bound adds 1e-6 |M| + 1e-30.
 #pragma once
 #include "track_common.h"
 
 namespace ar {
 
-// Floats needed by the cascade region for upsampling `up` (power of two, 1..64).
-__host__ __device__ inline int cascade_floats(int up) {
-    // header: patch(32) rowmax(96) ub(64) order/scalars(64)
-    int hdr = 256;
-    if (up <= 1) return hdr;
-    // materialised levels: Y (final materialised level), X (previous), T (vertical tmp)
+#define AR_TILE 16        // final-level tile edge
+#define AR_TILE_WS 416    // per-wave tile scratch: vertical tmp 16 x 16 + block 10 x 16
+
+// Region layout (floats): patch [0, 32), scalars [32, 64), per-tile 8x row ranges lo [64, 96)
+// and hi [96, 128), tile flags [128, 256) as bytes, tile list [256, 656) as ints, then the levels.
+#define AR_CASC_HDR 656
+
+__host__ __device__ inline int casc_levels(int up) {
     int L = 0;
     while ((1 << L) < up) L++;
-    int LM = L < 4 ? L : 4;
+    return L;
+}
+
+// Floats needed by the cascade region for upsampling `up` (power of two, 1..64).
+__host__ __device__ inline int cascade_floats(int up) {
+    if (up <= 1) return AR_CASC_HDR;
+    // materialised levels: Y (last materialised level), X (previous), T (vertical tmp)
+    int L = casc_levels(up);
+    int LM = L < 3 ? L : 3;
     int nM = 5 << LM;
     int nP = 5 << (LM > 0 ? LM - 1 : 0);
     int Y = nM * nM;
     int X = nP * nP;
     int T = nM * nP;
-    int stream = 0;
-    if (L == 5) stream = 16 * 80;                         // tmp5 strip
-    if (L == 6) stream = 12 * 160 + 12 * 80 + 16 * 160;  // level-5 rows, tmp5 rows, tmp6 strip
-    int body = Y + (X + T > stream ? X + T : stream);
-    return hdr + body;
+    int tiles = L > 3 ? AR_NWAVES * AR_TILE_WS : 0;   // aliases X and T once Y is complete
+    return AR_CASC_HDR + Y + (X + T > tiles ? X + T : tiles);
 }
 
 struct CascadeOut {
@@ -115,40 +129,6 @@ __device__ inline void horiz_pairs(const float* src, float* dst, int rows) {
     }
 }
 
-// Horizontal pass of the final level straight into the argmax: row r of the strip is output row
-// row0 + r; sample (row, col) has row-major index row * 2WS + col.
-template <int WS>
-__device__ inline Best horiz_pairs_argmax(const float* src, int rows, int row0) {
-    Best best = best_empty();
-    const int total = rows * WS;
-    for (int t0 = threadIdx.x; t0 < total; t0 += AR_U * AR_WG) {
-        float a[AR_U], b[AR_U], q[AR_U];
-        int idx[AR_U];
-#pragma unroll
-        for (int u = 0; u < AR_U; u++) {
-            int t = t0 + u * AR_WG;
-            t = t < total ? t : total - 1;
-            int r = t / WS, k = t - r * WS;
-            int km, kq;
-            pair_src(k, WS, km, kq);
-            const float* row = src + r * WS;
-            a[u] = row[km];
-            b[u] = row[k];
-            q[u] = row[kq];
-            idx[u] = (row0 + r) * 2 * WS + 2 * k;
-        }
-#pragma unroll
-        for (int u = 0; u < AR_U; u++) {
-            if (t0 + u * AR_WG < total) {
-                float e = tap3(a[u], b[u], q[u]), o = tap2(b[u], q[u]);
-                if (better(e, idx[u], best)) { best.v = e; best.idx = idx[u]; }
-                if (better(o, idx[u] + 1, best)) { best.v = o; best.idx = idx[u] + 1; }
-            }
-        }
-    }
-    return best;
-}
-
 // One full pyrUp step in LDS: src (N x N, row-major) -> dst (2N x 2N), via tmp (2N x N).
 template <int N>
 __device__ inline void pyrup_full(const float* src, float* tmp, float* dst) {
@@ -163,153 +143,245 @@ __device__ inline float ub_margin(float m) {
     return m + fabsf(m) * 1e-6f + 1e-30f;
 }
 
-// Source-row range [lo, hi] (0-based) feeding output rows [a, b) of a pass over n source rows.
-__device__ inline void feed_rows(int a, int b, int n, int& lo, int& hi) {
-    lo = 0x7fffffff;
-    hi = -1;
-    for (int i = a; i < b; i++) {
-        int m, c, q, nt;
-        taps(i, n, m, c, q, nt);
-        int l = m < c ? m : c;
-        l = l < q ? l : q;
-        int h = m > c ? m : c;
-        h = h > q ? h : q;
-        lo = l < lo ? l : lo;
-        hi = h > hi ? h : hi;
+// Source range [lo, hi] (0-based, inclusive) feeding outputs [a, b] of a pyrUp axis over n >= 2
+// source samples: the smallest range holding every tap of every output in [a, b] (taps above:
+// output i reads i/2 - 1, i/2, i/2 + 1 for even i, i/2 and i/2 + 1 for odd i, reflected at 0
+// and clamped at n - 1).
+__host__ __device__ inline void feed_range(int a, int b, int n, int& lo, int& hi) {
+    int l = ((a + 1) >> 1) - 1;
+    lo = l > 0 ? l : 0;
+    int h = (b >> 1) + 1;
+    hi = h < n - 1 ? h : n - 1;
+}
+
+// Intra-wave LDS hand-off: the LDS operations of one wave complete in order, so ordering the
+// compiler's accesses is enough.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        float u = __shfl_xor(v, o);
+        v = u > v ? u : v;
     }
+    return v;
+}
+
+// One wave computes the final-level tile (ti, tj) of level F from the materialised 8x level Z
+// (40 x 40, row-major), through its blocks of the levels in between (pitch 16 in `ws`). With
+// `prune`, it stops after an intermediate block whose bound is below `lb`. Returns the tile's
+// argmax under the peak_index rule (wave-uniform; empty if pruned or all NaN).
+template <int F>
+__device__ inline Best tile_run(const float* Z, float* ws, int ti, int tj, float lb, bool prune) {
+    const int lane = threadIdx.x & 63;
+    int rlo[F + 1], rhi[F + 1], clo[F + 1], chi[F + 1];
+    rlo[F] = AR_TILE * ti;
+    rhi[F] = rlo[F] + AR_TILE - 1;
+    clo[F] = AR_TILE * tj;
+    chi[F] = clo[F] + AR_TILE - 1;
+#pragma unroll
+    for (int L = F; L > 3; L--) {
+        feed_range(rlo[L], rhi[L], 5 << (L - 1), rlo[L - 1], rhi[L - 1]);
+        feed_range(clo[L], chi[L], 5 << (L - 1), clo[L - 1], chi[L - 1]);
+    }
+    float* tmp = ws;         // vertical pass of the level being built: rows x 16
+    float* blk = ws + 256;   // last intermediate block: rows x 16
+    const float* src = Z;
+    int sr0 = 0, sc0 = 0, sp = 40;
+    Best best = best_empty();
+#pragma unroll
+    for (int L = 4; L <= F; L++) {
+        const int n = 5 << (L - 1);
+        const int nr = rhi[L] - rlo[L] + 1;
+        const int ncs = chi[L - 1] - clo[L - 1] + 1;
+        for (int t = lane; t < nr * 16; t += 64) {
+            int r = t >> 4, c = t & 15;
+            if (c < ncs) {
+                int m, cc, q, nt;
+                taps(rlo[L] + r, n, m, cc, q, nt);
+                const float* col = src + (clo[L - 1] + c - sc0);
+                float va = col[(m - sr0) * sp], vb = col[(cc - sr0) * sp], vq = col[(q - sr0) * sp];
+                tmp[t] = nt == 3 ? tap3(va, vb, vq) : tap2(vb, vq);
+            }
+        }
+        wave_sync();
+        const float* row0 = tmp - clo[L - 1];
+        if (L < F) {
+            const int nc = chi[L] - clo[L] + 1;
+            float mx = -INFINITY;
+            for (int t = lane; t < nr * 16; t += 64) {
+                int r = t >> 4, c = t & 15;
+                if (c < nc) {
+                    int m, cc, q, nt;
+                    taps(clo[L] + c, n, m, cc, q, nt);
+                    const float* row = row0 + r * 16;
+                    float v = nt == 3 ? tap3(row[m], row[cc], row[q]) : tap2(row[cc], row[q]);
+                    blk[t] = v;
+                    mx = v > mx ? v : mx;
+                }
+            }
+            const bool cut = prune && ub_margin(wave_max(mx)) < lb;
+            wave_sync();
+            if (cut) return best;
+            src = blk;
+            sr0 = rlo[L];
+            sc0 = clo[L];
+            sp = 16;
+        } else {
+            const int nf = 5 << F;
+            for (int t = lane; t < AR_TILE * AR_TILE; t += 64) {
+                int r = t >> 4, c = t & 15;
+                int m, cc, q, nt;
+                taps(clo[F] + c, n, m, cc, q, nt);
+                const float* row = row0 + r * 16;
+                float v = nt == 3 ? tap3(row[m], row[cc], row[q]) : tap2(row[cc], row[q]);
+                int idx = (rlo[F] + r) * nf + clo[F] + c;
+                if (better(v, idx, best)) { best.v = v; best.idx = idx; }
+            }
+            best = wave_best(best);
+            wave_sync();   // tmp is rewritten by the wave's next tile
+        }
+    }
+    return best;
+}
+
+// Final levels 16x..64x (F = 4..6) from the materialised 8x level in Y.
+template <int F>
+__device__ inline CascadeOut cascade_tiles(float* reg, const float* Y, float* ws, Best* red) {
+    const int tid = threadIdx.x, w = tid >> 6;
+    int* sc = (int*)(reg + 32);
+    int* r3lo = (int*)(reg + 64);
+    int* r3hi = (int*)(reg + 96);
+    unsigned char* flags = (unsigned char*)(reg + 128);
+    int* list = (int*)(reg + 256);
+    const int nf = 5 << F, NT = nf / AR_TILE;
+
+    if (tid < NT) {
+        // 8x rows feeding tile row tid (the same ranges serve columns)
+        int lo = AR_TILE * tid, hi = lo + AR_TILE - 1;
+        for (int L = F; L > 3; L--) feed_range(lo, hi, 5 << (L - 1), lo, hi);
+        r3lo[tid] = lo;
+        r3hi[tid] = hi;
+    }
+    for (int t = tid; t < NT * NT; t += AR_WG) flags[t] = 0;
+    if (tid == 0) sc[0] = 0;
+
+    // Phase 1: the 2x2 tiles around the image of the 8x maximum, one per wave, no pruning.
+    Best b3 = best_empty();
+    for (int t = tid; t < 40 * 40; t += AR_WG) {
+        float v = Y[t];
+        if (better(v, t, b3)) { b3.v = v; b3.idx = t; }
+    }
+    b3 = block_best(b3, red);
+    int i3 = 20, j3 = 20;
+    if (b3.idx != 0x7fffffff) {
+        i3 = b3.idx / 40;
+        j3 = b3.idx - 40 * i3;
+    }
+    int ti0 = ((i3 << (F - 3)) + AR_TILE / 2) / AR_TILE - 1;
+    int tj0 = ((j3 << (F - 3)) + AR_TILE / 2) / AR_TILE - 1;
+    ti0 = ti0 < 0 ? 0 : (ti0 > NT - 2 ? NT - 2 : ti0);
+    tj0 = tj0 < 0 ? 0 : (tj0 > NT - 2 ? NT - 2 : tj0);
+    Best wb = tile_run<F>(Y, ws, ti0 + (w >> 1), tj0 + (w & 1), -INFINITY, false);
+    const Best pb = block_best(wb, red);
+    const float LB = pb.v;
+
+    // Phase 2: flag every tile fed by an 8x sample whose bound reaches LB (the tile bound is
+    // the bound of its largest feeder; NaN feeders never raise it).
+    for (int t = tid; t < 40 * 40; t += AR_WG) {
+        float v = Y[t];
+        if (ub_margin(v) >= LB) {
+            int r3 = t / 40, c3 = t - 40 * r3;
+            int a0 = NT, a1 = -1, b0 = NT, b1 = -1;
+            for (int k = 0; k < NT; k++) {
+                int lo = r3lo[k], hi = r3hi[k];
+                if (lo <= r3 && r3 <= hi) { a0 = k < a0 ? k : a0; a1 = k; }
+                if (lo <= c3 && c3 <= hi) { b0 = k < b0 ? k : b0; b1 = k; }
+            }
+            for (int ti = a0; ti <= a1; ti++)
+                for (int tj = b0; tj <= b1; tj++) flags[ti * NT + tj] = 1;
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < NT * NT; t += AR_WG) {
+        int ti = t / NT, tj = t - ti * NT;
+        bool first = (unsigned)(ti - ti0) < 2u && (unsigned)(tj - tj0) < 2u;
+        if (flags[t] && !first) list[atomicAdd(&sc[0], 1)] = t;
+    }
+    __syncthreads();
+    const int cnt = sc[0];
+
+    // Phase 3: the flagged tiles, wave-strided, each pruned against the best value known.
+    Best best = pb;
+    float lb = LB;
+    for (int k = w; k < cnt; k += AR_NWAVES) {
+        int t = list[k];
+        int ti = t / NT, tj = t - ti * NT;
+        Best r = tile_run<F>(Y, ws, ti, tj, lb, true);
+        best = best_merge(best, r);
+        lb = best.v > lb ? best.v : lb;
+    }
+    best = block_best(best, red);
+
+    CascadeOut out;
+    // All samples NaN: the reference's scan leaves (1, 1), whose value is NaN.
+    if (best.idx == 0x7fffffff) {
+        best.idx = 0;
+        best.v = __int_as_float(0x7fc00000);
+    }
+    out.si = best.idx / nf;
+    out.sj = best.idx - out.si * nf;
+    out.val = best.v;
+    return out;
 }
 
 // The cascade. `patch` (5x5, row-major) is in LDS; `reg` is the cascade region
 // (cascade_floats(up) floats, 16-byte aligned). Returns the final-level argmax under the
 // peak_index rule; identical for every thread.
 __device__ inline CascadeOut cascade(float* reg, int up, Best* red) {
-    float* patch = reg;          // 32
-    float* rowmax = reg + 32;    // 96
-    float* ubs = reg + 128;      // 64
-    int* sc = (int*)(reg + 192); // 64 ints of scalars
-    float* body = reg + 256;
-
-    int L = 0;
-    while ((1 << L) < up) L++;
-    int LM = L < 4 ? L : 4;
-    int nM = 5 << LM;
-    int nP = 5 << (LM > 0 ? LM - 1 : 0);
+    const float* patch = reg;
+    float* body = reg + AR_CASC_HDR;
+    const int L = casc_levels(up);
+    const int LM = L < 3 ? L : 3;
+    const int nM = 5 << LM;
+    const int nP = 5 << (LM > 0 ? LM - 1 : 0);
     float* Y = body;                       // nM x nM
-    float* X = Y + nM * nM;                // nP x nP; after level LM, the streaming buffers
-    float* T = X + nP * nP;                // vertical-pass tmp (2n x n) while materialising
+    float* X = Y + nM * nM;                // nP x nP
+    float* T = X + nP * nP;                // vertical-pass tmp (2n x n)
 
     // Levels 1..LM, ping-ponging so that level LM lands in Y.
     const float* cur = patch;
     if (LM >= 1) { float* d = ((LM - 1) & 1) ? X : Y; pyrup_full<5>(cur, T, d); cur = d; }
     if (LM >= 2) { float* d = ((LM - 2) & 1) ? X : Y; pyrup_full<10>(cur, T, d); cur = d; }
     if (LM >= 3) { float* d = ((LM - 3) & 1) ? X : Y; pyrup_full<20>(cur, T, d); cur = d; }
-    if (LM >= 4) { float* d = ((LM - 4) & 1) ? X : Y; pyrup_full<40>(cur, T, d); cur = d; }
-    int n = 5 << LM;
 
-    CascadeOut out;
-    if (L <= 4) {
-        // final level materialised: plain argmax
-        Best b = best_empty();
-        int nf = n;
-        for (int t = threadIdx.x; t < nf * nf; t += AR_WG) {
-            float v = cur[t];
-            if (better(v, t, b)) { b.v = v; b.idx = t; }
-        }
-        b = block_best(b, red);
-        if (b.idx == 0x7fffffff) b.idx = 0;
-        out.si = b.idx / nf;
-        out.sj = b.idx - out.si * nf;
-        out.val = cur[b.idx];
+    if (L >= 4) {
+        float* ws = X + (threadIdx.x >> 6) * AR_TILE_WS;
+        CascadeOut out;
+        if (L == 4) out = cascade_tiles<4>(reg, Y, ws, red);
+        else if (L == 5) out = cascade_tiles<5>(reg, Y, ws, red);
+        else out = cascade_tiles<6>(reg, Y, ws, red);
         __syncthreads();
         return out;
     }
 
-    // Row maxima of level 4 (80 x 80, in Y), ignoring NaN.
-    const int n4 = 80;
-    for (int r = threadIdx.x; r < n4; r += AR_WG) {
-        float m = -INFINITY;
-        for (int c = 0; c < n4; c++) {
-            float v = Y[r * n4 + c];
-            m = v > m ? v : m;
-        }
-        rowmax[r] = m;
+    // final level materialised: plain argmax
+    const int nfin = 5 << LM;
+    Best b = best_empty();
+    for (int t = threadIdx.x; t < nfin * nfin; t += AR_WG) {
+        float v = cur[t];
+        if (better(v, t, b)) { b.v = v; b.idx = t; }
     }
+    b = block_best(b, red);
+    if (b.idx == 0x7fffffff) b.idx = 0;
+    CascadeOut out;
+    out.si = b.idx / nfin;
+    out.sj = b.idx - out.si * nfin;
+    out.val = cur[b.idx];
     __syncthreads();
-
-    const int nfin = 5 * up;          // 160 or 320
-    const int NS = nfin / 16;         // strips
-    // Upper bound per strip from the level-4 rows that feed it.
-    for (int s = threadIdx.x; s < NS; s += AR_WG) {
-        int lo, hi;
-        if (L == 5) {
-            feed_rows(16 * s, 16 * s + 16, n4, lo, hi);
-        } else {
-            int lo5, hi5;
-            feed_rows(16 * s, 16 * s + 16, 160, lo5, hi5);
-            feed_rows(lo5, hi5 + 1, n4, lo, hi);
-        }
-        float m = -INFINITY;
-        for (int r = lo; r <= hi; r++) m = rowmax[r] > m ? rowmax[r] : m;
-        ubs[s] = ub_margin(m);
-    }
-    __syncthreads();
-
-    Best best = best_empty();
-    float* lv5 = X;                  // level-5 rows (L == 6)
-    float* t5 = X + 12 * 160;        // tmp5 rows
-    float* t6 = t5 + 12 * 80;        // tmp6 strip
-    if (L == 5) t5 = X;
-
-    for (int iter = 0; iter < NS; iter++) {
-        // pick the unprocessed strip with the largest bound (lowest index on ties)
-        if (threadIdx.x == 0) {
-            int bs = -1;
-            float bu = 0;
-            for (int s = 0; s < NS; s++) {
-                float u = ubs[s];
-                if (u != u) continue;   // processed (NaN marker)
-                if (bs < 0 || u > bu) { bs = s; bu = u; }
-            }
-            int go = bs >= 0 && !(bu < best.v);
-            sc[0] = go ? bs : -1;
-            if (go) ubs[bs] = __int_as_float(0x7fc00000);
-        }
-        __syncthreads();
-        int s = sc[0];
-        if (s < 0) break;
-        int a = 16 * s;
-        Best b = best_empty();
-        if (L == 5) {
-            // level-5 rows [a, a+16) from level 4, then their samples into the argmax
-            vert_pairs<80>(Y, n4, 0, t5, a, a / 2, a / 2 + 8);
-            __syncthreads();
-            b = horiz_pairs_argmax<80>(t5, 16, a);
-        } else {
-            // level-6 rows [a, a+16) need level-5 rows [a/2 - 1, a/2 + 8] (clamped), produced
-            // as whole pairs from level 4
-            int lo5 = a / 2 - 1 < 0 ? 0 : a / 2 - 1, hi5 = a / 2 + 8 > 159 ? 159 : a / 2 + 8;
-            int plo = lo5 / 2, phi = hi5 / 2, base5 = 2 * plo, nr5 = 2 * (phi - plo + 1);
-            vert_pairs<80>(Y, n4, 0, t5, base5, plo, phi + 1);
-            __syncthreads();
-            horiz_pairs<80>(t5, lv5, nr5);
-            __syncthreads();
-            vert_pairs<160>(lv5, 160, base5, t6, a, a / 2, a / 2 + 8);
-            __syncthreads();
-            b = horiz_pairs_argmax<160>(t6, 16, a);
-        }
-        b = block_best(b, red);
-        best = best_merge(best, b);
-        __syncthreads();
-    }
-    // All strips NaN (or none processed): the reference's scan leaves (1, 1). Its value is
-    // recomputed from the first strip's first sample, which is NaN in that case.
-    if (best.idx == 0x7fffffff) {
-        best.idx = 0;
-        best.v = __int_as_float(0x7fc00000);
-    }
-    out.si = best.idx / nfin;
-    out.sj = best.idx - out.si * nfin;
-    out.val = best.v;
     return out;
 }
 

@@ -132,7 +132,6 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
     float* reg = (float*)smem;   // cascade region (aliases everything below once used)
     __shared__ Best red[AR_NWAVES];
     __shared__ int64_t red64[AR_NWAVES];
-    __shared__ int sc[8];
     int8_t* Wl = (int8_t*)(smem + L.off_w);
     int8_t* Tp = (int8_t*)(smem + L.off_t);
     int32_t* S1 = (int32_t*)(smem + L.off_s1);
