@@ -26,8 +26,8 @@ namespace ar {
 #define AR_TILE_WS 416    // per-wave tile scratch: vertical tmp 16 x 16 + block 10 x 16
 
 // Region layout (floats): patch [0, 32), scalars [32, 64), per-tile 8x row ranges lo [64, 96)
-// and hi [96, 128), tile flags [128, 256) as bytes, tile list [256, 656) as ints, then the levels.
-#define AR_CASC_HDR 656
+// and hi [96, 128), tile flags [128, 528) and tile list [528, 928) as ints, then the levels.
+#define AR_CASC_HDR 928
 
 __host__ __device__ inline int casc_levels(int up) {
     int L = 0;
@@ -99,9 +99,11 @@ __device__ inline void vert_pairs(const float* src, int n, int s_off, float* dst
     }
 }
 
-// Horizontal pass: `rows` rows of WS source columns -> 2 WS output columns.
-template <int WS>
-__device__ inline void horiz_pairs(const float* src, float* dst, int rows) {
+// Horizontal pass: `rows` rows of WS source columns -> 2 WS output columns. With ARGMAX, also
+// returns the thread's argmax over the samples it wrote (row-major index of the output).
+template <int WS, bool ARGMAX = false>
+__device__ inline Best horiz_pairs(const float* src, float* dst, int rows) {
+    Best best = best_empty();
     const int total = rows * WS;
     for (int t0 = threadIdx.x; t0 < total; t0 += AR_U * AR_WG) {
         float a[AR_U], b[AR_U], q[AR_U];
@@ -122,20 +124,27 @@ __device__ inline void horiz_pairs(const float* src, float* dst, int rows) {
 #pragma unroll
         for (int u = 0; u < AR_U; u++) {
             if (t0 + u * AR_WG < total) {
-                dst[o[u]] = tap3(a[u], b[u], q[u]);
-                dst[o[u] + 1] = tap2(b[u], q[u]);
+                float e = tap3(a[u], b[u], q[u]), d = tap2(b[u], q[u]);
+                dst[o[u]] = e;
+                dst[o[u] + 1] = d;
+                if (ARGMAX) {
+                    if (better(e, o[u], best)) { best.v = e; best.idx = o[u]; }
+                    if (better(d, o[u] + 1, best)) { best.v = d; best.idx = o[u] + 1; }
+                }
             }
         }
     }
+    return best;
 }
 
 // One full pyrUp step in LDS: src (N x N, row-major) -> dst (2N x 2N), via tmp (2N x N).
-template <int N>
-__device__ inline void pyrup_full(const float* src, float* tmp, float* dst) {
+template <int N, bool ARGMAX = false>
+__device__ inline Best pyrup_full(const float* src, float* tmp, float* dst) {
     vert_pairs<N>(src, N, 0, tmp, 0, 0, N);
     __syncthreads();
-    horiz_pairs<N>(tmp, dst, 2 * N);
+    Best b = horiz_pairs<N, ARGMAX>(tmp, dst, 2 * N);
     __syncthreads();
+    return b;
 }
 
 __device__ inline float ub_margin(float m) {
@@ -170,10 +179,16 @@ __device__ inline float wave_max(float v) {
     return v;
 }
 
+// Rows of the level-L block of a final-level tile: 16 on the final level F, then 10 and 7.
+__host__ __device__ constexpr int tile_rows(int L, int F) {
+    return L == F ? AR_TILE : (L == F - 1 ? 10 : 7);
+}
+
 // One wave computes the final-level tile (ti, tj) of level F from the materialised 8x level Z
 // (40 x 40, row-major), through its blocks of the levels in between (pitch 16 in `ws`). With
 // `prune`, it stops after an intermediate block whose bound is below `lb`. Returns the tile's
-// argmax under the peak_index rule (wave-uniform; empty if pruned or all NaN).
+// argmax under the peak_index rule (wave-uniform; empty if pruned or all NaN). Each pass issues
+// all of a lane's loads before any arithmetic.
 template <int F>
 __device__ inline Best tile_run(const float* Z, float* ws, int ti, int tj, float lb, bool prune) {
     const int lane = threadIdx.x & 63;
@@ -194,31 +209,58 @@ __device__ inline Best tile_run(const float* Z, float* ws, int ti, int tj, float
     Best best = best_empty();
 #pragma unroll
     for (int L = 4; L <= F; L++) {
+        const int NIT = (tile_rows(L, F) * 16 + 63) / 64;   // lane iterations covering the block
         const int n = 5 << (L - 1);
         const int nr = rhi[L] - rlo[L] + 1;
         const int ncs = chi[L - 1] - clo[L - 1] + 1;
-        for (int t = lane; t < nr * 16; t += 64) {
+        float xa[4], xb[4], xq[4];
+        bool x3[4];
+        // vertical pass: tmp[r][c] = level-L row rlo + r over level-(L-1) column clo + c
+#pragma unroll
+        for (int u = 0; u < NIT; u++) {
+            int t = lane + 64 * u;
             int r = t >> 4, c = t & 15;
-            if (c < ncs) {
-                int m, cc, q, nt;
-                taps(rlo[L] + r, n, m, cc, q, nt);
-                const float* col = src + (clo[L - 1] + c - sc0);
-                float va = col[(m - sr0) * sp], vb = col[(cc - sr0) * sp], vq = col[(q - sr0) * sp];
-                tmp[t] = nt == 3 ? tap3(va, vb, vq) : tap2(vb, vq);
-            }
+            r = r < nr ? r : nr - 1;
+            c = c < ncs ? c : ncs - 1;
+            int m, cc, q, nt;
+            taps(rlo[L] + r, n, m, cc, q, nt);
+            const float* col = src + (clo[L - 1] + c - sc0);
+            xa[u] = col[(m - sr0) * sp];
+            xb[u] = col[(cc - sr0) * sp];
+            xq[u] = col[(q - sr0) * sp];
+            x3[u] = nt == 3;
+        }
+#pragma unroll
+        for (int u = 0; u < NIT; u++) {
+            int t = lane + 64 * u;
+            if ((t >> 4) < nr && (t & 15) < ncs)
+                tmp[t] = x3[u] ? tap3(xa[u], xb[u], xq[u]) : tap2(xb[u], xq[u]);
         }
         wave_sync();
         const float* row0 = tmp - clo[L - 1];
+        const int nc = chi[L] - clo[L] + 1;
+        // horizontal pass: level-L samples (rlo + r, clo + c)
+#pragma unroll
+        for (int u = 0; u < NIT; u++) {
+            int t = lane + 64 * u;
+            int r = t >> 4, c = t & 15;
+            r = r < nr ? r : nr - 1;
+            c = c < nc ? c : nc - 1;
+            int m, cc, q, nt;
+            taps(clo[L] + c, n, m, cc, q, nt);
+            const float* row = row0 + r * 16;
+            xa[u] = row[m];
+            xb[u] = row[cc];
+            xq[u] = row[q];
+            x3[u] = nt == 3;
+        }
         if (L < F) {
-            const int nc = chi[L] - clo[L] + 1;
             float mx = -INFINITY;
-            for (int t = lane; t < nr * 16; t += 64) {
-                int r = t >> 4, c = t & 15;
-                if (c < nc) {
-                    int m, cc, q, nt;
-                    taps(clo[L] + c, n, m, cc, q, nt);
-                    const float* row = row0 + r * 16;
-                    float v = nt == 3 ? tap3(row[m], row[cc], row[q]) : tap2(row[cc], row[q]);
+#pragma unroll
+            for (int u = 0; u < NIT; u++) {
+                int t = lane + 64 * u;
+                if ((t >> 4) < nr && (t & 15) < nc) {
+                    float v = x3[u] ? tap3(xa[u], xb[u], xq[u]) : tap2(xb[u], xq[u]);
                     blk[t] = v;
                     mx = v > mx ? v : mx;
                 }
@@ -232,13 +274,11 @@ __device__ inline Best tile_run(const float* Z, float* ws, int ti, int tj, float
             sp = 16;
         } else {
             const int nf = 5 << F;
-            for (int t = lane; t < AR_TILE * AR_TILE; t += 64) {
-                int r = t >> 4, c = t & 15;
-                int m, cc, q, nt;
-                taps(clo[F] + c, n, m, cc, q, nt);
-                const float* row = row0 + r * 16;
-                float v = nt == 3 ? tap3(row[m], row[cc], row[q]) : tap2(row[cc], row[q]);
-                int idx = (rlo[F] + r) * nf + clo[F] + c;
+#pragma unroll
+            for (int u = 0; u < NIT; u++) {
+                int t = lane + 64 * u;
+                float v = x3[u] ? tap3(xa[u], xb[u], xq[u]) : tap2(xb[u], xq[u]);
+                int idx = (rlo[F] + (t >> 4)) * nf + clo[F] + (t & 15);
                 if (better(v, idx, best)) { best.v = v; best.idx = idx; }
             }
             best = wave_best(best);
@@ -248,19 +288,17 @@ __device__ inline Best tile_run(const float* Z, float* ws, int ti, int tj, float
     return best;
 }
 
-// Final levels 16x..64x (F = 4..6) from the materialised 8x level in Y.
+// Set-up of the tile search, run before the materialised levels (their barriers publish it):
+// per-tile 8x row ranges (the same serve columns), cleared flags and counter.
 template <int F>
-__device__ inline CascadeOut cascade_tiles(float* reg, const float* Y, float* ws, Best* red) {
-    const int tid = threadIdx.x, w = tid >> 6;
+__device__ inline void tiles_init(float* reg) {
+    const int tid = threadIdx.x;
     int* sc = (int*)(reg + 32);
     int* r3lo = (int*)(reg + 64);
     int* r3hi = (int*)(reg + 96);
-    unsigned char* flags = (unsigned char*)(reg + 128);
-    int* list = (int*)(reg + 256);
-    const int nf = 5 << F, NT = nf / AR_TILE;
-
+    int* flags = (int*)(reg + 128);
+    const int NT = (5 << F) / AR_TILE;
     if (tid < NT) {
-        // 8x rows feeding tile row tid (the same ranges serve columns)
         int lo = AR_TILE * tid, hi = lo + AR_TILE - 1;
         for (int L = F; L > 3; L--) feed_range(lo, hi, 5 << (L - 1), lo, hi);
         r3lo[tid] = lo;
@@ -268,13 +306,22 @@ __device__ inline CascadeOut cascade_tiles(float* reg, const float* Y, float* ws
     }
     for (int t = tid; t < NT * NT; t += AR_WG) flags[t] = 0;
     if (tid == 0) sc[0] = 0;
+}
+
+// Final levels 16x..64x (F = 4..6) from the materialised 8x level in Y; b3 is each thread's
+// argmax over the 8x samples it produced.
+template <int F>
+__device__ inline CascadeOut cascade_tiles(float* reg, const float* Y, float* ws, Best b3,
+                                           Best* red) {
+    const int tid = threadIdx.x, w = tid >> 6;
+    int* sc = (int*)(reg + 32);
+    const int* r3lo = (const int*)(reg + 64);
+    const int* r3hi = (const int*)(reg + 96);
+    int* flags = (int*)(reg + 128);
+    int* list = (int*)(reg + 528);
+    const int nf = 5 << F, NT = nf / AR_TILE;
 
     // Phase 1: the 2x2 tiles around the image of the 8x maximum, one per wave, no pruning.
-    Best b3 = best_empty();
-    for (int t = tid; t < 40 * 40; t += AR_WG) {
-        float v = Y[t];
-        if (better(v, t, b3)) { b3.v = v; b3.idx = t; }
-    }
     b3 = block_best(b3, red);
     int i3 = 20, j3 = 20;
     if (b3.idx != 0x7fffffff) {
@@ -289,12 +336,15 @@ __device__ inline CascadeOut cascade_tiles(float* reg, const float* Y, float* ws
     const Best pb = block_best(wb, red);
     const float LB = pb.v;
 
-    // Phase 2: flag every tile fed by an 8x sample whose bound reaches LB (the tile bound is
-    // the bound of its largest feeder; NaN feeders never raise it).
-    for (int t = tid; t < 40 * 40; t += AR_WG) {
-        float v = Y[t];
-        if (ub_margin(v) >= LB) {
-            int r3 = t / 40, c3 = t - 40 * r3;
+    // Phase 2: list every other tile fed by an 8x sample whose bound reaches LB (a tile's bound
+    // is that of its largest feeder; NaN feeders never raise it). The flags deduplicate.
+    for (int t4 = tid; t4 < 400; t4 += AR_WG) {
+        const float4 v4 = ((const float4*)Y)[t4];
+        const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            if (!(ub_margin(vv[e]) >= LB)) continue;
+            int t = 4 * t4 + e, r3 = t / 40, c3 = t - 40 * r3;
             int a0 = NT, a1 = -1, b0 = NT, b1 = -1;
             for (int k = 0; k < NT; k++) {
                 int lo = r3lo[k], hi = r3hi[k];
@@ -302,19 +352,17 @@ __device__ inline CascadeOut cascade_tiles(float* reg, const float* Y, float* ws
                 if (lo <= c3 && c3 <= hi) { b0 = k < b0 ? k : b0; b1 = k; }
             }
             for (int ti = a0; ti <= a1; ti++)
-                for (int tj = b0; tj <= b1; tj++) flags[ti * NT + tj] = 1;
+                for (int tj = b0; tj <= b1; tj++) {
+                    bool first = (unsigned)(ti - ti0) < 2u && (unsigned)(tj - tj0) < 2u;
+                    int f = ti * NT + tj;
+                    if (!first && atomicExch(&flags[f], 1) == 0) list[atomicAdd(&sc[0], 1)] = f;
+                }
         }
-    }
-    __syncthreads();
-    for (int t = tid; t < NT * NT; t += AR_WG) {
-        int ti = t / NT, tj = t - ti * NT;
-        bool first = (unsigned)(ti - ti0) < 2u && (unsigned)(tj - tj0) < 2u;
-        if (flags[t] && !first) list[atomicAdd(&sc[0], 1)] = t;
     }
     __syncthreads();
     const int cnt = sc[0];
 
-    // Phase 3: the flagged tiles, wave-strided, each pruned against the best value known.
+    // Phase 3: the listed tiles, wave-strided, each pruned against the best value known.
     Best best = pb;
     float lb = LB;
     for (int k = w; k < cnt; k += AR_NWAVES) {
@@ -352,18 +400,23 @@ __device__ inline CascadeOut cascade(float* reg, int up, Best* red) {
     float* X = Y + nM * nM;                // nP x nP
     float* T = X + nP * nP;                // vertical-pass tmp (2n x n)
 
+    if (L == 4) tiles_init<4>(reg);
+    else if (L == 5) tiles_init<5>(reg);
+    else if (L == 6) tiles_init<6>(reg);
+
     // Levels 1..LM, ping-ponging so that level LM lands in Y.
     const float* cur = patch;
+    Best b3 = best_empty();
     if (LM >= 1) { float* d = ((LM - 1) & 1) ? X : Y; pyrup_full<5>(cur, T, d); cur = d; }
     if (LM >= 2) { float* d = ((LM - 2) & 1) ? X : Y; pyrup_full<10>(cur, T, d); cur = d; }
-    if (LM >= 3) { float* d = ((LM - 3) & 1) ? X : Y; pyrup_full<20>(cur, T, d); cur = d; }
+    if (LM >= 3) { float* d = ((LM - 3) & 1) ? X : Y; b3 = pyrup_full<20, true>(cur, T, d); cur = d; }
 
     if (L >= 4) {
         float* ws = X + (threadIdx.x >> 6) * AR_TILE_WS;
         CascadeOut out;
-        if (L == 4) out = cascade_tiles<4>(reg, Y, ws, red);
-        else if (L == 5) out = cascade_tiles<5>(reg, Y, ws, red);
-        else out = cascade_tiles<6>(reg, Y, ws, red);
+        if (L == 4) out = cascade_tiles<4>(reg, Y, ws, b3, red);
+        else if (L == 5) out = cascade_tiles<5>(reg, Y, ws, b3, red);
+        else out = cascade_tiles<6>(reg, Y, ws, b3, red);
         __syncthreads();
         return out;
     }

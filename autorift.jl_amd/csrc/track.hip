@@ -80,6 +80,8 @@ __device__ inline bool patch_origin(int pi, int pj, int nr, int nc, int& i0, int
 
 // Upper bound on |approximate - exact| for the FP32 screening score (see u8 epilogue).
 #define AR_SCREEN_EPS 4e-6f
+// Columns whose global loads a lane issues before storing any (window and chip staging).
+#define AR_SB 4
 
 struct U8Point {
     int64_t n;       // chip area (power of two)
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
     const U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, a.up);
     float* reg = (float*)smem;   // cascade region (aliases everything below once used)
     __shared__ Best red[AR_NWAVES];
-    __shared__ int64_t red64[AR_NWAVES];
+    __shared__ int64_t red64[2 * AR_NWAVES];
     int8_t* Wl = (int8_t*)(smem + L.off_w);
     int8_t* Tp = (int8_t*)(smem + L.off_t);
     int32_t* S1 = (int32_t*)(smem + L.off_s1);
@@ -161,27 +163,43 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
             bool inside = orow >= 0 && ocol >= 0 && orow + winh <= a.ny && ocol + winw <= a.nx;
             if (inside) {
                 // Each lane moves 4 rows of one column as a dword: two aligned dword loads and a
-                // byte realignment (the column's start address has any alignment). Half-waves take
-                // alternate columns; rows past the window are tile padding (int8 0).
+                // byte realignment (the column's start address has any alignment). The 8
+                // half-waves take columns hw, hw + 8, ...; a lane issues the loads of AR_SB
+                // columns before storing any. Rows past the window are tile padding (int8 0).
                 const uint8_t* gb = ref + orow + ocol * a.ny;
                 const int nr4 = rows_used >> 2;
-                const int half = lane >> 5, l32 = lane & 31;
-                for (int c0 = 2 * wave; c0 < cols_used; c0 += 2 * AR_NWAVES) {
-                    const int c = c0 + half;
-                    const bool colok = c < cols_used;
-                    const uintptr_t cadr = (uintptr_t)(gb + (int64_t)c * a.ny);
-                    for (int r4 = l32; r4 < nr4; r4 += 32) {
-                        int r = 4 * r4;
-                        uint32_t v = 0;
-                        if (colok && c < winw && r < winh) {
-                            uintptr_t ad = cadr + (uintptr_t)r;
-                            const uint32_t* al = (const uint32_t*)(ad & ~(uintptr_t)3);
-                            v = __builtin_amdgcn_alignbyte(al[1], al[0], (uint32_t)(ad & 3));
-                            v ^= 0x80808080u;
-                            int keep = winh - r;   // rows of this dword inside the window
-                            if (keep < 4) v &= (1u << (8 * keep)) - 1u;
+                const int hw = tid >> 5, l32 = lane & 31;
+                for (int r4 = l32; r4 < nr4; r4 += 32) {
+                    const int r = 4 * r4;
+                    const bool rowok = r < winh;
+                    const int keep = winh - r;   // rows of this dword inside the window
+                    const uint32_t kmask = keep < 4 ? (1u << (8 * keep)) - 1u : 0xffffffffu;
+                    for (int c0 = hw; c0 < cols_used; c0 += 8 * AR_SB) {
+                        uint32_t lo[AR_SB], hi[AR_SB], sh[AR_SB];
+#pragma unroll
+                        for (int k = 0; k < AR_SB; k++) {
+                            const int c = c0 + 8 * k;
+                            const uintptr_t ad = (uintptr_t)(gb + (int64_t)c * a.ny) + (uintptr_t)r;
+                            sh[k] = (uint32_t)(ad & 3);
+                            lo[k] = 0;
+                            hi[k] = 0;
+                            if (rowok && c < winw) {
+                                const uint32_t* al = (const uint32_t*)(ad & ~(uintptr_t)3);
+                                lo[k] = al[0];
+                                hi[k] = al[1];
+                            }
                         }
-                        if (colok) *(uint32_t*)(Wl + c * L.ldw + r) = v;
+#pragma unroll
+                        for (int k = 0; k < AR_SB; k++) {
+                            const int c = c0 + 8 * k;
+                            if (c < cols_used) {
+                                uint32_t v = 0;
+                                if (rowok && c < winw)
+                                    v = (__builtin_amdgcn_alignbyte(hi[k], lo[k], sh[k]) ^ 0x80808080u) &
+                                        kmask;
+                                *(uint32_t*)(Wl + c * L.ldw + r) = v;
+                            }
+                        }
                     }
                 }
             } else {
@@ -196,30 +214,54 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
                     }
             }
         }
-        int64_t st = 0;
+        int64_t st = 0, st2 = 0;
         {
             int64_t orow = g.cr0 - a.pady - 1, ocol = g.cc0 - a.padx - 1;
             bool inside = orow >= 0 && ocol >= 0 && orow + ch <= a.ny && ocol + cw <= a.nx;
             if (inside && (ch & 3) == 0) {
                 // dword path: chip rows u = x - 16 for x in [16, 16 + ch), realigned as above;
-                // the zero band rows and the two trailing zero columns are written as zeros
+                // the zero band rows and the two trailing zero columns are written as zeros.
+                // Half-waves take columns hw, hw + 8, ...; loads of AR_SB columns go first.
                 const uint8_t* gb = sec + orow + ocol * a.ny;
                 const int nx4 = L.cpad >> 2;
-                uint32_t acc8 = 0;
-                for (int v = wave; v < cw + 2; v += AR_NWAVES)
-                    for (int x4 = lane; x4 < nx4; x4 += 64) {
-                        int u = 4 * x4 - 16;
-                        uint32_t w = 0;
-                        if (u >= 0 && u < ch && v < cw) {
-                            uintptr_t ad = (uintptr_t)(gb + (int64_t)v * a.ny + u);
-                            const uint32_t* al = (const uint32_t*)(ad & ~(uintptr_t)3);
-                            uint32_t raw = __builtin_amdgcn_alignbyte(al[1], al[0], (uint32_t)(ad & 3));
-                            acc8 = __builtin_amdgcn_sad_u8(raw, 0u, acc8);
-                            w = raw ^ 0x80808080u;
+                const int hw = tid >> 5, l32 = lane & 31;
+                uint32_t acc1 = 0, acc2 = 0;
+                for (int x4 = l32; x4 < nx4; x4 += 32) {
+                    const int u = 4 * x4 - 16;
+                    const bool rowok = u >= 0 && u < ch;
+                    for (int v0 = hw; v0 < cw + 2; v0 += 8 * AR_SB) {
+                        uint32_t lo[AR_SB], hi[AR_SB], sh[AR_SB];
+#pragma unroll
+                        for (int k = 0; k < AR_SB; k++) {
+                            const int v = v0 + 8 * k;
+                            const uintptr_t ad = (uintptr_t)(gb + (int64_t)v * a.ny + u);
+                            sh[k] = (uint32_t)(ad & 3);
+                            lo[k] = 0;
+                            hi[k] = 0;
+                            if (rowok && v < cw) {
+                                const uint32_t* al = (const uint32_t*)(ad & ~(uintptr_t)3);
+                                lo[k] = al[0];
+                                hi[k] = al[1];
+                            }
                         }
-                        *(uint32_t*)(Tp + v * L.cpad + 4 * x4) = w;
+#pragma unroll
+                        for (int k = 0; k < AR_SB; k++) {
+                            const int v = v0 + 8 * k;
+                            if (v < cw + 2) {
+                                uint32_t w = 0;
+                                if (rowok && v < cw) {
+                                    uint32_t raw = __builtin_amdgcn_alignbyte(hi[k], lo[k], sh[k]);
+                                    acc1 = __builtin_amdgcn_sad_u8(raw, 0u, acc1);
+                                    acc2 = __builtin_amdgcn_udot4(raw, raw, acc2, false);
+                                    w = raw ^ 0x80808080u;
+                                }
+                                *(uint32_t*)(Tp + v * L.cpad + 4 * x4) = w;
+                            }
+                        }
                     }
-                st = acc8;
+                }
+                st = acc1;
+                st2 = acc2;
             } else {
                 for (int v = wave; v < cw + 2; v += AR_NWAVES)   // two trailing zero columns
                     for (int x = lane; x < L.cpad; x += 64) {
@@ -229,6 +271,7 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
                             int64_t k = pix(a, g.cr0 + u, g.cc0 + v);
                             uint8_t raw = k >= 0 ? sec[k] : 0;
                             st += raw;
+                            st2 += (int64_t)raw * raw;
                             b = (int8_t)(raw ^ 0x80);
                         }
                         Tp[v * L.cpad + x] = b;
@@ -237,17 +280,10 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         }
         U8Point P;
         P.n = (int64_t)ch * cw;
-        P.sumT = block_sum<int64_t>(st, red64);
-        __syncthreads();
-        // ---- chip energy, exact: sum (n x - sumT)^2 over n^2
-        int64_t e = 0;
-        for (int v = wave; v < cw; v += AR_NWAVES)
-            for (int u = lane; u < ch; u += 64) {
-                int64_t x = (int64_t)(uint8_t)(Tp[v * L.cpad + 16 + u] ^ 0x80);
-                int64_t k = P.n * x - P.sumT;
-                e += k * k;
-            }
-        int64_t sumk2 = block_sum<int64_t>(e, red64);
+        // chip energy, exact in int64: sum (n x - sumT)^2 = n^2 sum x^2 - n sumT^2, over n^2
+        int64_t sumx2;
+        block_sum2<int64_t>(st, st2, red64, P.sumT, sumx2);
+        const int64_t sumk2 = P.n * P.n * sumx2 - P.n * P.sumT * P.sumT;
         double sq = (double)sumk2 / ((double)P.n * (double)P.n);
         bool ok = sq > 2.220446049250313e-16 * (double)ch * (double)cw;
         if (!ok) { write_skip(a, p, true); __syncthreads(); continue; }
@@ -327,6 +363,8 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
 #pragma unroll
             for (int k = 0; k < 4; k++) acc[k] = (v4i){0, 0, 0, 0};
             int v = g16 / nrb, rb = g16 - v * nrb;
+            // two steps per trip, so that the next step's fragment loads overlap these MFMAs
+#pragma unroll 2
             for (int m = 0; m < nm; m++) {
                 // B: 16 bytes of the zero-padded chip column v from row 16 rb - l16, realigned
                 // from five dwords

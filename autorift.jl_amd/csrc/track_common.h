@@ -158,6 +158,26 @@ __device__ inline T block_sum(T v, T* red) {
     return s;
 }
 
+// Two block sums in one exchange; `red` holds 2 AR_NWAVES elements.
+template <typename T>
+__device__ inline void block_sum2(T a, T b, T* red, T& sa, T& sb) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        red[w] = a;
+        red[AR_NWAVES + w] = b;
+    }
+    __syncthreads();
+    sa = red[0];
+    sb = red[AR_NWAVES];
+    for (int k = 1; k < AR_NWAVES; k++) {
+        sa += red[k];
+        sb += red[AR_NWAVES + k];
+    }
+}
+
 // Argmax key under peak_index's rule (src/peak.jl:32-61): larger value wins; equal values go to
 // the smaller row-major index; NaN never wins.
 struct Best {
