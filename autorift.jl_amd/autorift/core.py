@@ -210,3 +210,9 @@ class Context:
         ms, path = C.c_double(), C.c_int32()
         lib().ar_last_kernel(self._h, C.byref(ms), C.byref(path))
         return ms.value, path.value
+
+    def last_timing(self):
+        """(correlation kernel ms, refinement kernel ms) of the last tracking pass."""
+        t, r = C.c_double(), C.c_double()
+        lib().ar_last_timing(self._h, C.byref(t), C.byref(r))
+        return t.value, r.value

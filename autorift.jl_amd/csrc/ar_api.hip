@@ -19,6 +19,7 @@ namespace ar {
 
 // Defined in track.hip.
 __global__ void track_u8_kernel(TrackArgs a);
+__global__ void refine_kernel(TrackArgs a);
 template <int IS_U8>
 __global__ void track_generic_kernel(TrackArgs a);
 
@@ -217,7 +218,8 @@ ar_status ar_context_create(const ar_params* p, int32_t device, ar_context** out
     c->device = device;
     c->ncu = prop.multiProcessorCount;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->evm) != hipSuccess) {
         delete c;
         return fail(nullptr, AR_ERR_DEVICE, "stream/event creation failed");
     }
@@ -232,6 +234,7 @@ void ar_context_destroy(ar_context* ctx) {
     for (DevBuf* b : ctx->all_bufs()) b->release();
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
+    if (ctx->evm) hipEventDestroy(ctx->evm);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -381,9 +384,13 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     if (use_u8) {
         kern = track_u8_kernel;
     } else {
-        a.lds_bytes = (size_t)cascade_floats((int)up) * 4 + 64;
+        a.lds_bytes = 0;
         kern = ctx->pdtype == AR_U8 ? track_generic_kernel<1> : track_generic_kernel<0>;
     }
+    // refinement hand-off: 32 floats of patch and one packed origin per point
+    if ((st = ctx->refine.reserve(ctx, (size_t)n * 132))) return st;
+    a.rpatch = (float*)ctx->refine.ptr;
+    a.rmeta = (int32_t*)((char*)ctx->refine.ptr + (size_t)n * 128);
     HIPCHK(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)a.lds_bytes));
     int per_cu = 0;
@@ -397,14 +404,36 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         if ((st = ctx->scratch.reserve(ctx, G.total * nblocks))) return st;
         a.scratch = (unsigned char*)ctx->scratch.ptr;
     }
+    // refinement kernel: the cascade region only, so several workgroups share a CU
+    const size_t rlds = (size_t)cascade_floats((int)up) * 4;
+    int64_t rblocks = 0;
+    if (up > 1) {
+        HIPCHK(ctx, hipFuncSetAttribute((const void*)refine_kernel,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
+        int rpc = 0;
+        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&rpc, refine_kernel, AR_WG, rlds));
+        if (rpc < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "refinement kernel does not fit on a CU");
+        rblocks = (int64_t)ctx->ncu * rpc;
+        if (rblocks > n) rblocks = n;
+    }
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(AR_WG), a.lds_bytes, ctx->stream, a);
     HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->evm, ctx->stream));
+    if (up > 1) {
+        hipLaunchKernelGGL(refine_kernel, dim3((unsigned)rblocks), dim3(AR_WG), rlds, ctx->stream,
+                           a);
+        HIPCHK(ctx, hipGetLastError());
+    }
     HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    float ms = 0;
+    float ms = 0, ms_t = 0, ms_r = 0;
     hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    hipEventElapsedTime(&ms_t, ctx->ev0, ctx->evm);
+    hipEventElapsedTime(&ms_r, ctx->evm, ctx->ev1);
     ctx->last_ms = ms;
+    ctx->last_ms_track = ms_t;
+    ctx->last_ms_refine = up > 1 ? ms_r : 0.0;
     ctx->last_path = use_u8 ? 0 : 1;
     return AR_OK;
 }
@@ -483,6 +512,14 @@ extern "C" ar_status ar_last_kernel(const ar_context* ctx, double* ms, int32_t* 
     if (!ctx) return AR_ERR_INVALID_ARGUMENT;
     if (ms) *ms = ctx->last_ms;
     if (path) *path = ctx->last_path;
+    return AR_OK;
+}
+
+extern "C" ar_status ar_last_timing(const ar_context* ctx, double* track_ms,
+                                    double* refine_ms) {
+    if (!ctx) return AR_ERR_INVALID_ARGUMENT;
+    if (track_ms) *track_ms = ctx->last_ms_track;
+    if (refine_ms) *refine_ms = ctx->last_ms_refine;
     return AR_OK;
 }
 

@@ -22,7 +22,7 @@ struct ar_context {
     int device = 0;
     int ncu = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;
     std::string err;
 
     // prepared pair (what the tracker correlates), column-major ny x nx
@@ -39,13 +39,14 @@ struct ar_context {
 
     // scratch
     DevBuf flags, geom, pts, out, scratch, grid_a, grid_b;
+    DevBuf refine;              // per-point refinement hand-off: 5x5 patches and origins
 
-    double last_ms = 0;
+    double last_ms = 0, last_ms_track = 0, last_ms_refine = 0;
     int32_t last_path = -1;
 
     std::vector<DevBuf*> all_bufs() {
         return {&pref, &psec, &rmask, &smask, &okmask, &raw, &rawmask, &work_f32, &work_f64,
-                &flags, &geom, &pts, &out, &scratch, &grid_a, &grid_b};
+                &flags, &geom, &pts, &out, &scratch, &grid_a, &grid_b, &refine};
     }
 };
 

@@ -27,8 +27,7 @@ __host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx
     L.cpad = rup(16 * nrb + 32, 16);
     int nr = 2 * max_ry, nc = 2 * max_rx;
     int winw = max_cw + 2 * max_rx - 1;
-    size_t hdr = 1024;   // cascade header alias: patch and scalars live at the front
-    L.off_w = hdr;
+    L.off_w = 0;
     L.off_t = L.off_w + (size_t)L.cols * L.ldw;
     L.off_s1 = rup((int)(L.off_t + (size_t)(max_cw + 2) * L.cpad), 16);
     L.off_s2 = L.off_s1 + (size_t)nr * nc * 4;
@@ -36,8 +35,6 @@ __host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx
     size_t colsum = (size_t)nr * winw * 8;   // W and W^2 column sums
     size_t accsurf = (size_t)nr * nc * 8;   // int32 accumulators + float approximations
     L.total = L.off_tmp + (colsum > accsurf ? colsum : accsurf);
-    size_t casc = (size_t)cascade_floats(up) * 4 + 64;
-    if (casc > L.total) L.total = casc;
     return L;
 }
 

@@ -34,6 +34,7 @@ __device__ inline void write_skip(const TrackArgs& a, int64_t p, bool searched) 
         a.dy[p] = __int_as_float(0x7fc00000);
         a.corr[p] = __int_as_float(0x7fc00000);
         a.searched[p] = searched ? 1 : 0;
+        a.rmeta[p] = -1;
     }
 }
 
@@ -48,20 +49,48 @@ __device__ inline void write_result(const TrackArgs& a, int64_t p, const Geom& g
     }
 }
 
-// After the integer peak (pi, pj) (1-based) and its exact value are known, and with the 5x5
-// patch (exact values, row-major) in `reg[0..24]`, finish the point.
+// After the integer peak (pi, pj) (1-based) and its exact value are known: without refinement,
+// write the point's result; with it, hand the 5x5 patch (exact values; thread t < 25 holds
+// element t, row-major) and its origin to the refinement kernel.
 __device__ inline void finish_point(const TrackArgs& a, int64_t p, const Geom& g, int pi, int pj,
-                                    float pval, int i0, int j0, bool refine, float* reg,
-                                    Best* red) {
+                                    float pval, int i0, int j0, bool refine, float pv) {
     if (!refine) {
         write_result(a, p, g, (double)(pj - g.rx - 1), (double)(pi - g.ry - 1), pval);
+        if (threadIdx.x == 0) a.rmeta[p] = -1;
         return;
     }
-    CascadeOut c = cascade(reg, a.up, red);
-    double factor = (double)a.up;
-    double row = (double)c.si / factor + (double)(i0 - 1);
-    double col = (double)c.sj / factor + (double)(j0 - 1);
-    write_result(a, p, g, col - (double)g.rx, row - (double)g.ry, c.val);
+    if (threadIdx.x < 25) a.rpatch[32 * p + threadIdx.x] = pv;
+    if (threadIdx.x == 0) {
+        a.rmeta[p] = i0 | (j0 << 16);
+        a.searched[p] = 1;
+    }
+}
+
+// The pyrUp cascade of every point the correlation kernel handed over (src/peak.jl:456-522),
+// with the offsets of src/peak.jl:515-521 and the prior.
+__global__ __launch_bounds__(AR_WG) void refine_kernel(TrackArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* reg = (float*)smem;
+    __shared__ Best red[AR_NWAVES];
+    const PointRange pr = point_range(a.n);
+    for (int64_t p = pr.first; p < pr.end; p += pr.step) {
+        const int32_t m = a.rmeta[p];
+        if (m < 0) continue;
+        if (threadIdx.x < 25) reg[threadIdx.x] = a.rpatch[32 * p + threadIdx.x];
+        __syncthreads();
+        CascadeOut c = cascade(reg, a.up, red);
+        const int i0 = m & 0xffff, j0 = m >> 16;
+        Geom g;
+        g.pdx = a.dx_prior[p];
+        g.pdy = a.dy_prior[p];
+        g.rx = (int)a.radius_x[p];
+        g.ry = (int)a.radius_y[p];
+        double factor = (double)a.up;
+        double row = (double)c.si / factor + (double)(i0 - 1);
+        double col = (double)c.sj / factor + (double)(j0 - 1);
+        write_result(a, p, g, col - (double)g.rx, row - (double)g.ry, c.val);
+        __syncthreads();
+    }
 }
 
 // Patch origin of src/peak.jl:473-478 (1-based); returns false when the surface is smaller than
@@ -131,7 +160,6 @@ __device__ inline float approx_u8(const U8Point& P, int32_t acc, int32_t s1, int
 __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, a.up);
-    float* reg = (float*)smem;   // cascade region (aliases everything below once used)
     __shared__ Best red[AR_NWAVES];
     __shared__ int64_t red64[2 * AR_NWAVES];
     int8_t* Wl = (int8_t*)(smem + L.off_w);
@@ -363,8 +391,6 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
 #pragma unroll
             for (int k = 0; k < 4; k++) acc[k] = (v4i){0, 0, 0, 0};
             int v = g16 / nrb, rb = g16 - v * nrb;
-            // two steps per trip, so that the next step's fragment loads overlap these MFMAs
-#pragma unroll 2
             for (int m = 0; m < nm; m++) {
                 // B: 16 bytes of the zero-padded chip column v from row 16 rb - l16, realigned
                 // from five dwords
@@ -433,10 +459,7 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
             int o = jj * NR + ii;
             pv = exact_u8(P, accL[o], S1[o], S2[o], a.measure);
         }
-        __syncthreads();   // everything above is dead from here: the cascade may alias it
-        if (refine && tid < 25) reg[tid] = pv;
-        __syncthreads();
-        finish_point(a, p, g, pi, pj, pval, i0, j0, refine, reg, red);
+        finish_point(a, p, g, pi, pj, pval, i0, j0, refine, pv);
         __syncthreads();
     }
 }
@@ -454,8 +477,6 @@ __device__ inline float load_pix_f(const TrackArgs& a, const void* img, int is_u
 
 template <int IS_U8>
 __global__ __launch_bounds__(AR_WG) void track_generic_kernel(TrackArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    float* reg = (float*)smem;
     __shared__ Best red[AR_NWAVES];
     __shared__ double shd[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -562,9 +583,9 @@ __global__ __launch_bounds__(AR_WG) void track_generic_kernel(TrackArgs a) {
         float pval = surf[(pj - 1) * NR + (pi - 1)];
         int i0 = 1, j0 = 1;
         bool refine = a.up > 1 && patch_origin(pi, pj, NR, NC, i0, j0);
-        if (refine && tid < 25) reg[tid] = surf[(j0 - 1 + tid % 5) * NR + (i0 - 1 + tid / 5)];
-        __syncthreads();
-        finish_point(a, p, g, pi, pj, pval, i0, j0, refine, reg, red);
+        float pv = 0.0f;
+        if (refine && tid < 25) pv = surf[(j0 - 1 + tid % 5) * NR + (i0 - 1 + tid / 5)];
+        finish_point(a, p, g, pi, pj, pval, i0, j0, refine, pv);
         __syncthreads();
     }
 }

@@ -33,6 +33,11 @@ struct TrackArgs {
     float* dy;
     float* corr;
     uint8_t* searched;
+    // refinement hand-off between the correlation and refinement kernels: the 5x5 patch of
+    // point p at rpatch[32 p ..], its 1-based origin packed as i0 | j0 << 16 in rmeta[p]
+    // (-1: nothing to refine; the correlation kernel has written the result)
+    float* rpatch;
+    int32_t* rmeta;
     // per-pass maxima the LDS layout is carved for
     int32_t max_ch, max_cw, max_rx, max_ry;
     size_t lds_bytes;

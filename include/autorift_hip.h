@@ -139,9 +139,15 @@ ar_status ar_correlate_point(ar_context* ctx, const void* search, const void* ch
                              int64_t upsampling, int32_t measure, double* dx, double* dy,
                              float* corr);
 
-/* Device timing of the last ar_track / ar_track_device kernel, in milliseconds (HIP events on
- * the context's stream), and the kernel path it took: 0 u8 matrix-core path, 1 generic. */
+/* Device timing of the last ar_track / ar_track_device pass, in milliseconds (HIP events on
+ * the context's stream: correlation kernel plus refinement kernel), and the correlation path it
+ * took: 0 u8 matrix-core path, 1 generic. */
 ar_status ar_last_kernel(const ar_context* ctx, double* ms, int32_t* path);
+
+/* The same pass split by kernel: the correlation kernel (integer peak, exact 5x5 patch) and the
+ * refinement kernel (pyrUp cascade; 0 when upsampling is 1). No counterpart in the reference,
+ * whose track loop is not timed per stage. */
+ar_status ar_last_timing(const ar_context* ctx, double* track_ms, double* refine_ms);
 
 /* Message of the last failed call on ctx (or of the last failed ar_context_create when ctx is
  * NULL). Valid until the next call on the same context. */
