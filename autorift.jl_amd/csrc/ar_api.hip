@@ -404,14 +404,14 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         if ((st = ctx->scratch.reserve(ctx, G.total * nblocks))) return st;
         a.scratch = (unsigned char*)ctx->scratch.ptr;
     }
-    // refinement kernel: the cascade region only, so several workgroups share a CU
+    // refinement kernel: one wave per workgroup with its own cascade region
     const size_t rlds = (size_t)cascade_floats((int)up) * 4;
     int64_t rblocks = 0;
     if (up > 1) {
         HIPCHK(ctx, hipFuncSetAttribute((const void*)refine_kernel,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
         int rpc = 0;
-        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&rpc, refine_kernel, AR_WG, rlds));
+        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&rpc, refine_kernel, 64, rlds));
         if (rpc < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "refinement kernel does not fit on a CU");
         rblocks = (int64_t)ctx->ncu * rpc;
         if (rblocks > n) rblocks = n;
@@ -421,8 +421,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ctx->evm, ctx->stream));
     if (up > 1) {
-        hipLaunchKernelGGL(refine_kernel, dim3((unsigned)rblocks), dim3(AR_WG), rlds, ctx->stream,
-                           a);
+        hipLaunchKernelGGL(refine_kernel, dim3((unsigned)rblocks), dim3(64), rlds, ctx->stream, a);
         HIPCHK(ctx, hipGetLastError());
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));

@@ -67,18 +67,19 @@ __device__ inline void finish_point(const TrackArgs& a, int64_t p, const Geom& g
 }
 
 // The pyrUp cascade of every point the correlation kernel handed over (src/peak.jl:456-522),
-// with the offsets of src/peak.jl:515-521 and the prior.
-__global__ __launch_bounds__(AR_WG) void refine_kernel(TrackArgs a) {
+// with the offsets of src/peak.jl:515-521 and the prior. One wave per workgroup, one point at a
+// time; no workgroup barriers, so many points are in flight on each CU.
+__global__ __launch_bounds__(64) void refine_kernel(TrackArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     float* reg = (float*)smem;
-    __shared__ Best red[AR_NWAVES];
+    const int lane = threadIdx.x;
     const PointRange pr = point_range(a.n);
     for (int64_t p = pr.first; p < pr.end; p += pr.step) {
         const int32_t m = a.rmeta[p];
         if (m < 0) continue;
-        if (threadIdx.x < 25) reg[threadIdx.x] = a.rpatch[32 * p + threadIdx.x];
-        __syncthreads();
-        CascadeOut c = cascade(reg, a.up, red);
+        if (lane < 25) reg[lane] = a.rpatch[32 * p + lane];
+        wave_sync();
+        CascadeOut c = cascade(reg, a.up);
         const int i0 = m & 0xffff, j0 = m >> 16;
         Geom g;
         g.pdx = a.dx_prior[p];
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(AR_WG) void refine_kernel(TrackArgs a) {
         double row = (double)c.si / factor + (double)(i0 - 1);
         double col = (double)c.sj / factor + (double)(j0 - 1);
         write_result(a, p, g, col - (double)g.rx, row - (double)g.ry, c.val);
-        __syncthreads();
+        wave_sync();
     }
 }
 
