@@ -373,10 +373,12 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     a.max_rx = (int32_t)rx;
     a.max_ry = (int32_t)ry;
 
-    // the u8 kernel's integer identities hold in int32 for chip areas up to 128 x 128
-    bool use_u8 = ctx->pdtype == AR_U8 && pow2 && (int64_t)a.max_ch * a.max_cw <= 16384;
+    // the u8 kernel's integer identities hold in int32 for chip areas up to 128 x 128, and its
+    // uint16 column sums of W for chip heights up to 256
+    bool use_u8 = ctx->pdtype == AR_U8 && pow2 && (int64_t)a.max_ch * a.max_cw <= 16384 &&
+                  a.max_ch <= 256;
     if (use_u8) {
-        U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, (int)up);
+        U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
         if (L.total > kLdsLimit) use_u8 = false;
         else a.lds_bytes = L.total;
     }

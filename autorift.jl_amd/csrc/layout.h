@@ -2,20 +2,22 @@
 #pragma once
 #include <stddef.h>
 
-#include "refine.h"
 
 namespace ar {
 
 struct U8Layout {
-    int ldw, rows, cols;   // window: cols x ldw bytes, column-major, int8 (value ^ 0x80)
-    int cpad;              // chip: max_cw x cpad bytes; chip column v at v*cpad, row u at 16+u
+    int ldw, rows, cols;   // window: column pitch ldw bytes (int8, value ^ 0x80), column-major
+    int cpad;              // chip: max_cw + 2 columns of cpad bytes; row u at 16 + u
     size_t off_w, off_t, off_s1, off_s2, off_tmp, total;
 };
 
 __host__ __device__ inline int rup(int x, int m) { return (x + m - 1) / m * m; }
 
-__host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx, int max_ry,
-                                              int up) {
+// LDS of the u8 tracker. The matrix-core loop reads whole 16x16 tiles of the window: rows up to
+// 16 (nti + nrb) and columns up to 16 ntj + max_cw + 1. Only the max_cw + 2 max_rx - 1 real
+// columns are allocated; reads past them land on the chip and the sums, finite int8 data that
+// meets a zero chip column or a dropped shift.
+__host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx, int max_ry) {
     U8Layout L;
     int nti = (2 * max_ry + 15) / 16, ntj = (2 * max_rx + 15) / 16;
     int nrb = (max_ch + 15 + 15) / 16;   // 16-row blocks of the band, ceil((ch + 15) / 16)
@@ -23,18 +25,20 @@ __host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx
     // an odd number of 16-byte units per column keeps the 16 lanes of a b128 read on
     // distinct LDS banks
     L.ldw = ((L.rows / 16) & 1) ? L.rows : L.rows + 16;
-    L.cols = 16 * ntj + max_cw + 2;   // + 2 zero columns read by the padded last k-blocks
+    L.cols = 16 * ntj + max_cw + 2;   // columns the tile loop reads
     L.cpad = rup(16 * nrb + 32, 16);
     int nr = 2 * max_ry, nc = 2 * max_rx;
     int winw = max_cw + 2 * max_rx - 1;
     L.off_w = 0;
-    L.off_t = L.off_w + (size_t)L.cols * L.ldw;
+    L.off_t = (size_t)winw * L.ldw;
     L.off_s1 = rup((int)(L.off_t + (size_t)(max_cw + 2) * L.cpad), 16);
     L.off_s2 = L.off_s1 + (size_t)nr * nc * 4;
     L.off_tmp = L.off_s2 + (size_t)nr * nc * 4;
-    size_t colsum = (size_t)nr * winw * 8;   // W and W^2 column sums
-    size_t accsurf = (size_t)nr * nc * 8;   // int32 accumulators + float approximations
+    size_t colsum = (size_t)nr * winw * 4;   // column sums of W, then of W^2
+    size_t accsurf = (size_t)nr * nc * 8;    // int32 accumulators + float approximations
     L.total = L.off_tmp + (colsum > accsurf ? colsum : accsurf);
+    size_t reach = (size_t)L.cols * L.ldw;   // furthest window byte the tile loop reads
+    if (reach > L.total) L.total = reach;
     return L;
 }
 

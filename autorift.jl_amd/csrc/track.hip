@@ -160,7 +160,7 @@ __device__ inline float approx_u8(const U8Point& P, int32_t acc, int32_t s1, int
 
 __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, a.up);
+    const U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
     __shared__ Best red[AR_NWAVES];
     __shared__ int64_t red64[2 * AR_NWAVES];
     int8_t* Wl = (int8_t*)(smem + L.off_w);
@@ -182,7 +182,6 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         const int nti = (NR + 15) / 16, ntj = (NC + 15) / 16;
         const int nrb = (ch + 15 + 15) / 16;          // band rows r - i in [0, ch), i < 16
         const int rows_used = 16 * (nti + nrb);
-        const int cols_used = 16 * ntj + cw + 2;
 
         // ---- stage window (int8, value ^ 0x80; virtual zero padding -> 0x80) and chip
         {
@@ -203,7 +202,7 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
                     const bool rowok = r < winh;
                     const int keep = winh - r;   // rows of this dword inside the window
                     const uint32_t kmask = keep < 4 ? (1u << (8 * keep)) - 1u : 0xffffffffu;
-                    for (int c0 = hw; c0 < cols_used; c0 += 8 * AR_SB) {
+                    for (int c0 = hw; c0 < winw; c0 += 8 * AR_SB) {
                         uint32_t lo[AR_SB], hi[AR_SB], sh[AR_SB];
 #pragma unroll
                         for (int k = 0; k < AR_SB; k++) {
@@ -221,9 +220,9 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
 #pragma unroll
                         for (int k = 0; k < AR_SB; k++) {
                             const int c = c0 + 8 * k;
-                            if (c < cols_used) {
+                            if (c < winw) {
                                 uint32_t v = 0;
-                                if (rowok && c < winw)
+                                if (rowok)
                                     v = (__builtin_amdgcn_alignbyte(hi[k], lo[k], sh[k]) ^ 0x80808080u) &
                                         kmask;
                                 *(uint32_t*)(Wl + c * L.ldw + r) = v;
@@ -232,10 +231,10 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
                     }
                 }
             } else {
-                for (int c = wave; c < cols_used; c += AR_NWAVES)
+                for (int c = wave; c < winw; c += AR_NWAVES)
                     for (int r = lane; r < rows_used; r += 64) {
                         int8_t v = 0;
-                        if (r < winh && c < winw) {
+                        if (r < winh) {
                             int64_t k = pix(a, g.wr0 + r, g.wc0 + c);
                             v = (int8_t)((k >= 0 ? ref[k] : 0) ^ 0x80);
                         }
@@ -320,51 +319,45 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         P.inv_cnorm_f = (float)(1.0 / P.cnorm);
 
         // ---- integer box sums S1 (W) and S2 (W^2) over every shift: sliding sums, with each
-        // window column split into KC row chunks and each shift row into KR column chunks
+        // window column split into KC row chunks and each shift row into KR column chunks; W
+        // first, then W^2, through one buffer of column sums
         {
             constexpr int KC = 3, KR = 5;
-            int32_t* c1 = colS;
-            int32_t* c2 = colS + NR * winw;
+            int32_t* cs = colS;
             const int lc = (NR + KC - 1) / KC, lr = (NC + KR - 1) / KR;
-            for (int t = tid; t < winw * KC; t += AR_WG) {
-                int c = t / KC, k = t - c * KC;
-                int r0 = k * lc, r1 = r0 + lc < NR ? r0 + lc : NR;
-                if (r0 >= r1) continue;
-                const uint8_t* col = (const uint8_t*)(Wl + c * L.ldw);
-                int32_t a1 = 0, a2 = 0;
-                for (int u = 0; u < ch; u++) {
-                    int32_t w = col[r0 + u] ^ 0x80;
-                    a1 += w;
-                    a2 += w * w;
-                }
-                for (int r = r0; r < r1; r++) {
-                    c1[c * NR + r] = a1;
-                    c2[c * NR + r] = a2;
-                    int32_t wo = col[r] ^ 0x80, wi = col[r + ch] ^ 0x80;
-                    a1 += wi - wo;
-                    a2 += wi * wi - wo * wo;
-                }
-            }
-            __syncthreads();
-            for (int t = tid; t < NR * KR; t += AR_WG) {
-                int i = t / KR, k = t - i * KR;
-                int j0 = k * lr, j1 = j0 + lr < NC ? j0 + lr : NC;
-                if (j0 >= j1) continue;
-                int32_t a1 = 0, a2 = 0;
-                for (int v = 0; v < cw; v++) {
-                    a1 += c1[(j0 + v) * NR + i];
-                    a2 += c2[(j0 + v) * NR + i];
-                }
-                for (int j = j0; j < j1; j++) {
-                    S1[j * NR + i] = a1;
-                    S2[j * NR + i] = a2;
-                    if (j + 1 < j1) {
-                        a1 += c1[(j + cw) * NR + i] - c1[j * NR + i];
-                        a2 += c2[(j + cw) * NR + i] - c2[j * NR + i];
+#pragma unroll 1
+            for (int sq = 0; sq < 2; sq++) {
+                for (int t = tid; t < winw * KC; t += AR_WG) {
+                    int c = t / KC, k = t - c * KC;
+                    int r0 = k * lc, r1 = r0 + lc < NR ? r0 + lc : NR;
+                    if (r0 >= r1) continue;
+                    const uint8_t* col = (const uint8_t*)(Wl + c * L.ldw);
+                    int32_t acc = 0;
+                    for (int u = 0; u < ch; u++) {
+                        int32_t w = col[r0 + u] ^ 0x80;
+                        acc += sq ? w * w : w;
+                    }
+                    for (int r = r0; r < r1; r++) {
+                        cs[c * NR + r] = acc;
+                        int32_t wo = col[r] ^ 0x80, wi = col[r + ch] ^ 0x80;
+                        acc += sq ? wi * wi - wo * wo : wi - wo;
                     }
                 }
+                __syncthreads();
+                int32_t* Sx = sq ? S2 : S1;
+                for (int t = tid; t < NR * KR; t += AR_WG) {
+                    int i = t / KR, k = t - i * KR;
+                    int j0 = k * lr, j1 = j0 + lr < NC ? j0 + lr : NC;
+                    if (j0 >= j1) continue;
+                    int32_t acc = 0;
+                    for (int v = 0; v < cw; v++) acc += cs[(j0 + v) * NR + i];
+                    for (int j = j0; j < j1; j++) {
+                        Sx[j * NR + i] = acc;
+                        if (j + 1 < j1) acc += cs[(j + cw) * NR + i] - cs[j * NR + i];
+                    }
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
 
         // ---- numerator on int8 MFMA; epilogue scores every shift in FP32
