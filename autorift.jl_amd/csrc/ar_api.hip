@@ -107,6 +107,7 @@ __global__ void pass_geometry_kernel(ar_points pts, int64_t n, int64_t ny, int64
                                      unsigned long long* g, int* flags) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long cx = 0, cy = 0, rx = 0, ry = 0, px = 0, py = 0;
+    unsigned long long ex = ~0ULL, ey = ~0ULL;   // smallest even chip extents
     int fits = 1, pow2 = 1;
     for (; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         int64_t prx = pts.radius_x[i], pry = pts.radius_y[i];
@@ -114,6 +115,8 @@ __global__ void pass_geometry_kernel(ar_points pts, int64_t n, int64_t ny, int64
         int64_t csx = pts.chip_size_x[i], csy = pts.chip_size_y[i];
         cx = max(cx, (unsigned long long)(csx < 0 ? 0 : csx));
         cy = max(cy, (unsigned long long)(csy < 0 ? 0 : csy));
+        ex = min(ex, (unsigned long long)(csx < 0 ? 0 : 2 * (csx / 2)));
+        ey = min(ey, (unsigned long long)(csy < 0 ? 0 : 2 * (csy / 2)));
         rx = max(rx, (unsigned long long)prx);
         ry = max(ry, (unsigned long long)pry);
         int64_t qx = csx / 2 + prx + (int64_t)ceil(fabs(pts.dx_prior[i]));
@@ -134,6 +137,8 @@ __global__ void pass_geometry_kernel(ar_points pts, int64_t n, int64_t ny, int64
     atomicMax(&g[3], ry);
     atomicMax(&g[4], px);
     atomicMax(&g[5], py);
+    atomicMin(&g[6], ex);
+    atomicMin(&g[7], ey);
     if (!fits) atomicAnd(&flags[0], 0);
     if (!pow2) atomicAnd(&flags[1], 0);
 }
@@ -328,7 +333,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     if (n == 0) return AR_OK;
     ar_status st;
     if ((st = ctx->geom.reserve(ctx, 128))) return st;
-    unsigned long long g0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long g0[8] = {0, 0, 0, 0, 0, 0, ~0ULL, ~0ULL};
     int f0[2] = {1, 1};
     HIPCHK(ctx, hipMemcpyAsync(ctx->geom.ptr, g0, sizeof(g0), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync((char*)ctx->geom.ptr + 64, f0, sizeof(f0), hipMemcpyHostToDevice, ctx->stream));
@@ -341,6 +346,9 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     HIPCHK(ctx, hipMemcpyAsync(f, (char*)ctx->geom.ptr + 64, sizeof(f), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     int64_t cx = g[0], cy = g[1], rx = g[2], ry = g[3], px = g[4], py = g[5];
+    // one even chip extent for every searchable point
+    const bool uniform = g[6] == (unsigned long long)(2 * (cx / 2)) &&
+                         g[7] == (unsigned long long)(2 * (cy / 2));
     bool fits = f[0] != 0, pow2 = f[1] != 0;
     if (cx == 0) {
         skip_all_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dx, dy, corr, srch, n);
@@ -388,6 +396,25 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     } else {
         a.lds_bytes = 0;
         kern = ctx->pdtype == AR_U8 ? track_generic_kernel<1> : track_generic_kernel<0>;
+    }
+    // Box images of W and W^2 replace the per-point box sums when every point has the same chip
+    // and the points' shifts cover at least a quarter of the image's chip positions.
+    if (use_u8 && uniform && a.max_ch > 0 && a.max_cw > 0) {
+        const double positions = (double)(a.ny + 2 * a.pady - a.max_ch + 1) *
+                                 (double)(a.nx + 2 * a.padx - a.max_cw + 1);
+        const double shifts = (double)n * (2.0 * a.max_rx) * (2.0 * a.max_ry);
+        if (positions >= 1 && 4.0 * shifts >= positions) {
+            int32_t *b1 = nullptr, *b2 = nullptr;
+            int64_t ld = 0;
+            st = ar_box_images(ctx, a.max_ch, a.max_cw, a.pady, a.padx, &b1, &b2, &ld);
+            if (st == AR_ERR_OUT_OF_MEMORY) return fail(ctx, st, "out of device memory (box images)");
+            if (st == AR_ERR_DEVICE) return fail(ctx, st, "box image kernel failed to launch");
+            if (st == AR_OK) {
+                a.box1 = b1;
+                a.box2 = b2;
+                a.box_ld = ld;
+            }
+        }
     }
     // refinement hand-off: 32 floats of patch and one packed origin per point
     if ((st = ctx->refine.reserve(ctx, (size_t)n * 132))) return st;

@@ -40,13 +40,14 @@ struct ar_context {
     // scratch
     DevBuf flags, geom, pts, out, scratch, grid_a, grid_b;
     DevBuf refine;              // per-point refinement hand-off: 5x5 patches and origins
+    DevBuf box;                 // box images of W and W^2 for a pass with one chip size
 
     double last_ms = 0, last_ms_track = 0, last_ms_refine = 0;
     int32_t last_path = -1;
 
     std::vector<DevBuf*> all_bufs() {
         return {&pref, &psec, &rmask, &smask, &okmask, &raw, &rawmask, &work_f32, &work_f64,
-                &flags, &geom, &pts, &out, &scratch, &grid_a, &grid_b, &refine};
+                &flags, &geom, &pts, &out, &scratch, &grid_a, &grid_b, &refine, &box};
     }
 };
 
@@ -55,5 +56,7 @@ ar_status ar_upload_masks(ar_context* ctx, const uint8_t* ref_valid, const uint8
                           int64_t N);
 ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t up,
                         int32_t measure, float* dx, float* dy, float* corr, uint8_t* srch);
+ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t padx,
+                        int32_t** b1, int32_t** b2, int64_t* ld);
 ar_status ar_upload_points(ar_context* ctx, const ar_points* pts, int64_t n, ar_points* dp,
                            int64_t csx_fill, int64_t csy_fill);

@@ -242,6 +242,31 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
                     }
             }
         }
+        if (a.box1) {
+            // S1 / S2 of every shift from the pass's box images (exact integers); lanes take
+            // shift rows i, waves take shift columns j, AR_SB columns of loads in flight
+            const int64_t br = g.wr0 - 1, bc = g.wc0 - 1;
+            for (int i = lane; i < NR; i += 64) {
+                for (int j0 = wave; j0 < NC; j0 += AR_NWAVES * AR_SB) {
+                    int32_t v1[AR_SB], v2[AR_SB];
+#pragma unroll
+                    for (int k = 0; k < AR_SB; k++) {
+                        const int j = j0 + AR_NWAVES * k;
+                        const int64_t q = br + i + (bc + (j < NC ? j : NC - 1)) * a.box_ld;
+                        v1[k] = a.box1[q];
+                        v2[k] = a.box2[q];
+                    }
+#pragma unroll
+                    for (int k = 0; k < AR_SB; k++) {
+                        const int j = j0 + AR_NWAVES * k;
+                        if (j < NC) {
+                            S1[j * NR + i] = v1[k];
+                            S2[j * NR + i] = v2[k];
+                        }
+                    }
+                }
+            }
+        }
         int64_t st = 0, st2 = 0;
         {
             int64_t orow = g.cr0 - a.pady - 1, ocol = g.cc0 - a.padx - 1;
@@ -318,10 +343,10 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         P.cnorm = sqrt(sq);
         P.inv_cnorm_f = (float)(1.0 / P.cnorm);
 
-        // ---- integer box sums S1 (W) and S2 (W^2) over every shift: sliding sums, with each
-        // window column split into KC row chunks and each shift row into KR column chunks; W
-        // first, then W^2, through one buffer of column sums
-        {
+        // ---- integer box sums S1 (W) and S2 (W^2) over every shift (unless read from the box
+        // images above): sliding sums, with each window column split into KC row chunks and each
+        // shift row into KR column chunks; W first, then W^2, through one buffer of column sums
+        if (!a.box1) {
             constexpr int KC = 3, KR = 5;
             int32_t* cs = colS;
             const int lc = (NR + KC - 1) / KC, lr = (NC + KR - 1) / KR;

@@ -38,6 +38,12 @@ struct TrackArgs {
     // (-1: nothing to refine; the correlation kernel has written the result)
     float* rpatch;
     int32_t* rmeta;
+    // box images of W and W^2 for a pass with one chip size (u8 path), or nullptr: S1 / S2 of
+    // shift (i, j) of a point whose window starts at padded (wr0, wc0) are at
+    // (wr0 - 1 + i) + (wc0 - 1 + j) * box_ld
+    const int32_t* box1;
+    const int32_t* box2;
+    int64_t box_ld;
     // per-pass maxima the LDS layout is carved for
     int32_t max_ch, max_cw, max_rx, max_ry;
     size_t lds_bytes;
