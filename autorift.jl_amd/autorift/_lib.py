@@ -83,7 +83,8 @@ SIGNATURES = [
                                      C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_float)]),
     ("ar_last_kernel", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
-    ("ar_last_timing", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("ar_last_timing", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]),
     ("ar_last_error", C.c_char_p, [C.c_void_p]),
     ("ar_version", C.c_char_p, []),
 ]

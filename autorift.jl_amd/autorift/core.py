@@ -212,7 +212,7 @@ class Context:
         return ms.value, path.value
 
     def last_timing(self):
-        """(correlation kernel ms, refinement kernel ms) of the last tracking pass."""
-        t, r = C.c_double(), C.c_double()
-        lib().ar_last_timing(self._h, C.byref(t), C.byref(r))
-        return t.value, r.value
+        """(box images ms, correlation kernel ms, refinement kernel ms) of the last pass."""
+        b, t, r = C.c_double(), C.c_double(), C.c_double()
+        lib().ar_last_timing(self._h, C.byref(b), C.byref(t), C.byref(r))
+        return b.value, t.value, r.value

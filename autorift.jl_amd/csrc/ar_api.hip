@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -153,6 +154,16 @@ __global__ void skip_all_kernel(float* dx, float* dy, float* corr, uint8_t* s, i
     }
 }
 
+// AR_BOX_IMAGES=always|never overrides the choice between box images and per-point box sums
+// (both exact); unset or anything else: automatic.
+static int box_images_mode() {
+    const char* e = std::getenv("AR_BOX_IMAGES");
+    if (!e) return 0;
+    if (std::strcmp(e, "always") == 0) return 1;
+    if (std::strcmp(e, "never") == 0) return -1;
+    return 0;
+}
+
 static int grid_for(int64_t n, int per_block = 256) {
     int64_t b = (n + per_block - 1) / per_block;
     if (b < 1) b = 1;
@@ -224,7 +235,7 @@ ar_status ar_context_create(const ar_params* p, int32_t device, ar_context** out
     c->ncu = prop.multiProcessorCount;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreate(&c->evm) != hipSuccess) {
+        hipEventCreate(&c->evm) != hipSuccess || hipEventCreate(&c->evb) != hipSuccess) {
         delete c;
         return fail(nullptr, AR_ERR_DEVICE, "stream/event creation failed");
     }
@@ -240,6 +251,7 @@ void ar_context_destroy(ar_context* ctx) {
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
     if (ctx->evm) hipEventDestroy(ctx->evm);
+    if (ctx->evb) hipEventDestroy(ctx->evb);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -397,15 +409,21 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         a.lds_bytes = 0;
         kern = ctx->pdtype == AR_U8 ? track_generic_kernel<1> : track_generic_kernel<0>;
     }
-    // Box images of W and W^2 replace the per-point box sums when every point has the same chip
-    // and the points' shifts cover at least a quarter of the image's chip positions.
-    if (use_u8 && uniform && a.max_ch > 0 && a.max_cw > 0) {
+    bool box_timed = false;
+    const int bmode = box_images_mode();
+    // Box images of W and W^2 replace the per-point box sums when every point has the same chip,
+    // the points' shifts cover at least a quarter of the image's chip positions and there are at
+    // least 2^22 positions (below that the extra launch costs more than it saves).
+    if (bmode >= 0 && use_u8 && uniform && a.max_ch > 0 && a.max_cw > 0) {
         const double positions = (double)(a.ny + 2 * a.pady - a.max_ch + 1) *
                                  (double)(a.nx + 2 * a.padx - a.max_cw + 1);
         const double shifts = (double)n * (2.0 * a.max_rx) * (2.0 * a.max_ry);
-        if (positions >= 1 && 4.0 * shifts >= positions) {
+        if (positions >= 1 &&
+            (bmode == 1 || (positions >= 4194304.0 && 4.0 * shifts >= positions))) {
             int32_t *b1 = nullptr, *b2 = nullptr;
             int64_t ld = 0;
+            HIPCHK(ctx, hipEventRecord(ctx->evb, ctx->stream));
+            box_timed = true;
             st = ar_box_images(ctx, a.max_ch, a.max_cw, a.pady, a.padx, &b1, &b2, &ld);
             if (st == AR_ERR_OUT_OF_MEMORY) return fail(ctx, st, "out of device memory (box images)");
             if (st == AR_ERR_DEVICE) return fail(ctx, st, "box image kernel failed to launch");
@@ -459,7 +477,10 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     hipEventElapsedTime(&ms_t, ctx->ev0, ctx->evm);
     hipEventElapsedTime(&ms_r, ctx->evm, ctx->ev1);
+    float ms_b = 0;
+    if (box_timed) hipEventElapsedTime(&ms_b, ctx->evb, ctx->ev0);
     ctx->last_ms = ms;
+    ctx->last_ms_box = ms_b;
     ctx->last_ms_track = ms_t;
     ctx->last_ms_refine = up > 1 ? ms_r : 0.0;
     ctx->last_path = use_u8 ? 0 : 1;
@@ -543,9 +564,10 @@ extern "C" ar_status ar_last_kernel(const ar_context* ctx, double* ms, int32_t* 
     return AR_OK;
 }
 
-extern "C" ar_status ar_last_timing(const ar_context* ctx, double* track_ms,
+extern "C" ar_status ar_last_timing(const ar_context* ctx, double* box_ms, double* track_ms,
                                     double* refine_ms) {
     if (!ctx) return AR_ERR_INVALID_ARGUMENT;
+    if (box_ms) *box_ms = ctx->last_ms_box;
     if (track_ms) *track_ms = ctx->last_ms_track;
     if (refine_ms) *refine_ms = ctx->last_ms_refine;
     return AR_OK;

@@ -22,7 +22,7 @@ struct ar_context {
     int device = 0;
     int ncu = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr, evb = nullptr;
     std::string err;
 
     // prepared pair (what the tracker correlates), column-major ny x nx
@@ -42,7 +42,7 @@ struct ar_context {
     DevBuf refine;              // per-point refinement hand-off: 5x5 patches and origins
     DevBuf box;                 // box images of W and W^2 for a pass with one chip size
 
-    double last_ms = 0, last_ms_track = 0, last_ms_refine = 0;
+    double last_ms = 0, last_ms_box = 0, last_ms_track = 0, last_ms_refine = 0;
     int32_t last_path = -1;
 
     std::vector<DevBuf*> all_bufs() {

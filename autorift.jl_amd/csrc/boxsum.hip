@@ -6,9 +6,13 @@
 // B(wr0 - 1 + i, wc0 - 1 + j): the integer box sums of src/correlate.jl:574-600 (summed-area
 // tables, src/integral.jl:141-170) read instead of rebuilt per point, identical as integers.
 //
-// One workgroup per 32 x 64 tile of positions: the tile's input bytes go to LDS (coalesced
-// column reads), then vertical sliding sums of 8 rows per task, then horizontal sliding sums of
-// 8 columns per task, written column-major with leading dimension HB.
+// One wave per (band of 64 RPL padded rows, chunk of output columns); lane l holds rows
+// RPL l .. RPL l + RPL - 1 of the band. The wave walks the chunk's columns keeping each row's
+// horizontal running sums over cw columns in registers (loads of one column are coalesced
+// across lanes), then forms the vertical box of ch rows from an inclusive prefix over the band
+// (in-lane, then across lanes) as S(r + ch - 1) - S(r - 1), giving 64 RPL - ch + 1 output rows
+// per column, stored coalesced (column-major, leading dimension HB). Sums wrap modulo 2^32;
+// differences of box size are exact.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -18,85 +22,158 @@
 
 namespace ar {
 
-#define BX_TR 32
-#define BX_TC 64
+#define BX_CHUNK 128   // output columns per wave
+#define BX_G 8         // columns per group of loads
 
-__host__ __device__ inline int bx_rup(int x, int m) { return (x + m - 1) / m * m; }
+// rows per lane for chip height ch: the band of 64 RPL rows yields at least half as many outputs
+__host__ __device__ inline int box_rpl(int ch) { return ch <= 128 ? 4 : 8; }
 
-__host__ __device__ inline size_t box_lds_bytes(int ch, int cw) {
-    const int IR = BX_TR + ch - 1, IC = BX_TC + cw - 1;
-    return (size_t)bx_rup(IC * bx_rup(IR, 4), 16) + (size_t)BX_TR * IC * 8;
-}
-
+template <int RPL>
 __global__ __launch_bounds__(256) void box_u8_kernel(const uint8_t* img, int64_t ny, int64_t nx,
                                                     int64_t pady, int64_t padx, int ch, int cw,
                                                     int64_t HB, int64_t WB, int32_t* B1,
                                                     int32_t* B2) {
-    extern __shared__ __align__(16) unsigned char sm[];
-    const int IR = BX_TR + ch - 1, IC = BX_TC + cw - 1;
-    const int ldi = bx_rup(IR, 4);
-    uint8_t* in = sm;
-    int32_t* V1 = (int32_t*)(sm + bx_rup(IC * ldi, 16));   // V[c * BX_TR + r]
-    int32_t* V2 = V1 + BX_TR * IC;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t ntr = (HB + BX_TR - 1) / BX_TR, ntc = (WB + BX_TC - 1) / BX_TC;
-    for (int64_t tile = blockIdx.x; tile < ntr * ntc; tile += gridDim.x) {
-        const int64_t r0 = (tile % ntr) * BX_TR, c0 = (tile / ntr) * BX_TC;
-        // input bytes of the tile, zero outside the real image
-        for (int c = wave; c < IC; c += 4) {
-            const int64_t col = c0 + c - padx;
-            const bool colok = col >= 0 && col < nx;
-            for (int r = lane; r < IR; r += 64) {
-                const int64_t row = r0 + r - pady;
-                in[c * ldi + r] = (colok && row >= 0 && row < ny) ? img[row + col * ny] : 0;
+    __shared__ __align__(16) uint32_t sS[4][2][64 * RPL];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t* s1 = sS[wave][0];
+    uint32_t* s2 = sS[wave][1];
+    const int OUT = 64 * RPL - ch + 1;   // output rows per band
+    const int64_t nb = (HB + OUT - 1) / OUT, nchunks = (WB + BX_CHUNK - 1) / BX_CHUNK;
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    for (int64_t w = (int64_t)blockIdx.x * 4 + wave; w < nb * nchunks; w += nw) {
+        const int64_t band = w % nb, chunk = w / nb;
+        const int64_t R0 = band * OUT, C0 = chunk * BX_CHUNK;
+        const int64_t C1 = C0 + BX_CHUNK < WB ? C0 + BX_CHUNK : WB;
+        // image rows of this lane (padded rows R0 + RPL lane + k); the column walk is wave-uniform
+        const int64_t ir0 = R0 + (int64_t)RPL * lane - pady;
+        uint32_t rowok = 0;
+#pragma unroll
+        for (int k = 0; k < RPL; k++)
+            rowok |= (ir0 + k >= 0 && ir0 + k < ny) ? 1u << k : 0u;
+        const bool full = rowok == (1u << RPL) - 1;
+        // the lane's RPL bytes of padded column pc (zero outside the image), packed 4 per dword
+        auto load = [&](int64_t pc, uint32_t* w) {
+            const int64_t col = pc - padx;
+#pragma unroll
+            for (int q = 0; q < RPL / 4; q++) w[q] = 0;
+            if (col < 0 || col >= nx) return;
+            const uint8_t* base = img + col * ny + ir0;
+            if (full) {
+#pragma unroll
+                for (int q = 0; q < RPL / 4; q++) {
+                    const uintptr_t ad = (uintptr_t)(base + 4 * q);
+                    const uint32_t* al = (const uint32_t*)(ad & ~(uintptr_t)3);
+                    w[q] = __builtin_amdgcn_alignbyte(al[1], al[0], (uint32_t)(ad & 3));
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < RPL; k++)
+                    if ((rowok >> k) & 1) w[k >> 2] |= (uint32_t)base[k] << (8 * (k & 3));
             }
-        }
-        __syncthreads();
-        // vertical sums over ch rows: task (column c, rows 8k .. 8k + 7)
-        for (int t = tid; t < IC * (BX_TR / 8); t += 256) {
-            const int c = t / (BX_TR / 8), k = t - c * (BX_TR / 8);
-            const uint8_t* col = in + c * ldi + 8 * k;
-            int32_t a1 = 0, a2 = 0;
-            for (int u = 0; u < ch; u++) {
-                int32_t w = col[u];
-                a1 += w;
-                a2 += w * w;
-            }
-            int32_t* v1 = V1 + c * BX_TR + 8 * k;
-            int32_t* v2 = V2 + c * BX_TR + 8 * k;
-            for (int r = 0; r < 8; r++) {
-                v1[r] = a1;
-                v2[r] = a2;
-                if (r + 1 < 8) {
-                    int32_t wo = col[r], wi = col[r + ch];
-                    a1 += wi - wo;
-                    a2 += wi * wi - wo * wo;
+        };
+        auto byte = [](const uint32_t* w, int k) -> uint32_t { return (w[k >> 2] >> (8 * (k & 3))) & 0xffu; };
+        uint32_t h1[RPL], h2[RPL];
+#pragma unroll
+        for (int k = 0; k < RPL; k++) h1[k] = h2[k] = 0;
+        // horizontal sums of the first column, BX_G column loads in flight
+        for (int v0 = 0; v0 < cw; v0 += BX_G) {
+            uint32_t w[BX_G][RPL / 4];
+#pragma unroll
+            for (int g = 0; g < BX_G; g++) {
+                if (v0 + g < cw) {
+                    load(C0 + v0 + g, w[g]);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < RPL / 4; q++) w[g][q] = 0;
                 }
             }
-        }
-        __syncthreads();
-        // horizontal sums over cw columns: task (row r, columns 8k .. 8k + 7)
-        {
-            const int r = tid & (BX_TR - 1), k = tid / BX_TR;   // 256 tasks = 32 rows x 8
-            int32_t a1 = 0, a2 = 0;
-            for (int v = 0; v < cw; v++) {
-                a1 += V1[(8 * k + v) * BX_TR + r];
-                a2 += V2[(8 * k + v) * BX_TR + r];
-            }
-            const int64_t R = r0 + r;
-            for (int c = 0; c < 8; c++) {
-                const int64_t C = c0 + 8 * k + c;
-                if (R < HB && C < WB) {
-                    B1[R + C * HB] = a1;
-                    B2[R + C * HB] = a2;
+#pragma unroll
+            for (int g = 0; g < BX_G; g++)
+#pragma unroll
+                for (int k = 0; k < RPL; k++) {
+                    uint32_t x = byte(w[g], k);
+                    h1[k] += x;
+                    h2[k] += x * x;
                 }
-                if (c + 1 < 8) {
-                    a1 += V1[(8 * k + c + cw) * BX_TR + r] - V1[(8 * k + c) * BX_TR + r];
-                    a2 += V2[(8 * k + c + cw) * BX_TR + r] - V2[(8 * k + c) * BX_TR + r];
+        }
+        // columns leaving (c) and entering (c + cw) the horizontal window after output column c
+        auto load_group = [&](int64_t cg, uint32_t (*wo)[RPL / 4], uint32_t (*wi)[RPL / 4]) {
+#pragma unroll
+            for (int g = 0; g < BX_G; g++) {
+                if (cg + g < C1) {
+                    load(cg + g, wo[g]);
+                    load(cg + g + cw, wi[g]);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < RPL / 4; q++) wo[g][q] = wi[g][q] = 0;
                 }
             }
+        };
+        // output column c from the current horizontal sums, then slide them by one column
+        auto step = [&](int64_t c, const uint32_t* wo, const uint32_t* wi) {
+            // inclusive prefix over the band: in-lane, then an exclusive scan of lane totals
+            uint32_t p1[RPL], p2[RPL];
+            p1[0] = h1[0];
+            p2[0] = h2[0];
+#pragma unroll
+            for (int k = 1; k < RPL; k++) {
+                p1[k] = p1[k - 1] + h1[k];
+                p2[k] = p2[k - 1] + h2[k];
+            }
+            uint32_t t1 = p1[RPL - 1], t2 = p2[RPL - 1];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t u1 = __shfl_up(t1, o), u2 = __shfl_up(t2, o);
+                if (lane >= o) {
+                    t1 += u1;
+                    t2 += u2;
+                }
+            }
+            const uint32_t e1 = t1 - p1[RPL - 1], e2 = t2 - p2[RPL - 1];
+#pragma unroll
+            for (int q = 0; q < RPL / 4; q++) {
+                ((uint4*)s1)[(RPL / 4) * lane + q] =
+                    make_uint4(e1 + p1[4 * q], e1 + p1[4 * q + 1], e1 + p1[4 * q + 2], e1 + p1[4 * q + 3]);
+                ((uint4*)s2)[(RPL / 4) * lane + q] =
+                    make_uint4(e2 + p2[4 * q], e2 + p2[4 * q + 1], e2 + p2[4 * q + 2], e2 + p2[4 * q + 3]);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // output rows lane, lane + 64, ...: each store covers 64 consecutive rows
+#pragma unroll
+            for (int k = 0; k < RPL; k++) {
+                const int rb = lane + 64 * k;
+                const int64_t R = R0 + rb;
+                if (rb < OUT && R < HB) {
+                    const uint32_t lo1 = rb > 0 ? s1[rb - 1] : 0u, lo2 = rb > 0 ? s2[rb - 1] : 0u;
+                    B1[R + c * HB] = (int32_t)(s1[rb + ch - 1] - lo1);
+                    B2[R + c * HB] = (int32_t)(s2[rb + ch - 1] - lo2);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int k = 0; k < RPL; k++) {
+                const uint32_t xo = byte(wo, k), xi = byte(wi, k);
+                h1[k] += xi - xo;
+                h2[k] += xi * xi - xo * xo;
+            }
+        };
+        // two register sets: the loads of the next group are in flight while a group is used
+        uint32_t ao[BX_G][RPL / 4], ai[BX_G][RPL / 4], bo[BX_G][RPL / 4], bi[BX_G][RPL / 4];
+        load_group(C0, ao, ai);
+        for (int64_t cg = C0; cg < C1; cg += 2 * BX_G) {
+            load_group(cg + BX_G, bo, bi);
+#pragma unroll
+            for (int g = 0; g < BX_G; g++)
+                if (cg + g < C1) step(cg + g, ao[g], ai[g]);
+            load_group(cg + 2 * BX_G, ao, ai);
+#pragma unroll
+            for (int g = 0; g < BX_G; g++)
+                if (cg + BX_G + g < C1) step(cg + BX_G + g, bo[g], bi[g]);
         }
-        __syncthreads();
     }
 }
 
@@ -114,21 +191,23 @@ ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t p
     if (st) return st;
     int32_t* p1 = (int32_t*)ctx->box.ptr;
     int32_t* p2 = (int32_t*)((char*)ctx->box.ptr + each);
-    const size_t lds = box_lds_bytes(ch, cw);
-    if (hipFuncSetAttribute((const void*)box_u8_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds) != hipSuccess)
-        return AR_ERR_DEVICE;
+    const int rpl = box_rpl(ch);
+    const int OUT = 64 * rpl - ch + 1;
+    const int64_t waves = ((HB + OUT - 1) / OUT) * ((WB + BX_CHUNK - 1) / BX_CHUNK);
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, box_u8_kernel, 256, lds) !=
-            hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, rpl == 4 ? box_u8_kernel<4> : box_u8_kernel<8>, 256, 0) != hipSuccess ||
         per_cu < 1)
         return AR_ERR_DEVICE;
-    const int64_t tiles = ((HB + BX_TR - 1) / BX_TR) * ((WB + BX_TC - 1) / BX_TC);
     int64_t blocks = (int64_t)ctx->ncu * per_cu;
-    if (blocks > tiles) blocks = tiles;
-    hipLaunchKernelGGL(box_u8_kernel, dim3((unsigned)blocks), dim3(256), lds, ctx->stream,
-                       (const uint8_t*)ctx->pref.ptr, ctx->ny, ctx->nx, pady, padx, ch, cw, HB, WB,
-                       p1, p2);
+    if (blocks > (waves + 3) / 4) blocks = (waves + 3) / 4;
+    const uint8_t* img = (const uint8_t*)ctx->pref.ptr;
+    if (rpl == 4)
+        hipLaunchKernelGGL(box_u8_kernel<4>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                           img, ctx->ny, ctx->nx, pady, padx, ch, cw, HB, WB, p1, p2);
+    else
+        hipLaunchKernelGGL(box_u8_kernel<8>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                           img, ctx->ny, ctx->nx, pady, padx, ch, cw, HB, WB, p1, p2);
     if (hipGetLastError() != hipSuccess) return AR_ERR_DEVICE;
     *b1 = p1;
     *b2 = p2;

@@ -144,10 +144,12 @@ ar_status ar_correlate_point(ar_context* ctx, const void* search, const void* ch
  * took: 0 u8 matrix-core path, 1 generic. */
 ar_status ar_last_kernel(const ar_context* ctx, double* ms, int32_t* path);
 
-/* The same pass split by kernel: the correlation kernel (integer peak, exact 5x5 patch) and the
- * refinement kernel (pyrUp cascade; 0 when upsampling is 1). No counterpart in the reference,
- * whose track loop is not timed per stage. */
-ar_status ar_last_timing(const ar_context* ctx, double* track_ms, double* refine_ms);
+/* The same pass split by kernel, in milliseconds: the box images of W and W^2 (0 when the pass
+ * computes its box sums per point), the correlation kernel (integer peak, exact 5x5 patch) and
+ * the refinement kernel (pyrUp cascade; 0 when upsampling is 1). No counterpart in the
+ * reference, whose track loop is not timed per stage. */
+ar_status ar_last_timing(const ar_context* ctx, double* box_ms, double* track_ms,
+                         double* refine_ms);
 
 /* Message of the last failed call on ctx (or of the last failed ar_context_create when ctx is
  * NULL). Valid until the next call on the same context. */
