@@ -37,7 +37,7 @@ __host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx
     size_t colsum = (size_t)nr * winw * 4;   // column sums of W, then of W^2
     size_t accsurf = (size_t)nr * nc * 8;    // int32 accumulators + float approximations
     L.total = L.off_tmp + (colsum > accsurf ? colsum : accsurf);
-    size_t reach = (size_t)L.cols * L.ldw;   // furthest window byte the tile loop reads
+    size_t reach = (size_t)L.cols * L.ldw + 64;   // past the furthest window byte the tile loop reads
     if (reach > L.total) L.total = reach;
     return L;
 }

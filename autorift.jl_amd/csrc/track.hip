@@ -158,6 +158,49 @@ __device__ inline float approx_u8(const U8Point& P, int32_t acc, int32_t s1, int
     return (float)n_times_num(P, acc, s1) * P.inv_cnorm_f * __builtin_amdgcn_rsqf(d);
 }
 
+// Sum W*T of one batch of 4 row-shift tiles (ti0 .. ti0 + 3) of column-shift tile tj on the int8
+// matrix cores, for band height NRB 16-row blocks. K is walked by quads of chip columns: lane
+// group g16 carries chip column v = 4 m + g16 (columns cw, cw + 1 are zero), all groups the same
+// band block rb. The B fragment of (v, rb) -- 16 chip bytes from row 16 rb - l16 -- does not
+// depend on the tile, and the A fragment of window rows 16 rho .. 16 rho + 15 serves every
+// (tile t, rb) with t + rb = rho, so each m loads NRB B fragments and NRB + 3 A fragments for
+// 4 NRB MFMAs.
+template <int NRB>
+__device__ inline void mfma_batch(const int8_t* Wl, const int8_t* Tp, int ldw, int cpad, int cw,
+                                  int tj, int ti0, int l16, int g16, v4i (&acc)[4]) {
+    const int nmq = (cw + 3) >> 2;
+    int aoff = (16 * tj + l16 + g16) * ldw + 16 * ti0;
+    int boff = g16 * cpad + 16 - l16;
+    const uint32_t sb = (uint32_t)(boff & 3);   // the same for every m and rb
+    for (int m = 0; m < nmq; m++) {
+        v4i bf[NRB];
+#pragma unroll
+        for (int rb = 0; rb < NRB; rb++) {
+            const uint32_t* bp = (const uint32_t*)(Tp + ((boff + 16 * rb) & ~3));
+            uint32_t d0 = bp[0], d1 = bp[1], d2 = bp[2], d3 = bp[3], d4 = bp[4];
+            bf[rb] = (v4i){(int)__builtin_amdgcn_alignbyte(d1, d0, sb),
+                           (int)__builtin_amdgcn_alignbyte(d2, d1, sb),
+                           (int)__builtin_amdgcn_alignbyte(d3, d2, sb),
+                           (int)__builtin_amdgcn_alignbyte(d4, d3, sb)};
+        }
+        const int8_t* ap = Wl + aoff;
+#pragma unroll
+        for (int rho = 0; rho < NRB + 3; rho++) {
+            const v4i af = *(const v4i*)(ap + 16 * rho);
+#pragma unroll
+            for (int rb = 0; rb < NRB; rb++) {
+                const int t = rho - rb;
+                if (t >= 0 && t < 4)
+                    acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[rb], acc[t], 0, 0, 0);
+            }
+        }
+        aoff += 4 * ldw;
+        boff += 4 * cpad;
+    }
+}
+
+#define AR_NRB_MAX 9   // band heights with a dedicated MFMA loop (chips up to 130 rows)
+
 __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
@@ -388,60 +431,96 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         // ---- numerator on int8 MFMA; epilogue scores every shift in FP32
         int32_t* accL = colS;                          // int32 [NR*NC] (colS is dead now)
         float* apx = (float*)(colS + NR * NC);         // float [NR*NC]
-        const int ntiles = nti * ntj;
-        const int nq = cw * nrb;                       // 16-wide k-blocks
-        const int nm = (nq + 3) / 4;                   // MFMAs per tile
         const int g16 = lane >> 4, l16 = lane & 15;
-        // K is walked in 16-wide blocks q = 4m + g16 = (chip column v, band row block rb). Blocks
-        // past the last (nq) read the zero chip columns cw, cw + 1, so the loop has no branches.
-        // Each wave takes batches of 4 tiles (t = tb + 4k); a batch past the last tile repeats
-        // the last tile and its results are dropped.
-        for (int tb = wave; tb < ntiles; tb += 16) {
-            int tt[4], ti[4], tj[4], abase[4];
+        if (nrb <= AR_NRB_MAX) {
+            // batches of 4 row-shift tiles of one column-shift tile; tiles past nti in the last
+            // batch read finite LDS data and are dropped
+            const int nbt = (nti + 3) >> 2, nbatch = ntj * nbt;
+            for (int bt = wave; bt < nbatch; bt += AR_NWAVES) {
+                const int tj = bt / nbt, ti0 = 4 * (bt - tj * nbt);
+                v4i acc[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                tt[k] = tb + 4 * k;
-                int t = tt[k] < ntiles ? tt[k] : ntiles - 1;
-                ti[k] = t % nti;
-                tj[k] = t / nti;
-                abase[k] = (16 * tj[k] + l16) * L.ldw + 16 * ti[k];
+                for (int k = 0; k < 4; k++) acc[k] = (v4i){0, 0, 0, 0};
+                switch (nrb) {
+                    case 2: mfma_batch<2>(Wl, Tp, L.ldw, L.cpad, cw, tj, ti0, l16, g16, acc); break;
+                    case 3: mfma_batch<3>(Wl, Tp, L.ldw, L.cpad, cw, tj, ti0, l16, g16, acc); break;
+                    case 4: mfma_batch<4>(Wl, Tp, L.ldw, L.cpad, cw, tj, ti0, l16, g16, acc); break;
+                    case 5: mfma_batch<5>(Wl, Tp, L.ldw, L.cpad, cw, tj, ti0, l16, g16, acc); break;
+                    case 6: mfma_batch<6>(Wl, Tp, L.ldw, L.cpad, cw, tj, ti0, l16, g16, acc); break;
+                    case 7: mfma_batch<7>(Wl, Tp, L.ldw, L.cpad, cw, tj, ti0, l16, g16, acc); break;
+                    case 8: mfma_batch<8>(Wl, Tp, L.ldw, L.cpad, cw, tj, ti0, l16, g16, acc); break;
+                    default: mfma_batch<9>(Wl, Tp, L.ldw, L.cpad, cw, tj, ti0, l16, g16, acc); break;
+                }
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int i = 16 * (ti0 + t) + l16;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int j = 16 * tj + 4 * g16 + r;
+                        if (ti0 + t < nti && i < NR && j < NC) {
+                            const int o = j * NR + i;
+                            const int32_t cv = acc[t][r];
+                            accL[o] = cv;
+                            apx[o] = approx_u8(P, cv, S1[o], S2[o], a.measure);
+                        }
+                    }
+                }
             }
-            v4i acc[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) acc[k] = (v4i){0, 0, 0, 0};
-            int v = g16 / nrb, rb = g16 - v * nrb;
-            for (int m = 0; m < nm; m++) {
-                // B: 16 bytes of the zero-padded chip column v from row 16 rb - l16, realigned
-                // from five dwords
-                int off = v * L.cpad + 16 + 16 * rb - l16;
-                const uint32_t* bp = (const uint32_t*)(Tp + (off & ~3));
-                uint32_t d0 = bp[0], d1 = bp[1], d2 = bp[2], d3 = bp[3], d4 = bp[4];
-                uint32_t sb = (uint32_t)(off & 3);
-                v4i bf = (v4i){(int)__builtin_amdgcn_alignbyte(d1, d0, sb),
-                               (int)__builtin_amdgcn_alignbyte(d2, d1, sb),
-                               (int)__builtin_amdgcn_alignbyte(d3, d2, sb),
-                               (int)__builtin_amdgcn_alignbyte(d4, d3, sb)};
-                const int aoff = v * L.ldw + 16 * rb;
-                v4i af[4];
-#pragma unroll
-                for (int k = 0; k < 4; k++) af[k] = *(const v4i*)(Wl + abase[k] + aoff);
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    acc[k] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[k], bf, acc[k], 0, 0, 0);
-                rb += 4;
-                if (rb >= nrb) { rb -= nrb; v++; }
-                if (rb >= nrb) { rb -= nrb; v++; }
-            }
-            for (int k = 0; k < 4; k++) {
-                if (tt[k] >= ntiles) break;
-                int i = 16 * ti[k] + l16;
-                for (int r = 0; r < 4; r++) {
-                    int j = 16 * tj[k] + 4 * g16 + r;
-                    if (i < NR && j < NC) {
-                        int o = j * NR + i;
-                        int32_t cv = acc[k][r];
-                        accL[o] = cv;
-                        apx[o] = approx_u8(P, cv, S1[o], S2[o], a.measure);
+        } else {
+            const int ntiles = nti * ntj;
+            const int nq = cw * nrb;                       // 16-wide k-blocks
+            const int nm = (nq + 3) / 4;                   // MFMAs per tile
+            // Taller bands: K is walked in 16-wide blocks q = 4m + g16 = (chip column v, band row
+            // block rb). Blocks past the last (nq) read the zero chip columns cw, cw + 1, so the
+            // loop has no branches. Each wave takes batches of 4 tiles (t = tb + 4k); a batch past
+            // the last tile repeats the last tile and its results are dropped.
+            for (int tb = wave; tb < ntiles; tb += 16) {
+                int tt[4], ti[4], tj[4], abase[4];
+    #pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    tt[k] = tb + 4 * k;
+                    int t = tt[k] < ntiles ? tt[k] : ntiles - 1;
+                    ti[k] = t % nti;
+                    tj[k] = t / nti;
+                    abase[k] = (16 * tj[k] + l16) * L.ldw + 16 * ti[k];
+                }
+                v4i acc[4];
+    #pragma unroll
+                for (int k = 0; k < 4; k++) acc[k] = (v4i){0, 0, 0, 0};
+                int v = g16 / nrb, rb = g16 - v * nrb;
+                for (int m = 0; m < nm; m++) {
+                    // B: 16 bytes of the zero-padded chip column v from row 16 rb - l16, realigned
+                    // from five dwords
+                    int off = v * L.cpad + 16 + 16 * rb - l16;
+                    const uint32_t* bp = (const uint32_t*)(Tp + (off & ~3));
+                    uint32_t d0 = bp[0], d1 = bp[1], d2 = bp[2], d3 = bp[3], d4 = bp[4];
+                    uint32_t sb = (uint32_t)(off & 3);
+                    v4i bf = (v4i){(int)__builtin_amdgcn_alignbyte(d1, d0, sb),
+                                   (int)__builtin_amdgcn_alignbyte(d2, d1, sb),
+                                   (int)__builtin_amdgcn_alignbyte(d3, d2, sb),
+                                   (int)__builtin_amdgcn_alignbyte(d4, d3, sb)};
+                    const int aoff = v * L.ldw + 16 * rb;
+                    v4i af[4];
+    #pragma unroll
+                    for (int k = 0; k < 4; k++) af[k] = *(const v4i*)(Wl + abase[k] + aoff);
+    #pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        acc[k] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[k], bf, acc[k], 0, 0, 0);
+                    rb += 4;
+                    if (rb >= nrb) { rb -= nrb; v++; }
+                    if (rb >= nrb) { rb -= nrb; v++; }
+                }
+                for (int k = 0; k < 4; k++) {
+                    if (tt[k] >= ntiles) break;
+                    int i = 16 * ti[k] + l16;
+                    for (int r = 0; r < 4; r++) {
+                        int j = 16 * tj[k] + 4 * g16 + r;
+                        if (i < NR && j < NC) {
+                            int o = j * NR + i;
+                            int32_t cv = acc[k][r];
+                            accL[o] = cv;
+                            apx[o] = approx_u8(P, cv, S1[o], S2[o], a.measure);
+                        }
                     }
                 }
             }
