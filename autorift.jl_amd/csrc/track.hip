@@ -226,6 +226,28 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         const int nrb = (ch + 15 + 15) / 16;          // band rows r - i in [0, ch), i < 16
         const int rows_used = 16 * (nti + nrb);
 
+        // ---- S1 / S2 of every shift from the pass's box images (exact integers), loaded straight
+        // into LDS (global_load_lds: no registers; the barrier of the chip sums below waits for
+        // them). Wave w fills elements 64 (w + 4 k) + lane of the column-major arrays.
+        if (a.box1) {
+            const int64_t br = g.wr0 - 1, bc = g.wc0 - 1;
+            const int tot = NR * NC, dj = 256 / NR, di = 256 - dj * NR;
+            int j = (64 * wave + lane) / NR;
+            int i = 64 * wave + lane - j * NR;
+            for (int e0 = 64 * wave; e0 < tot; e0 += 256) {
+                if (e0 + lane < tot) {
+                    const int64_t q = br + i + (bc + j) * a.box_ld;
+                    __builtin_amdgcn_global_load_lds(a.box1 + q, S1 + e0, 4, 0, 0);
+                    __builtin_amdgcn_global_load_lds(a.box2 + q, S2 + e0, 4, 0, 0);
+                }
+                i += di;
+                j += dj;
+                if (i >= NR) {
+                    i -= NR;
+                    j++;
+                }
+            }
+        }
         // ---- stage window (int8, value ^ 0x80; virtual zero padding -> 0x80) and chip
         {
             // window origin in unpadded 0-based coordinates; interior windows skip the
@@ -283,31 +305,6 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
                         }
                         Wl[c * L.ldw + r] = v;
                     }
-            }
-        }
-        if (a.box1) {
-            // S1 / S2 of every shift from the pass's box images (exact integers); lanes take
-            // shift rows i, waves take shift columns j, AR_SB columns of loads in flight
-            const int64_t br = g.wr0 - 1, bc = g.wc0 - 1;
-            for (int i = lane; i < NR; i += 64) {
-                for (int j0 = wave; j0 < NC; j0 += AR_NWAVES * AR_SB) {
-                    int32_t v1[AR_SB], v2[AR_SB];
-#pragma unroll
-                    for (int k = 0; k < AR_SB; k++) {
-                        const int j = j0 + AR_NWAVES * k;
-                        const int64_t q = br + i + (bc + (j < NC ? j : NC - 1)) * a.box_ld;
-                        v1[k] = a.box1[q];
-                        v2[k] = a.box2[q];
-                    }
-#pragma unroll
-                    for (int k = 0; k < AR_SB; k++) {
-                        const int j = j0 + AR_NWAVES * k;
-                        if (j < NC) {
-                            S1[j * NR + i] = v1[k];
-                            S2[j * NR + i] = v2[k];
-                        }
-                    }
-                }
             }
         }
         int64_t st = 0, st2 = 0;
@@ -432,6 +429,7 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         int32_t* accL = colS;                          // int32 [NR*NC] (colS is dead now)
         float* apx = (float*)(colS + NR * NC);         // float [NR*NC]
         const int g16 = lane >> 4, l16 = lane & 15;
+        Best lb = best_empty();   // this lane's best screening score and its row-major index
         if (nrb <= AR_NRB_MAX) {
             // batches of 4 row-shift tiles of one column-shift tile; tiles past nti in the last
             // batch read finite LDS data and are dropped
@@ -460,8 +458,10 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
                         if (ti0 + t < nti && i < NR && j < NC) {
                             const int o = j * NR + i;
                             const int32_t cv = acc[t][r];
+                            const float sv = approx_u8(P, cv, S1[o], S2[o], a.measure);
                             accL[o] = cv;
-                            apx[o] = approx_u8(P, cv, S1[o], S2[o], a.measure);
+                            apx[o] = sv;
+                            if (better(sv, i * NC + j, lb)) { lb.v = sv; lb.idx = i * NC + j; }
                         }
                     }
                 }
@@ -518,8 +518,10 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
                         if (i < NR && j < NC) {
                             int o = j * NR + i;
                             int32_t cv = acc[k][r];
+                            const float sv = approx_u8(P, cv, S1[o], S2[o], a.measure);
                             accL[o] = cv;
-                            apx[o] = approx_u8(P, cv, S1[o], S2[o], a.measure);
+                            apx[o] = sv;
+                            if (better(sv, i * NC + j, lb)) { lb.v = sv; lb.idx = i * NC + j; }
                         }
                     }
                 }
@@ -528,33 +530,58 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         __syncthreads();
 
         // ---- integer peak: screen, then exact values for the contenders
-        Best b = best_empty();
-        for (int j = wave; j < NC; j += AR_NWAVES)
-            for (int i = lane; i < NR; i += 64) {
-                float v = apx[j * NR + i];
-                int idx = i * NC + j;
-                if (better(v, idx, b)) { b.v = v; b.idx = idx; }
+        // shifts this lane scored, in the order of the matrix-core path that produced them
+        auto lane_outputs = [&](auto&& fn) {
+            if (nrb <= AR_NRB_MAX) {
+                const int nbt = (nti + 3) >> 2, nbatch = ntj * nbt;
+                for (int bt = wave; bt < nbatch; bt += AR_NWAVES) {
+                    const int tj = bt / nbt, ti0 = 4 * (bt - tj * nbt);
+                    for (int t = 0; t < 4 && ti0 + t < nti; t++)
+                        for (int r = 0; r < 4; r++) {
+                            const int i = 16 * (ti0 + t) + l16, j = 16 * tj + 4 * g16 + r;
+                            if (i < NR && j < NC) fn(i, j);
+                        }
+                }
+            } else {
+                const int ntiles = nti * ntj;
+                for (int tb = wave; tb < ntiles; tb += 16)
+                    for (int k = 0; k < 4 && tb + 4 * k < ntiles; k++) {
+                        const int ti = (tb + 4 * k) % nti, tj = (tb + 4 * k) / nti;
+                        for (int r = 0; r < 4; r++) {
+                            const int i = 16 * ti + l16, j = 16 * tj + 4 * g16 + r;
+                            if (i < NR && j < NC) fn(i, j);
+                        }
+                    }
             }
-        b = block_best(b, red);
-        float thr = b.v - 2.0f * AR_SCREEN_EPS;
-        Best e2 = best_empty();
-        for (int j = wave; j < NC; j += AR_NWAVES)
-            for (int i = lane; i < NR; i += 64) {
-                int o = j * NR + i;
-                if (!(apx[o] >= thr)) continue;
-                float v = exact_u8(P, accL[o], S1[o], S2[o], a.measure);
-                int idx = i * NC + j;
-                if (better(v, idx, e2)) { e2.v = v; e2.idx = idx; }
-            }
-        e2 = block_best(e2, red);
-        int pi = e2.idx / NC + 1, pj = e2.idx % NC + 1;
-        float pval = e2.v;
+        };
+        const Best b = block_best(lb, red);
+        const float thr = b.v - 2.0f * AR_SCREEN_EPS;
+        // the patch around the screening peak, evaluated while the contenders are
         int i0 = 1, j0 = 1;
-        bool refine = a.up > 1 && patch_origin(pi, pj, NR, NC, i0, j0);
+        const bool srefine =
+            a.up > 1 && patch_origin(b.idx / NC + 1, b.idx % NC + 1, NR, NC, i0, j0);
         float pv = 0.0f;
-        if (refine && tid < 25) {
-            int ii = i0 - 1 + tid / 5, jj = j0 - 1 + tid % 5;
-            int o = jj * NR + ii;
+        if (srefine && tid < 25) {
+            const int o = (j0 - 1 + tid % 5) * NR + (i0 - 1 + tid / 5);
+            pv = exact_u8(P, accL[o], S1[o], S2[o], a.measure);
+        }
+        // contenders: only lanes whose own best reaches the threshold hold any
+        Best e2 = best_empty();
+        if (lb.v >= thr)
+            lane_outputs([&](int i, int j) {
+                const int o = j * NR + i;
+                if (apx[o] >= thr) {
+                    const float v = exact_u8(P, accL[o], S1[o], S2[o], a.measure);
+                    if (better(v, i * NC + j, e2)) { e2.v = v; e2.idx = i * NC + j; }
+                }
+            });
+        e2 = block_best(e2, red);
+        const int pi = e2.idx / NC + 1, pj = e2.idx % NC + 1;
+        const float pval = e2.v;
+        const int si0 = i0, sj0 = j0;
+        const bool refine = a.up > 1 && patch_origin(pi, pj, NR, NC, i0, j0);
+        if (refine && (i0 != si0 || j0 != sj0) && tid < 25) {
+            const int o = (j0 - 1 + tid % 5) * NR + (i0 - 1 + tid / 5);
             pv = exact_u8(P, accL[o], S1[o], S2[o], a.measure);
         }
         finish_point(a, p, g, pi, pj, pval, i0, j0, refine, pv);
