@@ -164,6 +164,17 @@ static int box_images_mode() {
     return 0;
 }
 
+// Four int8 copies of the UInt8 reference (value ^ 0x80), copy k shifted k bytes later
+// (copy_k[y] = ref[y - k] ^ 0x80, int8 zero outside), so any image offset starts a dword in one of
+// them.
+__global__ void refx_kernel(const uint8_t* src, uint8_t* dst, int64_t n, int64_t stride) {
+    int64_t y = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; y < stride; y += (int64_t)gridDim.x * blockDim.x)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            dst[k * stride + y] = (y >= k && y - k < n) ? (uint8_t)(src[y - k] ^ 0x80) : 0x80;
+}
+
 static int grid_for(int64_t n, int per_block = 256) {
     int64_t b = (n + per_block - 1) / per_block;
     if (b < 1) b = 1;
@@ -299,6 +310,7 @@ extern "C" ar_status ar_set_pair(ar_context* ctx, const void* ref, const void* s
         return st;
     HIPCHK(ctx, hipMemcpyAsync(ctx->pref.ptr, ref, N * es, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(ctx->psec.ptr, sec, N * es, hipMemcpyHostToDevice, ctx->stream));
+    ctx->ref_gen++;
     ctx->ny = ny;
     ctx->nx = nx;
     ctx->pdtype = dtype;
@@ -405,6 +417,18 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     void (*kern)(TrackArgs);
     if (use_u8) {
         kern = track_u8_kernel;
+        // the int8 copy of the reference the matrix cores read (made once per prepared image)
+        const int64_t N = ctx->ny * ctx->nx;
+        const int64_t stride = (N + 1024 + 255) / 256 * 256;   // room for a window column's tail
+        if (ctx->refx_gen != ctx->ref_gen) {
+            if ((st = ctx->refx.reserve(ctx, 4 * stride))) return st;
+            refx_kernel<<<grid_for(stride), 256, 0, ctx->stream>>>(
+                (const uint8_t*)ctx->pref.ptr, (uint8_t*)ctx->refx.ptr, N, stride);
+            HIPCHK(ctx, hipGetLastError());
+            ctx->refx_gen = ctx->ref_gen;
+        }
+        a.refx = (const uint8_t*)ctx->refx.ptr;
+        a.refx_stride = stride;
     } else {
         a.lds_bytes = 0;
         kern = ctx->pdtype == AR_U8 ? track_generic_kernel<1> : track_generic_kernel<0>;
@@ -618,5 +642,6 @@ extern "C" ar_status ar_correlate_point(ar_context* ctx, const void* search, con
     c->okmask.release();
     c->pref = sref; c->psec = ssec; c->rmask = srm; c->smask = ssm; c->okmask = sok;
     c->ny = sny; c->nx = snx; c->pdtype = sdt; c->all_valid = sall; c->has_pair = had;
+    c->ref_gen++;   // the restored reference is not the one the int8 copies were made from
     return st;
 }

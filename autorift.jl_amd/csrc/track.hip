@@ -255,44 +255,26 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
             int64_t orow = g.wr0 - a.pady - 1, ocol = g.wc0 - a.padx - 1;
             bool inside = orow >= 0 && ocol >= 0 && orow + winh <= a.ny && ocol + winw <= a.nx;
             if (inside) {
-                // Each lane moves 4 rows of one column as a dword: two aligned dword loads and a
-                // byte realignment (the column's start address has any alignment). The 8
-                // half-waves take columns hw, hw + 8, ...; a lane issues the loads of AR_SB
-                // columns before storing any. Rows past the window are tile padding (int8 0).
-                const uint8_t* gb = ref + orow + ocol * a.ny;
-                const int nr4 = rows_used >> 2;
-                const int hw = tid >> 5, l32 = lane & 31;
-                for (int r4 = l32; r4 < nr4; r4 += 32) {
-                    const int r = 4 * r4;
-                    const bool rowok = r < winh;
-                    const int keep = winh - r;   // rows of this dword inside the window
-                    const uint32_t kmask = keep < 4 ? (1u << (8 * keep)) - 1u : 0xffffffffu;
-                    for (int c0 = hw; c0 < winw; c0 += 8 * AR_SB) {
-                        uint32_t lo[AR_SB], hi[AR_SB], sh[AR_SB];
-#pragma unroll
-                        for (int k = 0; k < AR_SB; k++) {
-                            const int c = c0 + 8 * k;
-                            const uintptr_t ad = (uintptr_t)(gb + (int64_t)c * a.ny) + (uintptr_t)r;
-                            sh[k] = (uint32_t)(ad & 3);
-                            lo[k] = 0;
-                            hi[k] = 0;
-                            if (rowok && c < winw) {
-                                const uint32_t* al = (const uint32_t*)(ad & ~(uintptr_t)3);
-                                lo[k] = al[0];
-                                hi[k] = al[1];
-                            }
-                        }
-#pragma unroll
-                        for (int k = 0; k < AR_SB; k++) {
-                            const int c = c0 + 8 * k;
-                            if (c < winw) {
-                                uint32_t v = 0;
-                                if (rowok)
-                                    v = (__builtin_amdgcn_alignbyte(hi[k], lo[k], sh[k]) ^ 0x80808080u) &
-                                        kmask;
-                                *(uint32_t*)(Wl + c * L.ldw + r) = v;
-                            }
-                        }
+                // Straight into LDS (global_load_lds, a dword per lane): the window's columns
+                // are one contiguous LDS region of ldw-byte columns, filled in dwords
+                // d = 64 (wave + 4 k) + lane, column d / (ldw / 4). Each column reads from the
+                // int8 copy of the reference in which its first row starts a dword. Rows past
+                // the window bring image bytes that meet zero chip rows or dropped shifts.
+                const int cq = L.ldw >> 2, tot = winw * cq, dc = 256 / cq, dr = 256 - dc * cq;
+                int c = (64 * wave + lane) / cq;
+                int r4 = 64 * wave + lane - c * cq;
+                for (int d0 = 64 * wave; d0 < tot; d0 += 256) {
+                    if (d0 + lane < tot) {
+                        const int64_t off = orow + (ocol + c) * a.ny;
+                        const int sh = (int)((-off) & 3);
+                        const uint8_t* src = a.refx + sh * a.refx_stride + off + sh + 4 * r4;
+                        __builtin_amdgcn_global_load_lds(src, Wl + 4 * d0, 4, 0, 0);
+                    }
+                    r4 += dr;
+                    c += dc;
+                    if (r4 >= cq) {
+                        r4 -= cq;
+                        c++;
                     }
                 }
             } else {
