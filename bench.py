@@ -187,12 +187,13 @@ def main():
     for _ in range(args.warmup):
         ctx.track_device(fptr, npts, 64, 0, optr)
     torch.cuda.synchronize()
-    kms, tms, rms = [], [], []
+    kms, bms, tms, rms = [], [], [], []
     t1 = time.perf_counter()
     for _ in range(args.steps):
         ctx.track_device(fptr, npts, 64, 0, optr)
         kms.append(ctx.last_kernel()[0])
-        t, r = ctx.last_timing()
+        b, t, r = ctx.last_timing()
+        bms.append(b)
         tms.append(t)
         rms.append(r)
     torch.cuda.synchronize()
@@ -201,11 +202,12 @@ def main():
     searched = int(out[3].sum().item())
     value = searched / (ms_step / 1000)
     kavg, tavg, ravg = float(np.mean(kms)), float(np.mean(tms)), float(np.mean(rms))
+    bavg = float(np.mean(bms))
     # the matrix-core roofline belongs to the correlation kernel (HIP events on its stream)
     achieved = searched * FLOP_PER_POINT / (tavg / 1000) / 1e12
     dx = out[0].cpu().numpy()
     log(f"{npts} points, searched {searched}, {ms_step:.2f} ms/step, kernels {kavg:.2f} ms "
-        f"(track_u8_kernel {tavg:.2f}, refine_kernel {ravg:.2f}), "
+        f"(box_u8_kernel {bavg:.2f}, track_u8_kernel {tavg:.2f}, refine_kernel {ravg:.2f}), "
         f"measured {np.count_nonzero(~np.isnan(dx))}, path {ctx.last_kernel()[1]}")
 
     lat = None
@@ -253,7 +255,7 @@ def main():
         "config": {"workload": f"track pass, {n}x{n} UInt8 pair, {npts} grid points "
                                f"(spacing {args.spacing}), chip 32, r 25, PyramidRefine(64)",
                    "points": npts},
-        "kernels_ms": {"track_u8_kernel": tavg, "refine_kernel": ravg},
+        "kernels_ms": {"box_u8_kernel": bavg, "track_u8_kernel": tavg, "refine_kernel": ravg},
         "roofline": {"kernel": "track_u8_kernel", "bound": "mfma", "achieved": achieved,
                      "peak": INT8_PEAK_TOPS,
                      "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": traffic,
