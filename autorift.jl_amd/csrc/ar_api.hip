@@ -164,7 +164,7 @@ static int box_images_mode() {
     return 0;
 }
 
-// Four int8 copies of the UInt8 reference (value ^ 0x80), copy k shifted k bytes later
+// Four int8 copies of a UInt8 image (value ^ 0x80), copy k shifted k bytes later
 // (copy_k[y] = ref[y - k] ^ 0x80, int8 zero outside), so any image offset starts a dword in one of
 // them.
 __global__ void refx_kernel(const uint8_t* src, uint8_t* dst, int64_t n, int64_t stride) {
@@ -421,13 +421,17 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         const int64_t N = ctx->ny * ctx->nx;
         const int64_t stride = (N + 1024 + 255) / 256 * 256;   // room for a window column's tail
         if (ctx->refx_gen != ctx->ref_gen) {
-            if ((st = ctx->refx.reserve(ctx, 4 * stride))) return st;
+            if ((st = ctx->refx.reserve(ctx, 8 * stride))) return st;
             refx_kernel<<<grid_for(stride), 256, 0, ctx->stream>>>(
                 (const uint8_t*)ctx->pref.ptr, (uint8_t*)ctx->refx.ptr, N, stride);
+            HIPCHK(ctx, hipGetLastError());
+            refx_kernel<<<grid_for(stride), 256, 0, ctx->stream>>>(
+                (const uint8_t*)ctx->psec.ptr, (uint8_t*)ctx->refx.ptr + 4 * stride, N, stride);
             HIPCHK(ctx, hipGetLastError());
             ctx->refx_gen = ctx->ref_gen;
         }
         a.refx = (const uint8_t*)ctx->refx.ptr;
+        a.secx = (const uint8_t*)ctx->refx.ptr + 4 * stride;
         a.refx_stride = stride;
     } else {
         a.lds_bytes = 0;
@@ -642,6 +646,6 @@ extern "C" ar_status ar_correlate_point(ar_context* ctx, const void* search, con
     c->okmask.release();
     c->pref = sref; c->psec = ssec; c->rmask = srm; c->smask = ssm; c->okmask = sok;
     c->ny = sny; c->nx = snx; c->pdtype = sdt; c->all_valid = sall; c->has_pair = had;
-    c->ref_gen++;   // the restored reference is not the one the int8 copies were made from
+    c->ref_gen++;   // the restored pair is not the one the int8 copies were made from
     return st;
 }

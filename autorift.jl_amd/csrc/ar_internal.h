@@ -41,8 +41,8 @@ struct ar_context {
     DevBuf flags, geom, pts, out, scratch, grid_a, grid_b;
     DevBuf refine;              // per-point refinement hand-off: 5x5 patches and origins
     DevBuf box;                 // box images of W and W^2 for a pass with one chip size
-    DevBuf refx;                // UInt8 reference as int8 (value ^ 0x80), for the matrix cores
-    uint64_t ref_gen = 0, refx_gen = ~0ULL;   // versions of pref and of the refx made from it
+    DevBuf refx;                // int8 copies (value ^ 0x80) of the UInt8 pair, for the matrix cores
+    uint64_t ref_gen = 0, refx_gen = ~0ULL;   // versions of the pair and of the copies made from it
 
     double last_ms = 0, last_ms_box = 0, last_ms_track = 0, last_ms_refine = 0;
     int32_t last_path = -1;

@@ -7,7 +7,8 @@ namespace ar {
 
 struct U8Layout {
     int ldw, rows, cols;   // window: column pitch ldw bytes (int8, value ^ 0x80), column-major
-    int cpad;              // chip: max_cw + 2 columns of cpad bytes; row u at 16 + u
+    int cpad;              // chip: cols_t = max_cw + 2 columns of cpad bytes; row u at 16 + u
+    int cols_t;
     size_t off_w, off_t, off_s1, off_s2, off_tmp, total;
 };
 
@@ -27,6 +28,7 @@ __host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx
     L.ldw = ((L.rows / 16) & 1) ? L.rows : L.rows + 16;
     L.cols = 16 * ntj + max_cw + 2;   // columns the tile loop reads
     L.cpad = rup(16 * nrb + 32, 16);
+    L.cols_t = max_cw + 2;
     int nr = 2 * max_ry, nc = 2 * max_rx;
     int winw = max_cw + 2 * max_rx - 1;
     L.off_w = 0;

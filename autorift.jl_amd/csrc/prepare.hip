@@ -473,7 +473,7 @@ extern "C" ar_status ar_set_images(ar_context* ctx, const void* ref, const void*
         const uint8_t* m = slot == 0 ? ref_valid : sec_valid;
         if (!img) continue;
         void* dst = slot == 0 ? ctx->pref.ptr : ctx->psec.ptr;
-        if (slot == 0) ctx->ref_gen++;
+        ctx->ref_gen++;   // the int8 copies of the pair are stale
         uint8_t* dm = (uint8_t*)(slot == 0 ? ctx->rmask.ptr : ctx->smask.ptr);
         if (pdt == AR_U8 && p.preprocess == AR_PRE_NONE && p.quantize == AR_QUANT_NONE) {
             // an integer image passes through unchanged

@@ -110,8 +110,11 @@ __device__ inline bool patch_origin(int pi, int pj, int nr, int nc, int& i0, int
 
 // Upper bound on |approximate - exact| for the FP32 screening score (see u8 epilogue).
 #define AR_SCREEN_EPS 4e-6f
-// Columns whose global loads a lane issues before storing any (window and chip staging).
+// Columns whose global loads a lane issues before storing any (chip staging).
 #define AR_SB 4
+// s_waitcnt immediate for vmcnt(0) with expcnt and lgkmcnt left at their maxima (gfx9 encoding:
+// vmcnt [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
+#define AR_WAIT_VMCNT0 0x0F70
 
 struct U8Point {
     int64_t n;       // chip area (power of two)
@@ -215,6 +218,9 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
     const uint8_t* ref = (const uint8_t*)a.ref;
     const uint8_t* sec = (const uint8_t*)a.sec;
 
+    // chip band rows and trailing columns are zero; the fetched chip path writes only chip rows
+    for (int t = tid; t < (L.cols_t * L.cpad) >> 2; t += AR_WG) ((uint32_t*)Tp)[t] = 0;
+    __syncthreads();
     const PointRange pr = point_range(a.n);
     for (int64_t p = pr.first; p < pr.end; p += pr.step) {
         Geom g = point_geom(a, p);
@@ -293,7 +299,36 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         {
             int64_t orow = g.cr0 - a.pady - 1, ocol = g.cc0 - a.padx - 1;
             bool inside = orow >= 0 && ocol >= 0 && orow + ch <= a.ny && ocol + cw <= a.nx;
-            if (inside && (ch & 3) == 0) {
+            if (inside && (ch & 3) == 0 && a.box1) {
+                // One chip size in the pass, so the zero band rows and trailing columns written
+                // at kernel start stay valid: only the chip's rows are fetched, straight into LDS
+                // (a dword per lane from the int8 copy of the secondary in which the column
+                // starts a dword); wave w takes columns w, w + 4, ... and, once its loads have
+                // landed, sums x and x^2 over them.
+                const int64_t base = orow + ocol * a.ny;
+                const int cd = ch >> 2;
+                for (int v = wave; v < cw; v += AR_NWAVES) {
+                    const int64_t off = base + (int64_t)v * a.ny;
+                    const int sh = (int)((-off) & 3);
+                    for (int x0 = 0; x0 < cd; x0 += 64)
+                        if (x0 + lane < cd)
+                            __builtin_amdgcn_global_load_lds(
+                                a.secx + sh * a.refx_stride + off + sh + 4 * (x0 + lane),
+                                Tp + v * L.cpad + 16 + 4 * x0, 4, 0, 0);
+                }
+                __builtin_amdgcn_s_waitcnt(AR_WAIT_VMCNT0);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);   // keep the LDS reads below the wait
+                uint32_t acc1 = 0, acc2 = 0;
+                for (int v = wave; v < cw; v += AR_NWAVES)
+                    for (int x = lane; x < cd; x += 64) {
+                        const uint32_t raw =
+                            *(const uint32_t*)(Tp + v * L.cpad + 16 + 4 * x) ^ 0x80808080u;
+                        acc1 = __builtin_amdgcn_sad_u8(raw, 0u, acc1);
+                        acc2 = __builtin_amdgcn_udot4(raw, raw, acc2, false);
+                    }
+                st = acc1;
+                st2 = acc2;
+            } else if (inside && (ch & 3) == 0) {
                 // dword path: chip rows u = x - 16 for x in [16, 16 + ch), realigned as above;
                 // the zero band rows and the two trailing zero columns are written as zeros.
                 // Half-waves take columns hw, hw + 8, ...; loads of AR_SB columns go first.

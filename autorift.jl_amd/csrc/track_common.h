@@ -15,7 +15,8 @@ namespace ar {
 struct TrackArgs {
     const void* ref;          // prepared reference image, column-major ny x nx
     const uint8_t* refx;      // u8 path: 4 int8 copies of the reference (value ^ 0x80), copy k
-    int64_t refx_stride;      // shifted k bytes later, at refx + k * refx_stride
+    const uint8_t* secx;      // shifted k bytes later, at refx + k * refx_stride; secx likewise
+    int64_t refx_stride;      // for the secondary
     const void* sec;          // prepared secondary image
     const uint8_t* ok;        // valid(ref) & valid(sec); nullptr when every pixel is valid
     int64_t ny, nx;           // image size
