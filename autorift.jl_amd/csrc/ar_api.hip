@@ -132,6 +132,19 @@ __global__ void pass_geometry_kernel(ar_points pts, int64_t n, int64_t ny, int64
         int64_t area = (2 * hy) * (2 * hx);
         pow2 &= (area > 0 && (area & (area - 1)) == 0) ? 1 : 0;
     }
+    // reduce across the wave first: one atomic per wave and value instead of one per thread
+    for (int o = 32; o > 0; o >>= 1) {
+        cx = max(cx, (unsigned long long)__shfl_xor(cx, o));
+        cy = max(cy, (unsigned long long)__shfl_xor(cy, o));
+        rx = max(rx, (unsigned long long)__shfl_xor(rx, o));
+        ry = max(ry, (unsigned long long)__shfl_xor(ry, o));
+        px = max(px, (unsigned long long)__shfl_xor(px, o));
+        py = max(py, (unsigned long long)__shfl_xor(py, o));
+        ex = min(ex, (unsigned long long)__shfl_xor(ex, o));
+        ey = min(ey, (unsigned long long)__shfl_xor(ey, o));
+    }
+    const bool all_fit = __all(fits), all_pow2 = __all(pow2);
+    if ((threadIdx.x & 63) != 0) return;
     atomicMax(&g[0], cx);
     atomicMax(&g[1], cy);
     atomicMax(&g[2], rx);
@@ -140,8 +153,8 @@ __global__ void pass_geometry_kernel(ar_points pts, int64_t n, int64_t ny, int64
     atomicMax(&g[5], py);
     atomicMin(&g[6], ex);
     atomicMin(&g[7], ey);
-    if (!fits) atomicAnd(&flags[0], 0);
-    if (!pow2) atomicAnd(&flags[1], 0);
+    if (!all_fit) atomicAnd(&flags[0], 0);
+    if (!all_pow2) atomicAnd(&flags[1], 0);
 }
 
 __global__ void skip_all_kernel(float* dx, float* dy, float* corr, uint8_t* s, int64_t n) {

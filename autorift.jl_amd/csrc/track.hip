@@ -121,6 +121,10 @@ struct U8Point {
     int64_t sumT;    // sum of chip samples
     double cnorm;    // sqrt(sum (x - mean)^2), src/correlate.jl:440-452
     float inv_cnorm_f;
+    // screening constants: n and sumT as floating point, 128 sumT - 16384 n
+    double nd, sumTd;
+    float nf;
+    int32_t swt0;
 };
 
 // Every quantity below is an integer below 2^53, so the Float64 forms are exact:
@@ -152,13 +156,19 @@ __device__ inline float exact_u8(const U8Point& P, int32_t acc, int32_t s1, int3
     return den > 0 ? (float)(num / den) : 0.0f;
 }
 
-// FP32 screening score, relative error below ~5e-7: (n num) / (cnorm sqrt(n * n var)).
+// FP32 screening score, relative error below ~5e-7: (n num) / (cnorm sqrt(n * n var)), 0 for a
+// flat window. n num and n var are the exact integers of n_times_num / n_times_var (the fused
+// forms round nothing below 2^53); the flat-window case is a select, so the per-shift code has
+// no branches.
 __device__ inline float approx_u8(const U8Point& P, int32_t acc, int32_t s1, int32_t s2,
                                   int measure) {
-    double nv = n_times_var(P, s1, s2, measure);
-    if (nv == 0.0) return 0.0f;
-    float d = (float)nv * (float)P.n;
-    return (float)n_times_num(P, acc, s1) * P.inv_cnorm_f * __builtin_amdgcn_rsqf(d);
+    const double ds1 = (double)s1;
+    const double ns2 = P.nd * (double)s2;
+    const double nv = measure == 1 ? ns2 : fma(-ds1, ds1, ns2);
+    const int32_t swt = acc + 128 * s1 + P.swt0;
+    const double nn = fma(-P.sumTd, ds1, (double)swt * P.nd);
+    const float sc = (float)nn * P.inv_cnorm_f * __builtin_amdgcn_rsqf((float)nv * P.nf);
+    return nv != 0.0 ? sc : 0.0f;
 }
 
 // Sum W*T of one batch of 4 row-shift tiles (ti0 .. ti0 + 3) of column-shift tile tj on the int8
@@ -399,6 +409,10 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         if (!ok) { write_skip(a, p, true); __syncthreads(); continue; }
         P.cnorm = sqrt(sq);
         P.inv_cnorm_f = (float)(1.0 / P.cnorm);
+        P.nd = (double)P.n;
+        P.sumTd = (double)P.sumT;
+        P.nf = (float)P.n;
+        P.swt0 = 128 * (int32_t)P.sumT - 16384 * (int32_t)P.n;
 
         // ---- integer box sums S1 (W) and S2 (W^2) over every shift (unless read from the box
         // images above): sliding sums, with each window column split into KC row chunks and each
@@ -478,7 +492,11 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
                             const float sv = approx_u8(P, cv, S1[o], S2[o], a.measure);
                             accL[o] = cv;
                             apx[o] = sv;
-                            if (better(sv, i * NC + j, lb)) { lb.v = sv; lb.idx = i * NC + j; }
+                            // screening keeps the first maximum a lane meets: ties are settled by the
+                            // exact values below
+                            const bool up = sv > lb.v;
+                            lb.v = up ? sv : lb.v;
+                            lb.idx = up ? i * NC + j : lb.idx;
                         }
                     }
                 }
@@ -538,7 +556,11 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
                             const float sv = approx_u8(P, cv, S1[o], S2[o], a.measure);
                             accL[o] = cv;
                             apx[o] = sv;
-                            if (better(sv, i * NC + j, lb)) { lb.v = sv; lb.idx = i * NC + j; }
+                            // screening keeps the first maximum a lane meets: ties are settled by the
+                            // exact values below
+                            const bool up = sv > lb.v;
+                            lb.v = up ? sv : lb.v;
+                            lb.idx = up ? i * NC + j : lb.idx;
                         }
                     }
                 }
