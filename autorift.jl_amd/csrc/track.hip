@@ -456,11 +456,10 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
             }
         }
 
-        // ---- numerator on int8 MFMA; epilogue scores every shift in FP32
+        // ---- numerator on int8 MFMA, stored per shift (column-major o = j NR + i)
         int32_t* accL = colS;                          // int32 [NR*NC] (colS is dead now)
         float* apx = (float*)(colS + NR * NC);         // float [NR*NC]
         const int g16 = lane >> 4, l16 = lane & 15;
-        Best lb = best_empty();   // this lane's best screening score and its row-major index
         if (nrb <= AR_NRB_MAX) {
             // batches of 4 row-shift tiles of one column-shift tile; tiles past nti in the last
             // batch read finite LDS data and are dropped
@@ -486,18 +485,7 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         const int j = 16 * tj + 4 * g16 + r;
-                        if (ti0 + t < nti && i < NR && j < NC) {
-                            const int o = j * NR + i;
-                            const int32_t cv = acc[t][r];
-                            const float sv = approx_u8(P, cv, S1[o], S2[o], a.measure);
-                            accL[o] = cv;
-                            apx[o] = sv;
-                            // screening keeps the first maximum a lane meets: ties are settled by the
-                            // exact values below
-                            const bool up = sv > lb.v;
-                            lb.v = up ? sv : lb.v;
-                            lb.idx = up ? i * NC + j : lb.idx;
-                        }
+                        if (i < NR && j < NC) accL[j * NR + i] = acc[t][r];   // i < NR: tile kept
                     }
                 }
             }
@@ -552,15 +540,7 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
                         int j = 16 * tj[k] + 4 * g16 + r;
                         if (i < NR && j < NC) {
                             int o = j * NR + i;
-                            int32_t cv = acc[k][r];
-                            const float sv = approx_u8(P, cv, S1[o], S2[o], a.measure);
-                            accL[o] = cv;
-                            apx[o] = sv;
-                            // screening keeps the first maximum a lane meets: ties are settled by the
-                            // exact values below
-                            const bool up = sv > lb.v;
-                            lb.v = up ? sv : lb.v;
-                            lb.idx = up ? i * NC + j : lb.idx;
+                            accL[o] = acc[k][r];
                         }
                     }
                 }
@@ -568,37 +548,24 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         }
         __syncthreads();
 
-        // ---- integer peak: screen, then exact values for the contenders
-        // shifts this lane scored, in the order of the matrix-core path that produced them
-        auto lane_outputs = [&](auto&& fn) {
-            if (nrb <= AR_NRB_MAX) {
-                const int nbt = (nti + 3) >> 2, nbatch = ntj * nbt;
-                for (int bt = wave; bt < nbatch; bt += AR_NWAVES) {
-                    const int tj = bt / nbt, ti0 = 4 * (bt - tj * nbt);
-                    for (int t = 0; t < 4 && ti0 + t < nti; t++)
-                        for (int r = 0; r < 4; r++) {
-                            const int i = 16 * (ti0 + t) + l16, j = 16 * tj + 4 * g16 + r;
-                            if (i < NR && j < NC) fn(i, j);
-                        }
-                }
-            } else {
-                const int ntiles = nti * ntj;
-                for (int tb = wave; tb < ntiles; tb += 16)
-                    for (int k = 0; k < 4 && tb + 4 * k < ntiles; k++) {
-                        const int ti = (tb + 4 * k) % nti, tj = (tb + 4 * k) / nti;
-                        for (int r = 0; r < 4; r++) {
-                            const int i = 16 * ti + l16, j = 16 * tj + 4 * g16 + r;
-                            if (i < NR && j < NC) fn(i, j);
-                        }
-                    }
-            }
-        };
+        // ---- integer peak: screen every shift in FP32 (all lanes busy, the matrix-core tiles'
+        // padding left out), then exact values for the contenders. The screening best of a lane
+        // is the first maximum it meets, indexed by o: ties are settled by the exact values.
+        const int nsh = NR * NC;
+        Best lb = best_empty();
+        for (int o = tid; o < nsh; o += AR_WG) {
+            const float sv = approx_u8(P, accL[o], S1[o], S2[o], a.measure);
+            apx[o] = sv;
+            const bool up = sv > lb.v;
+            lb.v = up ? sv : lb.v;
+            lb.idx = up ? o : lb.idx;
+        }
         const Best b = block_best(lb, red);
         const float thr = b.v - 2.0f * AR_SCREEN_EPS;
         // the patch around the screening peak, evaluated while the contenders are
         int i0 = 1, j0 = 1;
-        const bool srefine =
-            a.up > 1 && patch_origin(b.idx / NC + 1, b.idx % NC + 1, NR, NC, i0, j0);
+        const int bo = b.idx < nsh ? b.idx : 0;
+        const bool srefine = a.up > 1 && patch_origin(bo % NR + 1, bo / NR + 1, NR, NC, i0, j0);
         float pv = 0.0f;
         if (srefine && tid < 25) {
             const int o = (j0 - 1 + tid % 5) * NR + (i0 - 1 + tid / 5);
@@ -607,13 +574,12 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         // contenders: only lanes whose own best reaches the threshold hold any
         Best e2 = best_empty();
         if (lb.v >= thr)
-            lane_outputs([&](int i, int j) {
-                const int o = j * NR + i;
+            for (int o = tid; o < nsh; o += AR_WG)
                 if (apx[o] >= thr) {
+                    const int j = o / NR, i = o - j * NR;
                     const float v = exact_u8(P, accL[o], S1[o], S2[o], a.measure);
                     if (better(v, i * NC + j, e2)) { e2.v = v; e2.idx = i * NC + j; }
                 }
-            });
         e2 = block_best(e2, red);
         const int pi = e2.idx / NC + 1, pj = e2.idx % NC + 1;
         const float pval = e2.v;
