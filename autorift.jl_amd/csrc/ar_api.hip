@@ -433,7 +433,12 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         // the int8 copy of the reference the matrix cores read (made once per prepared image)
         const int64_t N = ctx->ny * ctx->nx;
         const int64_t stride = (N + 1024 + 255) / 256 * 256;   // room for a window column's tail
-        if (ctx->refx_gen != ctx->ref_gen) {
+        // the kernel addresses a window or chip column of copy k at a 32-bit byte offset
+        // k (stride + 1) + column * ny + row from the window's origin in copy 0
+        const int64_t reach = 3 * (stride + 1) +
+                              (int64_t)(a.max_cw + 2 * a.max_rx + 2) * ctx->ny + 4096;
+        const bool copies = reach < ((int64_t)1 << 32);
+        if (copies && ctx->refx_gen != ctx->ref_gen) {
             if ((st = ctx->refx.reserve(ctx, 8 * stride))) return st;
             refx_kernel<<<grid_for(stride), 256, 0, ctx->stream>>>(
                 (const uint8_t*)ctx->pref.ptr, (uint8_t*)ctx->refx.ptr, N, stride);
@@ -443,9 +448,11 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
             HIPCHK(ctx, hipGetLastError());
             ctx->refx_gen = ctx->ref_gen;
         }
-        a.refx = (const uint8_t*)ctx->refx.ptr;
-        a.secx = (const uint8_t*)ctx->refx.ptr + 4 * stride;
-        a.refx_stride = stride;
+        if (copies) {
+            a.refx = (const uint8_t*)ctx->refx.ptr;
+            a.secx = (const uint8_t*)ctx->refx.ptr + 4 * stride;
+            a.refx_stride = stride;
+        }
     } else {
         a.lds_bytes = 0;
         kern = ctx->pdtype == AR_U8 ? track_generic_kernel<1> : track_generic_kernel<0>;
@@ -455,7 +462,11 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     // Box images of W and W^2 replace the per-point box sums when every point has the same chip,
     // the points' shifts cover at least a quarter of the image's chip positions and there are at
     // least 2^22 positions (below that the extra launch costs more than it saves).
-    if (bmode >= 0 && use_u8 && uniform && a.max_ch > 0 && a.max_cw > 0) {
+    // (The kernel reads a point's S1 / S2 at 32-bit byte offsets 4 (i + j ld) from its first
+    // shift, ld = ny + 2 pady - ch + 1.)
+    const int64_t box_reach = 4 * (2 * a.max_ry + 2 * a.max_rx * (ctx->ny + 2 * a.pady));
+    if (bmode >= 0 && use_u8 && uniform && a.max_ch > 0 && a.max_cw > 0 &&
+        box_reach < ((int64_t)1 << 32)) {
         const double positions = (double)(a.ny + 2 * a.pady - a.max_ch + 1) *
                                  (double)(a.nx + 2 * a.padx - a.max_cw + 1);
         const double shifts = (double)n * (2.0 * a.max_rx) * (2.0 * a.max_ry);

@@ -245,22 +245,29 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         // ---- S1 / S2 of every shift from the pass's box images (exact integers), loaded straight
         // into LDS (global_load_lds: no registers; the barrier of the chip sums below waits for
         // them). Wave w fills elements 64 (w + 4 k) + lane of the column-major arrays.
+        // Addresses are a per-point base (scalar) plus a 32-bit byte offset per lane, advanced
+        // incrementally; the dispatcher keeps every such offset below 2^32.
         if (a.box1) {
-            const int64_t br = g.wr0 - 1, bc = g.wc0 - 1;
+            const int64_t q0 = (g.wr0 - 1) + (g.wc0 - 1) * a.box_ld;
+            const char* b1 = uniform_ptr((const char*)(a.box1 + q0));
+            const char* b2 = uniform_ptr((const char*)(a.box2 + q0));
+            const uint32_t ld = (uint32_t)a.box_ld;
             const int tot = NR * NC, dj = 256 / NR, di = 256 - dj * NR;
-            int j = (64 * wave + lane) / NR;
+            const int j = (64 * wave + lane) / NR;
             int i = 64 * wave + lane - j * NR;
+            uint32_t off = 4u * ((uint32_t)i + (uint32_t)j * ld);
+            const uint32_t doff = 4u * ((uint32_t)di + (uint32_t)dj * ld);
+            const uint32_t wrap = 4u * (ld - (uint32_t)NR);   // i -= NR, j += 1
             for (int e0 = 64 * wave; e0 < tot; e0 += 256) {
                 if (e0 + lane < tot) {
-                    const int64_t q = br + i + (bc + j) * a.box_ld;
-                    __builtin_amdgcn_global_load_lds(a.box1 + q, S1 + e0, 4, 0, 0);
-                    __builtin_amdgcn_global_load_lds(a.box2 + q, S2 + e0, 4, 0, 0);
+                    __builtin_amdgcn_global_load_lds(b1 + off, S1 + e0, 4, 0, 0);
+                    __builtin_amdgcn_global_load_lds(b2 + off, S2 + e0, 4, 0, 0);
                 }
                 i += di;
-                j += dj;
+                off += doff;
                 if (i >= NR) {
                     i -= NR;
-                    j++;
+                    off += wrap;
                 }
             }
         }
@@ -270,27 +277,32 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
             // per-pixel padding test and keep several independent loads in flight
             int64_t orow = g.wr0 - a.pady - 1, ocol = g.wc0 - a.padx - 1;
             bool inside = orow >= 0 && ocol >= 0 && orow + winh <= a.ny && ocol + winw <= a.nx;
-            if (inside) {
+            if (inside && a.refx) {
                 // Straight into LDS (global_load_lds, a dword per lane): the window's columns
                 // are one contiguous LDS region of ldw-byte columns, filled in dwords
                 // d = 64 (wave + 4 k) + lane, column d / (ldw / 4). Each column reads from the
                 // int8 copy of the reference in which its first row starts a dword. Rows past
                 // the window bring image bytes that meet zero chip rows or dropped shifts.
+                // column c of copy sh starts at base + sh (stride + 1) + c ny
+                const int64_t off0 = orow + ocol * a.ny;
+                const uint8_t* base = uniform_ptr(a.refx + off0);
+                const uint32_t ny = (uint32_t)a.ny, st1 = (uint32_t)a.refx_stride + 1u;
                 const int cq = L.ldw >> 2, tot = winw * cq, dc = 256 / cq, dr = 256 - dc * cq;
-                int c = (64 * wave + lane) / cq;
+                const int c = (64 * wave + lane) / cq;
                 int r4 = 64 * wave + lane - c * cq;
+                uint32_t coff = (uint32_t)c * ny;
+                const uint32_t dco = (uint32_t)dc * ny, lo0 = (uint32_t)off0;
                 for (int d0 = 64 * wave; d0 < tot; d0 += 256) {
                     if (d0 + lane < tot) {
-                        const int64_t off = orow + (ocol + c) * a.ny;
-                        const int sh = (int)((-off) & 3);
-                        const uint8_t* src = a.refx + sh * a.refx_stride + off + sh + 4 * r4;
-                        __builtin_amdgcn_global_load_lds(src, Wl + 4 * d0, 4, 0, 0);
+                        const uint32_t sh = (0u - (lo0 + coff)) & 3u;
+                        __builtin_amdgcn_global_load_lds(base + (sh * st1 + coff + 4u * (uint32_t)r4),
+                                                         Wl + 4 * d0, 4, 0, 0);
                     }
                     r4 += dr;
-                    c += dc;
+                    coff += dco;
                     if (r4 >= cq) {
                         r4 -= cq;
-                        c++;
+                        coff += ny;
                     }
                 }
             } else {
@@ -309,21 +321,24 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         {
             int64_t orow = g.cr0 - a.pady - 1, ocol = g.cc0 - a.padx - 1;
             bool inside = orow >= 0 && ocol >= 0 && orow + ch <= a.ny && ocol + cw <= a.nx;
-            if (inside && (ch & 3) == 0 && a.box1) {
+            if (inside && (ch & 3) == 0 && a.box1 && a.secx) {
                 // One chip size in the pass, so the zero band rows and trailing columns written
                 // at kernel start stay valid: only the chip's rows are fetched, straight into LDS
                 // (a dword per lane from the int8 copy of the secondary in which the column
                 // starts a dword); wave w takes columns w, w + 4, ... and, once its loads have
                 // landed, sums x and x^2 over them.
-                const int64_t base = orow + ocol * a.ny;
+                const int64_t off0 = orow + ocol * a.ny;
+                const uint8_t* base = uniform_ptr(a.secx + off0);
+                const uint32_t ny = (uint32_t)a.ny, st1 = (uint32_t)a.refx_stride + 1u;
+                const uint32_t lo0 = (uint32_t)off0;
                 const int cd = ch >> 2;
                 for (int v = wave; v < cw; v += AR_NWAVES) {
-                    const int64_t off = base + (int64_t)v * a.ny;
-                    const int sh = (int)((-off) & 3);
+                    const uint32_t coff = (uint32_t)v * ny;
+                    const uint32_t sh = (0u - (lo0 + coff)) & 3u;
                     for (int x0 = 0; x0 < cd; x0 += 64)
                         if (x0 + lane < cd)
                             __builtin_amdgcn_global_load_lds(
-                                a.secx + sh * a.refx_stride + off + sh + 4 * (x0 + lane),
+                                base + (sh * st1 + coff + 4u * (uint32_t)(x0 + lane)),
                                 Tp + v * L.cpad + 16 + 4 * x0, 4, 0, 0);
                 }
                 __builtin_amdgcn_s_waitcnt(AR_WAIT_VMCNT0);

@@ -16,7 +16,8 @@ struct TrackArgs {
     const void* ref;          // prepared reference image, column-major ny x nx
     const uint8_t* refx;      // u8 path: 4 int8 copies of the reference (value ^ 0x80), copy k
     const uint8_t* secx;      // shifted k bytes later, at refx + k * refx_stride; secx likewise
-    int64_t refx_stride;      // for the secondary
+    int64_t refx_stride;      // for the secondary (both nullptr for images too large for the
+                              // kernel's 32-bit offsets)
     const void* sec;          // prepared secondary image
     const uint8_t* ok;        // valid(ref) & valid(sec); nullptr when every pixel is valid
     int64_t ny, nx;           // image size
@@ -147,6 +148,16 @@ __device__ inline bool any_valid(const TrackArgs& a, const Geom& g) {
             if (k >= 0 && a.ok[k]) found = 1;
         }
     return __syncthreads_or(found) != 0;
+}
+
+// A pointer every lane holds the same value of, moved to scalar registers (so that loads through
+// it can use the scalar-base + 32-bit offset addressing form).
+template <typename T>
+__device__ inline T* uniform_ptr(T* p) {
+    const uint64_t u = (uint64_t)(uintptr_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+    return (T*)(uintptr_t)(((uint64_t)hi << 32) | lo);
 }
 
 // ---------------------------------------------------------------------------------------------
