@@ -423,7 +423,8 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     bool use_u8 = ctx->pdtype == AR_U8 && pow2 && (int64_t)a.max_ch * a.max_cw <= 16384 &&
                   a.max_ch <= 256;
     if (use_u8) {
-        U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
+        // the layout with per-point box sums is the larger one
+        U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, false);
         if (L.total > kLdsLimit) use_u8 = false;
         else a.lds_bytes = L.total;
     }
@@ -486,6 +487,8 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
             }
         }
     }
+    if (use_u8 && a.box1)
+        a.lds_bytes = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, true).total;
     // refinement hand-off: 32 floats of patch and one packed origin per point
     if ((st = ctx->refine.reserve(ctx, (size_t)n * 132))) return st;
     a.rpatch = (float*)ctx->refine.ptr;

@@ -9,16 +9,32 @@ struct U8Layout {
     int ldw, rows, cols;   // window: column pitch ldw bytes (int8, value ^ 0x80), column-major
     int cpad;              // chip: cols_t = max_cw + 2 columns of cpad bytes; row u at 16 + u
     int cols_t;
-    size_t off_w, off_t, off_s1, off_s2, off_tmp, total;
+    bool acc_in_w;         // shift accumulators reuse the window's bytes once the MFMAs are done
+    int acc_cols_t;        // chip columns the accumulators may cover past the window (then their
+                           // zero rows are restored before the next chip is staged)
+    size_t off_w, off_t, off_s1, off_s2, off_acc, off_tmp, total;
 };
 
 __host__ __device__ inline int rup(int x, int m) { return (x + m - 1) / m * m; }
 
+// Matrix-core batches of 4 row-shift tiles per wave: with at most one per wave (and the templated
+// band loop), a wave holds its accumulators in registers until every wave is past the window.
+__host__ __device__ inline bool u8_one_batch(int max_ch, int max_rx, int max_ry) {
+    int nti = (2 * max_ry + 15) / 16, ntj = (2 * max_rx + 15) / 16;
+    int nrb = (max_ch + 15 + 15) / 16;
+    return nrb <= 9 && ((nti + 3) / 4) * ntj <= 4;
+}
+
 // LDS of the u8 tracker. The matrix-core loop reads whole 16x16 tiles of the window: rows up to
 // 16 (nti + nrb) and columns up to 16 ntj + max_cw + 1. Only the max_cw + 2 max_rx - 1 real
 // columns are allocated; reads past them land on the chip and the sums, finite int8 data that
-// meets a zero chip column or a dropped shift.
-__host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx, int max_ry) {
+// meets a zero chip column or a dropped shift. Chip fragments are 16-byte reads at rows
+// 16 rb - l16 + 16, rb < nrb, so a chip column needs 16 nrb + 16 bytes.
+// Regions: window, chip, S1, S2 (int32 per shift), then the per-shift int32 accumulators -- in
+// the window's bytes (running on into the chip's) when they fit there and a wave holds one batch
+// -- and, when the pass builds its box sums per point (box == false), the column sums of W / W^2.
+__host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx, int max_ry,
+                                             bool box) {
     U8Layout L;
     int nti = (2 * max_ry + 15) / 16, ntj = (2 * max_rx + 15) / 16;
     int nrb = (max_ch + 15 + 15) / 16;   // 16-row blocks of the band, ceil((ch + 15) / 16)
@@ -27,18 +43,28 @@ __host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx
     // distinct LDS banks
     L.ldw = ((L.rows / 16) & 1) ? L.rows : L.rows + 16;
     L.cols = 16 * ntj + max_cw + 2;   // columns the tile loop reads
-    L.cpad = rup(16 * nrb + 32, 16);
+    L.cpad = 16 * nrb + 16;
     L.cols_t = max_cw + 2;
     int nr = 2 * max_ry, nc = 2 * max_rx;
     int winw = max_cw + 2 * max_rx - 1;
+    size_t shifts = (size_t)nr * nc * 4;
     L.off_w = 0;
     L.off_t = (size_t)winw * L.ldw;
     L.off_s1 = rup((int)(L.off_t + (size_t)(max_cw + 2) * L.cpad), 16);
-    L.off_s2 = L.off_s1 + (size_t)nr * nc * 4;
-    L.off_tmp = L.off_s2 + (size_t)nr * nc * 4;
-    size_t colsum = (size_t)nr * winw * 4;   // column sums of W, then of W^2
-    size_t accsurf = (size_t)nr * nc * 8;    // int32 accumulators + float approximations
-    L.total = L.off_tmp + (colsum > accsurf ? colsum : accsurf);
+    L.off_s2 = L.off_s1 + shifts;
+    L.off_tmp = L.off_s2 + shifts;
+    L.acc_in_w = shifts <= L.off_s1 && u8_one_batch(max_ch, max_rx, max_ry);
+    L.acc_cols_t = 0;
+    if (L.acc_in_w && shifts > L.off_t) L.acc_cols_t = (int)((shifts - L.off_t + L.cpad - 1) / L.cpad);
+    size_t colsum = box ? 0 : (size_t)nr * winw * 4;   // column sums of W, then of W^2
+    size_t tmp = colsum;
+    if (L.acc_in_w) {
+        L.off_acc = L.off_w;
+    } else {
+        L.off_acc = L.off_tmp;                          // the column sums are dead by then
+        if (shifts > tmp) tmp = shifts;
+    }
+    L.total = L.off_tmp + tmp;
     size_t reach = (size_t)L.cols * L.ldw + 64;   // past the furthest window byte the tile loop reads
     if (reach > L.total) L.total = reach;
     return L;

@@ -33,16 +33,17 @@ __host__ __device__ inline int casc_levels(int up) {
 }
 
 // Floats of one wave's cascade region for upsampling `up` (power of two, 2..64): the patch, the
-// last materialised level Y, the previous level X and the vertical-pass tmp T (the tile scratch
-// aliases X and T once Y is complete).
+// last materialised level Y and the vertical-pass tmp T. The level before Y lives in Y's own
+// bytes (it is dead before the last horizontal pass writes Y), and the tile scratch reuses T once
+// Y is complete.
 __host__ __device__ inline int cascade_floats(int up) {
     int L = casc_levels(up);
     int LM = L < 3 ? L : 3;
     int nM = 5 << LM;
     int nP = 5 << (LM > 0 ? LM - 1 : 0);
-    int XT = nP * nP + nM * nP;
-    if (L > 3 && XT < AR_TILE_WS) XT = AR_TILE_WS;
-    return AR_CASC_HDR + nM * nM + XT;
+    int T = nM * nP;
+    if (L > 3 && T < AR_TILE_WS) T = AR_TILE_WS;
+    return AR_CASC_HDR + nM * nM + T;
 }
 
 struct CascadeOut {
@@ -82,10 +83,11 @@ __device__ inline void pair_src(int k, int n, int& km, int& kq) {
 
 // The passes below give each lane up to AR_U tasks at a time (t, t + 64, ...), issuing all
 // loads before any arithmetic.
-#define AR_U 4
+#define AR_U 2
 
 // One full pyrUp step by one wave: src (N x N, row-major) -> dst (2N x 2N), via tmp (2N x N),
-// vertical pass first. Returns the lane's argmax over the samples it wrote (row-major index).
+// vertical pass first; src may share dst's bytes (only the vertical pass reads it). Returns the
+// lane's argmax over the samples it wrote (row-major index).
 template <int N>
 __device__ inline Best wave_pyrup(const float* src, float* tmp, float* dst) {
     const int lane = threadIdx.x & 63;
@@ -334,24 +336,22 @@ __device__ inline CascadeOut cascade(float* reg, int up) {
     const int L = casc_levels(up);
     const int LM = L < 3 ? L : 3;
     const int nM = 5 << LM;
-    const int nP = 5 << (LM > 0 ? LM - 1 : 0);
     float* Y = reg + AR_CASC_HDR;          // nM x nM
-    float* X = Y + nM * nM;                // nP x nP
-    float* T = X + nP * nP;                // vertical-pass tmp (2n x n)
+    float* T = Y + nM * nM;                // vertical-pass tmp (2n x n), then the tile scratch
 
-    // Levels 1..LM, ping-ponging so that level LM lands in Y; the last step's argmax comes along.
-    const float* cur = patch;
+    // Levels 1..LM, each in Y's bytes: a step's vertical pass reads the level before into T, its
+    // horizontal pass writes the new level over it. The last step's argmax comes along.
     Best b = best_empty();
-    if (LM >= 1) { float* d = ((LM - 1) & 1) ? X : Y; b = wave_pyrup<5>(cur, T, d); cur = d; }
-    if (LM >= 2) { float* d = ((LM - 2) & 1) ? X : Y; b = wave_pyrup<10>(cur, T, d); cur = d; }
-    if (LM >= 3) { float* d = ((LM - 3) & 1) ? X : Y; b = wave_pyrup<20>(cur, T, d); cur = d; }
+    if (LM >= 1) b = wave_pyrup<5>(patch, T, Y);
+    if (LM >= 2) b = wave_pyrup<10>(Y, T, Y);
+    if (LM >= 3) b = wave_pyrup<20>(Y, T, Y);
     b = wave_best(b);
 
     int nf = 5 << LM;
     if (L >= 4) {
-        if (L == 4) b = cascade_tiles<4>(Y, X, b);
-        else if (L == 5) b = cascade_tiles<5>(Y, X, b);
-        else b = cascade_tiles<6>(Y, X, b);
+        if (L == 4) b = cascade_tiles<4>(Y, T, b);
+        else if (L == 5) b = cascade_tiles<5>(Y, T, b);
+        else b = cascade_tiles<6>(Y, T, b);
         nf = 5 << L;
     }
     CascadeOut out;

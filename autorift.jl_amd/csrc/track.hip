@@ -69,7 +69,8 @@ __device__ inline void finish_point(const TrackArgs& a, int64_t p, const Geom& g
 // The pyrUp cascade of every point the correlation kernel handed over (src/peak.jl:456-522),
 // with the offsets of src/peak.jl:515-521 and the prior. One wave per workgroup, one point at a
 // time; no workgroup barriers, so many points are in flight on each CU.
-__global__ __launch_bounds__(64) void refine_kernel(TrackArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void refine_kernel(
+    TrackArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     float* reg = (float*)smem;
     const int lane = threadIdx.x;
@@ -186,6 +187,7 @@ __device__ inline void mfma_batch(const int8_t* Wl, const int8_t* Tp, int ldw, i
     int boff = g16 * cpad + 16 - l16;
     const uint32_t sb = (uint32_t)(boff & 3);   // the same for every m and rb
     for (int m = 0; m < nmq; m++) {
+        // B fragments realigned from five dwords (a byte-misaligned ds_read_b128 costs more)
         v4i bf[NRB];
 #pragma unroll
         for (int rb = 0; rb < NRB; rb++) {
@@ -214,9 +216,11 @@ __device__ inline void mfma_batch(const int8_t* Wl, const int8_t* Tp, int ldw, i
 
 #define AR_NRB_MAX 9   // band heights with a dedicated MFMA loop (chips up to 130 rows)
 
-__global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
+// Four workgroups per CU (128 VGPRs) when the LDS layout allows it.
+__global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void track_u8_kernel(
+    TrackArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
+    const U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, a.box1 != nullptr);
     __shared__ Best red[AR_NWAVES];
     __shared__ int64_t red64[2 * AR_NWAVES];
     int8_t* Wl = (int8_t*)(smem + L.off_w);
@@ -224,7 +228,8 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
     int32_t* S1 = (int32_t*)(smem + L.off_s1);
     int32_t* S2 = (int32_t*)(smem + L.off_s2);
     int32_t* colS = (int32_t*)(smem + L.off_tmp);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the wave index in a scalar register: loops over a wave's share are scalar loops
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint8_t* ref = (const uint8_t*)a.ref;
     const uint8_t* sec = (const uint8_t*)a.sec;
 
@@ -318,6 +323,17 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
             }
         }
         int64_t st = 0, st2 = 0;
+        if (L.acc_cols_t > 0 && (ch & 3) == 0) {
+            // The last point's accumulators ran over the first chip columns: their band rows
+            // (and whole trailing columns) are zero again before the chip is staged. The paths
+            // that stage whole columns write the same zeros; the fetched path writes chip rows
+            // only.
+            const int nd = L.cpad >> 2, c0 = ch >> 2;   // dwords per column; rows 16 + u: u < ch
+            for (int t = tid; t < L.acc_cols_t * nd; t += AR_WG) {
+                const int v = t / nd, d = t - v * nd;
+                if (v >= cw || d < 4 || d >= 4 + c0) ((uint32_t*)Tp)[t] = 0u;
+            }
+        }
         {
             int64_t orow = g.cr0 - a.pady - 1, ocol = g.cc0 - a.padx - 1;
             bool inside = orow >= 0 && ocol >= 0 && orow + ch <= a.ny && ocol + cw <= a.nx;
@@ -472,16 +488,32 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         }
 
         // ---- numerator on int8 MFMA, stored per shift (column-major o = j NR + i)
-        int32_t* accL = colS;                          // int32 [NR*NC] (colS is dead now)
-        float* apx = (float*)(colS + NR * NC);         // float [NR*NC]
+        int32_t* accL = (int32_t*)(smem + L.off_acc);   // int32 [NR*NC]
         const int g16 = lane >> 4, l16 = lane & 15;
+        // the shifts of one tile batch, from a wave's accumulators
+        auto store_batch = [&](const v4i (&acc)[4], int tj, int ti0) {
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int i = 16 * (ti0 + t) + l16;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int j = 16 * tj + 4 * g16 + r;
+                    if (i < NR && j < NC) accL[j * NR + i] = acc[t][r];   // i < NR: tile kept
+                }
+            }
+        };
         if (nrb <= AR_NRB_MAX) {
             // batches of 4 row-shift tiles of one column-shift tile; tiles past nti in the last
-            // batch read finite LDS data and are dropped
+            // batch read finite LDS data and are dropped. With the accumulators in the window's
+            // bytes, a wave has at most one batch and stores it once every wave is past the
+            // window.
             const int nbt = (nti + 3) >> 2, nbatch = ntj * nbt;
+            v4i acc[4];
+            int btj = 0, bti0 = 0;
             for (int bt = wave; bt < nbatch; bt += AR_NWAVES) {
                 const int tj = bt / nbt, ti0 = 4 * (bt - tj * nbt);
-                v4i acc[4];
+                btj = tj;
+                bti0 = ti0;
 #pragma unroll
                 for (int k = 0; k < 4; k++) acc[k] = (v4i){0, 0, 0, 0};
                 switch (nrb) {
@@ -494,15 +526,11 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
                     case 8: mfma_batch<8>(Wl, Tp, L.ldw, L.cpad, cw, tj, ti0, l16, g16, acc); break;
                     default: mfma_batch<9>(Wl, Tp, L.ldw, L.cpad, cw, tj, ti0, l16, g16, acc); break;
                 }
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const int i = 16 * (ti0 + t) + l16;
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int j = 16 * tj + 4 * g16 + r;
-                        if (i < NR && j < NC) accL[j * NR + i] = acc[t][r];   // i < NR: tile kept
-                    }
-                }
+                if (!L.acc_in_w) store_batch(acc, tj, ti0);
+            }
+            if (L.acc_in_w) {
+                __syncthreads();
+                if (wave < nbatch) store_batch(acc, btj, bti0);
             }
         } else {
             const int ntiles = nti * ntj;
@@ -570,7 +598,6 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
         Best lb = best_empty();
         for (int o = tid; o < nsh; o += AR_WG) {
             const float sv = approx_u8(P, accL[o], S1[o], S2[o], a.measure);
-            apx[o] = sv;
             const bool up = sv > lb.v;
             lb.v = up ? sv : lb.v;
             lb.idx = up ? o : lb.idx;
@@ -586,11 +613,12 @@ __global__ __launch_bounds__(AR_WG) void track_u8_kernel(TrackArgs a) {
             const int o = (j0 - 1 + tid % 5) * NR + (i0 - 1 + tid / 5);
             pv = exact_u8(P, accL[o], S1[o], S2[o], a.measure);
         }
-        // contenders: only lanes whose own best reaches the threshold hold any
+        // contenders: only lanes whose own best reaches the threshold hold any; they score their
+        // shifts again (the same function of the same integers)
         Best e2 = best_empty();
         if (lb.v >= thr)
             for (int o = tid; o < nsh; o += AR_WG)
-                if (apx[o] >= thr) {
+                if (approx_u8(P, accL[o], S1[o], S2[o], a.measure) >= thr) {
                     const int j = o / NR, i = o - j * NR;
                     const float v = exact_u8(P, accL[o], S1[o], S2[o], a.measure);
                     if (better(v, i * NC + j, e2)) { e2.v = v; e2.idx = i * NC + j; }
