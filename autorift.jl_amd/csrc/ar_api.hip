@@ -102,59 +102,61 @@ __global__ void fill_u8_kernel(uint8_t* out, uint8_t v, int64_t n) {
     for (; i < n; i += (int64_t)gridDim.x * blockDim.x) out[i] = v;
 }
 
-// _pass_geometry (src/track.jl:176-197) plus the facts the dispatcher needs: every searchable
-// chip area a power of two. g[0..5] = cx, cy, rx, ry, px, py (max); g[6] = fits; g[7] = pow2.
-__global__ void pass_geometry_kernel(ar_points pts, int64_t n, int64_t ny, int64_t nx,
-                                     unsigned long long* g, int* flags) {
+// _pass_geometry (src/track.jl:176-197) plus the facts the dispatcher needs. Ten results, each
+// in its own 256-byte line of g (slot k at g[AR_GEOM_SLOT * k]) so that the blocks' atomics on
+// different results do not queue behind each other: 0..5 = cx, cy, rx, ry, px, py (max), 6..7 =
+// the smallest even chip extents (min), 8 = every window fits, 9 = every searchable chip area a
+// power of two (and). Each block reduces its points first and issues one atomic per result.
+#define AR_GEOM_SLOT 32
+__global__ __launch_bounds__(256) void pass_geometry_kernel(ar_points pts, int64_t n, int64_t ny,
+                                                            int64_t nx, unsigned long long* g) {
+    __shared__ unsigned long long red[4][10];
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long cx = 0, cy = 0, rx = 0, ry = 0, px = 0, py = 0;
-    unsigned long long ex = ~0ULL, ey = ~0ULL;   // smallest even chip extents
-    int fits = 1, pow2 = 1;
+    unsigned long long v[10] = {0, 0, 0, 0, 0, 0, ~0ULL, ~0ULL, 1, 1};
     for (; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         int64_t prx = pts.radius_x[i], pry = pts.radius_y[i];
         if (!(prx > 0 && pry > 0)) continue;
         int64_t csx = pts.chip_size_x[i], csy = pts.chip_size_y[i];
-        cx = max(cx, (unsigned long long)(csx < 0 ? 0 : csx));
-        cy = max(cy, (unsigned long long)(csy < 0 ? 0 : csy));
-        ex = min(ex, (unsigned long long)(csx < 0 ? 0 : 2 * (csx / 2)));
-        ey = min(ey, (unsigned long long)(csy < 0 ? 0 : 2 * (csy / 2)));
-        rx = max(rx, (unsigned long long)prx);
-        ry = max(ry, (unsigned long long)pry);
+        v[0] = max(v[0], (unsigned long long)(csx < 0 ? 0 : csx));
+        v[1] = max(v[1], (unsigned long long)(csy < 0 ? 0 : csy));
+        v[2] = max(v[2], (unsigned long long)prx);
+        v[3] = max(v[3], (unsigned long long)pry);
         int64_t qx = csx / 2 + prx + (int64_t)ceil(fabs(pts.dx_prior[i]));
         int64_t qy = csy / 2 + pry + (int64_t)ceil(fabs(pts.dy_prior[i]));
-        px = max(px, (unsigned long long)(qx < 0 ? 0 : qx));
-        py = max(py, (unsigned long long)(qy < 0 ? 0 : qy));
+        v[4] = max(v[4], (unsigned long long)(qx < 0 ? 0 : qx));
+        v[5] = max(v[5], (unsigned long long)(qy < 0 ? 0 : qy));
+        v[6] = min(v[6], (unsigned long long)(csx < 0 ? 0 : 2 * (csx / 2)));
+        v[7] = min(v[7], (unsigned long long)(csy < 0 ? 0 : 2 * (csy / 2)));
         int64_t hx = csx / 2, hy = csy / 2;
         int64_t fx = (int64_t)floor(pts.x[i]), fy = (int64_t)floor(pts.y[i]);
         int64_t r0 = fy - hy - pry, r1 = fy + hy + pry - 2;
         int64_t c0 = fx - hx - prx, c1 = fx + hx + prx - 2;
-        fits &= (r0 >= 1 && r1 <= ny && c0 >= 1 && c1 <= nx) ? 1 : 0;
+        if (!(r0 >= 1 && r1 <= ny && c0 >= 1 && c1 <= nx)) v[8] = 0;
         int64_t area = (2 * hy) * (2 * hx);
-        pow2 &= (area > 0 && (area & (area - 1)) == 0) ? 1 : 0;
+        if (!(area > 0 && (area & (area - 1)) == 0)) v[9] = 0;
     }
-    // reduce across the wave first: one atomic per wave and value instead of one per thread
     for (int o = 32; o > 0; o >>= 1) {
-        cx = max(cx, (unsigned long long)__shfl_xor(cx, o));
-        cy = max(cy, (unsigned long long)__shfl_xor(cy, o));
-        rx = max(rx, (unsigned long long)__shfl_xor(rx, o));
-        ry = max(ry, (unsigned long long)__shfl_xor(ry, o));
-        px = max(px, (unsigned long long)__shfl_xor(px, o));
-        py = max(py, (unsigned long long)__shfl_xor(py, o));
-        ex = min(ex, (unsigned long long)__shfl_xor(ex, o));
-        ey = min(ey, (unsigned long long)__shfl_xor(ey, o));
+#pragma unroll
+        for (int k = 0; k < 10; k++) {
+            const unsigned long long u = __shfl_xor(v[k], o);
+            v[k] = k < 6 ? max(v[k], u) : (k < 8 ? min(v[k], u) : (v[k] & u));
+        }
     }
-    const bool all_fit = __all(fits), all_pow2 = __all(pow2);
-    if ((threadIdx.x & 63) != 0) return;
-    atomicMax(&g[0], cx);
-    atomicMax(&g[1], cy);
-    atomicMax(&g[2], rx);
-    atomicMax(&g[3], ry);
-    atomicMax(&g[4], px);
-    atomicMax(&g[5], py);
-    atomicMin(&g[6], ex);
-    atomicMin(&g[7], ey);
-    if (!all_fit) atomicAnd(&flags[0], 0);
-    if (!all_pow2) atomicAnd(&flags[1], 0);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 10; k++) red[wave][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 10) {
+        const int k = threadIdx.x;
+        unsigned long long r = red[0][k];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++)
+            r = k < 6 ? max(r, red[w][k]) : (k < 8 ? min(r, red[w][k]) : (r & red[w][k]));
+        unsigned long long* slot = g + AR_GEOM_SLOT * k;
+        if (k < 6) atomicMax(slot, r);
+        else if (k < 8) atomicMin(slot, r);
+        else if (r == 0) atomicAnd(slot, 0ULL);
+    }
 }
 
 __global__ void skip_all_kernel(float* dx, float* dy, float* corr, uint8_t* s, int64_t n) {
@@ -369,24 +371,22 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         return fail(ctx, AR_ERR_INVALID_ARGUMENT, "measure must be ZNCC or NCC");
     if (n == 0) return AR_OK;
     ar_status st;
-    if ((st = ctx->geom.reserve(ctx, 128))) return st;
-    unsigned long long g0[8] = {0, 0, 0, 0, 0, 0, ~0ULL, ~0ULL};
-    int f0[2] = {1, 1};
-    HIPCHK(ctx, hipMemcpyAsync(ctx->geom.ptr, g0, sizeof(g0), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync((char*)ctx->geom.ptr + 64, f0, sizeof(f0), hipMemcpyHostToDevice, ctx->stream));
-    pass_geometry_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(
-        dp, n, ctx->ny, ctx->nx, (unsigned long long*)ctx->geom.ptr, (int*)((char*)ctx->geom.ptr + 64));
+    unsigned long long gs[10 * AR_GEOM_SLOT] = {0};
+    if ((st = ctx->geom.reserve(ctx, sizeof(gs)))) return st;
+    for (int k = 6; k < 10; k++) gs[AR_GEOM_SLOT * k] = k < 8 ? ~0ULL : 1ULL;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->geom.ptr, gs, sizeof(gs), hipMemcpyHostToDevice, ctx->stream));
+    pass_geometry_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, ctx->ny, ctx->nx,
+                                                               (unsigned long long*)ctx->geom.ptr);
     HIPCHK(ctx, hipGetLastError());
-    unsigned long long g[8];
-    int f[2];
-    HIPCHK(ctx, hipMemcpyAsync(g, ctx->geom.ptr, sizeof(g), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(f, (char*)ctx->geom.ptr + 64, sizeof(f), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(gs, ctx->geom.ptr, sizeof(gs), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    unsigned long long g[10];
+    for (int k = 0; k < 10; k++) g[k] = gs[AR_GEOM_SLOT * k];
     int64_t cx = g[0], cy = g[1], rx = g[2], ry = g[3], px = g[4], py = g[5];
     // one even chip extent for every searchable point
     const bool uniform = g[6] == (unsigned long long)(2 * (cx / 2)) &&
                          g[7] == (unsigned long long)(2 * (cy / 2));
-    bool fits = f[0] != 0, pow2 = f[1] != 0;
+    bool fits = g[8] != 0, pow2 = g[9] != 0;
     if (cx == 0) {
         skip_all_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dx, dy, corr, srch, n);
         HIPCHK(ctx, hipGetLastError());

@@ -16,6 +16,8 @@
 //    arithmetic in its own order -- sequential chip mean and energy (src/correlate.jl:423-453),
 //    the summed-area recurrence of src/integral.jl:141-170 and the direct numerator of
 //    src/correlate.jl:799-814 -- with per-block scratch in global memory.
+#include <type_traits>
+
 #include "layout.h"
 #include "refine.h"
 #include "track_common.h"
@@ -121,11 +123,10 @@ struct U8Point {
     int64_t n;       // chip area (power of two)
     int64_t sumT;    // sum of chip samples
     double cnorm;    // sqrt(sum (x - mean)^2), src/correlate.jl:440-452
-    float inv_cnorm_f;
-    // screening constants: n and sumT as floating point, 128 sumT - 16384 n
-    double nd, sumTd;
-    float nf;
-    int32_t swt0;
+    // screening constants: n, 128 n - sumT and n (128 sumT - 16384 n) as Float64 (exact
+    // integers), 1 / (cnorm sqrt(n)) as Float32
+    double nd, cd, dd;
+    float k;
 };
 
 // Every quantity below is an integer below 2^53, so the Float64 forms are exact:
@@ -157,19 +158,19 @@ __device__ inline float exact_u8(const U8Point& P, int32_t acc, int32_t s1, int3
     return den > 0 ? (float)(num / den) : 0.0f;
 }
 
-// FP32 screening score, relative error below ~5e-7: (n num) / (cnorm sqrt(n * n var)), 0 for a
-// flat window. n num and n var are the exact integers of n_times_num / n_times_var (the fused
-// forms round nothing below 2^53); the flat-window case is a select, so the per-shift code has
-// no branches.
-__device__ inline float approx_u8(const U8Point& P, int32_t acc, int32_t s1, int32_t s2,
-                                  int measure) {
+// FP32 screening score (measure M: 0 ZNCC, 1 NCC): (n num) / (cnorm sqrt(n) sqrt(n var)),
+// 0 for a flat window. n num = n acc + (128 n - sumT) s1 + n (128 sumT - 16384 n) and n var are
+// the exact integers of n_times_num / n_times_var (every partial sum of the fused forms is an
+// integer below 2^53). Two conversions to Float32, v_rsq_f32 (1 ulp), the rounded constant and
+// two products: relative error below 6 * 2^-24 < 4e-7. A flat window has n var = 0 and n num = 0,
+// so flooring n var at 1e-30 (every other n var is >= 1) makes its score 0 without a branch.
+template <int M>
+__device__ inline float approx_u8(const U8Point& P, int32_t acc, int32_t s1, int32_t s2) {
     const double ds1 = (double)s1;
     const double ns2 = P.nd * (double)s2;
-    const double nv = measure == 1 ? ns2 : fma(-ds1, ds1, ns2);
-    const int32_t swt = acc + 128 * s1 + P.swt0;
-    const double nn = fma(-P.sumTd, ds1, (double)swt * P.nd);
-    const float sc = (float)nn * P.inv_cnorm_f * __builtin_amdgcn_rsqf((float)nv * P.nf);
-    return nv != 0.0 ? sc : 0.0f;
+    const double nv = M == 1 ? ns2 : fma(-ds1, ds1, ns2);
+    const double nn = fma(P.cd, ds1, fma(P.nd, (double)acc, P.dd));
+    return (float)nn * P.k * __builtin_amdgcn_rsqf(fmaxf((float)nv, 1e-30f));
 }
 
 // Sum W*T of one batch of 4 row-shift tiles (ti0 .. ti0 + 3) of column-shift tile tj on the int8
@@ -439,11 +440,10 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
         bool ok = sq > 2.220446049250313e-16 * (double)ch * (double)cw;
         if (!ok) { write_skip(a, p, true); __syncthreads(); continue; }
         P.cnorm = sqrt(sq);
-        P.inv_cnorm_f = (float)(1.0 / P.cnorm);
         P.nd = (double)P.n;
-        P.sumTd = (double)P.sumT;
-        P.nf = (float)P.n;
-        P.swt0 = 128 * (int32_t)P.sumT - 16384 * (int32_t)P.n;
+        P.cd = (double)(128 * P.n - P.sumT);
+        P.dd = (double)(P.n * (128 * P.sumT - 16384 * P.n));
+        P.k = (float)(1.0 / (P.cnorm * sqrt(P.nd)));
 
         // ---- integer box sums S1 (W) and S2 (W^2) over every shift (unless read from the box
         // images above): sliding sums, with each window column split into KC row chunks and each
@@ -596,12 +596,16 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
         // is the first maximum it meets, indexed by o: ties are settled by the exact values.
         const int nsh = NR * NC;
         Best lb = best_empty();
-        for (int o = tid; o < nsh; o += AR_WG) {
-            const float sv = approx_u8(P, accL[o], S1[o], S2[o], a.measure);
-            const bool up = sv > lb.v;
-            lb.v = up ? sv : lb.v;
-            lb.idx = up ? o : lb.idx;
-        }
+        auto screen = [&](auto mc) {
+            for (int o = tid; o < nsh; o += AR_WG) {
+                const float sv = approx_u8<decltype(mc)::value>(P, accL[o], S1[o], S2[o]);
+                const bool up = sv > lb.v;
+                lb.v = up ? sv : lb.v;
+                lb.idx = up ? o : lb.idx;
+            }
+        };
+        if (a.measure == 1) screen(std::integral_constant<int, 1>());
+        else screen(std::integral_constant<int, 0>());
         const Best b = block_best(lb, red);
         const float thr = b.v - 2.0f * AR_SCREEN_EPS;
         // the patch around the screening peak, evaluated while the contenders are
@@ -616,13 +620,18 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
         // contenders: only lanes whose own best reaches the threshold hold any; they score their
         // shifts again (the same function of the same integers)
         Best e2 = best_empty();
-        if (lb.v >= thr)
+        auto contend = [&](auto mc) {
             for (int o = tid; o < nsh; o += AR_WG)
-                if (approx_u8(P, accL[o], S1[o], S2[o], a.measure) >= thr) {
+                if (approx_u8<decltype(mc)::value>(P, accL[o], S1[o], S2[o]) >= thr) {
                     const int j = o / NR, i = o - j * NR;
                     const float v = exact_u8(P, accL[o], S1[o], S2[o], a.measure);
                     if (better(v, i * NC + j, e2)) { e2.v = v; e2.idx = i * NC + j; }
                 }
+        };
+        if (lb.v >= thr) {
+            if (a.measure == 1) contend(std::integral_constant<int, 1>());
+            else contend(std::integral_constant<int, 0>());
+        }
         e2 = block_best(e2, red);
         const int pi = e2.idx / NC + 1, pj = e2.idx % NC + 1;
         const float pval = e2.v;
