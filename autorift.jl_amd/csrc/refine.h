@@ -83,7 +83,7 @@ __device__ inline void pair_src(int k, int n, int& km, int& kq) {
 
 // The passes below give each lane up to AR_U tasks at a time (t, t + 64, ...), issuing all
 // loads before any arithmetic.
-#define AR_U 2
+#define AR_U 1
 
 // One full pyrUp step by one wave: src (N x N, row-major) -> dst (2N x 2N), via tmp (2N x N),
 // vertical pass first; src may share dst's bytes (only the vertical pass reads it). Returns the
@@ -159,10 +159,14 @@ __host__ __device__ inline void feed_range(int a, int b, int n, int& lo, int& hi
     hi = h < n - 1 ? h : n - 1;
 }
 
-// Rows of the level-L block of a final-level tile: 16 on the final level F, then 10 and 7.
+// Rows (and columns) of the level-L block of a final-level tile: 16 on the final level F, then
+// 10 and 7 -- each the widest source range feed_range gives for the level above.
 __host__ __device__ constexpr int tile_rows(int L, int F) {
     return L == F ? AR_TILE : (L == F - 1 ? 10 : 7);
 }
+
+// Widest source range of w consecutive outputs of a pyrUp axis (feed_range): (w + 1) / 2 + 2.
+__host__ __device__ constexpr int feed_width(int w) { return (w + 1) / 2 + 2; }
 
 // The wave computes the final-level tile (ti, tj) of level F from the materialised 8x level Z
 // (40 x 40, row-major), through its blocks of the levels in between (pitch 16 in `ws`). With
@@ -189,7 +193,10 @@ __device__ inline Best tile_run(const float* Z, float* ws, int ti, int tj, float
     Best best = best_empty();
 #pragma unroll
     for (int L = 4; L <= F; L++) {
-        const int NIT = (tile_rows(L, F) * 16 + 63) / 64;   // lane iterations covering the block
+        // Lanes cover each pass's own extent: the block is W x W (at most), its vertical pass
+        // W rows by WS source columns. Blocks keep a pitch of 16 floats.
+        const int W = tile_rows(L, F), WS = feed_width(W);
+        const int NV = (W * WS + 63) / 64, NH = (W * W + 63) / 64;   // lane iterations per pass
         const int n = 5 << (L - 1);
         const int nr = rhi[L] - rlo[L] + 1;
         const int ncs = chi[L - 1] - clo[L - 1] + 1;
@@ -197,9 +204,9 @@ __device__ inline Best tile_run(const float* Z, float* ws, int ti, int tj, float
         bool x3[4];
         // vertical pass: tmp[r][c] = level-L row rlo + r over level-(L-1) column clo + c
 #pragma unroll
-        for (int u = 0; u < NIT; u++) {
+        for (int u = 0; u < NV; u++) {
             int t = lane + 64 * u;
-            int r = t >> 4, c = t & 15;
+            int r = t / WS, c = t - r * WS;
             r = r < nr ? r : nr - 1;
             c = c < ncs ? c : ncs - 1;
             int m, cc, q, nt;
@@ -211,19 +218,20 @@ __device__ inline Best tile_run(const float* Z, float* ws, int ti, int tj, float
             x3[u] = nt == 3;
         }
 #pragma unroll
-        for (int u = 0; u < NIT; u++) {
+        for (int u = 0; u < NV; u++) {
             int t = lane + 64 * u;
-            if ((t >> 4) < nr && (t & 15) < ncs)
-                tmp[t] = x3[u] ? tap3(xa[u], xb[u], xq[u]) : tap2(xb[u], xq[u]);
+            int r = t / WS, c = t - r * WS;
+            if (r < nr && c < ncs)
+                tmp[r * 16 + c] = x3[u] ? tap3(xa[u], xb[u], xq[u]) : tap2(xb[u], xq[u]);
         }
         wave_sync();
         const float* row0 = tmp - clo[L - 1];
         const int nc = chi[L] - clo[L] + 1;
         // horizontal pass: level-L samples (rlo + r, clo + c)
 #pragma unroll
-        for (int u = 0; u < NIT; u++) {
+        for (int u = 0; u < NH; u++) {
             int t = lane + 64 * u;
-            int r = t >> 4, c = t & 15;
+            int r = t / W, c = t - r * W;
             r = r < nr ? r : nr - 1;
             c = c < nc ? c : nc - 1;
             int m, cc, q, nt;
@@ -237,11 +245,12 @@ __device__ inline Best tile_run(const float* Z, float* ws, int ti, int tj, float
         if (L < F) {
             float mx = -INFINITY;
 #pragma unroll
-            for (int u = 0; u < NIT; u++) {
+            for (int u = 0; u < NH; u++) {
                 int t = lane + 64 * u;
-                if ((t >> 4) < nr && (t & 15) < nc) {
+                int r = t / W, c = t - r * W;
+                if (r < nr && c < nc) {
                     float v = x3[u] ? tap3(xa[u], xb[u], xq[u]) : tap2(xb[u], xq[u]);
-                    blk[t] = v;
+                    blk[r * 16 + c] = v;
                     mx = v > mx ? v : mx;
                 }
             }
@@ -255,7 +264,7 @@ __device__ inline Best tile_run(const float* Z, float* ws, int ti, int tj, float
         } else {
             const int nf = 5 << F;
 #pragma unroll
-            for (int u = 0; u < NIT; u++) {
+            for (int u = 0; u < NH; u++) {
                 int t = lane + 64 * u;
                 float v = x3[u] ? tap3(xa[u], xb[u], xq[u]) : tap2(xb[u], xq[u]);
                 int idx = (rlo[F] + (t >> 4)) * nf + clo[F] + (t & 15);
