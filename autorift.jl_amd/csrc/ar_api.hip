@@ -159,6 +159,18 @@ __global__ __launch_bounds__(256) void pass_geometry_kernel(ar_points pts, int64
     }
 }
 
+// Per-point geometry records of a pass (point_geom once per point instead of once per wave).
+__global__ __launch_bounds__(256) void point_prep_kernel(TrackArgs a) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; p < a.n; p += (int64_t)gridDim.x * blockDim.x) {
+        int32_t r[AR_PREC];
+        point_record(point_geom(a, p), r);
+        int4* q = (int4*)(a.prec + AR_PREC * p);
+        q[0] = make_int4(r[0], r[1], r[2], r[3]);
+        q[1] = make_int4(r[4], r[5], r[6], r[7]);
+    }
+}
+
 __global__ void skip_all_kernel(float* dx, float* dy, float* corr, uint8_t* s, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -517,6 +529,12 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         if (rpc < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "refinement kernel does not fit on a CU");
         rblocks = (int64_t)ctx->ncu * rpc;
         if (rblocks > n) rblocks = n;
+    }
+    if (use_u8) {
+        if ((st = ctx->prec.reserve(ctx, (size_t)n * AR_PREC * 4))) return st;
+        a.prec = (const int32_t*)ctx->prec.ptr;
+        point_prep_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(a);
+        HIPCHK(ctx, hipGetLastError());
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(AR_WG), a.lds_bytes, ctx->stream, a);

@@ -42,6 +42,7 @@ struct ar_context {
     DevBuf refine;              // per-point refinement hand-off: 5x5 patches and origins
     DevBuf box;                 // box images of W and W^2 for a pass with one chip size
     DevBuf refx;                // int8 copies (value ^ 0x80) of the UInt8 pair, for the matrix cores
+    DevBuf prec;                // per-point geometry records of the current pass (u8 path)
     uint64_t ref_gen = 0, refx_gen = ~0ULL;   // versions of the pair and of the copies made from it
 
     double last_ms = 0, last_ms_box = 0, last_ms_track = 0, last_ms_refine = 0;
@@ -49,7 +50,7 @@ struct ar_context {
 
     std::vector<DevBuf*> all_bufs() {
         return {&pref, &psec, &rmask, &smask, &okmask, &raw, &rawmask, &work_f32, &work_f64,
-                &flags, &geom, &pts, &out, &scratch, &grid_a, &grid_b, &refine, &box, &refx};
+                &flags, &geom, &pts, &out, &scratch, &grid_a, &grid_b, &refine, &box, &refx, &prec};
     }
 };
 

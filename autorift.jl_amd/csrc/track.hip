@@ -44,8 +44,8 @@ __device__ inline void write_skip(const TrackArgs& a, int64_t p, bool searched) 
 __device__ inline void write_result(const TrackArgs& a, int64_t p, const Geom& g, double ddx,
                                     double ddy, float val) {
     if (threadIdx.x == 0) {
-        a.dx[p] = (float)(ddx + g.pdx);
-        a.dy[p] = (float)(ddy + g.pdy);
+        a.dx[p] = (float)(ddx + a.dx_prior[p]);
+        a.dy[p] = (float)(ddy + a.dy_prior[p]);
         a.corr[p] = val;
         a.searched[p] = 1;
     }
@@ -85,8 +85,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void re
         CascadeOut c = cascade(reg, a.up);
         const int i0 = m & 0xffff, j0 = m >> 16;
         Geom g;
-        g.pdx = a.dx_prior[p];
-        g.pdy = a.dy_prior[p];
         g.rx = (int)a.radius_x[p];
         g.ry = (int)a.radius_y[p];
         double factor = (double)a.up;
@@ -239,7 +237,7 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
     __syncthreads();
     const PointRange pr = point_range(a.n);
     for (int64_t p = pr.first; p < pr.end; p += pr.step) {
-        Geom g = point_geom(a, p);
+        Geom g = load_geom(a, p);
         if (!g.status) { write_skip(a, p, false); continue; }
         if (!any_valid(a, g)) { write_skip(a, p, false); continue; }
         const int ch = g.ch, cw = g.cw, NR = 2 * g.ry, NC = 2 * g.rx;

@@ -20,6 +20,7 @@ struct TrackArgs {
                               // kernel's 32-bit offsets)
     const void* sec;          // prepared secondary image
     const uint8_t* ok;        // valid(ref) & valid(sec); nullptr when every pixel is valid
+    const int32_t* prec;      // u8 path: per-point geometry records (point_record), 8 per point
     int64_t ny, nx;           // image size
     int64_t padx, pady;       // virtual zero padding of the pass (0 when every window fits)
     const double* x;          // point fields (src/points.jl:50-74), 1-based coordinates
@@ -98,6 +99,40 @@ __device__ inline Geom point_geom(const TrackArgs& a, int64_t p) {
     if (!chip_in || !win_in) return g;
     if (g.ch == 0 || g.cw == 0) return g;   // empty chip: _any_valid is false
     g.status = 1;
+    return g;
+}
+
+// The geometry of a point as 8 int32 (written once per pass by point_prep_kernel, read by the
+// u8 tracker with one scalar load): status, cr0, cc0, wr0, wc0, ch | cw << 16, rx | ry << 16, 0.
+// A searchable point lies inside the padded image, so its coordinates fit in int32.
+#define AR_PREC 8
+__device__ inline void point_record(const Geom& g, int32_t* r) {
+    r[0] = g.status;
+    r[1] = g.status ? (int32_t)g.cr0 : 0;
+    r[2] = g.status ? (int32_t)g.cc0 : 0;
+    r[3] = g.status ? (int32_t)g.wr0 : 0;
+    r[4] = g.status ? (int32_t)g.wc0 : 0;
+    r[5] = g.status ? (g.ch | (g.cw << 16)) : 0;
+    r[6] = (g.rx & 0xffff) | (g.ry << 16);
+    r[7] = 0;
+}
+
+// Geom from a point's record (the priors are read where the result is written).
+__device__ inline Geom load_geom(const TrackArgs& a, int64_t p) {
+    const int4* q = (const int4*)(a.prec + AR_PREC * p);
+    const int4 u = q[0], v = q[1];
+    Geom g;
+    g.status = u.x;
+    g.cr0 = u.y;
+    g.cc0 = u.z;
+    g.wr0 = u.w;
+    g.wc0 = v.x;
+    g.ch = v.y & 0xffff;
+    g.cw = v.y >> 16;
+    g.rx = v.z & 0xffff;
+    g.ry = v.z >> 16;
+    g.pdx = 0.0;
+    g.pdy = 0.0;
     return g;
 }
 
