@@ -220,8 +220,9 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
     TrackArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, a.box1 != nullptr);
-    __shared__ Best red[AR_NWAVES];
-    __shared__ int64_t red64[2 * AR_NWAVES];
+    // one set of reduction slots per call site (see block_sum2_u32 / block_best1)
+    __shared__ Best red_screen[AR_NWAVES], red_exact[AR_NWAVES];
+    __shared__ uint32_t red_chip[2 * AR_NWAVES];
     int8_t* Wl = (int8_t*)(smem + L.off_w);
     int8_t* Tp = (int8_t*)(smem + L.off_t);
     int32_t* S1 = (int32_t*)(smem + L.off_s1);
@@ -431,8 +432,11 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
         U8Point P;
         P.n = (int64_t)ch * cw;
         // chip energy, exact in int64: sum (n x - sumT)^2 = n^2 sum x^2 - n sumT^2, over n^2
-        int64_t sumx2;
-        block_sum2<int64_t>(st, st2, red64, P.sumT, sumx2);
+        // (sum x and sum x^2 of at most 16384 UInt8 samples fit in uint32)
+        uint32_t sum1, sum2;
+        block_sum2_u32((uint32_t)st, (uint32_t)st2, red_chip, sum1, sum2);
+        P.sumT = sum1;
+        const int64_t sumx2 = sum2;
         const int64_t sumk2 = P.n * P.n * sumx2 - P.n * P.sumT * P.sumT;
         double sq = (double)sumk2 / ((double)P.n * (double)P.n);
         bool ok = sq > 2.220446049250313e-16 * (double)ch * (double)cw;
@@ -604,7 +608,7 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
         };
         if (a.measure == 1) screen(std::integral_constant<int, 1>());
         else screen(std::integral_constant<int, 0>());
-        const Best b = block_best(lb, red);
+        const Best b = block_best1<true>(lb, red_screen);
         const float thr = b.v - 2.0f * AR_SCREEN_EPS;
         // the patch around the screening peak, evaluated while the contenders are
         int i0 = 1, j0 = 1;
@@ -630,7 +634,7 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
             if (a.measure == 1) contend(std::integral_constant<int, 1>());
             else contend(std::integral_constant<int, 0>());
         }
-        e2 = block_best(e2, red);
+        e2 = block_best1<false>(e2, red_exact);
         const int pi = e2.idx / NC + 1, pj = e2.idx % NC + 1;
         const float pval = e2.v;
         const int si0 = i0, sj0 = j0;

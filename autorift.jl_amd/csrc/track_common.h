@@ -272,6 +272,89 @@ __device__ inline Best block_best(Best b, Best* red) {
     return r;
 }
 
+// ---- one-barrier block reductions for a full workgroup (all lanes active). A row of 16 lanes
+// is reduced by DPP (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror, row_mirror: afterwards
+// every lane of the row holds the row's result), the four rows by readlane, and the waves through
+// `slots` (AR_NWAVES entries owned by one call site, so no barrier has to guard them before the
+// write: the call site is reached again only after later barriers).
+__device__ inline int dpp_row(int v, int ctrl) {
+    switch (ctrl) {
+        case 0: return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+        case 1: return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+        case 2: return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);
+        default: return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);
+    }
+}
+
+__device__ inline uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) v += (uint32_t)dpp_row((int)v, k);
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 0) + (uint32_t)__builtin_amdgcn_readlane((int)v, 16) +
+           (uint32_t)__builtin_amdgcn_readlane((int)v, 32) + (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+}
+
+__device__ inline float wave_max_f32(float v) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) v = fmaxf(v, __int_as_float(dpp_row(__float_as_int(v), k)));
+    const float a = fmaxf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0)),
+                          __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16)));
+    const float b = fmaxf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32)),
+                          __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48)));
+    return fmaxf(a, b);
+}
+
+__device__ inline int wave_min_i32(int v) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) v = min(v, dpp_row(v, k));
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+// Sums of two uint32 over the workgroup (each total below 2^32).
+__device__ inline void block_sum2_u32(uint32_t a, uint32_t b, uint32_t* slots, uint32_t& sa,
+                                      uint32_t& sb) {
+    a = wave_sum_u32(a);
+    b = wave_sum_u32(b);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        slots[w] = a;
+        slots[AR_NWAVES + w] = b;
+    }
+    __syncthreads();
+    sa = slots[0];
+    sb = slots[AR_NWAVES];
+    for (int k = 1; k < AR_NWAVES; k++) {
+        sa += slots[k];
+        sb += slots[AR_NWAVES + k];
+    }
+}
+
+// Workgroup argmax under `better` (exact: the peak_index rule) or, with ANY, the largest value
+// at any one of the lanes that hold it (ties left open; NaN never wins either way).
+template <bool ANY>
+__device__ inline Best block_best1(Best b, Best* slots) {
+    const float m = wave_max_f32(b.v);
+    Best r;
+    r.v = m;
+    if (ANY) {
+        const uint64_t hit = __ballot(b.v == m);
+        r.idx = hit ? __builtin_amdgcn_readlane(b.idx, __ffsll((unsigned long long)hit) - 1)
+                    : 0x7fffffff;
+    } else {
+        r.idx = wave_min_i32(b.v == m ? b.idx : 0x7fffffff);
+    }
+    if (!(m > -INFINITY)) {   // empty (or all NaN) wave: the empty key, which any value beats
+        r.v = -INFINITY;
+        r.idx = 0x7fffffff;
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) slots[w] = r;
+    __syncthreads();
+    Best q = slots[0];
+    for (int k = 1; k < AR_NWAVES; k++) q = best_merge(q, slots[k]);
+    return q;
+}
+
 __device__ inline Best best_empty() {
     Best b;
     b.v = -INFINITY;
