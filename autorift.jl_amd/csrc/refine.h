@@ -59,12 +59,21 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ inline float wave_max(float v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        float u = __shfl_xor(v, o);
-        v = u > v ? u : v;
+// Largest non-NaN value of the wave (-inf if none). A pass's output is NaN whenever one of its
+// taps is, so its non-NaN outputs are bounded by the largest non-NaN input.
+__device__ inline float wave_max(float v) { return wave_max_f32(v); }
+
+// The wave's argmax under the peak_index rule: the largest value, then the smallest index; NaN
+// never wins; (-inf, INT32_MAX) when no lane holds a value.
+__device__ inline Best wave_best_dpp(Best b) {
+    Best r;
+    r.v = wave_max_f32(b.v);
+    r.idx = wave_min_i32(b.v == r.v ? b.idx : 0x7fffffff);
+    if (!(r.v > -INFINITY)) {
+        r.v = -INFINITY;
+        r.idx = 0x7fffffff;
     }
-    return v;
+    return r;
 }
 
 __device__ inline float ub_margin(float m) {
@@ -270,7 +279,7 @@ __device__ inline Best tile_run(const float* Z, float* ws, int ti, int tj, float
                 int idx = (rlo[F] + (t >> 4)) * nf + clo[F] + (t & 15);
                 if (better(v, idx, best)) { best.v = v; best.idx = idx; }
             }
-            best = wave_best(best);
+            best = wave_best_dpp(best);
             wave_sync();   // tmp is rewritten by the next tile
         }
     }
@@ -354,7 +363,7 @@ __device__ inline CascadeOut cascade(float* reg, int up) {
     if (LM >= 1) b = wave_pyrup<5>(patch, T, Y);
     if (LM >= 2) b = wave_pyrup<10>(Y, T, Y);
     if (LM >= 3) b = wave_pyrup<20>(Y, T, Y);
-    b = wave_best(b);
+    b = wave_best_dpp(b);
 
     int nf = 5 << LM;
     if (L >= 4) {
