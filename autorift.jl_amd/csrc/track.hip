@@ -297,17 +297,33 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
                 int r4 = 64 * wave + lane - c * cq;
                 uint32_t coff = (uint32_t)c * ny;
                 const uint32_t dco = (uint32_t)dc * ny, lo0 = (uint32_t)off0;
-                for (int d0 = 64 * wave; d0 < tot; d0 += 256) {
-                    if (d0 + lane < tot) {
-                        const uint32_t sh = (0u - (lo0 + coff)) & 3u;
-                        __builtin_amdgcn_global_load_lds(base + (sh * st1 + coff + 4u * (uint32_t)r4),
-                                                         Wl + 4 * d0, 4, 0, 0);
+                if ((ny & 3u) == 0) {
+                    // every column starts at the window's alignment: one copy, one offset per lane
+                    const uint8_t* bs = uniform_ptr(base + (uint64_t)((0u - lo0) & 3u) * st1);
+                    uint32_t off = coff + 4u * (uint32_t)r4;
+                    const uint32_t doff = dco + 4u * (uint32_t)dr, wrap = ny - 4u * (uint32_t)cq;
+                    for (int d0 = 64 * wave; d0 < tot; d0 += 256) {
+                        if (d0 + lane < tot) __builtin_amdgcn_global_load_lds(bs + off, Wl + 4 * d0, 4, 0, 0);
+                        r4 += dr;
+                        off += doff;
+                        if (r4 >= cq) {
+                            r4 -= cq;
+                            off += wrap;
+                        }
                     }
-                    r4 += dr;
-                    coff += dco;
-                    if (r4 >= cq) {
-                        r4 -= cq;
-                        coff += ny;
+                } else {
+                    for (int d0 = 64 * wave; d0 < tot; d0 += 256) {
+                        if (d0 + lane < tot) {
+                            const uint32_t sh = (0u - (lo0 + coff)) & 3u;
+                            __builtin_amdgcn_global_load_lds(
+                                base + (sh * st1 + coff + 4u * (uint32_t)r4), Wl + 4 * d0, 4, 0, 0);
+                        }
+                        r4 += dr;
+                        coff += dco;
+                        if (r4 >= cq) {
+                            r4 -= cq;
+                            coff += ny;
+                        }
                     }
                 }
             } else {

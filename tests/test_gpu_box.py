@@ -74,3 +74,18 @@ def test_box_tall_chip(ctx, box_mode):
     pts = autorift.gridpoints_xy(xs, ys, chip_size=64, search_radius=8)
     pts = pts.rebuild(chip_size_y=np.full(pts.shape, 256, np.int64))
     _compare(ctx, ref, sec, pts, 64, box_mode)
+
+
+@pytest.mark.parametrize("ny", [200, 201, 202, 203])
+def test_box_row_counts(ctx, box_mode, ny):
+    # every residue of the column length mod 4: the window and chip columns start at each byte
+    # alignment in the int8 copies the tracker fetches from (all windows inside the image)
+    ref, sec = shifted_pair(210, (-1.75, 2.5), seed=14 + ny, dtype=np.uint8)
+    ref = np.ascontiguousarray(ref[:ny, :197])
+    sec = np.ascontiguousarray(sec[:ny, :197])
+    xs = np.arange(45, 160, 9, dtype=float)
+    ys = np.arange(45, ny - 44, 9, dtype=float)
+    pts = autorift.gridpoints_xy(xs, ys, chip_size=32, search_radius=16)
+    for up in (1, 64):
+        g = _compare(ctx, ref, sec, pts, up, box_mode)
+        assert g.searched.all()
