@@ -23,7 +23,6 @@
 namespace ar {
 
 #define BX_CHUNK 128   // output columns per wave
-#define BX_G 8         // columns per group of loads
 
 // rows per lane for chip height ch: the band of 64 RPL rows yields at least half as many outputs
 __host__ __device__ inline int box_rpl(int ch) { return ch <= 128 ? 4 : 8; }
@@ -33,6 +32,8 @@ __global__ __launch_bounds__(256) void box_u8_kernel(const uint8_t* img, int64_t
                                                     int64_t pady, int64_t padx, int ch, int cw,
                                                     int64_t HB, int64_t WB, int32_t* B1,
                                                     int32_t* B2) {
+    // columns per group of loads: 16 loaded dwords per lane in flight whatever the band height
+    constexpr int BX_G = 32 / RPL;
     __shared__ __align__(16) uint32_t sS[4][2][64 * RPL];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t* s1 = sS[wave][0];
@@ -194,20 +195,16 @@ ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t p
     const int rpl = box_rpl(ch);
     const int OUT = 64 * rpl - ch + 1;
     const int64_t waves = ((HB + OUT - 1) / OUT) * ((WB + BX_CHUNK - 1) / BX_CHUNK);
+    auto kern = rpl == 4 ? box_u8_kernel<4> : (rpl == 8 ? box_u8_kernel<8> : box_u8_kernel<16>);
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, rpl == 4 ? box_u8_kernel<4> : box_u8_kernel<8>, 256, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess ||
         per_cu < 1)
         return AR_ERR_DEVICE;
     int64_t blocks = (int64_t)ctx->ncu * per_cu;
     if (blocks > (waves + 3) / 4) blocks = (waves + 3) / 4;
     const uint8_t* img = (const uint8_t*)ctx->pref.ptr;
-    if (rpl == 4)
-        hipLaunchKernelGGL(box_u8_kernel<4>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
-                           img, ctx->ny, ctx->nx, pady, padx, ch, cw, HB, WB, p1, p2);
-    else
-        hipLaunchKernelGGL(box_u8_kernel<8>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
-                           img, ctx->ny, ctx->nx, pady, padx, ch, cw, HB, WB, p1, p2);
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, img, ctx->ny,
+                       ctx->nx, pady, padx, ch, cw, HB, WB, p1, p2);
     if (hipGetLastError() != hipSuccess) return AR_ERR_DEVICE;
     *b1 = p1;
     *b2 = p2;

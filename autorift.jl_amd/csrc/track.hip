@@ -250,29 +250,23 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
         // ---- S1 / S2 of every shift from the pass's box images (exact integers), loaded straight
         // into LDS (global_load_lds: no registers; the barrier of the chip sums below waits for
         // them). Wave w fills elements 64 (w + 4 k) + lane of the column-major arrays.
-        // Addresses are a per-point base (scalar) plus a 32-bit byte offset per lane, advanced
-        // incrementally; the dispatcher keeps every such offset below 2^32.
+        // Addresses are a per-point base (scalar) plus a 32-bit byte offset per lane; the
+        // dispatcher keeps every such offset below 2^32. Wave w takes shift columns w, w + 4, ...
+        // with lane = shift row (in chunks of 64 rows): the lane offsets never change, the
+        // column base moves in scalar registers.
         if (a.box1) {
             const int64_t q0 = (g.wr0 - 1) + (g.wc0 - 1) * a.box_ld;
             const char* b1 = uniform_ptr((const char*)(a.box1 + q0));
             const char* b2 = uniform_ptr((const char*)(a.box2 + q0));
-            const uint32_t ld = (uint32_t)a.box_ld;
-            const int tot = NR * NC, dj = 256 / NR, di = 256 - dj * NR;
-            const int j = (64 * wave + lane) / NR;
-            int i = 64 * wave + lane - j * NR;
-            uint32_t off = 4u * ((uint32_t)i + (uint32_t)j * ld);
-            const uint32_t doff = 4u * ((uint32_t)di + (uint32_t)dj * ld);
-            const uint32_t wrap = 4u * (ld - (uint32_t)NR);   // i -= NR, j += 1
-            for (int e0 = 64 * wave; e0 < tot; e0 += 256) {
-                if (e0 + lane < tot) {
-                    __builtin_amdgcn_global_load_lds(b1 + off, S1 + e0, 4, 0, 0);
-                    __builtin_amdgcn_global_load_lds(b2 + off, S2 + e0, 4, 0, 0);
-                }
-                i += di;
-                off += doff;
-                if (i >= NR) {
-                    i -= NR;
-                    off += wrap;
+            const uint32_t ldb = 4u * (uint32_t)a.box_ld;   // bytes between box-image columns
+            for (int i0 = 0; i0 < NR; i0 += 64) {
+                if (i0 + lane < NR) {
+                    const uint32_t lo = 4u * (uint32_t)(i0 + lane);
+                    for (int j = wave; j < NC; j += AR_NWAVES) {
+                        const uint32_t co = (uint32_t)j * ldb;
+                        __builtin_amdgcn_global_load_lds(b1 + co + lo, S1 + j * NR + i0, 4, 0, 0);
+                        __builtin_amdgcn_global_load_lds(b2 + co + lo, S2 + j * NR + i0, 4, 0, 0);
+                    }
                 }
             }
         }
@@ -297,8 +291,23 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
                 int r4 = 64 * wave + lane - c * cq;
                 uint32_t coff = (uint32_t)c * ny;
                 const uint32_t dco = (uint32_t)dc * ny, lo0 = (uint32_t)off0;
-                if ((ny & 3u) == 0) {
-                    // every column starts at the window's alignment: one copy, one offset per lane
+                const int kc = 64 / cq;   // whole window columns one DMA instruction covers
+                if ((ny & 3u) == 0 && kc >= 1) {
+                    // Every column starts at the window's alignment, so one copy serves them all;
+                    // each instruction fills kc whole columns (lane = column cin, dword r4 of
+                    // it): the lane offsets never change, the column base moves in scalar
+                    // registers.
+                    const uint8_t* bs = uniform_ptr(base + (uint64_t)((0u - lo0) & 3u) * st1);
+                    const int cin = lane / cq, r4l = lane - cin * cq;
+                    if (cin < kc) {
+                        const uint32_t lo = (uint32_t)cin * ny + 4u * (uint32_t)r4l;
+                        for (int c0 = wave * kc; c0 < winw; c0 += AR_NWAVES * kc)
+                            if (c0 + cin < winw)
+                                __builtin_amdgcn_global_load_lds(bs + (uint64_t)c0 * ny + lo,
+                                                                 Wl + c0 * L.ldw, 4, 0, 0);
+                    }
+                } else if ((ny & 3u) == 0) {
+                    // columns longer than 64 dwords: dwords d = 64 (wave + 4 k) + lane in order
                     const uint8_t* bs = uniform_ptr(base + (uint64_t)((0u - lo0) & 3u) * st1);
                     uint32_t off = coff + 4u * (uint32_t)r4;
                     const uint32_t doff = dco + 4u * (uint32_t)dr, wrap = ny - 4u * (uint32_t)cq;
