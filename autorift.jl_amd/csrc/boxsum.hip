@@ -22,10 +22,23 @@
 
 namespace ar {
 
-#define BX_CHUNK 128   // output columns per wave
+#define BX_CHUNK 64    // output columns per wave
 
 // rows per lane for chip height ch: the band of 64 RPL rows yields at least half as many outputs
 __host__ __device__ inline int box_rpl(int ch) { return ch <= 128 ? 4 : 8; }
+
+// Inclusive prefix sum over the 64 lanes by DPP: shifts by 1, 2, 4, 8 within each row of 16
+// lanes, then the last lane of rows 0 and 2 into rows 1 and 3 (row_bcast:15) and the last lane
+// of row 1 into rows 2 and 3 (row_bcast:31). Lanes a shift leaves without a source add 0.
+__device__ inline uint32_t wave_scan_u32(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);
+    return v;
+}
 
 template <int RPL>
 __global__ __launch_bounds__(256) void box_u8_kernel(const uint8_t* img, int64_t ny, int64_t nx,
@@ -121,15 +134,7 @@ __global__ __launch_bounds__(256) void box_u8_kernel(const uint8_t* img, int64_t
                 p1[k] = p1[k - 1] + h1[k];
                 p2[k] = p2[k - 1] + h2[k];
             }
-            uint32_t t1 = p1[RPL - 1], t2 = p2[RPL - 1];
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                uint32_t u1 = __shfl_up(t1, o), u2 = __shfl_up(t2, o);
-                if (lane >= o) {
-                    t1 += u1;
-                    t2 += u2;
-                }
-            }
+            const uint32_t t1 = wave_scan_u32(p1[RPL - 1]), t2 = wave_scan_u32(p2[RPL - 1]);
             const uint32_t e1 = t1 - p1[RPL - 1], e2 = t2 - p2[RPL - 1];
 #pragma unroll
             for (int q = 0; q < RPL / 4; q++) {
