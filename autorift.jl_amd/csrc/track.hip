@@ -125,6 +125,7 @@ struct U8Point {
     // integers), 1 / (cnorm sqrt(n)) as Float32
     double nd, cd, dd;
     float k;
+    double invn;     // 1 / n (exact: n is a power of two)
 };
 
 // Every quantity below is an integer below 2^53, so the Float64 forms are exact:
@@ -144,7 +145,7 @@ __device__ inline double n_times_var(const U8Point& P, int32_t s1, int32_t s2, i
 // The reference's normalisation (src/correlate.jl:591-600) on the exact sums.
 __device__ inline float exact_u8(const U8Point& P, int32_t acc, int32_t s1, int32_t s2,
                                  int measure) {
-    double invn = 1.0 / (double)P.n;                // exact: n = 2^k
+    const double invn = P.invn;                     // exact: n = 2^k
     double num = n_times_num(P, acc, s1) * invn;
     double den;
     if (measure == 1) {
@@ -468,6 +469,7 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
         if (!ok) { write_skip(a, p, true); __syncthreads(); continue; }
         P.cnorm = sqrt(sq);
         P.nd = (double)P.n;
+        P.invn = 1.0 / P.nd;
         P.cd = (double)(128 * P.n - P.sumT);
         P.dd = (double)(P.n * (128 * P.sumT - 16384 * P.n));
         P.k = (float)(1.0 / (P.cnorm * sqrt(P.nd)));
@@ -620,13 +622,16 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
 
         // ---- integer peak: screen every shift in FP32 (all lanes busy, the matrix-core tiles'
         // padding left out), then exact values for the contenders. The screening best of a lane
-        // is the first maximum it meets, indexed by o: ties are settled by the exact values.
+        // is the first maximum it meets, indexed by o: ties are settled by the exact values. It also
+        // keeps its second-largest score, so that a lane with one contender needs no rescan.
         const int nsh = NR * NC;
         Best lb = best_empty();
+        float lb2 = -INFINITY;
         auto screen = [&](auto mc) {
             for (int o = tid; o < nsh; o += AR_WG) {
                 const float sv = approx_u8<decltype(mc)::value>(P, accL[o], S1[o], S2[o]);
                 const bool up = sv > lb.v;
+                lb2 = up ? lb.v : fmaxf(lb2, sv);
                 lb.v = up ? sv : lb.v;
                 lb.idx = up ? o : lb.idx;
             }
@@ -656,8 +661,16 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
                 }
         };
         if (lb.v >= thr) {
-            if (a.measure == 1) contend(std::integral_constant<int, 1>());
-            else contend(std::integral_constant<int, 0>());
+            if (lb2 < thr) {
+                // the lane's only contender is its best shift
+                const int o = lb.idx, j = o / NR, i = o - j * NR;
+                e2.v = exact_u8(P, accL[o], S1[o], S2[o], a.measure);
+                e2.idx = i * NC + j;
+            } else if (a.measure == 1) {
+                contend(std::integral_constant<int, 1>());
+            } else {
+                contend(std::integral_constant<int, 0>());
+            }
         }
         e2 = block_best1<false>(e2, red_exact);
         const int pi = e2.idx / NC + 1, pj = e2.idx % NC + 1;
