@@ -16,6 +16,7 @@ AR_OK, AR_ERR_INVALID_ARGUMENT, AR_ERR_DIMENSION_MISMATCH = 0, 1, 2
 AR_ERR_OUT_OF_MEMORY, AR_ERR_DEVICE, AR_ERR_UNSUPPORTED, AR_ERR_NO_DEVICE = 3, 4, 5, 6
 AR_U8, AR_F32, AR_F64, AR_I16 = 0, 1, 2, 3
 AR_ZNCC, AR_NCC = 0, 1
+AR_PREC_EXACT, AR_PREC_BF16, AR_PREC_FP16 = 0, 1, 2
 
 
 class AutoRIFTError(RuntimeError):
@@ -51,6 +52,7 @@ class Params(C.Structure):
         ("min_search_radius", C.c_int64), ("coarse_stride", C.c_int64),
         ("coarse_buffer", C.c_int64), ("min_coarse_valid_fraction", C.c_double),
         ("dx_prior", C.c_double), ("dy_prior", C.c_double), ("fill_window", C.c_int64),
+        ("precision", C.c_int32), ("_pad1", C.c_int32),
     ]
 
 

@@ -20,6 +20,9 @@ _PREPROCESS = {"highpass": 1, "none": 0}
 _QUANTIZE = {"uint8": 1, "none": 0}
 _SUBPIXEL = ("pyramid", "none")
 _OUTLIERS = ("gardner", "none")
+# arithmetic of the correlation contraction for Float32 prepared images (no reference
+# counterpart: the reference always correlates in Float32 / Float64)
+_PRECISION = {"exact": 0, "bf16": 1, "fp16": 2}
 
 
 def _resolve(table, x, kw):
@@ -55,9 +58,15 @@ def params(similarity="zncc", preprocess="highpass", quantize="uint8", subpixel=
            dy_prior=0.0, coarse_stride=4, coarse_buffer=8, min_coarse_valid_fraction=0.01,
            outliers="gardner", rotation=None, outlier_window=None, outlier_iterations=None,
            min_agree_fraction=None, agree_tolerance=None, mad_scale=None, fill_window=3,
-           threaded=False, progress=False, rng_seed=0) -> Params:
-    """Resolve and validate keywords into the flattened parameter set (src/params.jl:306-392)."""
+           threaded=False, progress=False, rng_seed=0, precision="exact") -> Params:
+    """Resolve and validate keywords into the flattened parameter set (src/params.jl:306-392).
+
+    `precision` (:exact, :bf16, :fp16) is this library's addition: with `quantize = :none` the
+    correlation contraction runs on the matrix cores with 16-bit operands and FP32 accumulation;
+    UInt8 prepared images always take the exact integer path.
+    """
     p = _lib.default_params()
+    p.precision = _PRECISION[_resolve(_PRECISION, precision, "precision")]
     p.measure = _SIMILARITY[_resolve(_SIMILARITY, similarity, "similarity")]
     pre = _resolve(_PREPROCESS, preprocess, "preprocess")
     p.preprocess = _PREPROCESS[pre]

@@ -466,7 +466,24 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
             a.secx = (const uint8_t*)ctx->refx.ptr + 4 * stride;
             a.refx_stride = stride;
         }
-    } else {
+    }
+    // Float32 images with a 16-bit contraction requested: the mixed-precision kernel, which
+    // carves its LDS for one chip size per pass
+    const bool use_mp = !use_u8 && ctx->pdtype == AR_F32 && ctx->params.precision != AR_PREC_EXACT;
+    if (use_mp) {
+        if (ctx->params.precision != AR_PREC_BF16 && ctx->params.precision != AR_PREC_FP16)
+            return fail(ctx, AR_ERR_INVALID_ARGUMENT, "precision must be EXACT, BF16 or FP16");
+        if (!uniform)
+            return fail(ctx, AR_ERR_UNSUPPORTED,
+                        "the BF16 / FP16 tracker needs one chip size per pass");
+        size_t lds = 0;
+        st = ar_mp_setup(ctx, a, &kern, &lds);
+        if (st == AR_ERR_UNSUPPORTED)
+            return fail(ctx, st, "chip or search window too large for the BF16 / FP16 tracker");
+        if (st == AR_ERR_OUT_OF_MEMORY) return fail(ctx, st, "out of device memory (box images)");
+        if (st) return fail(ctx, st, "mixed-precision set-up failed");
+        a.lds_bytes = lds;
+    } else if (!use_u8) {
         a.lds_bytes = 0;
         kern = ctx->pdtype == AR_U8 ? track_generic_kernel<1> : track_generic_kernel<0>;
     }
@@ -512,7 +529,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     if (per_cu < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "tracker kernel does not fit on a CU");
     int64_t nblocks = (int64_t)ctx->ncu * per_cu;
     if (nblocks > n) nblocks = n;
-    if (!use_u8) {
+    if (!use_u8 && !use_mp) {
         GenScratch G = gen_scratch(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
         a.scratch_per_block = G.total;
         if ((st = ctx->scratch.reserve(ctx, G.total * nblocks))) return st;
@@ -556,7 +573,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     ctx->last_ms_box = ms_b;
     ctx->last_ms_track = ms_t;
     ctx->last_ms_refine = up > 1 ? ms_r : 0.0;
-    ctx->last_path = use_u8 ? 0 : 1;
+    ctx->last_path = use_u8 ? 0 : (use_mp ? 2 : 1);
     return AR_OK;
 }
 

@@ -44,6 +44,8 @@ struct ar_context {
     DevBuf refx;                // int8 copies (value ^ 0x80) of the UInt8 pair, for the matrix cores
     DevBuf prec;                // per-point geometry records of the current pass (u8 path)
     uint64_t ref_gen = 0, refx_gen = ~0ULL;   // versions of the pair and of the copies made from it
+    float absmax = 0.0f;        // largest |value| of the prepared Float32 pair (FP16 operand scale)
+    uint64_t absmax_gen = ~0ULL;
 
     double last_ms = 0, last_ms_box = 0, last_ms_track = 0, last_ms_refine = 0;
     int32_t last_path = -1;
@@ -61,5 +63,9 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
                         int32_t measure, float* dx, float* dy, float* corr, uint8_t* srch);
 ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t padx,
                         int32_t** b1, int32_t** b2, int64_t* ld);
+namespace ar { struct TrackArgs; }
+// mixed-precision pass set-up (track_mp.hip): operand scales, Float64 box images, kernel choice
+ar_status ar_mp_setup(ar_context* ctx, ar::TrackArgs& a, void (**kern)(ar::TrackArgs),
+                      size_t* lds);
 ar_status ar_upload_points(ar_context* ctx, const ar_points* pts, int64_t n, ar_points* dp,
                            int64_t csx_fill, int64_t csy_fill);

@@ -86,4 +86,55 @@ __host__ __device__ inline GenScratch gen_scratch(int max_ch, int max_cw, int ma
     return G;
 }
 
+// ---- mixed-precision tracker (BF16 / FP16 operands, FP32 accumulation)
+
+// Band heights (in 8-row blocks, ceil((ch + 15) / 8) rounded up) with a dedicated MFMA loop;
+// chips up to 128 rows. 0: the chip is too tall for the mixed-precision kernel.
+__host__ __device__ inline int mp_nrb(int ch) {
+    const int need = (ch + 15 + 7) / 8;
+    const int opts[6] = {4, 6, 8, 10, 14, 18};
+    for (int k = 0; k < 6; k++)
+        if (need <= opts[k]) return opts[k];
+    return 0;
+}
+
+struct MpLayout {
+    int nti, ntj, nbt;   // row-shift tiles, column-shift tiles, batches of 4 row-shift tiles
+    int ldw;             // window column pitch in 16-bit elements (an odd multiple of 8)
+    int cols;            // window columns stored: every column the tile loop reads
+    int cpad;            // chip column pitch in elements; chip row u at 16 + u, zeros around it
+    int cw4;             // chip columns stored (a multiple of 4; the extra ones zero)
+    bool keep_chip;      // the Float32 mean-removed chip stays in LDS for the exact FP64 values
+    size_t off_w, off_num, off_c, off_f, total;   // byte offsets
+};
+
+// LDS of track_mp_kernel: window (16-bit, column-major, rows past the window zero), the shift
+// numerators (FP32, column-major over the row shift i), the 16-bit chip, and the Float32 chip
+// (staged first in the window's bytes when there is no room to keep it).
+__host__ __device__ inline MpLayout mp_layout(int max_ch, int max_cw, int max_rx, int max_ry,
+                                             int nrb) {
+    MpLayout L;
+    L.nti = (2 * max_ry + 15) / 16;
+    L.ntj = (2 * max_rx + 15) / 16;
+    L.nbt = (L.nti + 3) / 4;
+    const int rows = 64 * L.nbt + 8 * nrb;
+    L.ldw = ((rows / 8) & 1) ? rows : rows + 8;
+    L.cw4 = (max_cw + 3) / 4 * 4;
+    L.cols = 16 * L.ntj + L.cw4;
+    L.cpad = 8 * nrb + 24;
+    const size_t wbytes = (size_t)L.cols * L.ldw * 2;
+    const size_t numbytes = (size_t)(64 * L.nbt) * (16 * L.ntj) * 4;
+    const size_t chipf = (size_t)max_ch * max_cw * 4;
+    L.off_w = 0;
+    L.off_num = (size_t)rup((int)wbytes, 16);
+    L.off_c = (size_t)rup((int)(L.off_num + numbytes), 16);
+    if (L.off_c < chipf) L.off_c = (size_t)rup((int)chipf, 16);
+    L.total = L.off_c + (size_t)L.cw4 * L.cpad * 2;
+    L.off_f = (size_t)rup((int)L.total, 16);
+    L.keep_chip = L.off_f + chipf <= 96 * 1024;
+    if (L.keep_chip) L.total = L.off_f + chipf;
+    else L.off_f = 0;
+    return L;
+}
+
 }  // namespace ar

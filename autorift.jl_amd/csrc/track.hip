@@ -30,44 +30,6 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 // shared tail: peak, refinement, outputs
 // ---------------------------------------------------------------------------------------------
 
-__device__ inline void write_skip(const TrackArgs& a, int64_t p, bool searched) {
-    if (threadIdx.x == 0) {
-        a.dx[p] = __int_as_float(0x7fc00000);
-        a.dy[p] = __int_as_float(0x7fc00000);
-        a.corr[p] = __int_as_float(0x7fc00000);
-        a.searched[p] = searched ? 1 : 0;
-        a.rmeta[p] = -1;
-    }
-}
-
-// Offsets of src/peak.jl:102-106 and src/peak.jl:515-521, plus the prior (src/track.jl:293-295).
-__device__ inline void write_result(const TrackArgs& a, int64_t p, const Geom& g, double ddx,
-                                    double ddy, float val) {
-    if (threadIdx.x == 0) {
-        a.dx[p] = (float)(ddx + a.dx_prior[p]);
-        a.dy[p] = (float)(ddy + a.dy_prior[p]);
-        a.corr[p] = val;
-        a.searched[p] = 1;
-    }
-}
-
-// After the integer peak (pi, pj) (1-based) and its exact value are known: without refinement,
-// write the point's result; with it, hand the 5x5 patch (exact values; thread t < 25 holds
-// element t, row-major) and its origin to the refinement kernel.
-__device__ inline void finish_point(const TrackArgs& a, int64_t p, const Geom& g, int pi, int pj,
-                                    float pval, int i0, int j0, bool refine, float pv) {
-    if (!refine) {
-        write_result(a, p, g, (double)(pj - g.rx - 1), (double)(pi - g.ry - 1), pval);
-        if (threadIdx.x == 0) a.rmeta[p] = -1;
-        return;
-    }
-    if (threadIdx.x < 25) a.rpatch[32 * p + threadIdx.x] = pv;
-    if (threadIdx.x == 0) {
-        a.rmeta[p] = i0 | (j0 << 16);
-        a.searched[p] = 1;
-    }
-}
-
 // The pyrUp cascade of every point the correlation kernel handed over (src/peak.jl:456-522),
 // with the offsets of src/peak.jl:515-521 and the prior. One wave per workgroup, one point at a
 // time; no workgroup barriers, so many points are in flight on each CU.
@@ -93,16 +55,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void re
         write_result(a, p, g, col - (double)g.rx, row - (double)g.ry, c.val);
         wave_sync();
     }
-}
-
-// Patch origin of src/peak.jl:473-478 (1-based); returns false when the surface is smaller than
-// the patch, in which case the integer peak is reported.
-__device__ inline bool patch_origin(int pi, int pj, int nr, int nc, int& i0, int& j0) {
-    int hr = nr - 5 + 1 > 1 ? nr - 5 + 1 : 1;
-    int hc = nc - 5 + 1 > 1 ? nc - 5 + 1 : 1;
-    i0 = pi - 2 < 1 ? 1 : (pi - 2 > hr ? hr : pi - 2);
-    j0 = pj - 2 < 1 ? 1 : (pj - 2 > hc ? hc : pj - 2);
-    return !(nr < 5 || nc < 5);
 }
 
 // ---------------------------------------------------------------------------------------------

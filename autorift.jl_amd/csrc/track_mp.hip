@@ -1,0 +1,415 @@
+// Mixed-precision tracker for Float32 prepared images (quantize = :none).
+//
+// The correlation contraction sum(chip' * W) runs on the matrix cores with BF16 or FP16 operands
+// and FP32 accumulation (v_mfma_f32_16x16x32_{bf16,f16}); everything around it keeps the
+// reference's precision:
+//   * the chip mean and energy are the reference's sequential Float64 sums and the chip is
+//     Float32(x - mean) (src/correlate.jl:423-453) before it is rounded to the 16-bit type;
+//   * the box sums S1 = sum W and S2 = sum W^2 come from Float64 box images of the pass
+//     (box_f64_colpass_kernel / box_f64_rowpass_kernel), and the ZNCC / NCC normalisation is the
+//     reference's Float64 formula (src/correlate.jl:575-630);
+//   * the integer peak's value and the 5x5 patch the subpixel fit starts from are recomputed
+//     with the reference's Float64 direct numerator in its own loop order
+//     (src/correlate.jl:799-814), so the pyrUp cascade (src/peak.jl:456-522) starts from
+//     full-precision samples.
+// The argmax over the low-precision surface follows peak_index's rule (src/peak.jl:32-61).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "ar_internal.h"
+#include "layout.h"
+#include "track_common.h"
+
+namespace ar {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+
+// Operand type traits: Float32 -> 16-bit bits (round to nearest even) and the MFMA.
+template <int PREC> struct MpType;
+template <> struct MpType<AR_PREC_BF16> {
+    static __device__ inline uint16_t bits(float v) {
+        return __builtin_bit_cast(uint16_t, (__bf16)v);
+    }
+    static __device__ inline v4f mfma(v4i a, v4i b, v4f c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),
+                                                      __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
+    }
+};
+template <> struct MpType<AR_PREC_FP16> {
+    static __device__ inline uint16_t bits(float v) {
+        return __builtin_bit_cast(uint16_t, (_Float16)v);
+    }
+    static __device__ inline v4f mfma(v4i a, v4i b, v4f c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, a),
+                                                     __builtin_bit_cast(v8h, b), c, 0, 0, 0);
+    }
+};
+
+// Prepared Float32 pixel at padded 1-based (r, c); 0 in the padding.
+__device__ inline float mp_pix(const TrackArgs& a, const float* img, int64_t r, int64_t c) {
+    const int64_t k = pix(a, r, c);
+    return k < 0 ? 0.0f : img[k];
+}
+
+// sum chip' * W of one batch of 4 row-shift tiles (ti0 .. ti0 + 3) of column-shift tile tj.
+// Output rows are column shifts j, output columns row shifts i; K = 32 is four chip columns
+// (lane group g16 carries chip column v = 4 m + g16) times 8 window rows. The B fragment of
+// (v, band block rb) -- chip rows 8 rb - l16 .. 8 rb - l16 + 7 -- does not depend on the tile,
+// and the A fragment of window rows 16 ti0 + 8 rho .. + 7 serves every (tile t, rb) with
+// rho = 2 t + rb: per chip-column quad NRB B and NRB + 6 A fragments feed 4 NRB MFMAs.
+template <int PREC, int NRB>
+__device__ inline void mp_batch(const uint16_t* Wl, const uint16_t* Cl, int ldw, int cpad, int cw,
+                                int tj, int ti0, int l16, int g16, v4f (&acc)[4]) {
+    const int nmq = (cw + 3) >> 2;
+    int aoff = (16 * tj + l16 + g16) * ldw + 16 * ti0;   // elements
+    int boff = g16 * cpad + 16 - l16;                    // elements
+    const uint32_t sb = (uint32_t)(boff & 1) * 2;        // byte shift, the same for every m, rb
+    for (int m = 0; m < nmq; m++) {
+        v4i bf[NRB];
+#pragma unroll
+        for (int rb = 0; rb < NRB; rb++) {
+            const uint32_t* bp = (const uint32_t*)(Cl + ((boff + 8 * rb) & ~1));
+            const uint32_t d0 = bp[0], d1 = bp[1], d2 = bp[2], d3 = bp[3], d4 = bp[4];
+            bf[rb] = (v4i){(int)__builtin_amdgcn_alignbyte(d1, d0, sb),
+                           (int)__builtin_amdgcn_alignbyte(d2, d1, sb),
+                           (int)__builtin_amdgcn_alignbyte(d3, d2, sb),
+                           (int)__builtin_amdgcn_alignbyte(d4, d3, sb)};
+        }
+        const uint16_t* ap = Wl + aoff;
+#pragma unroll
+        for (int rho = 0; rho < NRB + 6; rho++) {
+            const v4i af = *(const v4i*)(ap + 8 * rho);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int rb = rho - 2 * t;
+                if (rb >= 0 && rb < NRB) acc[t] = MpType<PREC>::mfma(af, bf[rb], acc[t]);
+            }
+        }
+        aoff += 4 * ldw;
+        boff += 4 * cpad;
+    }
+}
+
+// The reference's normalisation (src/correlate.jl:591-600 ZNCC, :618-627 NCC) of a numerator.
+__device__ inline float mp_normalise(double num, double s1, double s2, double n, double cnorm,
+                                     int measure) {
+    double den;
+    if (measure == 1) {
+        den = cnorm * sqrt(s2 < 0.0 ? 0.0 : s2);
+    } else {
+        const double wv = s2 - s1 * s1 / n;
+        den = cnorm * sqrt(wv < 0.0 ? 0.0 : wv);
+    }
+    return den > 0 ? (float)(num / den) : 0.0f;
+}
+
+template <int PREC, int NRB>
+__global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double shd[2];
+    __shared__ float pk;
+    __shared__ Best red[AR_NWAVES];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const MpLayout L = mp_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, NRB);
+    uint16_t* Wl = (uint16_t*)(smem + L.off_w);
+    float* num = (float*)(smem + L.off_num);
+    uint16_t* Cl = (uint16_t*)(smem + L.off_c);
+    float* chipf = (float*)(smem + (L.keep_chip ? L.off_f : L.off_w));
+    const float* ref = (const float*)a.ref;
+    const float* sec = (const float*)a.sec;
+    const int g16 = lane >> 4, l16 = lane & 15;
+
+    const PointRange pr = point_range(a.n);
+    for (int64_t p = pr.first; p < pr.end; p += pr.step) {
+        const Geom g = point_geom(a, p);
+        if (!g.status || !any_valid(a, g)) { write_skip(a, p, false); continue; }
+        const int ch = g.ch, cw = g.cw, NR = 2 * g.ry, NC = 2 * g.rx;
+        const int winh = ch + NR - 1, winw = cw + NC - 1;
+
+        // ---- chip: Float32 samples, then the reference's sequential Float64 mean and energy
+        for (int v = wave; v < cw; v += AR_NWAVES)
+            for (int u = lane; u < ch; u += 64) chipf[u + v * ch] = mp_pix(a, sec, g.cr0 + u, g.cc0 + v);
+        __syncthreads();
+        if (tid == 0) {
+            double s = 0.0;
+            for (int k = 0; k < ch * cw; k++) s += (double)chipf[k];
+            const double mean = s / (double)(ch * cw);
+            double sq = 0.0;
+            for (int k = 0; k < ch * cw; k++) {
+                const double d = (double)chipf[k] - mean;
+                sq += d * d;
+            }
+            shd[0] = mean;
+            shd[1] = sq;
+        }
+        __syncthreads();
+        const double mean = shd[0], sq = shd[1];
+        if (!(sq > 2.220446049250313e-16 * (double)ch * (double)cw)) {   // degenerate chip
+            write_skip(a, p, true);
+            __syncthreads();
+            continue;
+        }
+        const double cnorm = sqrt(sq);
+        // mean-removed chip Float32(x - mean) (kept in place when it has its own bytes), and its
+        // scaled 16-bit copy in full padded columns
+        for (int k = tid; k < L.cw4 * L.cpad; k += AR_WG) {
+            const int v = k / L.cpad, u = k - v * L.cpad - 16;
+            float c = 0.0f;
+            if (v < cw && u >= 0 && u < ch) c = (float)((double)chipf[u + v * ch] - mean);
+            Cl[k] = MpType<PREC>::bits(c * a.cscale);
+        }
+        __syncthreads();
+        if (L.keep_chip)
+            for (int k = tid; k < ch * cw; k += AR_WG) chipf[k] = (float)((double)chipf[k] - mean);
+        // ---- window, scaled to the 16-bit type; rows and columns past it zero
+        for (int c = wave; c < L.cols; c += AR_NWAVES)
+            for (int r = lane; r < L.ldw; r += 64) {
+                float w = 0.0f;
+                if (c < winw && r < winh) w = mp_pix(a, ref, g.wr0 + r, g.wc0 + c);
+                Wl[c * L.ldw + r] = MpType<PREC>::bits(w * a.wscale);
+            }
+        __syncthreads();
+
+        // ---- numerators on the matrix cores
+        const int nti = (NR + 15) >> 4, ntj = (NC + 15) >> 4, nbt = (nti + 3) >> 2;
+        for (int item = wave; item < ntj * nbt; item += AR_NWAVES) {
+            const int tj = item % ntj, ti0 = 4 * (item / ntj);
+            v4f acc[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[t] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+            mp_batch<PREC, NRB>(Wl, Cl, L.ldw, L.cpad, cw, tj, ti0, l16, g16, acc);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int i = 16 * (ti0 + t) + l16;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int j = 16 * tj + 4 * g16 + r;
+                    if (i < NR && j < NC) num[j * NR + i] = acc[t][r];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- Float64 normalisation per shift and the argmax (peak_index rule)
+        const double n = (double)(ch * cw);
+        const int64_t bb = (g.wr0 - 1) + (g.wc0 - 1) * a.box_ld;
+        Best b = best_empty();
+        for (int o = tid; o < NR * NC; o += AR_WG) {
+            const int j = o / NR, i = o - j * NR;
+            const int64_t q = bb + i + (int64_t)j * a.box_ld;
+            const float v = mp_normalise((double)num[o] * a.numscale, a.fbox1[q], a.fbox2[q], n,
+                                         cnorm, a.measure);
+            const int idx = i * NC + j;
+            if (better(v, idx, b)) { b.v = v; b.idx = idx; }
+        }
+        b = block_best(b, red);
+        if (b.idx == 0x7fffffff) b.idx = 0;
+        const int pi = b.idx / NC + 1, pj = b.idx % NC + 1;
+        int i0 = 1, j0 = 1;
+        const bool refine = a.up > 1 && patch_origin(pi, pj, NR, NC, i0, j0);
+
+        // ---- exact Float64 values: the 5x5 patch (threads 0..24, row-major) or the peak (25)
+        float pv = 0.0f;
+        if (tid < 26 && (tid < 25) == refine) {
+            const int si = tid < 25 ? i0 - 1 + tid / 5 : pi - 1;
+            const int sj = tid < 25 ? j0 - 1 + tid % 5 : pj - 1;
+            double acc = 0.0;
+            for (int jj = 0; jj < cw; jj++)
+                for (int ii = 0; ii < ch; ii++) {
+                    const float c = L.keep_chip
+                                        ? chipf[ii + jj * ch]
+                                        : (float)((double)mp_pix(a, sec, g.cr0 + ii, g.cc0 + jj) - mean);
+                    const float w = mp_pix(a, ref, g.wr0 + si + ii, g.wc0 + sj + jj);
+                    acc += (double)c * (double)w;
+                }
+            const int64_t q = bb + si + (int64_t)sj * a.box_ld;
+            pv = mp_normalise(acc, a.fbox1[q], a.fbox2[q], n, cnorm, a.measure);
+            if (tid == 25) pk = pv;
+        }
+        __syncthreads();
+        finish_point(a, p, g, pi, pj, pk, i0, j0, refine, pv);
+        __syncthreads();
+    }
+}
+
+#define AR_MP_INST(P, N) template __global__ void track_mp_kernel<P, N>(TrackArgs);
+AR_MP_INST(AR_PREC_BF16, 4) AR_MP_INST(AR_PREC_BF16, 6) AR_MP_INST(AR_PREC_BF16, 8)
+AR_MP_INST(AR_PREC_BF16, 10) AR_MP_INST(AR_PREC_BF16, 14) AR_MP_INST(AR_PREC_BF16, 18)
+AR_MP_INST(AR_PREC_FP16, 4) AR_MP_INST(AR_PREC_FP16, 6) AR_MP_INST(AR_PREC_FP16, 8)
+AR_MP_INST(AR_PREC_FP16, 10) AR_MP_INST(AR_PREC_FP16, 14) AR_MP_INST(AR_PREC_FP16, 18)
+#undef AR_MP_INST
+
+// ---------------------------------------------------------------------------------------------
+// Float64 box images of the zero-padded Float32 reference over every chip position
+// ---------------------------------------------------------------------------------------------
+
+// Padded 0-based (r, c) -> pixel value, 0 in the padding.
+__device__ inline double boxf_pix(const float* img, int64_t ny, int64_t nx, int64_t pady,
+                                  int64_t padx, int64_t r, int64_t c) {
+    const int64_t orow = r - pady, ocol = c - padx;
+    if (orow < 0 || orow >= ny || ocol < 0 || ocol >= nx) return 0.0;
+    return (double)img[orow + ocol * ny];
+}
+
+#define AR_BOXF_BLK 64
+// V1 (r, c) = sum_{u < ch} W(r + u, c), V2 likewise of W^2, r < HB, every padded column c. One
+// thread per (column, block of 64 rows): a fresh sum at the block start, then a sliding window,
+// so rounding drifts over at most 64 steps.
+__global__ __launch_bounds__(256) void box_f64_colpass_kernel(const float* img, int64_t ny,
+                                                              int64_t nx, int64_t pady,
+                                                              int64_t padx, int ch, int64_t HB,
+                                                              int64_t NX, double* V1, double* V2) {
+    const int64_t nblk = (HB + AR_BOXF_BLK - 1) / AR_BOXF_BLK;
+    const int64_t total = nblk * NX;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = t / nblk, r0 = (t - c * nblk) * AR_BOXF_BLK;
+        const int64_t r1 = r0 + AR_BOXF_BLK < HB ? r0 + AR_BOXF_BLK : HB;
+        double s1 = 0.0, s2 = 0.0;
+        for (int u = 0; u < ch; u++) {
+            const double w = boxf_pix(img, ny, nx, pady, padx, r0 + u, c);
+            s1 += w;
+            s2 += w * w;
+        }
+        for (int64_t r = r0; r < r1; r++) {
+            V1[r + c * HB] = s1;
+            V2[r + c * HB] = s2;
+            if (r + 1 < r1) {
+                const double wi = boxf_pix(img, ny, nx, pady, padx, r + ch, c);
+                const double wo = boxf_pix(img, ny, nx, pady, padx, r, c);
+                s1 = (s1 + wi) - wo;
+                s2 = (s2 + wi * wi) - wo * wo;
+            }
+        }
+    }
+}
+
+// B (r, c) = sum_{v < cw} V(r, c + v), c < WB. One thread per (row, block of 64 columns);
+// consecutive threads take consecutive rows.
+__global__ __launch_bounds__(256) void box_f64_rowpass_kernel(const double* V1, const double* V2,
+                                                              int cw, int64_t HB, int64_t WB,
+                                                              double* B1, double* B2) {
+    const int64_t nblk = (WB + AR_BOXF_BLK - 1) / AR_BOXF_BLK;
+    const int64_t total = nblk * HB;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t blk = t / HB, r = t - blk * HB;
+        const int64_t c0 = blk * AR_BOXF_BLK;
+        const int64_t c1 = c0 + AR_BOXF_BLK < WB ? c0 + AR_BOXF_BLK : WB;
+        double s1 = 0.0, s2 = 0.0;
+        for (int v = 0; v < cw; v++) {
+            s1 += V1[r + (c0 + v) * HB];
+            s2 += V2[r + (c0 + v) * HB];
+        }
+        for (int64_t c = c0; c < c1; c++) {
+            B1[r + c * HB] = s1;
+            B2[r + c * HB] = s2;
+            if (c + 1 < c1) {
+                s1 = (s1 + V1[r + (c + cw) * HB]) - V1[r + c * HB];
+                s2 = (s2 + V2[r + (c + cw) * HB]) - V2[r + c * HB];
+            }
+        }
+    }
+}
+
+// Largest |value| of a Float32 image (non-finite values excluded), as the bits of a
+// non-negative float (ordered like unsigned integers) in *out, which starts at 0.
+__global__ __launch_bounds__(256) void absmax_f32_kernel(const float* img, int64_t n,
+                                                         unsigned int* out) {
+    unsigned int m = 0;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const float v = fabsf(img[k]);
+        if (v <= 3.402823466e38f) m = max(m, __float_as_uint(v));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned int)__shfl_xor((int)m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+}  // namespace ar
+
+using namespace ar;
+
+static int64_t mp_grid(int64_t work) {
+    int64_t b = (work + 255) / 256;
+    return b < 1 ? 1 : (b > 65536 ? 65536 : b);
+}
+
+// Set-up of a mixed-precision pass: operand scales, the pass's Float64 box images and the
+// kernel instance. `a` already holds the pass geometry (max extents, padding).
+ar_status ar_mp_setup(ar_context* ctx, TrackArgs& a, void (**kern)(TrackArgs), size_t* lds) {
+    const int prec = ctx->params.precision;
+    const int nrb = mp_nrb(a.max_ch);
+    if (nrb == 0) return AR_ERR_UNSUPPORTED;
+    const MpLayout L = mp_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, nrb);
+    if (L.total > 160 * 1024 - 2048) return AR_ERR_UNSUPPORTED;
+    *lds = L.total;
+    // operand scales: FP16 keeps |W| <= 2^14 and |chip'| <= 2^14 (|chip'| <= 2 max|W|)
+    float wscale = 1.0f, cscale = 1.0f;
+    double numscale = 1.0;
+    if (prec == AR_PREC_FP16) {
+        if (ctx->absmax_gen != ctx->ref_gen) {
+            if (ar_status st = ctx->flags.reserve(ctx, 64)) return st;
+            if (hipMemsetAsync(ctx->flags.ptr, 0, 4, ctx->stream) != hipSuccess) return AR_ERR_DEVICE;
+            const int64_t N = ctx->ny * ctx->nx;
+            absmax_f32_kernel<<<mp_grid(N), 256, 0, ctx->stream>>>((const float*)ctx->pref.ptr, N,
+                                                                   (unsigned int*)ctx->flags.ptr);
+            absmax_f32_kernel<<<mp_grid(N), 256, 0, ctx->stream>>>((const float*)ctx->psec.ptr, N,
+                                                                   (unsigned int*)ctx->flags.ptr);
+            if (hipGetLastError() != hipSuccess) return AR_ERR_DEVICE;
+            unsigned int bits = 0;
+            if (hipMemcpyAsync(&bits, ctx->flags.ptr, 4, hipMemcpyDeviceToHost, ctx->stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess)
+                return AR_ERR_DEVICE;
+            float m;
+            std::memcpy(&m, &bits, 4);
+            ctx->absmax = m;
+            ctx->absmax_gen = ctx->ref_gen;
+        }
+        int e = 0;
+        if (ctx->absmax > 0.0f) {
+            std::frexp(ctx->absmax, &e);   // absmax < 2^e
+            e = e > 14 ? e - 14 : 0;
+        }
+        wscale = std::ldexp(1.0f, -e);
+        cscale = std::ldexp(1.0f, -(e + 1));
+        numscale = std::ldexp(1.0, 2 * e + 1);
+    }
+    a.wscale = wscale;
+    a.cscale = cscale;
+    a.numscale = numscale;
+    // Float64 box images over the padded image's chip positions
+    const int64_t NY = ctx->ny + 2 * a.pady, NX = ctx->nx + 2 * a.padx;
+    const int64_t HB = NY - a.max_ch + 1, WB = NX - a.max_cw + 1;
+    if (HB < 1 || WB < 1) return AR_ERR_UNSUPPORTED;
+    const size_t vbytes = (size_t)HB * NX * 8, bbytes = (size_t)HB * WB * 8;
+    if (ar_status st = ctx->box.reserve(ctx, 2 * vbytes + 2 * bbytes)) return st;
+    double* V1 = (double*)ctx->box.ptr;
+    double* V2 = (double*)((char*)ctx->box.ptr + vbytes);
+    double* B1 = (double*)((char*)ctx->box.ptr + 2 * vbytes);
+    double* B2 = (double*)((char*)ctx->box.ptr + 2 * vbytes + bbytes);
+    const int64_t cwork = NX * ((HB + AR_BOXF_BLK - 1) / AR_BOXF_BLK);
+    const int64_t rwork = HB * ((WB + AR_BOXF_BLK - 1) / AR_BOXF_BLK);
+    box_f64_colpass_kernel<<<mp_grid(cwork), 256, 0, ctx->stream>>>(
+        (const float*)ctx->pref.ptr, ctx->ny, ctx->nx, a.pady, a.padx, a.max_ch, HB, NX, V1, V2);
+    box_f64_rowpass_kernel<<<mp_grid(rwork), 256, 0, ctx->stream>>>(V1, V2, a.max_cw, HB, WB, B1, B2);
+    if (hipGetLastError() != hipSuccess) return AR_ERR_DEVICE;
+    a.fbox1 = B1;
+    a.fbox2 = B2;
+    a.box_ld = HB;
+#define AR_MP_PICK(P)                                                          \
+    *kern = nrb == 4 ? track_mp_kernel<P, 4> : nrb == 6 ? track_mp_kernel<P, 6> \
+          : nrb == 8 ? track_mp_kernel<P, 8> : nrb == 10 ? track_mp_kernel<P, 10> \
+          : nrb == 14 ? track_mp_kernel<P, 14> : track_mp_kernel<P, 18>;
+    if (prec == AR_PREC_BF16) { AR_MP_PICK(AR_PREC_BF16) }
+    else { AR_MP_PICK(AR_PREC_FP16) }
+#undef AR_MP_PICK
+    return AR_OK;
+}

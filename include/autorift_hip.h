@@ -38,6 +38,12 @@ typedef enum { AR_U8 = 0, AR_F32 = 1, AR_F64 = 2, AR_I16 = 3 } ar_dtype;
 typedef enum { AR_ZNCC = 0, AR_NCC = 1 } ar_measure;
 typedef enum { AR_PRE_NONE = 0, AR_PRE_HIGHPASS = 1 } ar_preprocess;
 typedef enum { AR_QUANT_NONE = 0, AR_QUANT_UINT8 = 1 } ar_quantize;
+/* Arithmetic of the correlation contraction for Float32 prepared images (quantize = :none).
+ * EXACT follows the reference's Float64 direct numerator (src/correlate.jl:799-814); BF16 and
+ * FP16 run it on the matrix cores with 16-bit operands and FP32 accumulation, keeping Float64
+ * for the box sums, the normalisation and the integer-peak / subpixel-patch values. UInt8
+ * prepared images always take the exact integer path. No counterpart in the reference. */
+typedef enum { AR_PREC_EXACT = 0, AR_PREC_BF16 = 1, AR_PREC_FP16 = 2 } ar_precision;
 
 /* Flattened Params (src/types.jl:781-829) with the defaults of params() (src/params.jl:306-392).
  * Built by the host from its resolved parameter set; the library never parses strings. */
@@ -68,6 +74,8 @@ typedef struct {
     double dx_prior;              /* 0.0 */
     double dy_prior;              /* 0.0 */
     int64_t fill_window;          /* 3 */
+    int32_t precision;            /* ar_precision, default AR_PREC_EXACT (not in the reference) */
+    int32_t _pad1;
 } ar_params;
 
 /* PointSet fields (src/points.jl:50-74), structure of arrays of equal length. */
@@ -141,7 +149,7 @@ ar_status ar_correlate_point(ar_context* ctx, const void* search, const void* ch
 
 /* Device timing of the last ar_track / ar_track_device pass, in milliseconds (HIP events on
  * the context's stream: correlation kernel plus refinement kernel), and the correlation path it
- * took: 0 u8 matrix-core path, 1 generic. */
+ * took: 0 u8 matrix-core path, 1 generic Float64, 2 BF16 / FP16 matrix-core path. */
 ar_status ar_last_kernel(const ar_context* ctx, double* ms, int32_t* path);
 
 /* The same pass split by kernel, in milliseconds: the box images of W and W^2 (0 when the pass
