@@ -11,11 +11,12 @@ from .api import (Cache, ImagePair, autorift, autorift_, autorift_with_grid, chi
 from .core import Context, DisplacementField, MultichipResult
 from .points import (PointSet, chip_bounds, gridpoints, gridpoints_xy, pointset, sanitize_,
                      search_bounds)
+from .serve import Server, ServerError
 
 __all__ = [
     "AR_NCC", "AR_ZNCC", "ArgumentError", "AutoRIFTError", "DimensionMismatch", "NoDeviceError",
     "Params", "default_params", "lib", "Cache", "ImagePair", "autorift", "autorift_",
     "autorift_with_grid", "chip_sizes", "init", "params", "reinit_", "track", "Context",
     "DisplacementField", "MultichipResult", "PointSet", "chip_bounds", "gridpoints",
-    "gridpoints_xy", "pointset", "sanitize_", "search_bounds",
+    "gridpoints_xy", "pointset", "sanitize_", "search_bounds", "Server", "ServerError",
 ]

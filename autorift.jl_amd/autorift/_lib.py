@@ -89,6 +89,16 @@ SIGNATURES = [
                                  C.POINTER(C.c_double)]),
     ("ar_last_error", C.c_char_p, [C.c_void_p]),
     ("ar_version", C.c_char_p, []),
+    ("ar_server_create", C.c_int, [C.POINTER(Params), C.c_int32, C.c_int32, C.c_int64,
+                                   C.POINTER(C.c_void_p)]),
+    ("ar_server_destroy", None, [C.c_void_p]),
+    ("ar_server_submit", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_int64, C.c_int64, C.c_int32, C.POINTER(Points), C.c_int64,
+                                   C.c_int64, C.POINTER(C.c_int64)]),
+    ("ar_server_wait", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
+    ("ar_server_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    ("ar_server_last_error", C.c_char_p, [C.c_void_p]),
 ]
 
 _lib = None

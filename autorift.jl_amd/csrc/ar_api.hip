@@ -296,6 +296,24 @@ void ar_context_destroy(ar_context* ctx) {
 
 }  // extern "C"
 
+// Device bytes the context holds.
+size_t ar_context_bytes(ar_context* ctx) {
+    size_t t = 0;
+    for (DevBuf* b : ctx->all_bufs()) t += b->cap;
+    return t;
+}
+
+// Gives every device buffer back; the context keeps its stream and parameters and needs a new
+// pair before the next pass.
+void ar_context_release(ar_context* ctx) {
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (DevBuf* b : ctx->all_bufs()) b->release();
+    ctx->has_pair = false;
+    ctx->refx_gen = ~0ULL;
+    ctx->absmax_gen = ~0ULL;
+}
+
 // Upload masks (or derive them) and combine; sets ctx->all_valid.
 ar_status ar_upload_masks(ar_context* ctx, const uint8_t* ref_valid, const uint8_t* sec_valid,
                           int64_t N) {

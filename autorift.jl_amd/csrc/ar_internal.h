@@ -63,6 +63,9 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
                         int32_t measure, float* dx, float* dy, float* corr, uint8_t* srch);
 ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t padx,
                         int32_t** b1, int32_t** b2, int64_t* ld);
+// device bytes a context holds, and giving them all back (serving: idle slots make room)
+size_t ar_context_bytes(ar_context* ctx);
+void ar_context_release(ar_context* ctx);
 namespace ar { struct TrackArgs; }
 // mixed-precision pass set-up (track_mp.hip): operand scales, Float64 box images, kernel choice
 ar_status ar_mp_setup(ar_context* ctx, ar::TrackArgs& a, void (**kern)(ar::TrackArgs),

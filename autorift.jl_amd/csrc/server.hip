@@ -1,0 +1,349 @@
+// Serving loop: image pairs of different sizes arrive at any time; each is prepared and run
+// through the multi-chip search (src/api.jl:165-170 -> src/multichip.jl:106-137) on one of a
+// fixed set of slots. A slot is a worker thread with its own context (device buffers and HIP
+// stream), so pairs admitted to different slots overlap on the device; a slot keeps its device
+// buffers between pairs and reuses them.
+//
+// Admission (FIFO): the head of the queue goes to the first idle slot once the device memory it
+// needs fits the budget, counting what every other slot holds. A slot's need for a pair is the
+// larger of what it already holds and the pair's footprint estimate; when the head does not fit,
+// idle slots give their buffers back first. A pair whose footprint alone exceeds the budget is
+// refused at submission (AR_ERR_OUT_OF_MEMORY).
+//
+// The submitting thread's buffers are copied at submission (the library keeps no host pointer);
+// results wait in the server until ar_server_wait collects them.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "ar_internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    int64_t ticket = 0;
+    int64_t ny = 0, nx = 0, nr = 0, nc = 0;
+    int32_t dtype = 0;
+    std::vector<uint8_t> ref, sec, rmask, smask;
+    std::vector<double> x, y, pdx, pdy;
+    std::vector<int64_t> rx, ry;
+    size_t footprint = 0;
+    // results
+    std::vector<float> dx, dy, corr;
+    std::vector<uint16_t> cs;
+    std::vector<uint8_t> interp;
+    ar_status status = AR_OK;
+    std::string err;
+    bool done = false;
+    int slot = -1;
+    Clock::time_point t_submit, t_admit, t_done;
+};
+
+size_t elem_size(int32_t dtype) {
+    switch (dtype) {
+        case AR_U8: return 1;
+        case AR_I16: return 2;
+        case AR_F32: return 4;
+        case AR_F64: return 8;
+        default: return 0;
+    }
+}
+
+}  // namespace
+
+struct ar_server {
+    ar_params params;
+    int device = 0;
+    int64_t budget = 0;
+    std::vector<ar_context*> ctx;
+    std::vector<size_t> held;     // device bytes each slot holds (or has been granted)
+    std::vector<bool> busy;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::unordered_map<int64_t, Request*> reqs;
+    int64_t next_ticket = 1;
+    bool stop = false;
+    // counters
+    int64_t completed = 0, peak_running = 0, running = 0;
+    size_t peak_held = 0;
+    std::string err;
+};
+
+// Device bytes a context needs for one pair: raw upload and preparation work images, the
+// prepared pair and masks, box images (two int32 planes), the four byte-shifted int8 copies of
+// each prepared image, per-point buffers and the multi-chip grid planes, plus a fixed margin.
+// Pessimistic on purpose: the estimate only decides admission.
+static size_t pair_footprint(int64_t ny, int64_t nx, int32_t dtype, int64_t npts) {
+    const size_t N = (size_t)ny * (size_t)nx;
+    const size_t es = elem_size(dtype);
+    const size_t raw = 2 * N * es + 2 * N;          // raw pair and masks
+    const size_t work = N * (4 + 8);                // Float32 / Float64 preparation planes
+    const size_t prep = 2 * N * 4 + 3 * N;          // prepared pair (Float32 at most), masks
+    const size_t box = 2 * N * 8;                   // box images (Float64 for the 16-bit path)
+    const size_t copies = 8 * N + 4096;             // int8 copies of the UInt8 pair
+    const size_t pts = (size_t)npts * 512;          // point fields, outputs, hand-off, grid planes
+    return raw + work + prep + box + copies + pts + ((size_t)64 << 20);
+}
+
+static void worker(ar_server* S, int s) {
+    ar_context* ctx = S->ctx[s];
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (;;) {
+        Request* r = nullptr;
+        for (;;) {
+            if (S->stop) return;
+            if (!S->queue.empty()) {
+                Request* h = S->queue.front();
+                size_t others = 0;
+                for (size_t t = 0; t < S->held.size(); t++)
+                    if ((int)t != s) others += S->held[t];
+                const size_t need = h->footprint > S->held[s] ? h->footprint : S->held[s];
+                if (others + need > (size_t)S->budget) {
+                    // idle slots give their buffers back, then check again
+                    for (size_t t = 0; t < S->held.size(); t++)
+                        if ((int)t != s && !S->busy[t] && S->held[t] > 0) {
+                            ar_context_release(S->ctx[t]);
+                            S->held[t] = 0;
+                        }
+                    others = 0;
+                    for (size_t t = 0; t < S->held.size(); t++)
+                        if ((int)t != s) others += S->held[t];
+                }
+                if (others + need <= (size_t)S->budget) {
+                    r = h;
+                    S->queue.pop_front();
+                    S->held[s] = need;
+                    break;
+                }
+            }
+            S->cv_work.wait(lk);
+        }
+        S->busy[s] = true;
+        S->running++;
+        if (S->running > S->peak_running) S->peak_running = S->running;
+        size_t tot = 0;
+        for (size_t h : S->held) tot += h;
+        if (tot > S->peak_held) S->peak_held = tot;
+        r->slot = s;
+        r->t_admit = Clock::now();
+        lk.unlock();
+
+        // the pair: prepare on the device, then the multi-chip search
+        const int64_t n = r->nr * r->nc;
+        r->dx.resize(n);
+        r->dy.resize(n);
+        r->corr.resize(n);
+        r->cs.resize(n);
+        r->interp.resize(n);
+        ar_status st = ar_set_images(ctx, r->ref.data(), r->sec.data(),
+                                     r->rmask.empty() ? nullptr : r->rmask.data(),
+                                     r->smask.empty() ? nullptr : r->smask.data(), r->ny, r->nx,
+                                     r->dtype);
+        if (st == AR_OK) {
+            ar_points g;
+            g.x = r->x.data();
+            g.y = r->y.data();
+            g.dx_prior = r->pdx.data();
+            g.dy_prior = r->pdy.data();
+            g.radius_x = r->rx.data();
+            g.radius_y = r->ry.data();
+            g.chip_size_x = nullptr;
+            g.chip_size_y = nullptr;
+            st = ar_run_multichip(ctx, &g, r->nr, r->nc, r->dx.data(), r->dy.data(),
+                                  r->corr.data(), r->cs.data(), r->interp.data());
+        }
+        const size_t actual = ar_context_bytes(ctx);
+
+        lk.lock();
+        r->status = st;
+        if (st != AR_OK) r->err = ar_last_error(ctx);
+        // inputs are no longer needed
+        std::vector<uint8_t>().swap(r->ref);
+        std::vector<uint8_t>().swap(r->sec);
+        std::vector<uint8_t>().swap(r->rmask);
+        std::vector<uint8_t>().swap(r->smask);
+        r->t_done = Clock::now();
+        r->done = true;
+        S->held[s] = actual;
+        S->busy[s] = false;
+        S->running--;
+        S->completed++;
+        S->cv_done.notify_all();
+        S->cv_work.notify_all();
+    }
+}
+
+extern "C" {
+
+ar_status ar_server_create(const ar_params* p, int32_t device, int32_t slots, int64_t hbm_budget,
+                           ar_server** out) {
+    if (!out) return AR_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (slots < 1 || slots > 64) return AR_ERR_INVALID_ARGUMENT;
+    ar_server* S = new ar_server();
+    if (p) S->params = *p;
+    else ar_params_default(&S->params);
+    S->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete S;
+        return AR_ERR_NO_DEVICE;
+    }
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
+        delete S;
+        return AR_ERR_DEVICE;
+    }
+    // default budget: 90 % of the memory free now
+    S->budget = hbm_budget > 0 ? hbm_budget : (int64_t)(freeb / 10 * 9);
+    for (int s = 0; s < slots; s++) {
+        ar_context* c = nullptr;
+        ar_status st = ar_context_create(&S->params, device, &c);
+        if (st != AR_OK) {
+            for (ar_context* k : S->ctx) ar_context_destroy(k);
+            delete S;
+            return st;
+        }
+        S->ctx.push_back(c);
+    }
+    S->held.assign(slots, 0);
+    S->busy.assign(slots, false);
+    for (int s = 0; s < slots; s++) S->workers.emplace_back(worker, S, s);
+    *out = S;
+    return AR_OK;
+}
+
+void ar_server_destroy(ar_server* S) {
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    for (std::thread& t : S->workers) t.join();
+    for (ar_context* c : S->ctx) ar_context_destroy(c);
+    for (auto& kv : S->reqs) delete kv.second;
+    for (Request* r : S->queue) (void)r;   // owned through reqs
+    delete S;
+}
+
+ar_status ar_server_submit(ar_server* S, const void* ref, const void* sec, const uint8_t* ref_valid,
+                           const uint8_t* sec_valid, int64_t ny, int64_t nx, int32_t dtype,
+                           const ar_points* grid, int64_t nr, int64_t nc, int64_t* ticket) {
+    if (!S || !ref || !sec || !grid || !ticket) return AR_ERR_INVALID_ARGUMENT;
+    if (ny < 1 || nx < 1 || nr < 0 || nc < 0) return AR_ERR_INVALID_ARGUMENT;
+    const size_t es = elem_size(dtype);
+    if (es == 0) return AR_ERR_UNSUPPORTED;
+    if (!grid->x || !grid->y || !grid->radius_x || !grid->radius_y || !grid->dx_prior ||
+        !grid->dy_prior)
+        return AR_ERR_INVALID_ARGUMENT;
+    Request* r = new Request();
+    r->ny = ny;
+    r->nx = nx;
+    r->nr = nr;
+    r->nc = nc;
+    r->dtype = dtype;
+    const size_t N = (size_t)ny * nx;
+    const int64_t n = nr * nc;
+    r->footprint = pair_footprint(ny, nx, dtype, n);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (r->footprint > (size_t)S->budget) {
+            S->err = "pair footprint exceeds the server's device-memory budget";
+            delete r;
+            return AR_ERR_OUT_OF_MEMORY;
+        }
+    }
+    r->ref.assign((const uint8_t*)ref, (const uint8_t*)ref + N * es);
+    r->sec.assign((const uint8_t*)sec, (const uint8_t*)sec + N * es);
+    if (ref_valid) r->rmask.assign(ref_valid, ref_valid + N);
+    if (sec_valid) r->smask.assign(sec_valid, sec_valid + N);
+    r->x.assign(grid->x, grid->x + n);
+    r->y.assign(grid->y, grid->y + n);
+    r->pdx.assign(grid->dx_prior, grid->dx_prior + n);
+    r->pdy.assign(grid->dy_prior, grid->dy_prior + n);
+    r->rx.assign(grid->radius_x, grid->radius_x + n);
+    r->ry.assign(grid->radius_y, grid->radius_y + n);
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        r->ticket = S->next_ticket++;
+        S->reqs[r->ticket] = r;
+        S->queue.push_back(r);
+        *ticket = r->ticket;
+    }
+    S->cv_work.notify_all();
+    return AR_OK;
+}
+
+ar_status ar_server_wait(ar_server* S, int64_t ticket, float* dx, float* dy, float* corr,
+                         uint16_t* chip_size, uint8_t* interpolated, double* timing) {
+    if (!S) return AR_ERR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(ticket);
+    if (it == S->reqs.end()) {
+        S->err = "unknown ticket";
+        return AR_ERR_INVALID_ARGUMENT;
+    }
+    Request* r = it->second;
+    S->cv_done.wait(lk, [&] { return r->done; });
+    S->reqs.erase(it);
+    lk.unlock();
+    const size_t n = r->dx.size();
+    if (r->status == AR_OK) {
+        if (dx) std::memcpy(dx, r->dx.data(), n * 4);
+        if (dy) std::memcpy(dy, r->dy.data(), n * 4);
+        if (corr) std::memcpy(corr, r->corr.data(), n * 4);
+        if (chip_size) std::memcpy(chip_size, r->cs.data(), n * 2);
+        if (interpolated) std::memcpy(interpolated, r->interp.data(), n);
+    }
+    if (timing) {
+        using ms = std::chrono::duration<double, std::milli>;
+        timing[0] = ms(r->t_done - r->t_submit).count();   // latency
+        timing[1] = ms(r->t_admit - r->t_submit).count();  // queueing
+        timing[2] = ms(r->t_done - r->t_admit).count();    // service
+        timing[3] = (double)r->slot;
+    }
+    const ar_status st = r->status;
+    if (st != AR_OK) {
+        std::lock_guard<std::mutex> g(S->mu);
+        S->err = r->err;
+    }
+    delete r;
+    return st;
+}
+
+ar_status ar_server_stats(ar_server* S, int64_t* stats) {
+    if (!S || !stats) return AR_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(S->mu);
+    size_t tot = 0;
+    for (size_t h : S->held) tot += h;
+    stats[0] = S->completed;
+    stats[1] = (int64_t)S->queue.size();
+    stats[2] = S->running;
+    stats[3] = S->peak_running;
+    stats[4] = (int64_t)tot;
+    stats[5] = (int64_t)S->peak_held;
+    stats[6] = S->budget;
+    stats[7] = (int64_t)S->ctx.size();
+    return AR_OK;
+}
+
+const char* ar_server_last_error(ar_server* S) {
+    if (!S) return "null server";
+    std::lock_guard<std::mutex> lk(S->mu);
+    return S->err.c_str();
+}
+
+}  // extern "C"

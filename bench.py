@@ -109,6 +109,131 @@ def pair_latency(autorift, npairs, warm=2):
             "expected_dx_dy": [-3.30, 1.70]}
 
 
+def serve_pool(count, seed=6, lo=512, hi=10000):
+    """Pairs for the serving sweep: side log-uniform in [lo, hi]; blurred noise (sigma 1.5) scaled
+    to 0..255, the secondary translated by a random subpixel shift (|d| <= 15 px, bilinear)
+    plus N(0, 2^2) noise; Float32."""
+    from scipy import ndimage
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        a = ndimage.gaussian_filter(rng.standard_normal((n, n), dtype=np.float32), 1.5,
+                                    truncate=4.0, mode="mirror")
+        a -= a.min()
+        a *= 255.0 / a.max()
+        ang, mag = rng.uniform(0, 2 * np.pi), rng.uniform(0, 15)
+        d = (mag * np.sin(ang), mag * np.cos(ang))            # (dy, dx)
+        b = ndimage.shift(a, d, order=1, mode="nearest")
+        b += rng.normal(0, 2, b.shape).astype(np.float32)
+        out.append((a, b, d))
+    return out
+
+
+def serving_sweep(autorift, rates, nreq, slots, pool_size, seed=6):
+    """Poisson arrivals of pairs from the pool at each rate into one Server; per rate: p50 / p99
+    latency (submission to result), pairs/s, and the server's peak concurrency."""
+    import threading
+    t0 = time.time()
+    pool = serve_pool(pool_size, seed)
+    log(f"serving pool: sides {[a.shape[0] for a, _, _ in pool]} in {time.time() - t0:.1f}s")
+    p = autorift.params()
+    rng = np.random.default_rng(seed + 1)
+    res = {"slots": slots, "pool_sides": [int(a.shape[0]) for a, _, _ in pool], "rates": []}
+    with autorift.Server(p, slots=slots) as srv:
+        for a, b, _ in pool[:2]:          # warm the slots' buffers and kernels
+            srv.result(srv.submit(a, b))
+        for rate in rates:
+            order = rng.integers(0, len(pool), nreq)
+            gaps = rng.exponential(1.0 / rate, nreq)
+            tickets = [None] * nreq
+            ready = threading.Semaphore(0)
+
+            def submit():
+                t = time.perf_counter()
+                for k in range(nreq):
+                    t += gaps[k]
+                    d = t - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                    a, b, _ = pool[order[k]]
+                    tickets[k] = srv.submit(a, b)
+                    ready.release()
+
+            th = threading.Thread(target=submit)
+            start = time.perf_counter()
+            th.start()
+            lat, errs = [], []
+            for k in range(nreq):
+                ready.acquire()
+                r, tm = srv.result(tickets[k])
+                lat.append(tm["latency_ms"])
+                dy, dx = pool[order[k]][2]
+                ok = np.isfinite(r.dx)
+                if ok.any():   # outputs are secondary -> reference offsets
+                    errs.append(float(np.hypot(np.median(r.dx[ok]) + dx,
+                                               np.median(r.dy[ok]) + dy)))
+            th.join()
+            wall = time.perf_counter() - start
+            st = srv.stats()
+            lat = np.array(lat)
+            row = {"rate": rate, "requests": nreq, "p50_ms": float(np.percentile(lat, 50)),
+                   "p99_ms": float(np.percentile(lat, 99)), "pairs_per_s": nreq / wall,
+                   "peak_running": st["peak_running"],
+                   "median_shift_error_px": float(np.median(errs)) if errs else None}
+            res["rates"].append(row)
+            log(f"serve rate {rate}/s: p50 {row['p50_ms']:.1f} ms p99 {row['p99_ms']:.1f} ms, "
+                f"{row['pairs_per_s']:.1f} pairs/s, peak running {st['peak_running']}")
+        res["peak_bytes_held"] = srv.stats()["peak_bytes_held"]
+    return res
+
+
+def mixed_block(autorift, tex, sec, pts, steps):
+    """16-bit contraction on the Float32 pair (quantize = :none): throughput of the BF16 and FP16
+    matrix-core tracker over the full grid, the exact Float64 path on a sample of it, and the
+    agreement on the 512^2 shifted pair."""
+    out = {}
+    sample = pts[::7, ::7]           # every 7th grid row and column
+    for prec in ("bf16", "fp16", "exact"):
+        ctx = autorift.Context(autorift.params(quantize="none", precision=prec))
+        ctx.set_images(tex, sec)
+        use = pts if prec != "exact" else sample
+        ctx.track(use, upsampling=64)                 # warm-up (box images, kernels)
+        t = []
+        for _ in range(steps if prec != "exact" else 1):
+            r = ctx.track(use, upsampling=64)
+            t.append(ctx.last_kernel()[0])
+        ms = float(np.median(t))
+        out[prec] = {"points": len(use), "ms": ms,
+                     "points_per_s": int(np.count_nonzero(r.searched)) / (ms / 1000),
+                     "path": ctx.last_kernel()[1]}
+        ctx.close()
+        log(f"mixed {prec}: {out[prec]['points_per_s']:.0f} points/s over {len(use)} points")
+    out["speedup_bf16_vs_exact"] = out["bf16"]["points_per_s"] / out["exact"]["points_per_s"]
+    # agreement on the 512^2 pair shifted by (3.30, -1.70)
+    a, b = cfg1_pair(1)
+    xs = np.arange(40.0, 473.0, 16.0)
+    g = autorift.gridpoints_xy(xs, xs, chip_size=32, search_radius=25)
+    res = {}
+    for prec in ("exact", "bf16", "fp16"):
+        with autorift.Context(autorift.params(quantize="none", precision=prec)) as c:
+            c.set_images(a, b)
+            res[prec] = (c.track(g, upsampling=1), c.track(g, upsampling=64))
+    e1, e64 = res["exact"]
+    ok = np.isfinite(e64.dx)
+    for prec in ("bf16", "fp16"):
+        r1, r64 = res[prec]
+        same = (r1.dx[ok] == e1.dx[ok]) & (r1.dy[ok] == e1.dy[ok])
+        err = np.hypot(r64.dx[ok] - e64.dx[ok], r64.dy[ok] - e64.dy[ok])
+        tru = np.hypot(r64.dx[ok] + 3.30, r64.dy[ok] - 1.70)
+        out[prec]["cfg1_peak_agreement"] = float(same.mean())
+        out[prec]["cfg1_subpixel_rms_vs_fp64_px"] = float(np.sqrt(np.mean(err ** 2)))
+        out[prec]["cfg1_rms_vs_known_shift_px"] = float(np.sqrt(np.mean(tru ** 2)))
+    out["exact"]["cfg1_rms_vs_known_shift_px"] = float(
+        np.sqrt(np.mean(np.hypot(e64.dx[ok] + 3.30, e64.dy[ok] - 1.70) ** 2)))
+    return out
+
+
 def profiled_traffic():
     """HBM bytes per tracker launch from the committed counter summary (None if absent)."""
     try:
@@ -143,6 +268,12 @@ def main():
                     help="points in the CPU baseline sample (0: every grid point, ~20 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-pairs", type=int, default=20)
+    ap.add_argument("--serve-rates", default="2,10,50",
+                    help="Poisson arrival rates (pairs/s) of the serving sweep; empty: skip")
+    ap.add_argument("--serve-requests", type=int, default=30)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-pool", type=int, default=6)
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device copy of this many bytes (profiling calibration)")
     args = ap.parse_args()
@@ -170,7 +301,6 @@ def main():
     t0 = time.perf_counter()
     ctx.set_images(tex, sec)          # default pipeline on the device: highpass(5) + UInt8
     prep_s = time.perf_counter() - t0
-    del tex, sec
     ref8, sec8, _, _ = ctx.get_pair()
     log(f"device preparation (upload + highpass + quantize, both images) {prep_s:.2f}s")
 
@@ -209,6 +339,17 @@ def main():
     log(f"{npts} points, searched {searched}, {ms_step:.2f} ms/step, kernels {kavg:.2f} ms "
         f"(box_u8_kernel {bavg:.2f}, track_u8_kernel {tavg:.2f}, refine_kernel {ravg:.2f}), "
         f"measured {np.count_nonzero(~np.isnan(dx))}, path {ctx.last_kernel()[1]}")
+
+    mixed = None
+    if not args.no_mixed:
+        mixed = mixed_block(autorift, tex, sec, pts, min(args.steps, 3))
+    del tex, sec
+
+    serving = None
+    if args.serve_rates:
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        serving = serving_sweep(autorift, rates, args.serve_requests, args.serve_slots,
+                                args.serve_pool)
 
     lat = None
     if args.latency_pairs > 0:
@@ -262,6 +403,8 @@ def main():
                      "traffic_source": TRAFFIC_PROFILE if traffic is not None else None},
         "cpu_baseline": cpu,
         "pair_latency": lat,
+        "serving": serving,
+        "mixed_precision": mixed,
         "prepare_s": prep_s,
     }
     print(json.dumps(line), flush=True)

@@ -166,6 +166,35 @@ const char* ar_last_error(const ar_context* ctx);
 /* Library version string. */
 const char* ar_version(void);
 
+/* ---- Serving (no counterpart in the reference, which correlates one pair per call).
+ * A server owns `slots` contexts, each with its own worker thread and HIP stream. Pairs of any
+ * size are submitted at any time and run as init + autorift! (src/api.jl:66-76, 165-170):
+ * preparation, then the multi-chip search over the caller's grid. The head of the FIFO queue is
+ * admitted to an idle slot once its device-memory footprint fits `hbm_budget` (bytes; <= 0:
+ * 90 % of the memory free at creation) next to what the other slots hold; pairs on different
+ * slots overlap on the device. Input buffers are copied at submission. */
+typedef struct ar_server ar_server;
+ar_status ar_server_create(const ar_params* p, int32_t device, int32_t slots, int64_t hbm_budget,
+                           ar_server** out);
+/* Stops the workers after the pairs they are running; pairs still queued are dropped. */
+void ar_server_destroy(ar_server* s);
+/* Queues one pair with its gridded point set (nr x nc, column-major fields as for
+ * ar_run_multichip; chip sizes are set per level). AR_ERR_OUT_OF_MEMORY: the pair's footprint
+ * alone exceeds the budget (admission refused). */
+ar_status ar_server_submit(ar_server* s, const void* ref, const void* sec,
+                           const uint8_t* ref_valid, const uint8_t* sec_valid, int64_t ny,
+                           int64_t nx, int32_t dtype, const ar_points* grid, int64_t nr,
+                           int64_t nc, int64_t* ticket);
+/* Blocks until the pair of `ticket` is done and copies its MultichipResult planes out (any
+ * pointer may be NULL). timing (4 doubles, may be NULL): latency from submission to completion,
+ * time queued, time on its slot (ms), and the slot index. The ticket is consumed. */
+ar_status ar_server_wait(ar_server* s, int64_t ticket, float* dx, float* dy, float* corr,
+                         uint16_t* chip_size, uint8_t* interpolated, double* timing);
+/* 8 counters: completed, queued, running, peak running, device bytes held by the slots, peak
+ * bytes held, budget, slots. */
+ar_status ar_server_stats(ar_server* s, int64_t* stats);
+const char* ar_server_last_error(ar_server* s);
+
 #ifdef __cplusplus
 }
 #endif
