@@ -488,6 +488,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     // Float32 images with a 16-bit contraction requested: the mixed-precision kernel, which
     // carves its LDS for one chip size per pass
     const bool use_mp = !use_u8 && ctx->pdtype == AR_F32 && ctx->params.precision != AR_PREC_EXACT;
+    bool mp_box_timed = false;
     if (use_mp) {
         if (ctx->params.precision != AR_PREC_BF16 && ctx->params.precision != AR_PREC_FP16)
             return fail(ctx, AR_ERR_INVALID_ARGUMENT, "precision must be EXACT, BF16 or FP16");
@@ -495,6 +496,8 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
             return fail(ctx, AR_ERR_UNSUPPORTED,
                         "the BF16 / FP16 tracker needs one chip size per pass");
         size_t lds = 0;
+        HIPCHK(ctx, hipEventRecord(ctx->evb, ctx->stream));   // the box images count in the pass
+        mp_box_timed = true;
         st = ar_mp_setup(ctx, a, &kern, &lds);
         if (st == AR_ERR_UNSUPPORTED)
             return fail(ctx, st, "chip or search window too large for the BF16 / FP16 tracker");
@@ -505,7 +508,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         a.lds_bytes = 0;
         kern = ctx->pdtype == AR_U8 ? track_generic_kernel<1> : track_generic_kernel<0>;
     }
-    bool box_timed = false;
+    bool box_timed = mp_box_timed;
     const int bmode = box_images_mode();
     // Box images of W and W^2 replace the per-point box sums when every point has the same chip,
     // the points' shifts cover at least a quarter of the image's chip positions and there are at

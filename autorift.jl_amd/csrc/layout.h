@@ -104,13 +104,14 @@ struct MpLayout {
     int cols;            // window columns stored: every column the tile loop reads
     int cpad;            // chip column pitch in elements; chip row u at 16 + u, zeros around it
     int cw4;             // chip columns stored (a multiple of 4; the extra ones zero)
-    bool keep_chip;      // the Float32 mean-removed chip stays in LDS for the exact FP64 values
-    size_t off_w, off_num, off_c, off_f, total;   // byte offsets
+    size_t off_w, off_num, off_c, off_pc, total;   // byte offsets
 };
 
 // LDS of track_mp_kernel: window (16-bit, column-major, rows past the window zero), the shift
-// numerators (FP32, column-major over the row shift i), the 16-bit chip, and the Float32 chip
-// (staged first in the window's bytes when there is no room to keep it).
+// numerators (FP32, column-major over the row shift i) and the 16-bit chip. The Float32 chip is
+// staged first in the window's bytes; after the argmax the LDS from offset 0 holds the Float32
+// window block under the 5x5 patch ((ch + 4) x (cw + 4)) and the mean-removed chip (at off_pc)
+// for the exact Float64 values.
 __host__ __device__ inline MpLayout mp_layout(int max_ch, int max_cw, int max_rx, int max_ry,
                                              int nrb) {
     MpLayout L;
@@ -128,12 +129,11 @@ __host__ __device__ inline MpLayout mp_layout(int max_ch, int max_cw, int max_rx
     L.off_w = 0;
     L.off_num = (size_t)rup((int)wbytes, 16);
     L.off_c = (size_t)rup((int)(L.off_num + numbytes), 16);
-    if (L.off_c < chipf) L.off_c = (size_t)rup((int)chipf, 16);
+    L.off_pc = (size_t)rup((max_ch + 4) * (max_cw + 4) * 4, 16);
+    const size_t post = L.off_pc + chipf;
+    if (L.off_c < chipf) L.off_c = (size_t)rup((int)chipf, 16);   // raw chip read while Cl is built
     L.total = L.off_c + (size_t)L.cw4 * L.cpad * 2;
-    L.off_f = (size_t)rup((int)L.total, 16);
-    L.keep_chip = L.off_f + chipf <= 96 * 1024;
-    if (L.keep_chip) L.total = L.off_f + chipf;
-    else L.off_f = 0;
+    if (L.total < post) L.total = post;   // the post-argmax region may cover the 16-bit chip
     return L;
 }
 

@@ -10,8 +10,9 @@
 // idle slots give their buffers back first. A pair whose footprint alone exceeds the budget is
 // refused at submission (AR_ERR_OUT_OF_MEMORY).
 //
-// The submitting thread's buffers are copied at submission (the library keeps no host pointer);
-// results wait in the server until ar_server_wait collects them.
+// The submitting thread's buffers are copied at submission (the library keeps no host pointer)
+// into a page-locked staging buffer from a pool the server keeps (16 GiB at most; beyond it,
+// pageable memory); results wait in the server until ar_server_wait collects them.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -34,7 +35,13 @@ struct Request {
     int64_t ticket = 0;
     int64_t ny = 0, nx = 0, nr = 0, nc = 0;
     int32_t dtype = 0;
-    std::vector<uint8_t> ref, sec, rmask, smask;
+    // images and masks, packed [ref | sec | ref_valid | sec_valid] in one staging buffer:
+    // page-locked (from the server's pool) when possible, so the upload runs at DMA speed
+    uint8_t* stage = nullptr;
+    size_t stage_bytes = 0;
+    int pinned = -1;                  // pool index, or -1: `spill` holds the bytes
+    std::vector<uint8_t> spill;
+    bool has_rmask = false, has_smask = false;
     std::vector<double> x, y, pdx, pdy;
     std::vector<int64_t> rx, ry;
     size_t footprint = 0;
@@ -61,8 +68,17 @@ size_t elem_size(int32_t dtype) {
 
 }  // namespace
 
+struct PinnedBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    bool used = false;
+};
+
 struct ar_server {
     ar_params params;
+    std::vector<PinnedBuf> pinned;    // page-locked staging buffers, reused across requests
+    size_t pinned_total = 0;
+    size_t pinned_limit = (size_t)16 << 30;
     int device = 0;
     int64_t budget = 0;
     std::vector<ar_context*> ctx;
@@ -147,9 +163,10 @@ static void worker(ar_server* S, int s) {
         r->corr.resize(n);
         r->cs.resize(n);
         r->interp.resize(n);
-        ar_status st = ar_set_images(ctx, r->ref.data(), r->sec.data(),
-                                     r->rmask.empty() ? nullptr : r->rmask.data(),
-                                     r->smask.empty() ? nullptr : r->smask.data(), r->ny, r->nx,
+        const size_t N = (size_t)r->ny * r->nx, ib = N * elem_size(r->dtype);
+        const uint8_t* base = r->stage;
+        ar_status st = ar_set_images(ctx, base, base + ib, r->has_rmask ? base + 2 * ib : nullptr,
+                                     r->has_smask ? base + 2 * ib + N : nullptr, r->ny, r->nx,
                                      r->dtype);
         if (st == AR_OK) {
             ar_points g;
@@ -169,11 +186,11 @@ static void worker(ar_server* S, int s) {
         lk.lock();
         r->status = st;
         if (st != AR_OK) r->err = ar_last_error(ctx);
-        // inputs are no longer needed
-        std::vector<uint8_t>().swap(r->ref);
-        std::vector<uint8_t>().swap(r->sec);
-        std::vector<uint8_t>().swap(r->rmask);
-        std::vector<uint8_t>().swap(r->smask);
+        // the inputs are no longer needed: the staging buffer goes back to the pool
+        if (r->pinned >= 0) S->pinned[r->pinned].used = false;
+        r->pinned = -1;
+        r->stage = nullptr;
+        std::vector<uint8_t>().swap(r->spill);
         r->t_done = Clock::now();
         r->done = true;
         S->held[s] = actual;
@@ -233,8 +250,8 @@ void ar_server_destroy(ar_server* S) {
     S->cv_work.notify_all();
     for (std::thread& t : S->workers) t.join();
     for (ar_context* c : S->ctx) ar_context_destroy(c);
-    for (auto& kv : S->reqs) delete kv.second;
-    for (Request* r : S->queue) (void)r;   // owned through reqs
+    for (auto& kv : S->reqs) delete kv.second;   // queued and finished requests alike
+    for (PinnedBuf& b : S->pinned) hipHostFree(b.ptr);
     delete S;
 }
 
@@ -265,10 +282,47 @@ ar_status ar_server_submit(ar_server* S, const void* ref, const void* sec, const
             return AR_ERR_OUT_OF_MEMORY;
         }
     }
-    r->ref.assign((const uint8_t*)ref, (const uint8_t*)ref + N * es);
-    r->sec.assign((const uint8_t*)sec, (const uint8_t*)sec + N * es);
-    if (ref_valid) r->rmask.assign(ref_valid, ref_valid + N);
-    if (sec_valid) r->smask.assign(sec_valid, sec_valid + N);
+    r->has_rmask = ref_valid != nullptr;
+    r->has_smask = sec_valid != nullptr;
+    r->stage_bytes = 2 * N * es + 2 * N;
+    {
+        // smallest free page-locked buffer that fits, else a new one within the pool limit
+        std::unique_lock<std::mutex> lk(S->mu);
+        int best = -1;
+        for (size_t k = 0; k < S->pinned.size(); k++)
+            if (!S->pinned[k].used && S->pinned[k].cap >= r->stage_bytes &&
+                (best < 0 || S->pinned[k].cap < S->pinned[best].cap))
+                best = (int)k;
+        if (best < 0 && S->pinned_total + r->stage_bytes <= S->pinned_limit) {
+            S->pinned_total += r->stage_bytes;
+            lk.unlock();
+            void* p = nullptr;
+            const bool ok = hipHostMalloc(&p, r->stage_bytes, hipHostMallocDefault) == hipSuccess;
+            lk.lock();
+            if (ok) {
+                PinnedBuf b;
+                b.ptr = p;
+                b.cap = r->stage_bytes;
+                S->pinned.push_back(b);
+                best = (int)S->pinned.size() - 1;
+            } else {
+                S->pinned_total -= r->stage_bytes;
+            }
+        }
+        if (best >= 0) {
+            S->pinned[best].used = true;
+            r->pinned = best;
+            r->stage = (uint8_t*)S->pinned[best].ptr;
+        }
+    }
+    if (!r->stage) {
+        r->spill.resize(r->stage_bytes);
+        r->stage = r->spill.data();
+    }
+    std::memcpy(r->stage, ref, N * es);
+    std::memcpy(r->stage + N * es, sec, N * es);
+    if (ref_valid) std::memcpy(r->stage + 2 * N * es, ref_valid, N);
+    if (sec_valid) std::memcpy(r->stage + 2 * N * es + N, sec_valid, N);
     r->x.assign(grid->x, grid->x + n);
     r->y.assign(grid->y, grid->y + n);
     r->pdx.assign(grid->dx_prior, grid->dx_prior + n);

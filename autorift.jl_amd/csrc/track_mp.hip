@@ -120,7 +120,9 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
     uint16_t* Wl = (uint16_t*)(smem + L.off_w);
     float* num = (float*)(smem + L.off_num);
     uint16_t* Cl = (uint16_t*)(smem + L.off_c);
-    float* chipf = (float*)(smem + (L.keep_chip ? L.off_f : L.off_w));
+    float* chipf = (float*)(smem + L.off_w);      // raw Float32 chip, before the window
+    float* pwin = (float*)(smem + L.off_w);       // after the argmax: window block under the patch
+    float* pchip = (float*)(smem + L.off_pc);     // and the mean-removed chip
     const float* ref = (const float*)a.ref;
     const float* sec = (const float*)a.sec;
     const int g16 = lane >> 4, l16 = lane & 15;
@@ -137,11 +139,16 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
             for (int u = lane; u < ch; u += 64) chipf[u + v * ch] = mp_pix(a, sec, g.cr0 + u, g.cc0 + v);
         __syncthreads();
         if (tid == 0) {
+            // sequential sums; ch * cw is a multiple of 4 (even extents), so the loads go in
+            // batches ahead of the dependent additions
+            const int nn = ch * cw;
             double s = 0.0;
-            for (int k = 0; k < ch * cw; k++) s += (double)chipf[k];
-            const double mean = s / (double)(ch * cw);
+#pragma unroll 8
+            for (int k = 0; k < nn; k++) s += (double)chipf[k];
+            const double mean = s / (double)nn;
             double sq = 0.0;
-            for (int k = 0; k < ch * cw; k++) {
+#pragma unroll 8
+            for (int k = 0; k < nn; k++) {
                 const double d = (double)chipf[k] - mean;
                 sq += d * d;
             }
@@ -156,8 +163,8 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
             continue;
         }
         const double cnorm = sqrt(sq);
-        // mean-removed chip Float32(x - mean) (kept in place when it has its own bytes), and its
-        // scaled 16-bit copy in full padded columns
+        // the mean-removed chip Float32(x - mean), scaled, as a 16-bit copy in full padded
+        // columns
         for (int k = tid; k < L.cw4 * L.cpad; k += AR_WG) {
             const int v = k / L.cpad, u = k - v * L.cpad - 16;
             float c = 0.0f;
@@ -165,8 +172,6 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
             Cl[k] = MpType<PREC>::bits(c * a.cscale);
         }
         __syncthreads();
-        if (L.keep_chip)
-            for (int k = tid; k < ch * cw; k += AR_WG) chipf[k] = (float)((double)chipf[k] - mean);
         // ---- window, scaled to the 16-bit type; rows and columns past it zero
         for (int c = wave; c < L.cols; c += AR_NWAVES)
             for (int r = lane; r < L.ldw; r += 64) {
@@ -214,20 +219,31 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
         int i0 = 1, j0 = 1;
         const bool refine = a.up > 1 && patch_origin(pi, pj, NR, NC, i0, j0);
 
-        // ---- exact Float64 values: the 5x5 patch (threads 0..24, row-major) or the peak (25)
+        // ---- exact Float64 values: the 5x5 patch (threads 0..24, row-major) or the peak (25).
+        // The window block under them (top-left shift (bi, bj)) and the mean-removed chip are
+        // staged in LDS first (the 16-bit window and the scores are dead by now).
+        const int bi = refine ? i0 - 1 : pi - 1, bj = refine ? j0 - 1 : pj - 1;
+        const int ph = ch + 4, pw = cw + 4;
+        for (int c = wave; c < pw; c += AR_NWAVES)
+            for (int r = lane; r < ph; r += 64)
+                pwin[r + c * ph] = (bi + r < winh && bj + c < winw)
+                                       ? mp_pix(a, ref, g.wr0 + bi + r, g.wc0 + bj + c)
+                                       : 0.0f;
+        for (int v = wave; v < cw; v += AR_NWAVES)
+            for (int u = lane; u < ch; u += 64)
+                pchip[u + v * ch] = (float)((double)mp_pix(a, sec, g.cr0 + u, g.cc0 + v) - mean);
+        __syncthreads();
         float pv = 0.0f;
         if (tid < 26 && (tid < 25) == refine) {
-            const int si = tid < 25 ? i0 - 1 + tid / 5 : pi - 1;
-            const int sj = tid < 25 ? j0 - 1 + tid % 5 : pj - 1;
+            const int di = tid < 25 ? tid / 5 : 0, dj = tid < 25 ? tid % 5 : 0;
             double acc = 0.0;
-            for (int jj = 0; jj < cw; jj++)
-                for (int ii = 0; ii < ch; ii++) {
-                    const float c = L.keep_chip
-                                        ? chipf[ii + jj * ch]
-                                        : (float)((double)mp_pix(a, sec, g.cr0 + ii, g.cc0 + jj) - mean);
-                    const float w = mp_pix(a, ref, g.wr0 + si + ii, g.wc0 + sj + jj);
-                    acc += (double)c * (double)w;
-                }
+            for (int jj = 0; jj < cw; jj++) {
+                const float* wc = pwin + (dj + jj) * ph + di;
+                const float* cc = pchip + jj * ch;
+#pragma unroll 8
+                for (int ii = 0; ii < ch; ii++) acc += (double)cc[ii] * (double)wc[ii];
+            }
+            const int si = bi + di, sj = bj + dj;
             const int64_t q = bb + si + (int64_t)sj * a.box_ld;
             pv = mp_normalise(acc, a.fbox1[q], a.fbox2[q], n, cnorm, a.measure);
             if (tid == 25) pk = pv;

@@ -126,7 +126,7 @@ def serve_pool(count, seed=6, lo=512, hi=10000):
         d = (mag * np.sin(ang), mag * np.cos(ang))            # (dy, dx)
         b = ndimage.shift(a, d, order=1, mode="nearest")
         b += rng.normal(0, 2, b.shape).astype(np.float32)
-        out.append((a, b, d))
+        out.append((np.asfortranarray(a), np.asfortranarray(b), d))   # the ABI's layout
     return out
 
 
@@ -202,7 +202,8 @@ def mixed_block(autorift, tex, sec, pts, steps):
         t = []
         for _ in range(steps if prec != "exact" else 1):
             r = ctx.track(use, upsampling=64)
-            t.append(ctx.last_kernel()[0])
+            box_ms = ctx.last_timing()[0]          # the pass's Float64 box images (mixed path)
+            t.append(ctx.last_kernel()[0] + box_ms)
         ms = float(np.median(t))
         out[prec] = {"points": len(use), "ms": ms,
                      "points_per_s": int(np.count_nonzero(r.searched)) / (ms / 1000),
