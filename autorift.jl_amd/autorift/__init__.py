@@ -11,6 +11,7 @@ from .api import (Cache, ImagePair, autorift, autorift_, autorift_with_grid, chi
 from .core import Context, DisplacementField, MultichipResult
 from .points import (PointSet, chip_bounds, gridpoints, gridpoints_xy, pointset, sanitize_,
                      search_bounds)
+from .batch import Batch
 from .serve import Server, ServerError
 
 __all__ = [
@@ -18,5 +19,5 @@ __all__ = [
     "Params", "default_params", "lib", "Cache", "ImagePair", "autorift", "autorift_",
     "autorift_with_grid", "chip_sizes", "init", "params", "reinit_", "track", "Context",
     "DisplacementField", "MultichipResult", "PointSet", "chip_bounds", "gridpoints",
-    "gridpoints_xy", "pointset", "sanitize_", "search_bounds", "Server", "ServerError",
+    "gridpoints_xy", "pointset", "sanitize_", "search_bounds", "Server", "ServerError", "Batch",
 ]

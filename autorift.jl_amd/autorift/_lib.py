@@ -17,6 +17,7 @@ AR_ERR_OUT_OF_MEMORY, AR_ERR_DEVICE, AR_ERR_UNSUPPORTED, AR_ERR_NO_DEVICE = 3, 4
 AR_U8, AR_F32, AR_F64, AR_I16 = 0, 1, 2, 3
 AR_ZNCC, AR_NCC = 0, 1
 AR_PREC_EXACT, AR_PREC_BF16, AR_PREC_FP16 = 0, 1, 2
+AR_BATCH_GRAPH, AR_BATCH_SPLIT_SET = 1, 2
 
 
 class AutoRIFTError(RuntimeError):
@@ -89,6 +90,18 @@ SIGNATURES = [
                                  C.POINTER(C.c_double)]),
     ("ar_last_error", C.c_char_p, [C.c_void_p]),
     ("ar_version", C.c_char_p, []),
+    ("ar_batch_create", C.c_int, [C.POINTER(Params), C.c_int32, C.POINTER(C.c_void_p)]),
+    ("ar_batch_destroy", None, [C.c_void_p]),
+    ("ar_batch_add_pair", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_int64, C.c_int64, C.c_int32, C.POINTER(C.c_int32)]),
+    ("ar_batch_track", C.c_int, [C.c_void_p, C.POINTER(Points), C.c_void_p, C.c_int64,
+                                 C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p]),
+    ("ar_batch_track_device", C.c_int, [C.c_void_p, C.POINTER(Points), C.c_void_p, C.c_int64,
+                                        C.c_int64, C.c_int32, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_int32]),
+    ("ar_batch_info", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    ("ar_batch_last_error", C.c_char_p, [C.c_void_p]),
     ("ar_server_create", C.c_int, [C.POINTER(Params), C.c_int32, C.c_int32, C.c_int64,
                                    C.POINTER(C.c_void_p)]),
     ("ar_server_destroy", None, [C.c_void_p]),

@@ -1,0 +1,122 @@
+"""Batched passes: many prepared pairs resident on the device and one tracking pass over points
+of all of them (the library's ar_batch). Each point names its pair; chip sizes and search radii
+may differ from point to point. There is no counterpart in the reference, which tracks one pair
+per call (src/track.jl:103-160); per point the work is that of `track`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Params, lib
+from .api import params as _params
+from .core import _DTYPES, DisplacementField, _fortran, _points_struct, _ptr
+
+
+class Batch:
+    """Resident UInt8-prepared pairs on `device` (the parameters must quantise to UInt8)."""
+
+    def __init__(self, p: Params | None = None, device: int = 0):
+        self.params = p if p is not None else _params()
+        self._h = C.c_void_p()
+        st = lib().ar_batch_create(C.byref(self.params), int(device), C.byref(self._h))
+        if st != _lib.AR_OK:
+            _lib.check(st, None)
+        self.shapes = []
+
+    def close(self):
+        if self._h:
+            lib().ar_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.shapes)
+
+    def _check(self, st):
+        if st == _lib.AR_OK:
+            return
+        msg = lib().ar_batch_last_error(self._h).decode(errors="replace")
+        if st == _lib.AR_ERR_INVALID_ARGUMENT:
+            raise _lib.ArgumentError(msg)
+        if st == _lib.AR_ERR_DIMENSION_MISMATCH:
+            raise _lib.DimensionMismatch(msg)
+        raise _lib.AutoRIFTError(st, msg)
+
+    def add_pair(self, reference, secondary, reference_valid=None, secondary_valid=None) -> int:
+        """Prepare a raw pair (as init does) and keep it; returns its pair index."""
+        ref = _fortran(reference)
+        sec = _fortran(secondary, ref.dtype)
+        if ref.shape != sec.shape:
+            raise _lib.DimensionMismatch(f"reference is {ref.shape} but secondary is {sec.shape}")
+        if ref.dtype not in _DTYPES:
+            raise _lib.ArgumentError(f"unsupported image element type {ref.dtype}")
+        rv = None if reference_valid is None else _fortran(reference_valid, np.uint8)
+        sv = None if secondary_valid is None else _fortran(secondary_valid, np.uint8)
+        k = C.c_int32()
+        ny, nx = ref.shape
+        self._check(lib().ar_batch_add_pair(self._h, _ptr(ref), _ptr(sec), _ptr(rv), _ptr(sv),
+                                            ny, nx, _DTYPES[ref.dtype], C.byref(k)))
+        self.shapes.append((ny, nx))
+        return k.value
+
+    def add_pair_device(self, ref_ptr: int, sec_ptr: int, ny: int, nx: int,
+                        dtype=_lib.AR_F32) -> int:
+        """As add_pair with the raw images already in device memory (column-major)."""
+        k = C.c_int32()
+        self._check(lib().ar_batch_add_pair(self._h, C.c_void_p(ref_ptr), C.c_void_p(sec_ptr),
+                                            None, None, ny, nx, dtype, C.byref(k)))
+        self.shapes.append((ny, nx))
+        return k.value
+
+    def track(self, pts, pair, upsampling: int | None = None, measure: int | None = None):
+        """One pass over every point of `pts` (any shape); pair[i] is point i's pair index."""
+        up = self.params.upsampling if upsampling is None else int(upsampling)
+        m = self.params.measure if measure is None else int(measure)
+        s, keep = _points_struct(pts)
+        shape = np.shape(pts.x)
+        n = int(np.prod(shape))
+        pr = np.ascontiguousarray(np.ravel(np.asarray(pair), order="F"), dtype=np.int32)
+        if pr.size != n:
+            raise _lib.DimensionMismatch(f"{pr.size} pair indices for {n} points")
+        dx = np.empty(n, np.float32)
+        dy = np.empty(n, np.float32)
+        cc = np.empty(n, np.float32)
+        sr = np.empty(n, np.uint8)
+        self._check(lib().ar_batch_track(self._h, C.byref(s), _ptr(pr), n, up, m, _ptr(dx),
+                                         _ptr(dy), _ptr(cc), _ptr(sr)))
+        r = lambda a: a.reshape(shape, order="F")
+        return DisplacementField(r(dx), r(dy), r(cc), r(sr).astype(bool))
+
+    def track_device(self, fields, pair_ptr: int, n: int, upsampling: int, measure: int, out,
+                     graph: bool = False, split: int | None = None):
+        """Device pointers: fields = (x, y, radius_x, radius_y, dx_prior, dy_prior, chip_size_x,
+        chip_size_y), out = (dx, dy, correlation, searched). graph: capture the pass and replay
+        it while the buffers (and the points' radii) stay the same."""
+        s = _lib.Points(*[C.c_void_p(f) for f in fields])
+        flags = (_lib.AR_BATCH_GRAPH if graph else 0)
+        if split is not None:
+            flags |= _lib.AR_BATCH_SPLIT_SET | (int(split) << 8)
+        self._check(lib().ar_batch_track_device(self._h, C.byref(s), C.c_void_p(pair_ptr), n,
+                                                upsampling, measure,
+                                                *[C.c_void_p(o) for o in out], flags))
+
+    def info(self) -> dict:
+        ms = C.c_double()
+        v = (C.c_int32 * 4)()
+        lib().ar_batch_info(self._h, C.byref(ms), v)
+        return {"ms": ms.value, "pairs": v[0], "split": v[1], "launches": v[2],
+                "replayed": bool(v[3])}

@@ -19,6 +19,7 @@
 namespace ar {
 
 // Defined in track.hip.
+template <bool BATCH>
 __global__ void track_u8_kernel(TrackArgs a);
 __global__ void refine_kernel(TrackArgs a);
 template <int IS_U8>
@@ -109,7 +110,9 @@ __global__ void fill_u8_kernel(uint8_t* out, uint8_t v, int64_t n) {
 // power of two (and). Each block reduces its points first and issues one atomic per result.
 #define AR_GEOM_SLOT 32
 __global__ __launch_bounds__(256) void pass_geometry_kernel(ar_points pts, int64_t n, int64_t ny,
-                                                            int64_t nx, unsigned long long* g) {
+                                                            int64_t nx, unsigned long long* g,
+                                                            const PairDesc* pairs,
+                                                            const int32_t* pair_of) {
     __shared__ unsigned long long red[4][10];
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long v[10] = {0, 0, 0, 0, 0, 0, ~0ULL, ~0ULL, 1, 1};
@@ -131,7 +134,9 @@ __global__ __launch_bounds__(256) void pass_geometry_kernel(ar_points pts, int64
         int64_t fx = (int64_t)floor(pts.x[i]), fy = (int64_t)floor(pts.y[i]);
         int64_t r0 = fy - hy - pry, r1 = fy + hy + pry - 2;
         int64_t c0 = fx - hx - prx, c1 = fx + hx + prx - 2;
-        if (!(r0 >= 1 && r1 <= ny && c0 >= 1 && c1 <= nx)) v[8] = 0;
+        const int64_t pny = pairs ? pairs[pair_of[i]].ny : ny;   // batched: the point's pair
+        const int64_t pnx = pairs ? pairs[pair_of[i]].nx : nx;
+        if (!(r0 >= 1 && r1 <= pny && c0 >= 1 && c1 <= pnx)) v[8] = 0;
         int64_t area = (2 * hy) * (2 * hx);
         if (!(area > 0 && (area & (area - 1)) == 0)) v[9] = 0;
     }
@@ -164,7 +169,13 @@ __global__ __launch_bounds__(256) void point_prep_kernel(TrackArgs a) {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (; p < a.n; p += (int64_t)gridDim.x * blockDim.x) {
         int32_t r[AR_PREC];
-        point_record(point_geom(a, p), r);
+        if (a.pairs) {   // batched pass: bounds against the point's own pair
+            TrackArgs b = a;
+            use_pair(b, p);
+            point_record(point_geom(b, p), r);
+        } else {
+            point_record(point_geom(a, p), r);
+        }
         int4* q = (int4*)(a.prec + AR_PREC * p);
         q[0] = make_int4(r[0], r[1], r[2], r[3]);
         q[1] = make_int4(r[4], r[5], r[6], r[7]);
@@ -406,7 +417,8 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     for (int k = 6; k < 10; k++) gs[AR_GEOM_SLOT * k] = k < 8 ? ~0ULL : 1ULL;
     HIPCHK(ctx, hipMemcpyAsync(ctx->geom.ptr, gs, sizeof(gs), hipMemcpyHostToDevice, ctx->stream));
     pass_geometry_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, ctx->ny, ctx->nx,
-                                                               (unsigned long long*)ctx->geom.ptr);
+                                                               (unsigned long long*)ctx->geom.ptr,
+                                                               nullptr, nullptr);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipMemcpyAsync(gs, ctx->geom.ptr, sizeof(gs), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -460,7 +472,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     }
     void (*kern)(TrackArgs);
     if (use_u8) {
-        kern = track_u8_kernel;
+        kern = track_u8_kernel<false>;
         // the int8 copy of the reference the matrix cores read (made once per prepared image)
         const int64_t N = ctx->ny * ctx->nx;
         const int64_t stride = (N + 1024 + 255) / 256 * 256;   // room for a window column's tail
@@ -732,3 +744,497 @@ extern "C" ar_status ar_correlate_point(ar_context* ctx, const void* search, con
     c->ref_gen++;   // the restored pair is not the one the int8 copies were made from
     return st;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Batched passes: many prepared UInt8 pairs resident at once, and one tracking pass over the
+// points of all of them (each point names its pair). The pairs keep their own buffers (prepared
+// images, validity, int8 copies); the pass shares one virtual zero padding, large enough for
+// every point, which leaves each point's result as in a pass over its own pair alone (the
+// reference pads every window into bounds, src/track.jl:121-138, and padding is invalid).
+// ---------------------------------------------------------------------------------------------
+
+#include <map>
+#include <tuple>
+
+namespace {
+
+struct BatchPair {
+    DevBuf ref, sec, ok, refx;
+    int64_t ny = 0, nx = 0, stride = 0;
+    bool all_valid = true;
+    bool copies = false;
+};
+
+struct GraphKey {
+    const void* p[11];
+    int64_t n, up;
+    int32_t measure, split;
+    bool operator==(const GraphKey& o) const {
+        return std::memcmp(p, o.p, sizeof(p)) == 0 && n == o.n && up == o.up &&
+               measure == o.measure && split == o.split;
+    }
+};
+
+}  // namespace
+
+struct ar_batch {
+    ar_context* ctx = nullptr;          // preparation, stream, pass scratch, error text
+    std::vector<BatchPair*> pairs;
+    DevBuf desc, lists, counts, pidx;   // descriptor table, class lists and counts, pair indices
+    bool desc_dirty = true;
+    // radius split chosen by the tuner, per pass shape (max ch, cw, rx, ry, points / 4096)
+    std::map<std::tuple<int, int, int, int, int64_t>, int> tuned;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    GraphKey gkey;
+    bool has_graph = false;
+    int graph_launches = 0, graph_split = 0;
+    double last_ms = 0;
+    int last_split = 0, last_launches = 0, last_replayed = 0;
+};
+
+// Points of class 0 (max radius <= R) and class 1 (the rest), each list in any order.
+__global__ __launch_bounds__(256) void batch_classify_kernel(ar_points pts, int64_t n, int R,
+                                                             int32_t* lists, uint32_t* counts) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = max(pts.radius_x[i], pts.radius_y[i]);
+        const int c = r <= R ? 0 : 1;
+        const uint32_t k = atomicAdd(counts + c, 1u);
+        lists[c * n + k] = (int32_t)i;
+    }
+}
+
+__global__ void batch_iota_kernel(int32_t* out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (int32_t)i;
+}
+
+__global__ void batch_zero_kernel(uint32_t* c, int k) {
+    if ((int)threadIdx.x < k) c[threadIdx.x] = 0u;
+}
+
+// Counts pair indices outside [0, npairs).
+__global__ __launch_bounds__(256) void batch_check_kernel(const int32_t* pair_of, int64_t n,
+                                                          int64_t npairs, uint32_t* bad) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (pair_of[i] < 0 || pair_of[i] >= npairs) atomicAdd(bad, 1u);
+}
+
+static ar_status batch_sync_desc(ar_batch* B) {
+    ar_context* ctx = B->ctx;
+    if (!B->desc_dirty) return AR_OK;
+    std::vector<PairDesc> d(B->pairs.size());
+    for (size_t k = 0; k < B->pairs.size(); k++) {
+        BatchPair* p = B->pairs[k];
+        d[k].ref = (const uint8_t*)p->ref.ptr;
+        d[k].sec = (const uint8_t*)p->sec.ptr;
+        d[k].refx = p->copies ? (const uint8_t*)p->refx.ptr : nullptr;
+        d[k].secx = p->copies ? (const uint8_t*)p->refx.ptr + 4 * p->stride : nullptr;
+        d[k].ok = p->all_valid ? nullptr : (const uint8_t*)p->ok.ptr;
+        d[k].ny = p->ny;
+        d[k].nx = p->nx;
+        d[k].refx_stride = p->stride;
+    }
+    ar_status st;
+    if ((st = B->desc.reserve(ctx, d.size() * sizeof(PairDesc) + 64))) return st;
+    HIPCHK(ctx, hipMemcpyAsync(B->desc.ptr, d.data(), d.size() * sizeof(PairDesc),
+                               hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    B->desc_dirty = false;
+    return AR_OK;
+}
+
+// One class launch of the batched tracker: the points in `list` (count m), LDS carved for the
+// class's largest radius.
+static ar_status batch_launch(ar_batch* B, TrackArgs a, const int32_t* list, int64_t m, int rcap) {
+    ar_context* ctx = B->ctx;
+    if (m == 0) return AR_OK;
+    a.order = list;
+    a.n = m;
+    a.max_rx = a.max_rx < rcap ? a.max_rx : rcap;
+    a.max_ry = a.max_ry < rcap ? a.max_ry : rcap;
+    a.lds_bytes = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, false).total;
+    auto kern = track_u8_kernel<true>;
+    HIPCHK(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)a.lds_bytes));
+    int per_cu = 0;
+    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, AR_WG, a.lds_bytes));
+    if (per_cu < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "batched tracker does not fit on a CU");
+    int64_t nb = (int64_t)ctx->ncu * per_cu;
+    if (nb > m) nb = m;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(AR_WG), a.lds_bytes, ctx->stream, a);
+    HIPCHK(ctx, hipGetLastError());
+    B->last_launches++;
+    return AR_OK;
+}
+
+// The batched pass on device-resident points, pair indices and outputs. split: radius threshold
+// of the small class (0: one launch over every point), or -1: the tuner's choice.
+static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpair, int64_t n,
+                            int64_t up, int32_t measure, float* dx, float* dy, float* corr,
+                            uint8_t* srch, int split, bool replay) {
+    ar_context* ctx = B->ctx;
+    if (B->pairs.empty()) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "the batch holds no pair");
+    if (n < 0) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "negative point count");
+    if (n > 0x7fffffffLL) return fail(ctx, AR_ERR_UNSUPPORTED, "more than 2^31 - 1 points");
+    if (up < 1 || up > 64 || (up & (up - 1)) != 0)
+        return fail(ctx, AR_ERR_UNSUPPORTED,
+                    "upsampling must be a power of two in 1..64 on the device path");
+    if (measure != AR_ZNCC && measure != AR_NCC)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "measure must be ZNCC or NCC");
+    B->last_launches = 0;
+    B->last_replayed = 0;
+    if (n == 0) return AR_OK;
+    ar_status st;
+    if ((st = batch_sync_desc(B))) return st;
+    GraphKey key;
+    const void* ps[11] = {dp.x, dp.y, dp.dx_prior, dp.dy_prior, dp.radius_x, dp.radius_y,
+                          dp.chip_size_x, dp.chip_size_y, dpair, dx, (const void*)srch};
+    std::memcpy(key.p, ps, sizeof(ps));
+    key.n = n;
+    key.up = up;
+    key.measure = measure;
+    key.split = split;
+    if (replay && B->has_graph && B->gkey == key) {
+        // the captured pass over the same buffers (contents may differ only in values)
+        HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        HIPCHK(ctx, hipGraphLaunch(B->gexec, ctx->stream));
+        HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        float ms = 0;
+        hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        B->last_ms = ms;
+        B->last_replayed = 1;
+        B->last_launches = B->graph_launches;
+        B->last_split = B->graph_split;
+        return AR_OK;
+    }
+    // pair indices in range
+    if ((st = B->counts.reserve(ctx, 256))) return st;
+    uint32_t* cnt = (uint32_t*)B->counts.ptr;
+    batch_zero_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 4);
+    batch_check_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dpair, n, (int64_t)B->pairs.size(),
+                                                             cnt + 2);
+    HIPCHK(ctx, hipGetLastError());
+    uint32_t bad = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&bad, cnt + 2, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (bad) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "pair index outside the batch");
+    // pass geometry over every pair
+    unsigned long long gs[10 * AR_GEOM_SLOT] = {0};
+    if ((st = ctx->geom.reserve(ctx, sizeof(gs)))) return st;
+    for (int k = 6; k < 10; k++) gs[AR_GEOM_SLOT * k] = k < 8 ? ~0ULL : 1ULL;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->geom.ptr, gs, sizeof(gs), hipMemcpyHostToDevice, ctx->stream));
+    pass_geometry_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(
+        dp, n, 0, 0, (unsigned long long*)ctx->geom.ptr, (const PairDesc*)B->desc.ptr, dpair);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(gs, ctx->geom.ptr, sizeof(gs), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    unsigned long long g[10];
+    for (int k = 0; k < 10; k++) g[k] = gs[AR_GEOM_SLOT * k];
+    const int64_t cx = g[0], cy = g[1], rx = g[2], ry = g[3], px = g[4], py = g[5];
+    const bool fits = g[8] != 0;
+    if (cx == 0) {
+        skip_all_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dx, dy, corr, srch, n);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        return AR_OK;
+    }
+    TrackArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.padx = fits ? 0 : px + 2;
+    a.pady = fits ? 0 : py + 2;
+    a.x = dp.x; a.y = dp.y; a.dx_prior = dp.dx_prior; a.dy_prior = dp.dy_prior;
+    a.radius_x = dp.radius_x; a.radius_y = dp.radius_y;
+    a.chip_size_x = dp.chip_size_x; a.chip_size_y = dp.chip_size_y;
+    a.n = n;
+    a.up = (int32_t)up;
+    a.measure = measure;
+    a.dx = dx; a.dy = dy; a.corr = corr; a.searched = srch;
+    a.max_ch = (int32_t)(2 * (cy / 2));
+    a.max_cw = (int32_t)(2 * (cx / 2));
+    a.max_rx = (int32_t)rx;
+    a.max_ry = (int32_t)ry;
+    a.pairs = (const PairDesc*)B->desc.ptr;
+    a.pair_of = dpair;
+    if ((int64_t)a.max_ch * a.max_cw > 16384 || a.max_ch > 256 ||
+        u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, false).total > kLdsLimit)
+        return fail(ctx, AR_ERR_UNSUPPORTED, "chip or search window too large for the batched tracker");
+    if ((st = ctx->refine.reserve(ctx, (size_t)n * 132))) return st;
+    a.rpatch = (float*)ctx->refine.ptr;
+    a.rmeta = (int32_t*)((char*)ctx->refine.ptr + (size_t)n * 128);
+    if ((st = ctx->prec.reserve(ctx, (size_t)n * AR_PREC * 4))) return st;
+    a.prec = (const int32_t*)ctx->prec.ptr;
+    if ((st = B->lists.reserve(ctx, (size_t)n * 8))) return st;
+    int32_t* lists = (int32_t*)B->lists.ptr;
+    // the tuner: time each candidate split once per pass shape, keep the fastest
+    const int rmax = (int)(rx > ry ? rx : ry);
+    if (split < 0) {
+        auto tk = std::make_tuple(a.max_ch, a.max_cw, a.max_rx, a.max_ry, n >> 12);
+        auto it = B->tuned.find(tk);
+        if (it != B->tuned.end()) {
+            split = it->second;
+        } else {
+            int best = 0;
+            double best_ms = 1e30;
+            const int cands[4] = {0, 16, 24, 32};
+            for (int c : cands) {
+                if (c != 0 && c >= rmax) continue;
+                if ((st = batch_pass(B, dp, dpair, n, up, measure, dx, dy, corr, srch, c, false)))
+                    return st;
+                if (B->last_ms < best_ms) {
+                    best_ms = B->last_ms;
+                    best = c;
+                }
+            }
+            B->tuned[tk] = best;
+            split = best;
+        }
+        key.split = split;
+    }
+    if (split >= rmax) split = 0;
+    // hipFuncSetAttribute / occupancy queries happen before a capture begins
+    auto kern = track_u8_kernel<true>;
+    HIPCHK(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry,
+                                                   false).total));
+    const size_t rlds = (size_t)cascade_floats((int)up) * 4;
+    int64_t rblocks = 0;
+    if (up > 1) {
+        HIPCHK(ctx, hipFuncSetAttribute((const void*)refine_kernel,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
+        int rpc = 0;
+        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&rpc, refine_kernel, 64, rlds));
+        if (rpc < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "refinement kernel does not fit on a CU");
+        rblocks = (int64_t)ctx->ncu * rpc;
+        if (rblocks > n) rblocks = n;
+    }
+    // class sizes (the class lists are rebuilt inside the pass, so a replay needs the same points)
+    int64_t m0 = n, m1 = 0;
+    if (split > 0) {
+        batch_zero_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 2);
+        batch_classify_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, split, lists, cnt);
+        HIPCHK(ctx, hipGetLastError());
+        uint32_t c2[2] = {0, 0};
+        HIPCHK(ctx, hipMemcpyAsync(c2, cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        m0 = c2[0];
+        m1 = c2[1];
+    }
+    const bool capture = replay;
+    if (capture) {
+        if (B->has_graph) {
+            hipGraphExecDestroy(B->gexec);
+            hipGraphDestroy(B->graph);
+            B->has_graph = false;
+        }
+        HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    } else {
+        HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    }
+    B->last_launches = 0;
+    // ---- the pass: point records, class lists, the tracker per class, the refinement
+    point_prep_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(a);
+    if (split > 0) {
+        batch_zero_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 2);
+        batch_classify_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, split, lists, cnt);
+    } else {
+        batch_iota_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(lists, n);
+    }
+    ar_status lst = batch_launch(B, a, lists, m0, split > 0 ? split : rmax);
+    if (lst == AR_OK && m1 > 0) lst = batch_launch(B, a, lists + n, m1, rmax);
+    if (lst == AR_OK && up > 1)
+        hipLaunchKernelGGL(refine_kernel, dim3((unsigned)rblocks), dim3(64), rlds, ctx->stream, a);
+    if (capture) {
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &gr);
+        if (lst != AR_OK) {
+            if (gr) hipGraphDestroy(gr);
+            return lst;
+        }
+        if (e != hipSuccess) return fail(ctx, AR_ERR_DEVICE, "graph capture of the batched pass failed");
+        hipGraphExec_t ex = nullptr;
+        if (hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) != hipSuccess) {
+            hipGraphDestroy(gr);
+            return fail(ctx, AR_ERR_DEVICE, "graph instantiation of the batched pass failed");
+        }
+        B->graph = gr;
+        B->gexec = ex;
+        B->gkey = key;
+        B->has_graph = true;
+        B->graph_launches = B->last_launches;
+        B->graph_split = split;
+        const int launches = B->last_launches;
+        // first run of the captured pass
+        HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        HIPCHK(ctx, hipGraphLaunch(ex, ctx->stream));
+        B->last_launches = launches;
+    } else if (lst != AR_OK) {
+        return lst;
+    }
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    B->last_ms = ms;
+    B->last_split = split;
+    return AR_OK;
+}
+
+extern "C" {
+
+ar_status ar_batch_create(const ar_params* p, int32_t device, ar_batch** out) {
+    if (!out) return fail(nullptr, AR_ERR_INVALID_ARGUMENT, "NULL output handle");
+    *out = nullptr;
+    ar_params q;
+    if (p) q = *p;
+    else ar_params_default(&q);
+    if (q.quantize != AR_QUANT_UINT8)
+        return fail(nullptr, AR_ERR_UNSUPPORTED, "the batched tracker takes UInt8-quantised pairs");
+    ar_context* c = nullptr;
+    ar_status st = ar_context_create(&q, device, &c);
+    if (st) return st;
+    ar_batch* B = new ar_batch();
+    B->ctx = c;
+    *out = B;
+    return AR_OK;
+}
+
+void ar_batch_destroy(ar_batch* B) {
+    if (!B) return;
+    hipSetDevice(B->ctx->device);
+    hipStreamSynchronize(B->ctx->stream);
+    if (B->has_graph) {
+        hipGraphExecDestroy(B->gexec);
+        hipGraphDestroy(B->graph);
+    }
+    for (BatchPair* p : B->pairs) {
+        p->ref.release();
+        p->sec.release();
+        p->ok.release();
+        p->refx.release();
+        delete p;
+    }
+    B->desc.release();
+    B->lists.release();
+    B->counts.release();
+    B->pidx.release();
+    ar_context_destroy(B->ctx);
+    delete B;
+}
+
+ar_status ar_batch_add_pair(ar_batch* B, const void* ref, const void* sec,
+                            const uint8_t* ref_valid, const uint8_t* sec_valid, int64_t ny,
+                            int64_t nx, int32_t dtype, int32_t* index) {
+    if (!B) return fail(nullptr, AR_ERR_INVALID_ARGUMENT, "batch is NULL");
+    ar_context* ctx = B->ctx;
+    if (!ref || !sec) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "a batched pair needs both images");
+    if (B->pairs.size() >= 0x7fffffff) return fail(ctx, AR_ERR_UNSUPPORTED, "too many pairs");
+    ar_status st = ar_set_images(ctx, ref, sec, ref_valid, sec_valid, ny, nx, dtype);
+    if (st) return st;
+    if (ctx->pdtype != AR_U8) return fail(ctx, AR_ERR_UNSUPPORTED, "prepared pair is not UInt8");
+    BatchPair* bp = new BatchPair();
+    const int64_t N = ny * nx;
+    bp->ny = ny;
+    bp->nx = nx;
+    bp->all_valid = ctx->all_valid;
+    bp->stride = (N + 1024 + 255) / 256 * 256;
+    // the tracker addresses the copies with 32-bit offsets (chips up to 256, radii up to 256)
+    bp->copies = 3 * (bp->stride + 1) + (int64_t)(256 + 2 * 256 + 2) * ny + 4096 < ((int64_t)1 << 32);
+    if ((st = bp->ref.reserve(ctx, N)) || (st = bp->sec.reserve(ctx, N)) ||
+        (!bp->all_valid && (st = bp->ok.reserve(ctx, N))) ||
+        (bp->copies && (st = bp->refx.reserve(ctx, 8 * bp->stride)))) {
+        bp->ref.release(); bp->sec.release(); bp->ok.release(); bp->refx.release();
+        delete bp;
+        return st;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(bp->ref.ptr, ctx->pref.ptr, N, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(bp->sec.ptr, ctx->psec.ptr, N, hipMemcpyDeviceToDevice, ctx->stream));
+    if (!bp->all_valid)
+        HIPCHK(ctx, hipMemcpyAsync(bp->ok.ptr, ctx->okmask.ptr, N, hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+    if (bp->copies) {
+        refx_kernel<<<grid_for(bp->stride), 256, 0, ctx->stream>>>(
+            (const uint8_t*)ctx->pref.ptr, (uint8_t*)bp->refx.ptr, N, bp->stride);
+        refx_kernel<<<grid_for(bp->stride), 256, 0, ctx->stream>>>(
+            (const uint8_t*)ctx->psec.ptr, (uint8_t*)bp->refx.ptr + 4 * bp->stride, N, bp->stride);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    B->pairs.push_back(bp);
+    B->desc_dirty = true;
+    if (index) *index = (int32_t)(B->pairs.size() - 1);
+    return AR_OK;
+}
+
+ar_status ar_batch_track_device(ar_batch* B, const ar_points* dev_pts, const int32_t* dev_pair,
+                                int64_t n, int64_t upsampling, int32_t measure, float* dev_dx,
+                                float* dev_dy, float* dev_corr, uint8_t* dev_searched,
+                                int32_t flags) {
+    if (!B) return fail(nullptr, AR_ERR_INVALID_ARGUMENT, "batch is NULL");
+    ar_context* ctx = B->ctx;
+    if (!dev_pts || !dev_pair || !dev_dx || !dev_dy || !dev_corr || !dev_searched)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int split = (flags & AR_BATCH_SPLIT_SET) ? (flags >> 8) & 0xff : -1;
+    return batch_pass(B, *dev_pts, dev_pair, n, upsampling, measure, dev_dx, dev_dy, dev_corr,
+                      dev_searched, split, (flags & AR_BATCH_GRAPH) != 0);
+}
+
+ar_status ar_batch_track(ar_batch* B, const ar_points* pts, const int32_t* pair, int64_t n,
+                         int64_t upsampling, int32_t measure, float* dx, float* dy, float* corr,
+                         uint8_t* searched) {
+    if (!B) return fail(nullptr, AR_ERR_INVALID_ARGUMENT, "batch is NULL");
+    ar_context* ctx = B->ctx;
+    if (!pts || !pair || !dx || !dy || !corr || !searched)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    if (!pts->x || !pts->y || !pts->radius_x || !pts->radius_y || !pts->dx_prior ||
+        !pts->dy_prior || !pts->chip_size_x || !pts->chip_size_y)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "NULL point field");
+    if (n == 0) return AR_OK;
+    for (int64_t i = 0; i < n; i++)
+        if (pair[i] < 0 || pair[i] >= (int64_t)B->pairs.size())
+            return fail(ctx, AR_ERR_INVALID_ARGUMENT, "pair index outside the batch");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    ar_status st;
+    ar_points dp;
+    if ((st = ar_upload_points(ctx, pts, n, &dp, 0, 0))) return st;
+    if ((st = B->pidx.reserve(ctx, 4 * (size_t)n))) return st;
+    HIPCHK(ctx, hipMemcpyAsync(B->pidx.ptr, pair, 4 * n, hipMemcpyHostToDevice, ctx->stream));
+    if ((st = ctx->out.reserve(ctx, 13 * n))) return st;
+    float* ddx = (float*)ctx->out.ptr;
+    float* ddy = ddx + n;
+    float* dc = ddy + n;
+    uint8_t* ds = (uint8_t*)(dc + n);
+    if ((st = batch_pass(B, dp, (const int32_t*)B->pidx.ptr, n, upsampling, measure, ddx, ddy,
+                         dc, ds, -1, false)))
+        return st;
+    HIPCHK(ctx, hipMemcpyAsync(dx, ddx, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(dy, ddy, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(corr, dc, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(searched, ds, n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return AR_OK;
+}
+
+ar_status ar_batch_info(const ar_batch* B, double* ms, int32_t* info) {
+    if (!B) return AR_ERR_INVALID_ARGUMENT;
+    if (ms) *ms = B->last_ms;
+    if (info) {
+        info[0] = (int32_t)B->pairs.size();
+        info[1] = B->last_split;
+        info[2] = B->last_launches;
+        info[3] = B->last_replayed;
+    }
+    return AR_OK;
+}
+
+const char* ar_batch_last_error(const ar_batch* B) {
+    return B ? B->ctx->err.c_str() : "batch is NULL";
+}
+
+}  // extern "C"

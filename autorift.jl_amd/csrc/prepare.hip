@@ -429,9 +429,10 @@ static ar_status prepare_slot(ar_context* ctx, const void* host_img, const uint8
     ar_status st;
     if ((st = ctx->raw.reserve(ctx, N * sizeof(T))) || (st = ctx->rawmask.reserve(ctx, N)))
         return st;
-    PCHK(hipMemcpyAsync(ctx->raw.ptr, host_img, N * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    // the caller's image may be host or device memory (unified addressing picks the direction)
+    PCHK(hipMemcpyAsync(ctx->raw.ptr, host_img, N * sizeof(T), hipMemcpyDefault, ctx->stream));
     if (host_mask) {
-        PCHK(hipMemcpyAsync(ctx->rawmask.ptr, host_mask, N, hipMemcpyHostToDevice, ctx->stream));
+        PCHK(hipMemcpyAsync(ctx->rawmask.ptr, host_mask, N, hipMemcpyDefault, ctx->stream));
     } else {
         finite_mask_t_kernel<T><<<gsz(N), 256, 0, ctx->stream>>>((const T*)ctx->raw.ptr,
                                                                  (uint8_t*)ctx->rawmask.ptr, N);
@@ -477,8 +478,8 @@ extern "C" ar_status ar_set_images(ar_context* ctx, const void* ref, const void*
         uint8_t* dm = (uint8_t*)(slot == 0 ? ctx->rmask.ptr : ctx->smask.ptr);
         if (pdt == AR_U8 && p.preprocess == AR_PRE_NONE && p.quantize == AR_QUANT_NONE) {
             // an integer image passes through unchanged
-            PCHK(hipMemcpyAsync(dst, img, N, hipMemcpyHostToDevice, ctx->stream));
-            if (m) PCHK(hipMemcpyAsync(dm, m, N, hipMemcpyHostToDevice, ctx->stream));
+            PCHK(hipMemcpyAsync(dst, img, N, hipMemcpyDefault, ctx->stream));
+            if (m) PCHK(hipMemcpyAsync(dm, m, N, hipMemcpyDefault, ctx->stream));
             else {
                 if ((st = ctx->rawmask.reserve(ctx, N))) return st;
                 std::vector<uint8_t> ones(N, 1);

@@ -169,8 +169,12 @@ __device__ inline void mfma_batch(const int8_t* Wl, const int8_t* Tp, int ldw, i
 #define AR_NRB_MAX 9   // band heights with a dedicated MFMA loop (chips up to 130 rows)
 
 // Four workgroups per CU (128 VGPRs) when the LDS layout allows it.
+// BATCH: many pairs in one launch. Point k of the launch is order[k]; its images come from its
+// pair's descriptor. The single-pair instance is the same code without the indirection.
+template <bool BATCH>
 __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void track_u8_kernel(
-    TrackArgs a) {
+    TrackArgs a0) {
+    TrackArgs a = a0;
     extern __shared__ __align__(16) unsigned char smem[];
     const U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, a.box1 != nullptr);
     // one set of reduction slots per call site (see block_sum2_u32 / block_best1)
@@ -183,14 +187,19 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
     int32_t* colS = (int32_t*)(smem + L.off_tmp);
     // the wave index in a scalar register: loops over a wave's share are scalar loops
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const uint8_t* ref = (const uint8_t*)a.ref;
-    const uint8_t* sec = (const uint8_t*)a.sec;
-
     // chip band rows and trailing columns are zero; the fetched chip path writes only chip rows
     for (int t = tid; t < (L.cols_t * L.cpad) >> 2; t += AR_WG) ((uint32_t*)Tp)[t] = 0;
     __syncthreads();
-    const PointRange pr = point_range(a.n);
-    for (int64_t p = pr.first; p < pr.end; p += pr.step) {
+    const PointRange pr = point_range(a0.n);
+    for (int64_t k = pr.first; k < pr.end; k += pr.step) {
+        int64_t p = k;
+        if (BATCH) {
+            p = a0.order[k];
+            a = a0;
+            use_pair(a, p);
+        }
+        const uint8_t* ref = (const uint8_t*)a.ref;
+        const uint8_t* sec = (const uint8_t*)a.sec;
         Geom g = load_geom(a, p);
         if (!g.status) { write_skip(a, p, false); continue; }
         if (!any_valid(a, g)) { write_skip(a, p, false); continue; }
@@ -637,6 +646,9 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
         __syncthreads();
     }
 }
+
+template __global__ void track_u8_kernel<false>(TrackArgs);
+template __global__ void track_u8_kernel<true>(TrackArgs);
 
 // ---------------------------------------------------------------------------------------------
 // generic path (FP64, reference order)

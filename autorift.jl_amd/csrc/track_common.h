@@ -12,6 +12,17 @@
 
 namespace ar {
 
+// One prepared UInt8 pair of a batch (batched pass): what a single-pair pass reads from
+// TrackArgs' image fields.
+struct PairDesc {
+    const uint8_t* ref;
+    const uint8_t* sec;
+    const uint8_t* refx;
+    const uint8_t* secx;
+    const uint8_t* ok;        // nullptr: every pixel valid
+    int64_t ny, nx, refx_stride;
+};
+
 struct TrackArgs {
     const void* ref;          // prepared reference image, column-major ny x nx
     const uint8_t* refx;      // u8 path: 4 int8 copies of the reference (value ^ 0x80), copy k
@@ -61,7 +72,25 @@ struct TrackArgs {
     // generic path: per-block global scratch
     unsigned char* scratch;
     size_t scratch_per_block;
+    // batched pass (many pairs in one launch): the pairs' descriptors, each point's pair, and the
+    // points this launch visits (order[k], k < n); nullptr for a single-pair pass
+    const PairDesc* pairs;
+    const int32_t* pair_of;
+    const int32_t* order;
 };
+
+// A batched pass reads point p's images from its pair's descriptor (uniform per workgroup).
+__device__ inline void use_pair(TrackArgs& a, int64_t p) {
+    const PairDesc d = a.pairs[a.pair_of[p]];
+    a.ref = d.ref;
+    a.sec = d.sec;
+    a.refx = d.refx;
+    a.secx = d.secx;
+    a.ok = d.ok;
+    a.ny = d.ny;
+    a.nx = d.nx;
+    a.refx_stride = d.refx_stride;
+}
 
 // Geometry of one point in padded coordinates (1-based, inclusive), src/points.jl:358-395 and
 // the pass shift of src/track.jl:207-210.

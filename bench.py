@@ -235,6 +235,120 @@ def mixed_block(autorift, tex, sec, pts, steps):
     return out
 
 
+def batched_block(autorift, steps, npairs=256, npoints=1_000_000, seed=5):
+    """The uneven batched workload: `npairs` pairs with sides uniform in [1024, 8192], each the
+    reference warped by its own affine field (|d| <= 15 px, bilinear) plus N(0, 2^2) noise, made
+    on the device; `npoints` scattered fractional centres spread evenly over the pairs, chip size
+    from {16, 32, 48, 64} and search radius uniform in 8..40 per point; one batched pass
+    (64x refinement) captured in a HIP graph and replayed."""
+    import torch
+    import torch.nn.functional as F
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    k1 = torch.arange(-6, 7, dtype=torch.float32, device=dev)
+    k1 = torch.exp(-k1 * k1 / (2 * 1.5 ** 2))
+    k1 = k1 / k1.sum()
+    bt = autorift.Batch(autorift.params())
+    sides = rng.integers(1024, 8193, (npairs, 2))
+    fields = []
+    t0 = time.time()
+    for k in range(npairs):
+        ny, nx = int(sides[k, 0]), int(sides[k, 1])
+        # column-major ny x nx image = row-major tensor (nx, ny): dim 0 is x, dim 1 is y
+        t = torch.randn((1, 1, nx, ny), generator=gen, device=dev)
+        # separable Gaussian blur as sums of shifted slices (elementwise kernels; a convolution
+        # library call would tune itself anew for every image shape)
+        tp = F.pad(t, (0, 0, 6, 6), mode="reflect")
+        t = sum(k1[j] * tp[:, :, j:j + nx, :] for j in range(13))
+        tp = F.pad(t, (6, 6, 0, 0), mode="reflect")
+        t = sum(k1[j] * tp[:, :, :, j:j + ny] for j in range(13))
+        del tp
+        t = (t - t.min()) * (255.0 / (t.max() - t.min()))
+        # affine field (u, v) in pixels about the centre; |d| <= 15 over the image
+        tr = rng.uniform(-9, 9, 2)
+        lin = rng.uniform(-1, 1, (2, 2)) * (5.0 / max(nx, ny))
+        cx, cy = (nx - 1) / 2, (ny - 1) / 2
+        xs = torch.arange(nx, dtype=torch.float32, device=dev).view(nx, 1)
+        ys = torch.arange(ny, dtype=torch.float32, device=dev).view(1, ny)
+        u = tr[0] + lin[0, 0] * (xs - cx) + lin[0, 1] * (ys - cy)
+        v = tr[1] + lin[1, 0] * (xs - cx) + lin[1, 1] * (ys - cy)
+        # secondary(x, y) = reference(x + u, y + v): the tracker reports (dx, dy) = (u, v)
+        gx = (ys + v) * (2.0 / (ny - 1)) - 1.0          # grid_sample x runs along dim 1 (y)
+        gy = (xs + u) * (2.0 / (nx - 1)) - 1.0
+        grid = torch.stack([gx.expand(nx, ny), gy.expand(nx, ny)], dim=-1).unsqueeze(0)
+        sec = F.grid_sample(t, grid, mode="bilinear", padding_mode="border", align_corners=True)
+        sec = sec + 2.0 * torch.randn(sec.shape, generator=gen, device=dev)
+        ref = t.contiguous()
+        sec = sec.contiguous()
+        bt.add_pair_device(ref.data_ptr(), sec.data_ptr(), ny, nx)
+        fields.append((tr, lin, cx, cy))
+        del t, ref, sec, grid, u, v, gx, gy
+        if (k + 1) % 32 == 0:
+            log(f"batched workload: {k + 1} of {npairs} pairs made and prepared "
+                f"({time.time() - t0:.1f}s)")
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+    # points spread evenly over the pairs, fractional 1-based centres
+    pair = rng.integers(0, npairs, npoints).astype(np.int32)
+    ny_of, nx_of = sides[pair, 0], sides[pair, 1]
+    x = 1 + rng.random(npoints) * (nx_of - 1)
+    y = 1 + rng.random(npoints) * (ny_of - 1)
+    cs = rng.choice(np.array([16, 32, 48, 64]), npoints).astype(np.int64)
+    r = rng.integers(8, 41, npoints).astype(np.int64)
+    flds = [torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+            for a in (x, y, r, r, np.zeros(npoints), np.zeros(npoints), cs, cs)]
+    pr = torch.from_numpy(pair).to(dev)
+    out = [torch.empty(npoints, dtype=torch.float32, device=dev) for _ in range(3)]
+    out.append(torch.empty(npoints, dtype=torch.uint8, device=dev))
+    fp = [f.data_ptr() for f in flds]
+    op = [o.data_ptr() for o in out]
+    log(f"batched workload: {npoints} points; tuning the radius split")
+    t1 = time.time()
+    bt.track_device(fp, pr.data_ptr(), npoints, 64, 0, op)       # tunes the radius split
+    tune_s = time.time() - t1
+    split = bt.info()["split"]
+    log(f"batched workload: split {split} chosen in {tune_s:.1f}s; capturing the pass")
+    bt.track_device(fp, pr.data_ptr(), npoints, 64, 0, op, graph=True, split=split)  # capture
+    torch.cuda.synchronize()
+    ms = []
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        bt.track_device(fp, pr.data_ptr(), npoints, 64, 0, op, graph=True, split=split)
+        ms.append(bt.info()["ms"])
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t2) / steps
+    info = bt.info()
+    dx = out[0].cpu().numpy()
+    dy = out[1].cpu().numpy()
+    searched = int(out[3].sum().item())
+    ok = np.isfinite(dx)
+    # expected (u, v) at each point (0-based pixel coordinates)
+    tr = np.array([f[0] for f in fields])[pair]
+    lin = np.array([f[1] for f in fields])[pair]
+    cxx = np.array([f[2] for f in fields])[pair]
+    cyy = np.array([f[3] for f in fields])[pair]
+    ex = tr[:, 0] + lin[:, 0, 0] * (x - 1 - cxx) + lin[:, 0, 1] * (y - 1 - cyy)
+    ey = tr[:, 1] + lin[:, 1, 0] * (x - 1 - cxx) + lin[:, 1, 1] * (y - 1 - cyy)
+    err = np.hypot(dx[ok] - ex[ok], dy[ok] - ey[ok])
+    flop = 2.0 * (2 * r) * (2 * r) * cs.astype(np.float64) ** 2
+    bt.close()
+    ms_med = float(np.median(ms))
+    res = {"pairs": npairs, "points": npoints, "searched": searched,
+           "measured": int(ok.sum()), "ms_per_pass": ms_med, "wall_ms_per_pass": wall * 1000,
+           "points_per_s": searched / (ms_med / 1000),
+           "alg_tflop_per_s": float(flop.sum() / (ms_med / 1000) / 1e12),
+           "radius_split": split, "launches": info["launches"], "graph_replayed": info["replayed"],
+           "median_error_px": float(np.median(err)), "p95_error_px": float(np.percentile(err, 95)),
+           "generate_s": gen_s, "tune_s": tune_s}
+    log(f"batched: {npairs} pairs, {npoints} points, {ms_med:.1f} ms/pass "
+        f"({res['points_per_s'] / 1e6:.1f} M points/s, {res['alg_tflop_per_s']:.0f} TFLOP/s "
+        f"algorithmic), split {split}, launches {info['launches']}, median error "
+        f"{res['median_error_px']:.3f} px (generate {gen_s:.1f}s, tune {tune_s:.1f}s)")
+    return res
+
+
 def profiled_traffic():
     """HBM bytes per tracker launch from the committed counter summary (None if absent)."""
     try:
@@ -275,6 +389,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-pool", type=int, default=6)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--batch-pairs", type=int, default=256,
+                    help="pairs of the batched workload (0: skip it)")
+    ap.add_argument("--batch-points", type=int, default=1_000_000)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a device copy of this many bytes (profiling calibration)")
     args = ap.parse_args()
@@ -346,6 +463,10 @@ def main():
         mixed = mixed_block(autorift, tex, sec, pts, min(args.steps, 3))
     del tex, sec
 
+    batched = None
+    if args.batch_pairs > 0:
+        batched = batched_block(autorift, min(args.steps, 5), args.batch_pairs, args.batch_points)
+
     serving = None
     if args.serve_rates:
         rates = [float(r) for r in args.serve_rates.split(",") if r]
@@ -405,6 +526,7 @@ def main():
         "cpu_baseline": cpu,
         "pair_latency": lat,
         "serving": serving,
+        "batched": batched,
         "mixed_precision": mixed,
         "prepare_s": prep_s,
     }

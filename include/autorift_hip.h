@@ -166,6 +166,39 @@ const char* ar_last_error(const ar_context* ctx);
 /* Library version string. */
 const char* ar_version(void);
 
+/* ---- Batched passes (no counterpart in the reference, which tracks one pair per call).
+ * A batch keeps many prepared UInt8 pairs resident (default pipeline: the batch's params must
+ * quantise to UInt8) and runs one tracking pass over points of all of them, each point naming
+ * its pair: track (src/track.jl:103-160) over every point, with uneven chip sizes and radii.
+ * For power-of-two chip areas the results equal a pass over each pair alone, bit for bit; for
+ * other even chip areas the numerator is the exact correlation of the unrounded mean-removed
+ * chip (the reference rounds it to Float32 first, src/correlate.jl:448-450). */
+typedef struct ar_batch ar_batch;
+#define AR_BATCH_GRAPH 1      /* capture the pass in a HIP graph, replay it for the same buffers */
+#define AR_BATCH_SPLIT_SET 2  /* bits 8..15 give the radius split (0: one launch) instead of
+                                 the tuner's choice */
+ar_status ar_batch_create(const ar_params* p, int32_t device, ar_batch** out);
+void ar_batch_destroy(ar_batch* b);
+/* Prepares a raw pair as ar_set_images does and keeps it in the batch; *index gets its pair
+ * index. The images may be host or device memory. */
+ar_status ar_batch_add_pair(ar_batch* b, const void* ref, const void* sec,
+                            const uint8_t* ref_valid, const uint8_t* sec_valid, int64_t ny,
+                            int64_t nx, int32_t dtype, int32_t* index);
+/* One pass over n points (host buffers; pair[i] is point i's pair index). The radius split of
+ * the launch is chosen by timing the candidates once per pass shape. */
+ar_status ar_batch_track(ar_batch* b, const ar_points* pts, const int32_t* pair, int64_t n,
+                         int64_t upsampling, int32_t measure, float* dx, float* dy, float* corr,
+                         uint8_t* searched);
+/* As ar_batch_track with every pointer in device memory; flags: AR_BATCH_GRAPH,
+ * AR_BATCH_SPLIT_SET | split << 8. */
+ar_status ar_batch_track_device(ar_batch* b, const ar_points* dev_pts, const int32_t* dev_pair,
+                                int64_t n, int64_t upsampling, int32_t measure, float* dev_dx,
+                                float* dev_dy, float* dev_corr, uint8_t* dev_searched,
+                                int32_t flags);
+/* Last pass: device time (ms); info = pairs, radius split, tracker launches, replayed (0/1). */
+ar_status ar_batch_info(const ar_batch* b, double* ms, int32_t* info);
+const char* ar_batch_last_error(const ar_batch* b);
+
 /* ---- Serving (no counterpart in the reference, which correlates one pair per call).
  * A server owns `slots` contexts, each with its own worker thread and HIP stream. Pairs of any
  * size are submitted at any time and run as init + autorift! (src/api.jl:66-76, 165-170):
