@@ -105,11 +105,13 @@ class Batch:
                      graph: bool = False, split: int | None = None):
         """Device pointers: fields = (x, y, radius_x, radius_y, dx_prior, dy_prior, chip_size_x,
         chip_size_y), out = (dx, dy, correlation, searched). graph: capture the pass and replay
-        it while the buffers (and the points' radii) stay the same."""
+        it while the buffers (and the points' radii) stay the same. split: radius classes R1 or
+        (R1, R2) instead of the tuner's choice."""
         s = _lib.Points(*[C.c_void_p(f) for f in fields])
         flags = (_lib.AR_BATCH_GRAPH if graph else 0)
-        if split is not None:
-            flags |= _lib.AR_BATCH_SPLIT_SET | (int(split) << 8)
+        if split is not None:     # R1, or (R1, R2): radius classes, one launch each
+            r1, r2 = (split, 0) if np.isscalar(split) else split
+            flags |= _lib.AR_BATCH_SPLIT_SET | (int(r1) << 8) | (int(r2) << 16)
         self._check(lib().ar_batch_track_device(self._h, C.byref(s), C.c_void_p(pair_ptr), n,
                                                 upsampling, measure,
                                                 *[C.c_void_p(o) for o in out], flags))
@@ -118,5 +120,5 @@ class Batch:
         ms = C.c_double()
         v = (C.c_int32 * 4)()
         lib().ar_batch_info(self._h, C.byref(ms), v)
-        return {"ms": ms.value, "pairs": v[0], "split": v[1], "launches": v[2],
-                "replayed": bool(v[3])}
+        return {"ms": ms.value, "pairs": v[0], "split": (v[1] & 0xff, v[1] >> 8),
+                "launches": v[2], "replayed": bool(v[3])}

@@ -793,15 +793,32 @@ struct ar_batch {
     int last_split = 0, last_launches = 0, last_replayed = 0;
 };
 
-// Points of class 0 (max radius <= R) and class 1 (the rest), each list in any order.
-__global__ __launch_bounds__(256) void batch_classify_kernel(ar_points pts, int64_t n, int R,
-                                                             int32_t* lists, uint32_t* counts) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = max(pts.radius_x[i], pts.radius_y[i]);
-        const int c = r <= R ? 0 : 1;
-        const uint32_t k = atomicAdd(counts + c, 1u);
-        lists[c * n + k] = (int32_t)i;
+// Radius classes of a batched pass: class 0 max(rx, ry) <= R1, class 1 <= R2 (R2 > R1, or 0:
+// no such class), class 2 the rest; each class list in any order. One atomic per wave and class
+// (ballot, then each lane's rank among the wave's lanes of its class).
+__global__ __launch_bounds__(256) void batch_classify_kernel(ar_points pts, int64_t n, int R1,
+                                                             int R2, int32_t* lists,
+                                                             uint32_t* counts) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        const int64_t i = i0 + threadIdx.x;
+        int c = -1;
+        if (i < n) {
+            const int64_t r = max(pts.radius_x[i], pts.radius_y[i]);
+            c = r <= R1 ? 0 : ((R2 > 0 && r <= R2) ? 1 : 2);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const uint64_t m = __ballot(c == k);
+            if (m == 0) continue;
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(counts + k, (uint32_t)__popcll(m));
+            base = (uint32_t)__shfl((int)base, leader);
+            if (c == k) lists[k * n + base + (uint32_t)__popcll(m & below)] = (int32_t)i;
+        }
     }
 }
 
@@ -968,7 +985,7 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     a.rmeta = (int32_t*)((char*)ctx->refine.ptr + (size_t)n * 128);
     if ((st = ctx->prec.reserve(ctx, (size_t)n * AR_PREC * 4))) return st;
     a.prec = (const int32_t*)ctx->prec.ptr;
-    if ((st = B->lists.reserve(ctx, (size_t)n * 8))) return st;
+    if ((st = B->lists.reserve(ctx, (size_t)n * 12))) return st;
     int32_t* lists = (int32_t*)B->lists.ptr;
     // the tuner: time each candidate split once per pass shape, keep the fastest
     const int rmax = (int)(rx > ry ? rx : ry);
@@ -980,9 +997,11 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
         } else {
             int best = 0;
             double best_ms = 1e30;
-            const int cands[4] = {0, 16, 24, 32};
+            // one launch; two classes split at 16, 24 or 32; three classes
+            const int cands[6] = {0, 16, 24, 32, 16 | (28 << 8), 20 | (32 << 8)};
             for (int c : cands) {
-                if (c != 0 && c >= rmax) continue;
+                const int c1 = c & 0xff, c2 = c >> 8;
+                if (c != 0 && (c1 >= rmax || (c2 > 0 && c2 >= rmax))) continue;
                 if ((st = batch_pass(B, dp, dpair, n, up, measure, dx, dy, corr, srch, c, false)))
                     return st;
                 if (B->last_ms < best_ms) {
@@ -995,7 +1014,10 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
         }
         key.split = split;
     }
-    if (split >= rmax) split = 0;
+    int R1 = split & 0xff, R2 = (split >> 8) & 0xff;
+    if (R2 > 0 && (R2 <= R1 || R2 >= rmax)) R2 = 0;
+    if (R1 >= rmax) R1 = R2 = 0;
+    split = R1 | (R2 << 8);
     // hipFuncSetAttribute / occupancy queries happen before a capture begins
     auto kern = track_u8_kernel<true>;
     HIPCHK(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1013,16 +1035,15 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
         if (rblocks > n) rblocks = n;
     }
     // class sizes (the class lists are rebuilt inside the pass, so a replay needs the same points)
-    int64_t m0 = n, m1 = 0;
-    if (split > 0) {
-        batch_zero_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 2);
-        batch_classify_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, split, lists, cnt);
+    int64_t m[3] = {n, 0, 0};
+    if (R1 > 0) {
+        batch_zero_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 3);
+        batch_classify_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, R1, R2, lists, cnt);
         HIPCHK(ctx, hipGetLastError());
-        uint32_t c2[2] = {0, 0};
-        HIPCHK(ctx, hipMemcpyAsync(c2, cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
+        uint32_t c3[3] = {0, 0, 0};
+        HIPCHK(ctx, hipMemcpyAsync(c3, cnt, 12, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        m0 = c2[0];
-        m1 = c2[1];
+        for (int k = 0; k < 3; k++) m[k] = c3[k];
     }
     const bool capture = replay;
     if (capture) {
@@ -1038,14 +1059,17 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     B->last_launches = 0;
     // ---- the pass: point records, class lists, the tracker per class, the refinement
     point_prep_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(a);
-    if (split > 0) {
-        batch_zero_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 2);
-        batch_classify_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, split, lists, cnt);
+    if (R1 > 0) {
+        batch_zero_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 3);
+        batch_classify_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, R1, R2, lists, cnt);
     } else {
         batch_iota_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(lists, n);
     }
-    ar_status lst = batch_launch(B, a, lists, m0, split > 0 ? split : rmax);
-    if (lst == AR_OK && m1 > 0) lst = batch_launch(B, a, lists + n, m1, rmax);
+    // class caps: R1, then R2 (or the largest radius when there is no middle class), then rmax
+    const int caps[3] = {R1 > 0 ? R1 : rmax, R2 > 0 ? R2 : rmax, rmax};
+    ar_status lst = AR_OK;
+    for (int k = 0; k < 3 && lst == AR_OK; k++)
+        if (m[k] > 0) lst = batch_launch(B, a, lists + k * n, m[k], caps[k]);
     if (lst == AR_OK && up > 1)
         hipLaunchKernelGGL(refine_kernel, dim3((unsigned)rblocks), dim3(64), rlds, ctx->stream, a);
     if (capture) {
@@ -1180,7 +1204,7 @@ ar_status ar_batch_track_device(ar_batch* B, const ar_points* dev_pts, const int
     if (!dev_pts || !dev_pair || !dev_dx || !dev_dy || !dev_corr || !dev_searched)
         return fail(ctx, AR_ERR_INVALID_ARGUMENT, "NULL pointer argument");
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    const int split = (flags & AR_BATCH_SPLIT_SET) ? (flags >> 8) & 0xff : -1;
+    const int split = (flags & AR_BATCH_SPLIT_SET) ? (flags >> 8) & 0xffff : -1;
     return batch_pass(B, *dev_pts, dev_pair, n, upsampling, measure, dev_dx, dev_dy, dev_corr,
                       dev_searched, split, (flags & AR_BATCH_GRAPH) != 0);
 }

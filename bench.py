@@ -339,7 +339,8 @@ def batched_block(autorift, steps, npairs=256, npoints=1_000_000, seed=5):
            "measured": int(ok.sum()), "ms_per_pass": ms_med, "wall_ms_per_pass": wall * 1000,
            "points_per_s": searched / (ms_med / 1000),
            "alg_tflop_per_s": float(flop.sum() / (ms_med / 1000) / 1e12),
-           "radius_split": split, "launches": info["launches"], "graph_replayed": info["replayed"],
+           "radius_classes": list(split), "launches": info["launches"],
+           "graph_replayed": info["replayed"],
            "median_error_px": float(np.median(err)), "p95_error_px": float(np.percentile(err, 95)),
            "generate_s": gen_s, "tune_s": tune_s}
     log(f"batched: {npairs} pairs, {npoints} points, {ms_med:.1f} ms/pass "

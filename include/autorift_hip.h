@@ -175,8 +175,9 @@ const char* ar_version(void);
  * chip (the reference rounds it to Float32 first, src/correlate.jl:448-450). */
 typedef struct ar_batch ar_batch;
 #define AR_BATCH_GRAPH 1      /* capture the pass in a HIP graph, replay it for the same buffers */
-#define AR_BATCH_SPLIT_SET 2  /* bits 8..15 give the radius split (0: one launch) instead of
-                                 the tuner's choice */
+#define AR_BATCH_SPLIT_SET 2  /* bits 8..15 (R1) and 16..23 (R2) give the radius classes
+                                 instead of the tuner's choice: one launch per class, max radius
+                                 <= R1, <= R2, the rest (R1 = 0: one launch; R2 = 0: two) */
 ar_status ar_batch_create(const ar_params* p, int32_t device, ar_batch** out);
 void ar_batch_destroy(ar_batch* b);
 /* Prepares a raw pair as ar_set_images does and keeps it in the batch; *index gets its pair
@@ -184,18 +185,21 @@ void ar_batch_destroy(ar_batch* b);
 ar_status ar_batch_add_pair(ar_batch* b, const void* ref, const void* sec,
                             const uint8_t* ref_valid, const uint8_t* sec_valid, int64_t ny,
                             int64_t nx, int32_t dtype, int32_t* index);
-/* One pass over n points (host buffers; pair[i] is point i's pair index). The radius split of
- * the launch is chosen by timing the candidates once per pass shape. */
+/* One pass over n points (host buffers; pair[i] is point i's pair index). The radius classes
+ * (one tracker launch each, LDS carved per class) are chosen by timing the candidates once per
+ * pass shape. */
 ar_status ar_batch_track(ar_batch* b, const ar_points* pts, const int32_t* pair, int64_t n,
                          int64_t upsampling, int32_t measure, float* dx, float* dy, float* corr,
                          uint8_t* searched);
 /* As ar_batch_track with every pointer in device memory; flags: AR_BATCH_GRAPH,
- * AR_BATCH_SPLIT_SET | split << 8. */
+ * AR_BATCH_SPLIT_SET | R1 << 8 | R2 << 16. A captured pass is replayed while the buffers and
+ * the points' radii stay the same (the class sizes are fixed at capture). */
 ar_status ar_batch_track_device(ar_batch* b, const ar_points* dev_pts, const int32_t* dev_pair,
                                 int64_t n, int64_t upsampling, int32_t measure, float* dev_dx,
                                 float* dev_dy, float* dev_corr, uint8_t* dev_searched,
                                 int32_t flags);
-/* Last pass: device time (ms); info = pairs, radius split, tracker launches, replayed (0/1). */
+/* Last pass: device time (ms); info = pairs, radius classes (R1 | R2 << 8), tracker launches,
+ * replayed (0/1). */
 ar_status ar_batch_info(const ar_batch* b, double* ms, int32_t* info);
 const char* ar_batch_last_error(const ar_batch* b);
 

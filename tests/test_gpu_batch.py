@@ -101,13 +101,13 @@ def test_batch_splits_and_graph_replay():
                 np.testing.assert_array_equal(_bits(t.cpu().numpy()), _bits(e), err_msg=tag)
             np.testing.assert_array_equal(out[3].cpu().numpy().astype(bool), ref.searched)
 
-        for split in (0, 16, 24, 32):
+        for split, launches in ((0, 1), (16, 2), (24, 2), (32, 2), ((16, 28), 3), ((20, 32), 3)):
             print(f"batch split {split}", flush=True)
             for t in out:
                 t.zero_()
             bt.track_device(fp, pr.data_ptr(), n, 64, 0, op, split=split)
             check(f"split {split}")
-            assert bt.info()["launches"] == (1 if split == 0 else 2)
+            assert bt.info()["launches"] == launches
         print("batch graph capture and replays", flush=True)
         for rep in range(3):
             for t in out:
