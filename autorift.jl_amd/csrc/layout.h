@@ -104,6 +104,8 @@ struct MpLayout {
     int cols;            // window columns stored: every column the tile loop reads
     int cpad;            // chip column pitch in elements; chip row u at 16 + u, zeros around it
     int cw4;             // chip columns stored (a multiple of 4; the extra ones zero)
+    bool num_in_w;       // numerators written over the window once every wave is past the
+                         // MFMAs (each wave holds one tile batch in registers until then)
     size_t off_w, off_num, off_c, off_pc, total;   // byte offsets
 };
 
@@ -127,8 +129,9 @@ __host__ __device__ inline MpLayout mp_layout(int max_ch, int max_cw, int max_rx
     const size_t numbytes = (size_t)(64 * L.nbt) * (16 * L.ntj) * 4;
     const size_t chipf = (size_t)max_ch * max_cw * 4;
     L.off_w = 0;
-    L.off_num = (size_t)rup((int)wbytes, 16);
-    L.off_c = (size_t)rup((int)(L.off_num + numbytes), 16);
+    L.num_in_w = L.ntj * L.nbt <= 4 && numbytes <= wbytes;
+    L.off_num = L.num_in_w ? 0 : (size_t)rup((int)wbytes, 16);
+    L.off_c = (size_t)rup((int)(L.num_in_w ? wbytes : L.off_num + numbytes), 16);
     L.off_pc = (size_t)rup((max_ch + 4) * (max_cw + 4) * 4, 16);
     const size_t post = L.off_pc + chipf;
     if (L.off_c < chipf) L.off_c = (size_t)rup((int)chipf, 16);   // raw chip read while Cl is built

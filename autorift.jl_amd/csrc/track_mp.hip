@@ -6,8 +6,9 @@
 //   * the chip mean and energy are the reference's sequential Float64 sums and the chip is
 //     Float32(x - mean) (src/correlate.jl:423-453) before it is rounded to the 16-bit type;
 //   * the box sums S1 = sum W and S2 = sum W^2 come from Float64 box images of the pass
-//     (box_f64_colpass_kernel / box_f64_rowpass_kernel), and the ZNCC / NCC normalisation is the
-//     reference's Float64 formula (src/correlate.jl:575-630);
+//     (box_f64_colpass_kernel: direct column sums; box_f64_rowpass_kernel: sliding row sums
+//     restarted every 64 columns), and the ZNCC / NCC normalisation is the reference's Float64
+//     formula (src/correlate.jl:575-630);
 //   * the integer peak's value and the 5x5 patch the subpixel fit starts from are recomputed
 //     with the reference's Float64 direct numerator in its own loop order
 //     (src/correlate.jl:799-814), so the pyrUp cascade (src/peak.jl:456-522) starts from
@@ -183,12 +184,7 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
 
         // ---- numerators on the matrix cores
         const int nti = (NR + 15) >> 4, ntj = (NC + 15) >> 4, nbt = (nti + 3) >> 2;
-        for (int item = wave; item < ntj * nbt; item += AR_NWAVES) {
-            const int tj = item % ntj, ti0 = 4 * (item / ntj);
-            v4f acc[4];
-#pragma unroll
-            for (int t = 0; t < 4; t++) acc[t] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
-            mp_batch<PREC, NRB>(Wl, Cl, L.ldw, L.cpad, cw, tj, ti0, l16, g16, acc);
+        auto store = [&](const v4f (&acc)[4], int tj, int ti0) {
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 const int i = 16 * (ti0 + t) + l16;
@@ -197,6 +193,27 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
                     const int j = 16 * tj + 4 * g16 + r;
                     if (i < NR && j < NC) num[j * NR + i] = acc[t][r];
                 }
+            }
+        };
+        if (L.num_in_w) {
+            // at most one tile batch per wave: its accumulators wait in registers until every
+            // wave is past the window, then go over it
+            v4f acc[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[t] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+            const bool mine = wave < ntj * nbt;
+            const int tj = wave % ntj, ti0 = 4 * (wave / ntj);
+            if (mine) mp_batch<PREC, NRB>(Wl, Cl, L.ldw, L.cpad, cw, tj, ti0, l16, g16, acc);
+            __syncthreads();
+            if (mine) store(acc, tj, ti0);
+        } else {
+            for (int item = wave; item < ntj * nbt; item += AR_NWAVES) {
+                const int tj = item % ntj, ti0 = 4 * (item / ntj);
+                v4f acc[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) acc[t] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+                mp_batch<PREC, NRB>(Wl, Cl, L.ldw, L.cpad, cw, tj, ti0, l16, g16, acc);
+                store(acc, tj, ti0);
             }
         }
         __syncthreads();
@@ -275,34 +292,30 @@ __device__ inline double boxf_pix(const float* img, int64_t ny, int64_t nx, int6
 
 #define AR_BOXF_BLK 64
 // V1 (r, c) = sum_{u < ch} W(r + u, c), V2 likewise of W^2, r < HB, every padded column c. One
-// thread per (column, block of 64 rows): a fresh sum at the block start, then a sliding window,
-// so rounding drifts over at most 64 steps.
+// thread per output row and column: consecutive lanes take consecutive rows of a column, so
+// every load and store is coalesced, and the ch terms of a sum are added directly (the
+// overlapping loads of neighbouring lanes hit in the caches).
 __global__ __launch_bounds__(256) void box_f64_colpass_kernel(const float* img, int64_t ny,
                                                               int64_t nx, int64_t pady,
                                                               int64_t padx, int ch, int64_t HB,
                                                               int64_t NX, double* V1, double* V2) {
-    const int64_t nblk = (HB + AR_BOXF_BLK - 1) / AR_BOXF_BLK;
-    const int64_t total = nblk * NX;
+    const int64_t total = HB * NX;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
          t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = t / nblk, r0 = (t - c * nblk) * AR_BOXF_BLK;
-        const int64_t r1 = r0 + AR_BOXF_BLK < HB ? r0 + AR_BOXF_BLK : HB;
+        const int64_t c = t / HB, r = t - c * HB;
         double s1 = 0.0, s2 = 0.0;
-        for (int u = 0; u < ch; u++) {
-            const double w = boxf_pix(img, ny, nx, pady, padx, r0 + u, c);
-            s1 += w;
-            s2 += w * w;
-        }
-        for (int64_t r = r0; r < r1; r++) {
-            V1[r + c * HB] = s1;
-            V2[r + c * HB] = s2;
-            if (r + 1 < r1) {
-                const double wi = boxf_pix(img, ny, nx, pady, padx, r + ch, c);
-                const double wo = boxf_pix(img, ny, nx, pady, padx, r, c);
-                s1 = (s1 + wi) - wo;
-                s2 = (s2 + wi * wi) - wo * wo;
+        const int64_t ocol = c - padx;
+        if (ocol >= 0 && ocol < nx) {
+            const float* col = img + ocol * ny;
+            for (int u = 0; u < ch; u++) {
+                const int64_t orow = r + u - pady;
+                const double w = (orow >= 0 && orow < ny) ? (double)col[orow] : 0.0;
+                s1 += w;
+                s2 += w * w;
             }
         }
+        V1[t] = s1;
+        V2[t] = s2;
     }
 }
 
@@ -356,6 +369,7 @@ static int64_t mp_grid(int64_t work) {
     int64_t b = (work + 255) / 256;
     return b < 1 ? 1 : (b > 65536 ? 65536 : b);
 }
+// (grid-stride loops: a grid of at most 65 536 workgroups covers any work count)
 
 // Set-up of a mixed-precision pass: operand scales, the pass's Float64 box images and the
 // kernel instance. `a` already holds the pass geometry (max extents, padding).
@@ -411,7 +425,7 @@ ar_status ar_mp_setup(ar_context* ctx, TrackArgs& a, void (**kern)(TrackArgs), s
     double* V2 = (double*)((char*)ctx->box.ptr + vbytes);
     double* B1 = (double*)((char*)ctx->box.ptr + 2 * vbytes);
     double* B2 = (double*)((char*)ctx->box.ptr + 2 * vbytes + bbytes);
-    const int64_t cwork = NX * ((HB + AR_BOXF_BLK - 1) / AR_BOXF_BLK);
+    const int64_t cwork = NX * HB;
     const int64_t rwork = HB * ((WB + AR_BOXF_BLK - 1) / AR_BOXF_BLK);
     box_f64_colpass_kernel<<<mp_grid(cwork), 256, 0, ctx->stream>>>(
         (const float*)ctx->pref.ptr, ctx->ny, ctx->nx, a.pady, a.padx, a.max_ch, HB, NX, V1, V2);
