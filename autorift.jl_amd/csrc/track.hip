@@ -195,8 +195,7 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
         int64_t p = k;
         if (BATCH) {
             p = a0.order[k];
-            a = a0;
-            use_pair(a, p);
+            use_pair(a, p);   // every field it sets is set again for each point
         }
         const uint8_t* ref = (const uint8_t*)a.ref;
         const uint8_t* sec = (const uint8_t*)a.sec;

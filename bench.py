@@ -188,6 +188,45 @@ def serving_sweep(autorift, rates, nreq, slots, pool_size, seed=6):
     return res
 
 
+def served_mixed_vs_plain(autorift, count=8, slots=4, seed=7):
+    """Float32 pairs (quantize = :none), default multi-chip search: the plain form (exact Float64
+    contraction, one pair at a time on one context) against the served mixed-precision form
+    (BF16 contraction, pairs submitted together to a Server with `slots` slots): pairs/s of each,
+    and how far the served results are from the plain ones."""
+    pool = serve_pool(count, seed, lo=1024, hi=4096)
+    log(f"served-vs-plain pool: sides {[a.shape[0] for a, _, _ in pool]}")
+    plain = []
+    with autorift.Context(autorift.params(quantize="none")) as ctx:
+        for a, b, _ in pool[:1]:                       # warm-up
+            ctx.set_images(a, b)
+            ctx.run_multichip(autorift.api._build_grid(a.shape, ctx.params))
+        t = time.perf_counter()
+        for a, b, _ in pool:
+            ctx.set_images(a, b)
+            plain.append(ctx.run_multichip(autorift.api._build_grid(a.shape, ctx.params)))
+        plain_s = time.perf_counter() - t
+    with autorift.Server(autorift.params(quantize="none", precision="bf16"), slots=slots) as srv:
+        srv.result(srv.submit(pool[0][0], pool[0][1]))   # warm-up
+        t = time.perf_counter()
+        tickets = [srv.submit(a, b) for a, b, _ in pool]
+        served = [srv.result(k)[0] for k in tickets]
+        served_s = time.perf_counter() - t
+    same, tot, sq = 0, 0, []
+    for r, e in zip(served, plain):
+        ok = np.isfinite(e.dx) & np.isfinite(r.dx)
+        tot += int(ok.sum())
+        same += int(np.count_nonzero((r.dx[ok] == e.dx[ok]) & (r.dy[ok] == e.dy[ok])))
+        sq.append((r.dx[ok] - e.dx[ok]) ** 2 + (r.dy[ok] - e.dy[ok]) ** 2)
+    res = {"pairs": count, "sides": [int(a.shape[0]) for a, _, _ in pool], "slots": slots,
+           "plain_fp32_pairs_per_s": count / plain_s, "served_bf16_pairs_per_s": count / served_s,
+           "speedup": plain_s / served_s, "identical_fraction": same / max(tot, 1),
+           "rms_vs_plain_px": float(np.sqrt(np.mean(np.concatenate(sq)))) if sq else None}
+    log(f"served BF16 {res['served_bf16_pairs_per_s']:.1f} pairs/s vs plain Float32 "
+        f"{res['plain_fp32_pairs_per_s']:.2f} pairs/s ({res['speedup']:.1f}x); identical "
+        f"{res['identical_fraction']:.4f}, RMS {res['rms_vs_plain_px']:.4f} px")
+    return res
+
+
 def mixed_block(autorift, tex, sec, pts, steps):
     """16-bit contraction on the Float32 pair (quantize = :none): throughput of the BF16 and FP16
     matrix-core tracker over the full grid, the exact Float64 path on a sample of it, and the
@@ -468,6 +507,10 @@ def main():
     if args.batch_pairs > 0:
         batched = batched_block(autorift, min(args.steps, 5), args.batch_pairs, args.batch_points)
 
+    served_vs_plain = None
+    if not args.no_mixed:
+        served_vs_plain = served_mixed_vs_plain(autorift)
+
     serving = None
     if args.serve_rates:
         rates = [float(r) for r in args.serve_rates.split(",") if r]
@@ -529,6 +572,7 @@ def main():
         "serving": serving,
         "batched": batched,
         "mixed_precision": mixed,
+        "served_mixed_vs_plain_fp32": served_vs_plain,
         "prepare_s": prep_s,
     }
     print(json.dumps(line), flush=True)
