@@ -3,17 +3,20 @@
 // The correlation contraction sum(chip' * W) runs on the matrix cores with BF16 or FP16 operands
 // and FP32 accumulation (v_mfma_f32_16x16x32_{bf16,f16}); everything around it keeps the
 // reference's precision:
-//   * the chip mean and energy are the reference's sequential Float64 sums and the chip is
-//     Float32(x - mean) (src/correlate.jl:423-453) before it is rounded to the 16-bit type;
+//   * the chip mean and energy are Float64 sums and the chip is Float32(x - mean)
+//     (src/correlate.jl:423-453) before it is rounded to the 16-bit type;
 //   * the box sums S1 = sum W and S2 = sum W^2 come from Float64 box images of the pass
 //     (box_f64_colpass_kernel: direct column sums; box_f64_rowpass_kernel: sliding row sums
 //     restarted every 64 columns), and the ZNCC / NCC normalisation is the reference's Float64
 //     formula (src/correlate.jl:575-630);
 //   * the integer peak's value and the 5x5 patch the subpixel fit starts from are recomputed
-//     with the reference's Float64 direct numerator in its own loop order
-//     (src/correlate.jl:799-814), so the pyrUp cascade (src/peak.jl:456-522) starts from
-//     full-precision samples.
-// The argmax over the low-precision surface follows peak_index's rule (src/peak.jl:32-61).
+//     with the Float64 direct numerator (src/correlate.jl:799-814), so the pyrUp cascade
+//     (src/peak.jl:456-522) starts from full-precision samples.
+// The Float64 sums (chip mean and energy, the 26 exact numerators) are block reductions in a
+// fixed order -- deterministic, but not the reference's left-to-right order, so a value can
+// differ from the exact path's in its last Float32 bit when it lies within ~1e-16 of a rounding
+// boundary. The argmax over the low-precision surface follows peak_index's rule
+// (src/peak.jl:32-61).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -112,8 +115,8 @@ __device__ inline float mp_normalise(double num, double s1, double s2, double n,
 template <int PREC, int NRB>
 __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double shd[2];
-    __shared__ float pk;
+    __shared__ double red_d[AR_NWAVES];
+    __shared__ float pvals[32];
     __shared__ Best red[AR_NWAVES];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -139,25 +142,17 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
         for (int v = wave; v < cw; v += AR_NWAVES)
             for (int u = lane; u < ch; u += 64) chipf[u + v * ch] = mp_pix(a, sec, g.cr0 + u, g.cc0 + v);
         __syncthreads();
-        if (tid == 0) {
-            // sequential sums; ch * cw is a multiple of 4 (even extents), so the loads go in
-            // batches ahead of the dependent additions
-            const int nn = ch * cw;
-            double s = 0.0;
-#pragma unroll 8
-            for (int k = 0; k < nn; k++) s += (double)chipf[k];
-            const double mean = s / (double)nn;
-            double sq = 0.0;
-#pragma unroll 8
-            for (int k = 0; k < nn; k++) {
-                const double d = (double)chipf[k] - mean;
-                sq += d * d;
-            }
-            shd[0] = mean;
-            shd[1] = sq;
+        // chip mean and energy: Float64 block sums in a fixed order
+        const int nn = ch * cw;
+        double s = 0.0;
+        for (int k = tid; k < nn; k += AR_WG) s += (double)chipf[k];
+        const double mean = block_sum(s, red_d) / (double)nn;
+        double q = 0.0;
+        for (int k = tid; k < nn; k += AR_WG) {
+            const double d = (double)chipf[k] - mean;
+            q += d * d;
         }
-        __syncthreads();
-        const double mean = shd[0], sq = shd[1];
+        const double sq = block_sum(q, red_d);
         if (!(sq > 2.220446049250313e-16 * (double)ch * (double)cw)) {   // degenerate chip
             write_skip(a, p, true);
             __syncthreads();
@@ -166,12 +161,13 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
         const double cnorm = sqrt(sq);
         // the mean-removed chip Float32(x - mean), scaled, as a 16-bit copy in full padded
         // columns
-        for (int k = tid; k < L.cw4 * L.cpad; k += AR_WG) {
-            const int v = k / L.cpad, u = k - v * L.cpad - 16;
-            float c = 0.0f;
-            if (v < cw && u >= 0 && u < ch) c = (float)((double)chipf[u + v * ch] - mean);
-            Cl[k] = MpType<PREC>::bits(c * a.cscale);
-        }
+        for (int v = wave; v < L.cw4; v += AR_NWAVES)
+            for (int r = lane; r < L.cpad; r += 64) {
+                const int u = r - 16;
+                float c = 0.0f;
+                if (v < cw && u >= 0 && u < ch) c = (float)((double)chipf[u + v * ch] - mean);
+                Cl[v * L.cpad + r] = MpType<PREC>::bits(c * a.cscale);
+            }
         __syncthreads();
         // ---- window, scaled to the 16-bit type; rows and columns past it zero
         for (int c = wave; c < L.cols; c += AR_NWAVES)
@@ -250,23 +246,24 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
             for (int u = lane; u < ch; u += 64)
                 pchip[u + v * ch] = (float)((double)mp_pix(a, sec, g.cr0 + u, g.cc0 + v) - mean);
         __syncthreads();
-        float pv = 0.0f;
-        if (tid < 26 && (tid < 25) == refine) {
-            const int di = tid < 25 ? tid / 5 : 0, dj = tid < 25 ? tid % 5 : 0;
+        // value v (0..24 the patch, row-major; 25 the peak alone) is wave v % 4's: its lanes
+        // take terms k = lane, lane + 64, ... (chip k, window k + 4 (k / ch) of the block)
+        for (int v = wave; v < 26; v += AR_NWAVES) {
+            if ((v < 25) != refine) continue;
+            const int di = v < 25 ? v / 5 : 0, dj = v < 25 ? v % 5 : 0;
+            const float* wb = pwin + dj * ph + di;
             double acc = 0.0;
-            for (int jj = 0; jj < cw; jj++) {
-                const float* wc = pwin + (dj + jj) * ph + di;
-                const float* cc = pchip + jj * ch;
-#pragma unroll 8
-                for (int ii = 0; ii < ch; ii++) acc += (double)cc[ii] * (double)wc[ii];
+            for (int k = lane; k < nn; k += 64)
+                acc += (double)pchip[k] * (double)wb[k + 4 * (k / ch)];
+            acc = wave_sum(acc);
+            if (lane == 0) {
+                const int64_t q = bb + (bi + di) + (int64_t)(bj + dj) * a.box_ld;
+                pvals[v] = mp_normalise(acc, a.fbox1[q], a.fbox2[q], n, cnorm, a.measure);
             }
-            const int si = bi + di, sj = bj + dj;
-            const int64_t q = bb + si + (int64_t)sj * a.box_ld;
-            pv = mp_normalise(acc, a.fbox1[q], a.fbox2[q], n, cnorm, a.measure);
-            if (tid == 25) pk = pv;
         }
         __syncthreads();
-        finish_point(a, p, g, pi, pj, pk, i0, j0, refine, pv);
+        const float pv = tid < 25 ? pvals[tid] : 0.0f;
+        finish_point(a, p, g, pi, pj, pvals[25], i0, j0, refine, pv);
         __syncthreads();
     }
 }
