@@ -113,6 +113,26 @@ static size_t pair_footprint(int64_t ny, int64_t nx, int32_t dtype, int64_t npts
     return raw + work + prep + box + copies + pts + ((size_t)64 << 20);
 }
 
+// Host copy split over up to 8 threads for large buffers: one thread's memcpy reaches only a
+// fraction of the host's memory bandwidth, and the submitting thread waits for the copy.
+static void par_copy(uint8_t* dst, const uint8_t* src, size_t n) {
+    const size_t chunk = (size_t)8 << 20;
+    int k = (int)((n + chunk - 1) / chunk);
+    if (k > 8) k = 8;
+    if (k <= 1) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const size_t per = (n + k - 1) / k;
+    std::vector<std::thread> ts;
+    for (int t = 1; t < k; t++) {
+        const size_t a = (size_t)t * per, b = a + per < n ? a + per : n;
+        if (a < b) ts.emplace_back([=] { std::memcpy(dst + a, src + a, b - a); });
+    }
+    std::memcpy(dst, src, per < n ? per : n);
+    for (std::thread& t : ts) t.join();
+}
+
 static void worker(ar_server* S, int s) {
     ar_context* ctx = S->ctx[s];
     std::unique_lock<std::mutex> lk(S->mu);
@@ -319,8 +339,8 @@ ar_status ar_server_submit(ar_server* S, const void* ref, const void* sec, const
         r->spill.resize(r->stage_bytes);
         r->stage = r->spill.data();
     }
-    std::memcpy(r->stage, ref, N * es);
-    std::memcpy(r->stage + N * es, sec, N * es);
+    par_copy(r->stage, (const uint8_t*)ref, N * es);
+    par_copy(r->stage + N * es, (const uint8_t*)sec, N * es);
     if (ref_valid) std::memcpy(r->stage + 2 * N * es, ref_valid, N);
     if (sec_valid) std::memcpy(r->stage + 2 * N * es + N, sec_valid, N);
     r->x.assign(grid->x, grid->x + n);
