@@ -12,6 +12,7 @@ from .core import Context, DisplacementField, MultichipResult
 from .points import (PointSet, chip_bounds, gridpoints, gridpoints_xy, pointset, sanitize_,
                      search_bounds)
 from .batch import Batch
+from .geo import autorift_geo, grid_coordinates, pixel_size, to_velocity, years
 from .serve import Server, ServerError
 
 __all__ = [
@@ -19,5 +20,6 @@ __all__ = [
     "Params", "default_params", "lib", "Cache", "ImagePair", "autorift", "autorift_",
     "autorift_with_grid", "chip_sizes", "init", "params", "reinit_", "track", "Context",
     "DisplacementField", "MultichipResult", "PointSet", "chip_bounds", "gridpoints",
-    "gridpoints_xy", "pointset", "sanitize_", "search_bounds", "Server", "ServerError", "Batch",
+    "gridpoints_xy", "pointset", "sanitize_", "search_bounds", "Server", "ServerError", "Batch", "autorift_geo", "grid_coordinates",
+    "pixel_size", "to_velocity", "years",
 ]
