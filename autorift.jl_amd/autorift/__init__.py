@@ -6,8 +6,8 @@ reference (src/api.jl, src/points.jl, src/track.jl, src/multichip.jl); Julia's m
 """
 from ._lib import (AR_NCC, AR_ZNCC, ArgumentError, AutoRIFTError, DimensionMismatch,
                    NoDeviceError, Params, default_params, lib)
-from .api import (Cache, ImagePair, autorift, autorift_, autorift_with_grid, chip_sizes, init,
-                  params, reinit_, track)
+from .api import (Cache, ImagePair, RotationSearch, angles, autorift, autorift_,
+                  autorift_with_grid, chip_sizes, init, params, reinit_, track)
 from .core import Context, DisplacementField, MultichipResult
 from .points import (PointSet, chip_bounds, gridpoints, gridpoints_xy, pointset, sanitize_,
                      search_bounds)
@@ -20,6 +20,6 @@ __all__ = [
     "Params", "default_params", "lib", "Cache", "ImagePair", "autorift", "autorift_",
     "autorift_with_grid", "chip_sizes", "init", "params", "reinit_", "track", "Context",
     "DisplacementField", "MultichipResult", "PointSet", "chip_bounds", "gridpoints",
-    "gridpoints_xy", "pointset", "sanitize_", "search_bounds", "Server", "ServerError", "Batch", "autorift_geo", "grid_coordinates",
+    "gridpoints_xy", "pointset", "sanitize_", "search_bounds", "Server", "ServerError", "Batch", "RotationSearch", "angles", "autorift_geo", "grid_coordinates",
     "pixel_size", "to_velocity", "years",
 ]

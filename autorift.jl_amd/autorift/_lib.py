@@ -53,7 +53,8 @@ class Params(C.Structure):
         ("min_search_radius", C.c_int64), ("coarse_stride", C.c_int64),
         ("coarse_buffer", C.c_int64), ("min_coarse_valid_fraction", C.c_double),
         ("dx_prior", C.c_double), ("dy_prior", C.c_double), ("fill_window", C.c_int64),
-        ("precision", C.c_int32), ("_pad1", C.c_int32),
+        ("precision", C.c_int32), ("rotation_count", C.c_int32),
+        ("rotation_angles", C.c_double * 8),
     ]
 
 

@@ -25,6 +25,36 @@ _OUTLIERS = ("gardner", "none")
 _PRECISION = {"exact": 0, "bf16": 1, "fp16": 2}
 
 
+class RotationSearch:
+    """Chip rotations tried at every point, the strongest peak kept (src/types.jl:426-552):
+    `angles` is the search window in degrees, `about` the scene rotation it is centred on."""
+
+    def __init__(self, angles=(-3.0, 0.0, 3.0), about=0.0):
+        if isinstance(about, (bool, np.bool_)) or not isinstance(about, (int, float, np.number)):
+            raise TypeError(f"`about` must be a real number, got {type(about).__name__}")
+        t = tuple(float(a) for a in angles)
+        if not t:
+            raise ArgumentError("`RotationSearch` needs at least one angle; use `rotation=None` "
+                                "to disable.")
+        if len(set(t)) != len(t):
+            raise ArgumentError(f"`RotationSearch` angles must be distinct, got {t} -- a repeated "
+                                "angle costs a full correlation and can never win.")
+        a = float(about)
+        if not (all(np.isfinite(t)) and np.isfinite(a)):
+            raise ArgumentError(f"`RotationSearch` needs finite angles and a finite `about`, got "
+                                f"angles {t} about {a}.")
+        self.angles = t
+        self.about = a
+
+    def __repr__(self):
+        return f"RotationSearch({self.angles}, about={self.about})"
+
+
+def angles(m: RotationSearch):
+    """The chip rotations a pass tries, `about` subtracted (src/types.jl:577)."""
+    return tuple(a - m.about for a in m.angles)
+
+
 def _resolve(table, x, kw):
     if isinstance(x, str) and x.lstrip(":") in table:
         return x.lstrip(":")
@@ -117,8 +147,16 @@ def params(similarity="zncc", preprocess="highpass", quantize="uint8", subpixel=
             raise ArgumentError(f"`mad_scale` must be positive, got {ms}")
         p.gardner_window, p.gardner_iterations = w, it
         p.min_agree_fraction, p.agree_tolerance, p.mad_scale = maf, at, ms
-    if rotation not in (None, False):
-        raise ArgumentError("rotation search is not available on the device path")
+    if rotation is not None and rotation is not False:
+        if not isinstance(rotation, RotationSearch):
+            raise ArgumentError(f"`rotation` must be a RotationSearch, None or False, got "
+                                f"{rotation!r}")
+        ang = angles(rotation)
+        if len(ang) > 8:
+            raise ArgumentError(f"at most 8 rotation angles on the device path, got {len(ang)}")
+        p.rotation_count = len(ang)
+        for k, v in enumerate(ang):
+            p.rotation_angles[k] = v
     base = _check_chip_size("chip_size", chip_size)
     cmin = base if chip_size_min is None else _check_chip_size("chip_size_min", chip_size_min)
     cmax = 4 * base if chip_size_max is None else _check_chip_size("chip_size_max",

@@ -460,10 +460,24 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     a.max_rx = (int32_t)rx;
     a.max_ry = (int32_t)ry;
 
+    // RotationSearch: rotated chips are Float32, so the pass takes the generic Float64 path;
+    // sines and cosines of deg * (pi / 180) are computed here, once
+    const int nrot = ctx->params.rotation_count;
+    if (nrot < 0 || nrot > 8)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "rotation_count must be in 0..8");
+    a.nrot = nrot;
+    for (int k = 0; k < nrot; k++) {
+        const double deg = ctx->params.rotation_angles[k];
+        if (!std::isfinite(deg)) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "rotation angles must be finite");
+        const double rad = deg * (M_PI / 180.0);
+        a.rot_s[k] = std::sin(rad);
+        a.rot_c[k] = std::cos(rad);
+        if (deg == 0.0) a.rot_zero |= 1u << k;
+    }
     // the u8 kernel's integer identities hold in int32 for chip areas up to 128 x 128, and its
     // uint16 column sums of W for chip heights up to 256
-    bool use_u8 = ctx->pdtype == AR_U8 && pow2 && (int64_t)a.max_ch * a.max_cw <= 16384 &&
-                  a.max_ch <= 256;
+    bool use_u8 = nrot == 0 && ctx->pdtype == AR_U8 && pow2 &&
+                  (int64_t)a.max_ch * a.max_cw <= 16384 && a.max_ch <= 256;
     if (use_u8) {
         // the layout with per-point box sums is the larger one
         U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, false);
@@ -499,7 +513,8 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     }
     // Float32 images with a 16-bit contraction requested: the mixed-precision kernel, which
     // carves its LDS for one chip size per pass
-    const bool use_mp = !use_u8 && ctx->pdtype == AR_F32 && ctx->params.precision != AR_PREC_EXACT;
+    const bool use_mp = !use_u8 && nrot == 0 && ctx->pdtype == AR_F32 &&
+                        ctx->params.precision != AR_PREC_EXACT;
     bool mp_box_timed = false;
     if (use_mp) {
         if (ctx->params.precision != AR_PREC_BF16 && ctx->params.precision != AR_PREC_FP16)
@@ -1119,6 +1134,8 @@ ar_status ar_batch_create(const ar_params* p, int32_t device, ar_batch** out) {
     else ar_params_default(&q);
     if (q.quantize != AR_QUANT_UINT8)
         return fail(nullptr, AR_ERR_UNSUPPORTED, "the batched tracker takes UInt8-quantised pairs");
+    if (q.rotation_count != 0)
+        return fail(nullptr, AR_ERR_UNSUPPORTED, "the batched tracker does not take a rotation search");
     ar_context* c = nullptr;
     ar_status st = ar_context_create(&q, device, &c);
     if (st) return st;

@@ -71,9 +71,12 @@ __host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx
 }
 
 struct GenScratch {
-    size_t off_chip, off_win, off_s, off_s2, off_surf, total;
+    size_t off_chip, off_win, off_s, off_s2, off_surf, off_chip0, off_best, total;
 };
 
+// Per-workgroup global scratch of the generic tracker: the chip being correlated, the window,
+// the two summed-area tables, the surface, and for a rotation search the unrotated chip and the
+// best surface so far.
 __host__ __device__ inline GenScratch gen_scratch(int max_ch, int max_cw, int max_rx, int max_ry) {
     GenScratch G;
     int winh = max_ch + 2 * max_ry - 1, winw = max_cw + 2 * max_rx - 1;
@@ -82,7 +85,9 @@ __host__ __device__ inline GenScratch gen_scratch(int max_ch, int max_cw, int ma
     G.off_s = G.off_win + rup(winh * winw * 4, 256);
     G.off_s2 = G.off_s + rup((winh + 1) * (winw + 1) * 8, 256);
     G.off_surf = G.off_s2 + rup((winh + 1) * (winw + 1) * 8, 256);
-    G.total = G.off_surf + rup(4 * max_rx * max_ry * 4, 256);
+    G.off_chip0 = G.off_surf + rup(4 * max_rx * max_ry * 4, 256);
+    G.off_best = G.off_chip0 + rup(max_ch * max_cw * 4, 256);
+    G.total = G.off_best + rup(4 * max_rx * max_ry * 4, 256);
     return G;
 }
 

@@ -660,10 +660,39 @@ __device__ inline float load_pix_f(const TrackArgs& a, const void* img, int is_u
     return is_u8 ? (float)((const uint8_t*)img)[k] : ((const float*)img)[k];
 }
 
+// _rotate_chip (src/track.jl:382-409): chip0 (ch x cw, column-major) rotated about its centre,
+// bilinear in Float64, Float32 out, zero where a source corner leaves the chip; s, c are the sine
+// and cosine of the angle.
+__device__ inline void rotate_chip(const float* chip0, float* out, int ch, int cw, double s,
+                                   double c) {
+    const double yc = (double)(ch + 1) / 2.0, xc = (double)(cw + 1) / 2.0;
+    for (int e = threadIdx.x; e < ch * cw; e += AR_WG) {
+        const int i = e % ch + 1, j = e / ch + 1;
+        const double dyy = (double)i - yc, dxx = (double)j - xc;
+        const double sy = yc + c * dyy + s * dxx;
+        const double sx = xc - s * dyy + c * dxx;
+        const int64_t i0 = jfloor(sy), j0 = jfloor(sx);
+        if (i0 < 1 || j0 < 1 || i0 + 1 > ch || j0 + 1 > cw) {
+            out[e] = 0.0f;
+            continue;
+        }
+        const double fy = sy - (double)i0, fx = sx - (double)j0;
+        const double v00 = (double)chip0[(i0 - 1) + (j0 - 1) * ch];
+        const double v10 = (double)chip0[i0 + (j0 - 1) * ch];
+        const double v01 = (double)chip0[(i0 - 1) + j0 * ch];
+        const double v11 = (double)chip0[i0 + j0 * ch];
+        out[e] = (float)((1 - fy) * ((1 - fx) * v00 + fx * v01) + fy * ((1 - fx) * v10 + fx * v11));
+    }
+}
+
+// Julia's maximum over a Float32 surface: NaN-propagating.
+__device__ inline float nan_max(float m, float v) { return (v != v || v > m) ? v : m; }
+
 template <int IS_U8>
 __global__ __launch_bounds__(AR_WG) void track_generic_kernel(TrackArgs a) {
     __shared__ Best red[AR_NWAVES];
     __shared__ double shd[4];
+    __shared__ float redm[AR_NWAVES];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const GenScratch G = gen_scratch(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
     unsigned char* base = a.scratch + (size_t)blockIdx.x * a.scratch_per_block;
@@ -672,6 +701,8 @@ __global__ __launch_bounds__(AR_WG) void track_generic_kernel(TrackArgs a) {
     double* S = (double*)(base + G.off_s);
     double* S2 = (double*)(base + G.off_s2);
     float* surf = (float*)(base + G.off_surf);
+    float* chip0 = (float*)(base + G.off_chip0);
+    float* bsurf = (float*)(base + G.off_best);
 
     const PointRange pr = point_range(a.n);
     for (int64_t p = pr.first; p < pr.end; p += pr.step) {
@@ -685,29 +716,10 @@ __global__ __launch_bounds__(AR_WG) void track_generic_kernel(TrackArgs a) {
                 win[c * winh + r] = load_pix_f(a, a.ref, IS_U8, g.wr0 + r, g.wc0 + c);
         for (int v = wave; v < cw; v += AR_NWAVES)
             for (int u = lane; u < ch; u += 64)
-                chip[v * ch + u] = load_pix_f(a, a.sec, IS_U8, g.cr0 + u, g.cc0 + v);
+                chip0[v * ch + u] = load_pix_f(a, a.sec, IS_U8, g.cr0 + u, g.cc0 + v);
         __syncthreads();
-        // prepare_chip!: sequential Float64 mean and energy in column-major order
-        if (tid == 0) {
-            double s = 0.0;
-            for (int k = 0; k < ch * cw; k++) s += (double)chip[k];
-            double mean = s / (double)(ch * cw);
-            double sq = 0.0;
-            for (int k = 0; k < ch * cw; k++) {
-                double d = (double)chip[k] - mean;
-                sq += d * d;
-            }
-            shd[0] = mean;
-            shd[1] = sq;
-        }
-        __syncthreads();
-        double mean = shd[0], sq = shd[1];
-        bool ok = sq > 2.220446049250313e-16 * (double)ch * (double)cw;
-        __syncthreads();
-        if (!ok) { write_skip(a, p, true); continue; }
-        double cnorm = sqrt(sq);
-        for (int k = tid; k < ch * cw; k += AR_WG) chip[k] = (float)((double)chip[k] - mean);
         // integral_both!: column running sums, then the row recurrence, in the reference order
+        // (the window is the same for every chip rotation)
         for (int j = tid; j <= winw; j += AR_WG) { S[j * ls] = 0.0; S2[j * ls] = 0.0; }
         for (int i = tid; i <= winh; i += AR_WG) { S[i] = 0.0; S2[i] = 0.0; }
         for (int j = tid; j < winw; j += AR_WG) {
@@ -728,48 +740,105 @@ __global__ __launch_bounds__(AR_WG) void track_generic_kernel(TrackArgs a) {
             }
         }
         __syncthreads();
-        // direct numerator (src/correlate.jl:799-814) and the normalisation, per shift
-        double n = (double)(ch * cw);
-        for (int j = wave; j < NC; j += AR_NWAVES)
-        for (int i = lane; i < NR; i += 64) {
-            int o = j * NR + i;
-            double acc = 0.0;
-            for (int jj = 0; jj < cw; jj++) {
-                const float* wc = win + (j + jj) * winh + i;
-                const float* cc = chip + jj * ch;
-                for (int ii = 0; ii < ch; ii++) acc += (double)cc[ii] * (double)wc[ii];
+
+        // one correlation, or one per chip rotation keeping the surface with the highest
+        // maximum (_correlate_rotations!, src/track.jl:316-351: an earlier angle wins a tie;
+        // every angle degenerate makes the point degenerate)
+        const int nang = a.nrot > 0 ? a.nrot : 1;
+        float best = -INFINITY;
+        bool found = false;
+        for (int k = 0; k < nang; k++) {
+            if (a.nrot > 0 && !((a.rot_zero >> k) & 1u))
+                rotate_chip(chip0, chip, ch, cw, a.rot_s[k], a.rot_c[k]);
+            else
+                for (int e = tid; e < ch * cw; e += AR_WG) chip[e] = chip0[e];
+            __syncthreads();
+            // prepare_chip!: sequential Float64 mean and energy in column-major order
+            if (tid == 0) {
+                double s = 0.0;
+                for (int q = 0; q < ch * cw; q++) s += (double)chip[q];
+                double mean = s / (double)(ch * cw);
+                double sq = 0.0;
+                for (int q = 0; q < ch * cw; q++) {
+                    double d = (double)chip[q] - mean;
+                    sq += d * d;
+                }
+                shd[0] = mean;
+                shd[1] = sq;
             }
-            int I = i + 1, J = j + 1;
+            __syncthreads();
+            const double mean = shd[0], sq = shd[1];
+            const bool ok = sq > 2.220446049250313e-16 * (double)ch * (double)cw;
+            __syncthreads();
+            if (!ok) continue;
+            const double cnorm = sqrt(sq);
+            for (int q = tid; q < ch * cw; q += AR_WG) chip[q] = (float)((double)chip[q] - mean);
+            __syncthreads();
+            // direct numerator (src/correlate.jl:799-814) and the normalisation, per shift
+            const double n = (double)(ch * cw);
+            float m = -INFINITY;
+            for (int j = wave; j < NC; j += AR_NWAVES)
+            for (int i = lane; i < NR; i += 64) {
+                int o = j * NR + i;
+                double acc = 0.0;
+                for (int jj = 0; jj < cw; jj++) {
+                    const float* wc = win + (j + jj) * winh + i;
+                    const float* cc = chip + jj * ch;
+                    for (int ii = 0; ii < ch; ii++) acc += (double)cc[ii] * (double)wc[ii];
+                }
+                int I = i + 1, J = j + 1;
 #define BOX(T) (T[(I + ch - 1) + (J + cw - 1) * ls] - T[(I - 1) + (J + cw - 1) * ls] - \
                 T[(I + ch - 1) + (J - 1) * ls] + T[(I - 1) + (J - 1) * ls])
-            double den;
-            if (a.measure == 1) {
-                double wsq = BOX(S2);
-                den = cnorm * sqrt(wsq < 0.0 ? 0.0 : wsq);
-            } else {
-                double s1 = BOX(S), s2 = BOX(S2);
-                double wv = s2 - s1 * s1 / n;
-                den = cnorm * sqrt(wv < 0.0 ? 0.0 : wv);
-            }
+                double den;
+                if (a.measure == 1) {
+                    double wsq = BOX(S2);
+                    den = cnorm * sqrt(wsq < 0.0 ? 0.0 : wsq);
+                } else {
+                    double s1 = BOX(S), s2 = BOX(S2);
+                    double wv = s2 - s1 * s1 / n;
+                    den = cnorm * sqrt(wv < 0.0 ? 0.0 : wv);
+                }
 #undef BOX
-            surf[o] = den > 0 ? (float)(acc / den) : 0.0f;
+                const float v = den > 0 ? (float)(acc / den) : 0.0f;
+                surf[o] = v;
+                m = nan_max(m, v);
+            }
+            if (a.nrot == 0) {
+                found = true;
+                break;
+            }
+            // the surface's maximum, and the best surface so far
+            for (int o = 32; o > 0; o >>= 1) m = nan_max(m, __shfl_xor(m, o));
+            __syncthreads();
+            if (lane == 0) redm[wave] = m;
+            __syncthreads();
+            float pk = redm[0];
+            for (int w = 1; w < AR_NWAVES; w++) pk = nan_max(pk, redm[w]);
+            if (pk > best) {
+                best = pk;
+                found = true;
+                for (int o = tid; o < NR * NC; o += AR_WG) bsurf[o] = surf[o];
+            }
+            __syncthreads();
         }
+        if (!found) { write_skip(a, p, true); __syncthreads(); continue; }
+        const float* fs = a.nrot > 0 ? bsurf : surf;
         __syncthreads();
         Best b = best_empty();
         for (int j = wave; j < NC; j += AR_NWAVES)
             for (int i = lane; i < NR; i += 64) {
-                float v = surf[j * NR + i];
+                float v = fs[j * NR + i];
                 int idx = i * NC + j;
                 if (better(v, idx, b)) { b.v = v; b.idx = idx; }
             }
         b = block_best(b, red);
         if (b.idx == 0x7fffffff) b.idx = 0;   // all NaN: (1, 1)
         int pi = b.idx / NC + 1, pj = b.idx % NC + 1;
-        float pval = surf[(pj - 1) * NR + (pi - 1)];
+        float pval = fs[(pj - 1) * NR + (pi - 1)];
         int i0 = 1, j0 = 1;
         bool refine = a.up > 1 && patch_origin(pi, pj, NR, NC, i0, j0);
         float pv = 0.0f;
-        if (refine && tid < 25) pv = surf[(j0 - 1 + tid % 5) * NR + (i0 - 1 + tid / 5)];
+        if (refine && tid < 25) pv = fs[(j0 - 1 + tid % 5) * NR + (i0 - 1 + tid / 5)];
         finish_point(a, p, g, pi, pj, pval, i0, j0, refine, pv);
         __syncthreads();
     }

@@ -72,6 +72,13 @@ struct TrackArgs {
     // generic path: per-block global scratch
     unsigned char* scratch;
     size_t scratch_per_block;
+    // RotationSearch (generic path): nrot chip rotations (0: none); per angle its sine and cosine
+    // (computed on the host) and, in bit k of rot_zero, whether angle k is exactly 0 (the chip is
+    // then used unrotated, as the reference does)
+    int32_t nrot;
+    uint32_t rot_zero;
+    double rot_s[8];
+    double rot_c[8];
     // batched pass (many pairs in one launch): the pairs' descriptors, each point's pair, and the
     // points this launch visits (order[k], k < n); nullptr for a single-pair pass
     const PairDesc* pairs;

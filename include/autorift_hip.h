@@ -75,7 +75,10 @@ typedef struct {
     double dy_prior;              /* 0.0 */
     int64_t fill_window;          /* 3 */
     int32_t precision;            /* ar_precision, default AR_PREC_EXACT (not in the reference) */
-    int32_t _pad1;
+    int32_t rotation_count;       /* RotationSearch (src/types.jl:426-577): 0 = NoRotationSearch,
+                                     else the number of chip rotations (at most 8) */
+    double rotation_angles[8];    /* the chip rotations in degrees, `about` already subtracted
+                                     (angles(m), src/types.jl:577) */
 } ar_params;
 
 /* PointSet fields (src/points.jl:50-74), structure of arrays of equal length. */

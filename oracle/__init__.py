@@ -81,7 +81,8 @@ class _Params(C.Structure):
                 ("min_search_radius", C.c_int64), ("coarse_stride", C.c_int64),
                 ("coarse_buffer", C.c_int64), ("min_coarse_valid_fraction", C.c_double),
                 ("fill_window", C.c_int64), ("upsampling", C.c_int64), ("measure", C.c_int),
-                ("outliers_on", C.c_int), ("gardner", _Gardner)]
+                ("outliers_on", C.c_int), ("gardner", _Gardner), ("nrot", C.c_int),
+                ("angles", C.c_double * 8)]
 
 
 def next_fft_size(n: int) -> int:
@@ -137,12 +138,19 @@ def subpixel_peak(surface, rx, ry, up):
     return dx.value, dy.value, v.value
 
 
+def _angles(rotation):
+    a = [] if rotation is None else [float(v) for v in rotation]
+    assert len(a) <= 8
+    return len(a), (C.c_double * 8)(*(a + [0.0] * (8 - len(a))))
+
+
 def track(ref, sec, ok_mask, x, y, rx, ry, csx, csy, pdx=0.0, pdy=0.0, up=1, measure=ZNCC,
-          num_mode=NUM_DIRECT, nthreads=1):
+          num_mode=NUM_DIRECT, nthreads=1, rotation=None):
     """track (src/track.jl:103-160) over a flat or gridded point set.
 
     Point fields broadcast to the shape of ``x``. Returns (dx, dy, corr, searched) shaped
-    like ``x``.
+    like ``x``. ``rotation``: the chip rotations of a RotationSearch (degrees, ``about``
+    already subtracted), or None.
     """
     ref = _f(ref)
     sec = _f(sec, ref.dtype)
@@ -157,9 +165,10 @@ def track(ref, sec, ok_mask, x, y, rx, ry, csx, csy, pdx=0.0, pdy=0.0, up=1, mea
     dy = np.empty(n, np.float32)
     corr = np.empty(n, np.float32)
     srch = np.empty(n, np.uint8)
+    nrot, ang = _angles(rotation)
     lib().orc_track(_p(ref), _p(sec), _p(ok), _DT[ref.dtype], C.c_int64(ref.shape[0]),
                     C.c_int64(ref.shape[1]), C.byref(pts), C.c_int64(n), C.c_int64(up), measure,
-                    num_mode, nthreads, _p(dx), _p(dy), _p(corr), _p(srch))
+                    num_mode, nthreads, nrot, ang, _p(dx), _p(dy), _p(corr), _p(srch))
     r = lambda a: a.reshape(shape, order="F")
     return r(dx), r(dy), r(corr), r(srch).astype(bool)
 
@@ -271,12 +280,23 @@ def fill_holes(dx, dy, w=3):
 def params(chip_size=32, chip_size_min=None, chip_size_max=None, chip_aspect=1.0,
            min_search_radius=6, coarse_stride=4, coarse_buffer=8,
            min_coarse_valid_fraction=0.01, fill_window=3, upsampling=64, measure=ZNCC,
-           outliers=True, g=None):
+           outliers=True, g=None, rotation=None):
     cmin = chip_size if chip_size_min is None else chip_size_min
     cmax = 4 * chip_size if chip_size_max is None else chip_size_max
+    nrot, ang = _angles(rotation)
     return _Params(chip_size, cmin, cmax, chip_aspect, min_search_radius, coarse_stride,
                    coarse_buffer, min_coarse_valid_fraction, fill_window, upsampling, measure,
-                   int(outliers), g or gardner())
+                   int(outliers), g or gardner(), nrot, ang)
+
+
+def rotate_chip(chip, deg):
+    """_rotate_chip (src/track.jl:382-409): Float32 chip rotated by ``deg`` degrees."""
+    chip = _f(chip)
+    out = np.empty(chip.shape, np.float32, order="F")
+    lib().orc_rotate_chip(_p(chip), C.c_int64(chip.shape[0]), _DT[chip.dtype],
+                          C.c_int64(chip.shape[0]), C.c_int64(chip.shape[1]), C.c_double(deg),
+                          _p(out))
+    return out
 
 
 def multichip(ref, sec, ok_mask, x, y, rx, ry, pdx=0.0, pdy=0.0, p=None, num_mode=NUM_DIRECT,

@@ -314,11 +314,11 @@ static void cws_free(corr_ws* w) {
 }
 
 static int correlate_ws(corr_ws* w, const void* search, int64_t lds, const void* chip,
-                        int64_t ldc, int dt, int64_t ch, int64_t cw, int64_t rx, int64_t ry,
-                        int measure, int num_mode, float* surface) {
+                        int64_t ldc, int dt, int cdt, int64_t ch, int64_t cw, int64_t rx,
+                        int64_t ry, int measure, int num_mode, float* surface) {
     int64_t nr = 2 * ry, nc = 2 * rx;
     double cnorm;
-    int ok = prepare_chip(chip, ldc, dt, ch, cw, w->chip, &cnorm);
+    int ok = prepare_chip(chip, ldc, cdt, ch, cw, w->chip, &cnorm);
     if (!ok) {
         for (int64_t k = 0; k < nr * nc; k++) surface[k] = 0.0f;
         return 0;
@@ -354,8 +354,8 @@ int orc_correlate(const void* search, int64_t ld_search, const void* chip, int64
                   int num_mode, float* surface) {
     corr_ws w;
     cws_init(&w, ch, cw, ry, rx);
-    int ok = correlate_ws(&w, search, ld_search, chip, ld_chip, dtype, ch, cw, rx, ry, measure,
-                          num_mode, surface);
+    int ok = correlate_ws(&w, search, ld_search, chip, ld_chip, dtype, dtype, ch, cw, rx, ry,
+                          measure, num_mode, surface);
     cws_free(&w);
     return ok;
 }
@@ -534,9 +534,41 @@ static void* zeropad(const void* a, int dt, int64_t nr, int64_t nc, int64_t px, 
     return out;
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* _rotate_chip -- src/track.jl:382-409: the chip rotated about its centre by `deg` degrees,   */
+/* bilinear in Float64 from the chip's samples, Float32 out, zero where a source corner falls  */
+/* outside the chip. deg2rad is deg * (pi / 180) (Julia's deg2rad for Float64); sin and cos    */
+/* are the C library's.                                                                        */
+/* ------------------------------------------------------------------------------------------ */
+void orc_rotate_chip(const void* chip, int64_t ldc, int dt, int64_t ch, int64_t cw, double deg,
+                     float* dst) {
+    const double rad = deg * (M_PI / 180.0);
+    const double s = sin(rad), c = cos(rad);
+    const double yc = (double)(ch + 1) / 2.0, xc = (double)(cw + 1) / 2.0;
+    for (int64_t j = 1; j <= cw; j++)
+        for (int64_t i = 1; i <= ch; i++) {
+            const double dyy = (double)i - yc, dxx = (double)j - xc;
+            const double sy = yc + c * dyy + s * dxx;
+            const double sx = xc - s * dyy + c * dxx;
+            const int64_t i0 = jfloor(sy), j0 = jfloor(sx);
+            float* o = dst + (i - 1) + (j - 1) * ch;
+            if (i0 < 1 || j0 < 1 || i0 + 1 > ch || j0 + 1 > cw) {
+                *o = 0.0f;
+                continue;
+            }
+            const double fy = sy - (double)i0, fx = sx - (double)j0;
+            const double v00 = ldv(chip, dt, (i0 - 1) + (j0 - 1) * ldc);
+            const double v10 = ldv(chip, dt, i0 + (j0 - 1) * ldc);
+            const double v01 = ldv(chip, dt, (i0 - 1) + j0 * ldc);
+            const double v11 = ldv(chip, dt, i0 + j0 * ldc);
+            *o = (float)((1 - fy) * ((1 - fx) * v00 + fx * v01) + fy * ((1 - fx) * v10 + fx * v11));
+        }
+}
+
 int orc_track(const void* ref, const void* sec, const uint8_t* ok_mask, int dt, int64_t ny,
               int64_t nx, const orc_points* pts, int64_t n, int64_t up, int measure,
-              int num_mode, int nthreads, float* dx, float* dy, float* corr, uint8_t* searched) {
+              int num_mode, int nthreads, int nrot, const double* angles, float* dx, float* dy,
+              float* corr, uint8_t* searched) {
     for (int64_t i = 0; i < n; i++) {
         dx[i] = NAN; dy[i] = NAN; corr[i] = NAN; searched[i] = 0;
     }
@@ -589,6 +621,8 @@ int orc_track(const void* ref, const void* sec, const uint8_t* ok_mask, int dt, 
         refine_ws rw;
         rws_init(&rw, up < 1 ? 1 : up);
         float* surf = (float*)malloc(sizeof(float) * (size_t)(4 * mrx * mry + 1));
+        float* hold = (float*)malloc(sizeof(float) * (size_t)(4 * mrx * mry + 1));
+        float* rot = (float*)malloc(sizeof(float) * (size_t)(cx * cy + 1));
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 8)
 #endif
@@ -618,8 +652,41 @@ int orc_track(const void* ref, const void* sec, const uint8_t* ok_mask, int dt, 
             int64_t ch = 2 * hy, cw = 2 * hx;
             const void* chip = elem_ptr(S, dt, (cr0 - 1) + (cc0 - 1) * NY);
             const void* win = elem_ptr(R, dt, (wr0 - 1) + (wc0 - 1) * NY);
-            int ok = correlate_ws(&w, win, NY, chip, NY, dt, ch, cw, prx, pry, measure, num_mode,
-                                  surf);
+            int ok;
+            if (nrot <= 0) {
+                ok = correlate_ws(&w, win, NY, chip, NY, dt, dt, ch, cw, prx, pry, measure,
+                                  num_mode, surf);
+            } else {
+                /* _correlate_rotations! -- src/track.jl:316-351: the surface whose maximum is
+                 * highest (strictly: an earlier angle wins a tie); every angle degenerate is a
+                 * degenerate chip */
+                float best = -INFINITY;
+                ok = 0;
+                const int64_t ns = 4 * prx * pry;
+                for (int k = 0; k < nrot; k++) {
+                    const void* cp = chip;
+                    int cdt = dt;
+                    int64_t cld = NY;
+                    if (angles[k] != 0.0) {
+                        orc_rotate_chip(chip, NY, dt, ch, cw, angles[k], rot);
+                        cp = rot;
+                        cdt = ORC_F32;
+                        cld = ch;
+                    }
+                    if (!correlate_ws(&w, win, NY, cp, cld, dt, cdt, ch, cw, prx, pry, measure,
+                                      num_mode, surf))
+                        continue;
+                    float pk = surf[0];
+                    for (int64_t q = 1; q < ns; q++)
+                        if (surf[q] > pk || surf[q] != surf[q]) pk = surf[q];
+                    if (pk > best) {
+                        best = pk;
+                        memcpy(hold, surf, sizeof(float) * (size_t)ns);
+                        ok = 1;
+                    }
+                }
+                if (ok) memcpy(surf, hold, sizeof(float) * (size_t)ns);
+            }
             if (!ok) continue;
             double pdx, pdy;
             float val;
@@ -632,6 +699,8 @@ int orc_track(const void* ref, const void* sec, const uint8_t* ok_mask, int dt, 
             corr[i] = val;
         }
         free(surf);
+        free(hold);
+        free(rot);
         rws_free(&rw);
         cws_free(&w);
     }
@@ -1222,7 +1291,8 @@ static int coarse_pass(const void* ref, const void* sec, const uint8_t* ok, int 
     cdx = malloc(sizeof(float) * M); cdy = malloc(sizeof(float) * M);
     ccorr = malloc(sizeof(float) * M); srch = malloc(M);
     orc_points cp = {x, y, rx, ry, pdx, pdy, cx, cy};
-    orc_track(ref, sec, ok, dt, ny, nx, &cp, M, 1, p->measure, num_mode, nthreads, cdx, cdy,
+    orc_track(ref, sec, ok, dt, ny, nx, &cp, M, 1, p->measure, num_mode, nthreads, p->nrot,
+              p->angles, cdx, cdy,
               ccorr, srch);
     meas = malloc(M); keep = malloc(M);
     int anym = 0;
@@ -1290,7 +1360,7 @@ int orc_multichip(const void* ref, const void* sec, const uint8_t* ok_mask, int 
             if (!cmask[k]) { rx[k] = 0; ry[k] = 0; }
         if (nsearchable(rx, ry, N) == 0) continue;
         orc_track(ref, sec, ok_mask, dt, ny, nx, &lv, N, p->upsampling, p->measure, num_mode,
-                  nthreads, fdx, fdy, fcorr, srch);
+                  nthreads, p->nrot, p->angles, fdx, fdy, fcorr, srch);
         /* _reject_and_fill! -- src/multichip.jl:318-330 */
         for (int64_t k = 0; k < N; k++) meas[k] = !isnan(fdx[k]);
         keep_from_filter(p, &p->gardner, fdx, fdy, rx, ry, meas, nr, nc, keep);

@@ -52,9 +52,14 @@ typedef struct {
     const int64_t* chip_size_y;
 } orc_points;
 
+/* nrot > 0: RotationSearch over angles[0..nrot) (degrees, `about` already subtracted). */
 int orc_track(const void* ref, const void* sec, const uint8_t* ok_mask, int dtype, int64_t ny,
               int64_t nx, const orc_points* pts, int64_t n, int64_t up, int measure,
-              int num_mode, int nthreads, float* dx, float* dy, float* corr, uint8_t* searched);
+              int num_mode, int nthreads, int nrot, const double* angles, float* dx, float* dy,
+              float* corr, uint8_t* searched);
+/* _rotate_chip: ch x cw chip (column-major, leading dimension ldc) rotated by deg degrees. */
+void orc_rotate_chip(const void* chip, int64_t ldc, int dtype, int64_t ch, int64_t cw,
+                     double deg, float* dst);
 
 /* Pre-filter and quantizer. */
 void orc_windowmean(const float* a, int64_t nr, int64_t nc, int64_t wx, int64_t wy, int hasnan,
@@ -97,6 +102,8 @@ typedef struct {
     int measure;
     int outliers_on;
     orc_gardner gardner;
+    int nrot;               /* RotationSearch angles (0: NoRotationSearch) */
+    double angles[8];
 } orc_params;
 int orc_multichip(const void* ref, const void* sec, const uint8_t* ok_mask, int dtype,
                   int64_t ny, int64_t nx, const orc_points* grid, int64_t nr, int64_t nc,

@@ -140,3 +140,32 @@ def test_pyrup_taps_closed_form():
     for n in list(range(2, 64)) + [80, 160, 320, 640]:
         for k in range(2 * n):
             assert _taps_closed(k, n) == _taps_reference(k, n), (k, n)
+
+
+def test_rotation_search_construction():
+    """RotationSearch and params(rotation=...) (test/firstguess.jl:107-153)."""
+    RS = autorift.RotationSearch
+    assert autorift.angles(RS()) == (-3.0, 0.0, 3.0)
+    assert autorift.angles(RS((-1, 0, 1))) == (-1.0, 0.0, 1.0)
+    assert autorift.angles(RS(about=8.0)) == (-11.0, -8.0, -5.0)
+    assert autorift.angles(RS((-1, 0, 1), about=-4.5)) == (3.5, 4.5, 5.5)
+    assert autorift.angles(RS((0.0,), about=-8.0)) == (8.0,)
+    for bad in ((), (0.0, 3.0, 0.0), (float("nan"), 0.0, 3.0), (-float("inf"), 0.0, 3.0)):
+        with pytest.raises(autorift.ArgumentError):
+            RS(bad)
+    with pytest.raises(autorift.ArgumentError):
+        RS(about=float("nan"))
+    with pytest.raises(autorift.ArgumentError):
+        RS((-3, 0, 3), about=float("inf"))
+    with pytest.raises(TypeError):
+        RS(about=[1.0])
+    assert autorift.params().rotation_count == 0
+    assert autorift.params(rotation=None).rotation_count == 0
+    assert autorift.params(rotation=False).rotation_count == 0
+    p = autorift.params(rotation=RS((-6, -3, 0, 3, 6), about=1.0))
+    assert p.rotation_count == 5
+    assert list(p.rotation_angles)[:5] == [-7.0, -4.0, -1.0, 2.0, 5.0]
+    with pytest.raises(autorift.ArgumentError):
+        autorift.params(rotation=RS(tuple(range(9))))
+    with pytest.raises(autorift.ArgumentError):
+        autorift.params(rotation="yes")
