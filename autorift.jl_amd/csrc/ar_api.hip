@@ -19,7 +19,7 @@
 namespace ar {
 
 // Defined in track.hip.
-template <bool BATCH>
+template <bool BATCH, int OCC>
 __global__ void track_u8_kernel(TrackArgs a);
 __global__ void refine_kernel(TrackArgs a);
 template <int IS_U8>
@@ -486,7 +486,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     }
     void (*kern)(TrackArgs);
     if (use_u8) {
-        kern = track_u8_kernel<false>;
+        kern = track_u8_kernel<false, 5>;
         // the int8 copy of the reference the matrix cores read (made once per prepared image)
         const int64_t N = ctx->ny * ctx->nx;
         const int64_t stride = (N + 1024 + 255) / 256 * 256;   // room for a window column's tail
@@ -889,7 +889,9 @@ static ar_status batch_launch(ar_batch* B, TrackArgs a, const int32_t* list, int
     a.max_rx = a.max_rx < rcap ? a.max_rx : rcap;
     a.max_ry = a.max_ry < rcap ? a.max_ry : rcap;
     a.lds_bytes = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, false).total;
-    auto kern = track_u8_kernel<true>;
+    // launches whose LDS leaves room for at most two workgroups per CU take the instance with
+    // twice the registers (no spills)
+    auto kern = a.lds_bytes > (160 * 1024) / 3 ? track_u8_kernel<true, 2> : track_u8_kernel<true, 4>;
     HIPCHK(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)a.lds_bytes));
     int per_cu = 0;
@@ -1034,10 +1036,11 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     if (R1 >= rmax) R1 = R2 = 0;
     split = R1 | (R2 << 8);
     // hipFuncSetAttribute / occupancy queries happen before a capture begins
-    auto kern = track_u8_kernel<true>;
-    HIPCHK(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry,
-                                                   false).total));
+    for (auto kern : {track_u8_kernel<true, 4>, track_u8_kernel<true, 2>})
+        HIPCHK(ctx, hipFuncSetAttribute((const void*)kern,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry,
+                                                       false).total));
     const size_t rlds = (size_t)cascade_floats((int)up) * 4;
     int64_t rblocks = 0;
     if (up > 1) {

@@ -171,8 +171,11 @@ __device__ inline void mfma_batch(const int8_t* Wl, const int8_t* Tp, int ldw, i
 // Four workgroups per CU (128 VGPRs) when the LDS layout allows it.
 // BATCH: many pairs in one launch. Point k of the launch is order[k]; its images come from its
 // pair's descriptor. The single-pair instance is the same code without the indirection.
-template <bool BATCH>
-__global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void track_u8_kernel(
+// OCC: waves per SIMD the registers are budgeted for (5: 96 VGPRs, five workgroups per CU at
+// chip 32 r 25, some spills; 4: 128; 2: 256, for launches whose LDS allows at most two
+// workgroups per CU, with no spills).
+template <bool BATCH, int OCC>
+__global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(OCC))) void track_u8_kernel(
     TrackArgs a0) {
     TrackArgs a = a0;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -646,8 +649,9 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(4))) void
     }
 }
 
-template __global__ void track_u8_kernel<false>(TrackArgs);
-template __global__ void track_u8_kernel<true>(TrackArgs);
+template __global__ void track_u8_kernel<false, 5>(TrackArgs);
+template __global__ void track_u8_kernel<true, 4>(TrackArgs);
+template __global__ void track_u8_kernel<true, 2>(TrackArgs);
 
 // ---------------------------------------------------------------------------------------------
 // generic path (FP64, reference order)
