@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -211,6 +212,38 @@ __global__ void refx_kernel(const uint8_t* src, uint8_t* dst, int64_t n, int64_t
 #pragma unroll
         for (int k = 0; k < 4; k++)
             dst[k * stride + y] = (y >= k && y - k < n) ? (uint8_t)(src[y - k] ^ 0x80) : 0x80;
+}
+
+// Every kernel that takes dynamic LDS gets the largest allowance once per process (the
+// attribute is process-wide, so lowering it for one context could break another's launch), and
+// its occupancy for a given size is cached per context: both calls cost host time on every pass
+// otherwise.
+static std::mutex g_attr_mu;
+static std::set<const void*> g_attr_set;
+
+static ar_status kernel_occupancy(ar_context* ctx, const void* kern, int threads, size_t lds,
+                                  int* per_cu) {
+    const auto key = std::make_tuple(kern, threads, lds);
+    auto it = ctx->occupancy.find(key);
+    if (it != ctx->occupancy.end()) {
+        *per_cu = it->second;
+        return AR_OK;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_attr_mu);
+        if (!g_attr_set.count(kern)) {
+            HIPCHK(ctx, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)kLdsLimit));
+            g_attr_set.insert(kern);
+        }
+    }
+    if (lds > kLdsLimit) {
+        *per_cu = 0;
+        return AR_OK;
+    }
+    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, kern, threads, lds));
+    ctx->occupancy[key] = *per_cu;
+    return AR_OK;
 }
 
 static int grid_for(int64_t n, int per_block = 256) {
@@ -570,10 +603,8 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     if ((st = ctx->refine.reserve(ctx, (size_t)n * 132))) return st;
     a.rpatch = (float*)ctx->refine.ptr;
     a.rmeta = (int32_t*)((char*)ctx->refine.ptr + (size_t)n * 128);
-    HIPCHK(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)a.lds_bytes));
     int per_cu = 0;
-    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, AR_WG, a.lds_bytes));
+    if ((st = kernel_occupancy(ctx, (const void*)kern, AR_WG, a.lds_bytes, &per_cu))) return st;
     if (per_cu < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "tracker kernel does not fit on a CU");
     int64_t nblocks = (int64_t)ctx->ncu * per_cu;
     if (nblocks > n) nblocks = n;
@@ -587,10 +618,8 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     const size_t rlds = (size_t)cascade_floats((int)up) * 4;
     int64_t rblocks = 0;
     if (up > 1) {
-        HIPCHK(ctx, hipFuncSetAttribute((const void*)refine_kernel,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
         int rpc = 0;
-        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&rpc, refine_kernel, 64, rlds));
+        if ((st = kernel_occupancy(ctx, (const void*)refine_kernel, 64, rlds, &rpc))) return st;
         if (rpc < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "refinement kernel does not fit on a CU");
         rblocks = (int64_t)ctx->ncu * rpc;
         if (rblocks > n) rblocks = n;
@@ -892,10 +921,9 @@ static ar_status batch_launch(ar_batch* B, TrackArgs a, const int32_t* list, int
     // launches whose LDS leaves room for at most two workgroups per CU take the instance with
     // twice the registers (no spills)
     auto kern = a.lds_bytes > (160 * 1024) / 3 ? track_u8_kernel<true, 2> : track_u8_kernel<true, 4>;
-    HIPCHK(ctx, hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)a.lds_bytes));
     int per_cu = 0;
-    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, AR_WG, a.lds_bytes));
+    if (ar_status st = kernel_occupancy(ctx, (const void*)kern, AR_WG, a.lds_bytes, &per_cu))
+        return st;
     if (per_cu < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "batched tracker does not fit on a CU");
     int64_t nb = (int64_t)ctx->ncu * per_cu;
     if (nb > m) nb = m;
@@ -1035,19 +1063,24 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     if (R2 > 0 && (R2 <= R1 || R2 >= rmax)) R2 = 0;
     if (R1 >= rmax) R1 = R2 = 0;
     split = R1 | (R2 << 8);
-    // hipFuncSetAttribute / occupancy queries happen before a capture begins
-    for (auto kern : {track_u8_kernel<true, 4>, track_u8_kernel<true, 2>})
-        HIPCHK(ctx, hipFuncSetAttribute((const void*)kern,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry,
-                                                       false).total));
+    // occupancy queries (and the once-per-process LDS allowances) happen before a capture
+    // begins: every class's size is queried here
+    {
+        int pc = 0;
+        const int caps0[3] = {R1 > 0 ? R1 : 1 << 30, R2 > 0 ? R2 : 1 << 30, 1 << 30};
+        for (int k = 0; k < 3; k++) {
+            const int rc = caps0[k];
+            const size_t l = u8_layout(a.max_ch, a.max_cw, a.max_rx < rc ? a.max_rx : rc,
+                                       a.max_ry < rc ? a.max_ry : rc, false).total;
+            auto kk = l > (160 * 1024) / 3 ? track_u8_kernel<true, 2> : track_u8_kernel<true, 4>;
+            if ((st = kernel_occupancy(ctx, (const void*)kk, AR_WG, l, &pc))) return st;
+        }
+    }
     const size_t rlds = (size_t)cascade_floats((int)up) * 4;
     int64_t rblocks = 0;
     if (up > 1) {
-        HIPCHK(ctx, hipFuncSetAttribute((const void*)refine_kernel,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
         int rpc = 0;
-        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&rpc, refine_kernel, 64, rlds));
+        if ((st = kernel_occupancy(ctx, (const void*)refine_kernel, 64, rlds, &rpc))) return st;
         if (rpc < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "refinement kernel does not fit on a CU");
         rblocks = (int64_t)ctx->ncu * rpc;
         if (rblocks > n) rblocks = n;

@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/autorift_hip.h"
@@ -46,6 +48,9 @@ struct ar_context {
     uint64_t ref_gen = 0, refx_gen = ~0ULL;   // versions of the pair and of the copies made from it
     float absmax = 0.0f;        // largest |value| of the prepared Float32 pair (FP16 operand scale)
     uint64_t absmax_gen = ~0ULL;
+
+    // workgroups per CU by (kernel, threads per workgroup, dynamic LDS bytes)
+    std::map<std::tuple<const void*, int, size_t>, int> occupancy;
 
     double last_ms = 0, last_ms_box = 0, last_ms_track = 0, last_ms_refine = 0;
     int32_t last_path = -1;

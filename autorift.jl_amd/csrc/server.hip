@@ -416,8 +416,11 @@ ar_status ar_server_stats(ar_server* S, int64_t* stats) {
 
 const char* ar_server_last_error(ar_server* S) {
     if (!S) return "null server";
+    // a copy for the calling thread: the server's own message may change under another thread
+    thread_local std::string msg;
     std::lock_guard<std::mutex> lk(S->mu);
-    return S->err.c_str();
+    msg = S->err;
+    return msg.c_str();
 }
 
 }  // extern "C"

@@ -233,6 +233,8 @@ ar_status ar_server_wait(ar_server* s, int64_t ticket, float* dx, float* dy, flo
 /* 8 counters: completed, queued, running, peak running, device bytes held by the slots, peak
  * bytes held, budget, slots. */
 ar_status ar_server_stats(ar_server* s, int64_t* stats);
+/* The server's last error message; the pointer stays valid in the calling thread until its next
+ * call of this function. */
 const char* ar_server_last_error(ar_server* s);
 
 #ifdef __cplusplus
