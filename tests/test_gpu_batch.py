@@ -145,3 +145,40 @@ def test_batch_errors():
             bt.track(ps, pair)               # indices 1..3 are not in the batch
     with pytest.raises(autorift.AutoRIFTError):
         autorift.Batch(autorift.params(quantize="none"))
+
+
+def test_batch_with_masks_and_gaps():
+    """Pairs with invalid pixels (NaN gaps and explicit masks): each point's result equals a pass
+    over its own pair alone, including the validity test of its chip (src/track.jl:416-421)."""
+    pairs, ps, pair = _scene(5, n_per=50)
+    masks = []
+    rng = np.random.default_rng(9)
+    for k, (a, b) in enumerate(pairs):
+        a = a.copy()
+        b = b.copy()
+        ny, nx = a.shape
+        if k % 2 == 0:
+            a[ny // 3:ny // 3 + 40, :nx // 2] = np.nan          # a gap: non-finite pixels
+            masks.append(None)
+        else:
+            m = rng.random((ny, nx)) > 0.02
+            m[:, : nx // 4] = False                          # a masked-off strip
+            masks.append(m)
+        pairs[k] = (a, b)
+    print("batch masks", flush=True)
+    exp = [np.full(len(ps), np.nan, np.float32) for _ in range(3)] + [np.zeros(len(ps), bool)]
+    with autorift.Context(autorift.params()) as ctx:
+        for k, (a, b) in enumerate(pairs):
+            sel = np.nonzero(pair == k)[0]
+            ctx.set_images(a, b, masks[k], masks[k])
+            r = ctx.track(ps[sel], upsampling=64)
+            for o, v in zip(exp, (r.dx, r.dy, r.correlation, r.searched)):
+                o[sel] = v
+    with autorift.Batch(autorift.params()) as bt:
+        for k, (a, b) in enumerate(pairs):
+            bt.add_pair(a, b, masks[k], masks[k])
+        got = bt.track(ps, pair, upsampling=64)
+    np.testing.assert_array_equal(got.searched, exp[3])
+    for g, e in zip((got.dx, got.dy, got.correlation), exp[:3]):
+        np.testing.assert_array_equal(_bits(g), _bits(e))
+    assert (~got.searched).any() and got.searched.any()

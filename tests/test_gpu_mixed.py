@@ -154,3 +154,28 @@ def test_mixed_chip_sizes_rejected():
         ctx.set_images(t, t)
         with pytest.raises(autorift.AutoRIFTError, match="one chip size"):
             ctx.track(pts, upsampling=1)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_gaps_and_masks(prec):
+    """NaN gaps in the raw images (the pre-filter zeroes them and marks them invalid) and an
+    explicit mask: NaN placement and searched flags equal the exact path's, peaks agree."""
+    a, b = cfg1_pair(2, n=320)
+    a = a.copy()
+    a[100:140, 50:200] = np.nan
+    m = np.ones(a.shape, bool)
+    m[:, 260:] = False
+    xs = np.arange(40.0, 281.0, 16.0)
+    pts = autorift.gridpoints_xy(xs, xs, chip_size=32, search_radius=12)
+    out = []
+    for pr in (prec, "exact"):
+        with autorift.Context(autorift.params(quantize="none", precision=pr)) as ctx:
+            ctx.set_images(a, b, m, m)
+            out.append(ctx.track(pts, upsampling=16))
+    got, exp = out
+    print(f"gaps {prec}", flush=True)
+    np.testing.assert_array_equal(got.searched, exp.searched)
+    np.testing.assert_array_equal(np.isnan(got.dx), np.isnan(exp.dx))
+    fin = np.isfinite(exp.dx)
+    err = np.hypot(got.dx[fin] - exp.dx[fin], got.dy[fin] - exp.dy[fin])
+    assert np.mean(err == 0) >= 0.98 and (~got.searched).any()

@@ -88,3 +88,22 @@ def test_unknown_ticket_and_bad_input():
             srv.result(12345)
         with pytest.raises(autorift.DimensionMismatch):
             srv.submit(np.zeros((64, 64), np.float32), np.zeros((64, 65), np.float32))
+
+
+def test_served_pairs_with_masks():
+    """Validity masks travel with a served pair: results equal the pair run alone with them."""
+    p = autorift.params()
+    pairs = _pairs()[:3]
+    rng = np.random.default_rng(4)
+    masks = [rng.random(a.shape) > 0.05 for a, _ in pairs]
+    alone = []
+    with autorift.Context(p) as ctx:
+        for (a, b), m in zip(pairs, masks):
+            ctx.set_images(a, b, m, m)
+            alone.append(ctx.run_multichip(autorift.api._build_grid(a.shape, p)))
+    print("serve masks", flush=True)
+    with autorift.Server(p, slots=2) as srv:
+        tickets = [srv.submit(a, b, m, m) for (a, b), m in zip(pairs, masks)]
+        for t, e in zip(tickets, alone):
+            r, _ = srv.result(t)
+            _same(r, e)
