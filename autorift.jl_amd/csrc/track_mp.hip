@@ -7,8 +7,10 @@
 //     (src/correlate.jl:423-453) before it is rounded to the 16-bit type;
 //   * the box sums S1 = sum W and S2 = sum W^2 come from Float64 box images of the pass
 //     (box_f64_colpass_kernel: direct column sums; box_f64_rowpass_kernel: sliding row sums
-//     restarted every 64 columns), and the ZNCC / NCC normalisation is the reference's Float64
-//     formula (src/correlate.jl:575-630);
+//     restarted every 64 columns) and the window variance s2 - s1^2 / n is formed in Float64;
+//     the screening score of each shift divides the FP32 numerator by cnorm sqrt(variance) in
+//     FP32, and the values reported (peak and 5x5 patch) use the reference's Float64
+//     normalisation (src/correlate.jl:575-630);
 //   * the integer peak's value and the 5x5 patch the subpixel fit starts from are recomputed
 //     with the Float64 direct numerator (src/correlate.jl:799-814), so the pyrUp cascade
 //     (src/peak.jl:456-522) starts from full-precision samples.
@@ -139,6 +141,7 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
         const int winh = ch + NR - 1, winw = cw + NC - 1;
 
         // ---- chip: Float32 samples, then the reference's sequential Float64 mean and energy
+#pragma unroll 4
         for (int v = wave; v < cw; v += AR_NWAVES)
             for (int u = lane; u < ch; u += 64) chipf[u + v * ch] = mp_pix(a, sec, g.cr0 + u, g.cc0 + v);
         __syncthreads();
@@ -169,13 +172,24 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
                 Cl[v * L.cpad + r] = MpType<PREC>::bits(c * a.cscale);
             }
         __syncthreads();
-        // ---- window, scaled to the 16-bit type; rows and columns past it zero
-        for (int c = wave; c < L.cols; c += AR_NWAVES)
-            for (int r = lane; r < L.ldw; r += 64) {
-                float w = 0.0f;
-                if (c < winw && r < winh) w = mp_pix(a, ref, g.wr0 + r, g.wc0 + c);
-                Wl[c * L.ldw + r] = MpType<PREC>::bits(w * a.wscale);
+        // ---- window, scaled to the 16-bit type; rows and columns past it zero. A window inside
+        // the image is read through one base pointer with no per-pixel padding test.
+        {
+            const int64_t orow = g.wr0 - a.pady - 1, ocol = g.wc0 - a.padx - 1;
+            const bool inside = orow >= 0 && ocol >= 0 && orow + winh <= a.ny &&
+                                ocol + winw <= a.nx;
+            const float* wb = ref + (inside ? orow + ocol * a.ny : 0);
+#pragma unroll 4
+            for (int c = wave; c < L.cols; c += AR_NWAVES) {
+                const float* col = wb + (int64_t)c * a.ny;
+                for (int r = lane; r < L.ldw; r += 64) {
+                    float w = 0.0f;
+                    if (c < winw && r < winh)
+                        w = inside ? col[r] : mp_pix(a, ref, g.wr0 + r, g.wc0 + c);
+                    Wl[c * L.ldw + r] = MpType<PREC>::bits(w * a.wscale);
+                }
             }
+        }
         __syncthreads();
 
         // ---- numerators on the matrix cores
@@ -214,17 +228,27 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
         }
         __syncthreads();
 
-        // ---- Float64 normalisation per shift and the argmax (peak_index rule)
+        // ---- normalisation per shift and the argmax (peak_index rule): the window variance
+        // s2 - s1^2 / n in Float64 (where it cancels), the rest of the 16-bit surface's score in
+        // FP32 (the exact values of the chosen peak and patch are Float64, below)
         const double n = (double)(ch * cw);
         const int64_t bb = (g.wr0 - 1) + (g.wc0 - 1) * a.box_ld;
+        const float inv_cn = (float)(1.0 / cnorm);
+        const float nsc = (float)a.numscale;
+        const double inv_n = 1.0 / n;
         Best b = best_empty();
-        for (int o = tid; o < NR * NC; o += AR_WG) {
-            const int j = o / NR, i = o - j * NR;
-            const int64_t q = bb + i + (int64_t)j * a.box_ld;
-            const float v = mp_normalise((double)num[o] * a.numscale, a.fbox1[q], a.fbox2[q], n,
-                                         cnorm, a.measure);
-            const int idx = i * NC + j;
-            if (better(v, idx, b)) { b.v = v; b.idx = idx; }
+        for (int j = wave; j < NC; j += AR_NWAVES) {
+            const double* b1 = a.fbox1 + bb + (int64_t)j * a.box_ld;
+            const double* b2 = a.fbox2 + bb + (int64_t)j * a.box_ld;
+            for (int i = lane; i < NR; i += 64) {
+                const double s2 = b2[i], s1 = b1[i];
+                const double wv = a.measure == 1 ? s2 : s2 - s1 * s1 * inv_n;
+                const float den = (float)(wv < 0.0 ? 0.0 : wv);
+                const float v =
+                    den > 0.0f ? num[j * NR + i] * nsc * inv_cn * __builtin_amdgcn_rsqf(den) : 0.0f;
+                const int idx = i * NC + j;
+                if (better(v, idx, b)) { b.v = v; b.idx = idx; }
+            }
         }
         b = block_best(b, red);
         if (b.idx == 0x7fffffff) b.idx = 0;
@@ -237,24 +261,34 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
         // staged in LDS first (the 16-bit window and the scores are dead by now).
         const int bi = refine ? i0 - 1 : pi - 1, bj = refine ? j0 - 1 : pj - 1;
         const int ph = ch + 4, pw = cw + 4;
+#pragma unroll 4
         for (int c = wave; c < pw; c += AR_NWAVES)
             for (int r = lane; r < ph; r += 64)
                 pwin[r + c * ph] = (bi + r < winh && bj + c < winw)
                                        ? mp_pix(a, ref, g.wr0 + bi + r, g.wc0 + bj + c)
                                        : 0.0f;
+#pragma unroll 4
         for (int v = wave; v < cw; v += AR_NWAVES)
             for (int u = lane; u < ch; u += 64)
                 pchip[u + v * ch] = (float)((double)mp_pix(a, sec, g.cr0 + u, g.cc0 + v) - mean);
         __syncthreads();
-        // value v (0..24 the patch, row-major; 25 the peak alone) is wave v % 4's: its lanes
-        // take terms k = lane, lane + 64, ... (chip k, window k + 4 (k / ch) of the block)
+        // value v (0..24 the patch, row-major; 25 the peak alone) is wave v % 4's. Lanes map to
+        // chip rows and column groups: for ch <= 64, lane = jg ch + ii takes rows ii of columns
+        // jg, jg + G, ... (G = 64 / ch); taller chips take rows lane and lane + 64 of every column
+        const int G = ch <= 64 ? 64 / ch : 1;
+        const int ii0 = ch <= 64 ? lane % ch : lane, jg = ch <= 64 ? lane / ch : 0;
+        const bool active = jg < G;
         for (int v = wave; v < 26; v += AR_NWAVES) {
             if ((v < 25) != refine) continue;
             const int di = v < 25 ? v / 5 : 0, dj = v < 25 ? v % 5 : 0;
             const float* wb = pwin + dj * ph + di;
             double acc = 0.0;
-            for (int k = lane; k < nn; k += 64)
-                acc += (double)pchip[k] * (double)wb[k + 4 * (k / ch)];
+            if (active)
+                for (int jj = jg; jj < cw; jj += G) {
+                    const float* cc = pchip + jj * ch;
+                    const float* wc = wb + jj * ph;
+                    for (int ii = ii0; ii < ch; ii += 64) acc += (double)cc[ii] * (double)wc[ii];
+                }
             acc = wave_sum(acc);
             if (lane == 0) {
                 const int64_t q = bb + (bi + di) + (int64_t)(bj + dj) * a.box_ld;
