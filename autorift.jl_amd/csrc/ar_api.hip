@@ -435,7 +435,8 @@ extern "C" ar_status ar_get_pair(ar_context* ctx, void* ref, void* sec, uint8_t*
 
 // The pass on device-resident points. All pointers device pointers.
 ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t up,
-                        int32_t measure, float* dx, float* dy, float* corr, uint8_t* srch) {
+                        int32_t measure, float* dx, float* dy, float* corr, uint8_t* srch,
+                        const ar_pass_hint* hint) {
     if (!ctx->has_pair) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "no image pair set");
     if (n < 0) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "negative point count");
     if (up < 1 || up > 64 || (up & (up - 1)) != 0)
@@ -445,18 +446,34 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         return fail(ctx, AR_ERR_INVALID_ARGUMENT, "measure must be ZNCC or NCC");
     if (n == 0) return AR_OK;
     ar_status st;
-    unsigned long long gs[10 * AR_GEOM_SLOT] = {0};
-    if ((st = ctx->geom.reserve(ctx, sizeof(gs)))) return st;
-    for (int k = 6; k < 10; k++) gs[AR_GEOM_SLOT * k] = k < 8 ? ~0ULL : 1ULL;
-    HIPCHK(ctx, hipMemcpyAsync(ctx->geom.ptr, gs, sizeof(gs), hipMemcpyHostToDevice, ctx->stream));
-    pass_geometry_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, ctx->ny, ctx->nx,
-                                                               (unsigned long long*)ctx->geom.ptr,
-                                                               nullptr, nullptr);
-    HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipMemcpyAsync(gs, ctx->geom.ptr, sizeof(gs), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     unsigned long long g[10];
-    for (int k = 0; k < 10; k++) g[k] = gs[AR_GEOM_SLOT * k];
+    if (hint) {
+        // the caller's bound: one chip size, every point's chip area that of the hint
+        const int64_t ex = 2 * (hint->cx / 2), ey = 2 * (hint->cy / 2), area = ex * ey;
+        g[0] = (unsigned long long)hint->cx;
+        g[1] = (unsigned long long)hint->cy;
+        g[2] = (unsigned long long)hint->rx;
+        g[3] = (unsigned long long)hint->ry;
+        g[4] = (unsigned long long)hint->px;
+        g[5] = (unsigned long long)hint->py;
+        g[6] = (unsigned long long)ex;
+        g[7] = (unsigned long long)ey;
+        g[8] = hint->fits ? 1 : 0;
+        g[9] = (area > 0 && (area & (area - 1)) == 0) ? 1 : 0;
+    } else {
+        unsigned long long gs[10 * AR_GEOM_SLOT] = {0};
+        if ((st = ctx->geom.reserve(ctx, sizeof(gs)))) return st;
+        for (int k = 6; k < 10; k++) gs[AR_GEOM_SLOT * k] = k < 8 ? ~0ULL : 1ULL;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->geom.ptr, gs, sizeof(gs), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        pass_geometry_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(
+            dp, n, ctx->ny, ctx->nx, (unsigned long long*)ctx->geom.ptr, nullptr, nullptr);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipMemcpyAsync(gs, ctx->geom.ptr, sizeof(gs), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        for (int k = 0; k < 10; k++) g[k] = gs[AR_GEOM_SLOT * k];
+    }
     int64_t cx = g[0], cy = g[1], rx = g[2], ry = g[3], px = g[4], py = g[5];
     // one even chip extent for every searchable point
     const bool uniform = g[6] == (unsigned long long)(2 * (cx / 2)) &&
@@ -639,19 +656,29 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         HIPCHK(ctx, hipGetLastError());
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timing_pending = true;
+    ctx->pending_box = box_timed;
+    ctx->pending_refine = up > 1;
+    ctx->last_path = use_u8 ? 0 : (use_mp ? 2 : 1);
+    if (hint) return AR_OK;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    float ms = 0, ms_t = 0, ms_r = 0;
+    ar_resolve_timing(ctx);
+    return AR_OK;
+}
+
+void ar_resolve_timing(ar_context* ctx) {
+    if (!ctx->timing_pending) return;
+    ctx->timing_pending = false;
+    if (hipEventSynchronize(ctx->ev1) != hipSuccess) return;
+    float ms = 0, ms_t = 0, ms_r = 0, ms_b = 0;
     hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     hipEventElapsedTime(&ms_t, ctx->ev0, ctx->evm);
     hipEventElapsedTime(&ms_r, ctx->evm, ctx->ev1);
-    float ms_b = 0;
-    if (box_timed) hipEventElapsedTime(&ms_b, ctx->evb, ctx->ev0);
+    if (ctx->pending_box) hipEventElapsedTime(&ms_b, ctx->evb, ctx->ev0);
     ctx->last_ms = ms;
     ctx->last_ms_box = ms_b;
     ctx->last_ms_track = ms_t;
-    ctx->last_ms_refine = up > 1 ? ms_r : 0.0;
-    ctx->last_path = use_u8 ? 0 : (use_mp ? 2 : 1);
-    return AR_OK;
+    ctx->last_ms_refine = ctx->pending_refine ? ms_r : 0.0;
 }
 
 extern "C" ar_status ar_track_device(ar_context* ctx, const ar_points* dev_pts, int64_t n,
@@ -726,6 +753,7 @@ extern "C" ar_status ar_track(ar_context* ctx, const ar_points* pts, int64_t n,
 
 extern "C" ar_status ar_last_kernel(const ar_context* ctx, double* ms, int32_t* path) {
     if (!ctx) return AR_ERR_INVALID_ARGUMENT;
+    ar_resolve_timing(const_cast<ar_context*>(ctx));
     if (ms) *ms = ctx->last_ms;
     if (path) *path = ctx->last_path;
     return AR_OK;
@@ -734,6 +762,7 @@ extern "C" ar_status ar_last_kernel(const ar_context* ctx, double* ms, int32_t* 
 extern "C" ar_status ar_last_timing(const ar_context* ctx, double* box_ms, double* track_ms,
                                     double* refine_ms) {
     if (!ctx) return AR_ERR_INVALID_ARGUMENT;
+    ar_resolve_timing(const_cast<ar_context*>(ctx));
     if (box_ms) *box_ms = ctx->last_ms_box;
     if (track_ms) *track_ms = ctx->last_ms_track;
     if (refine_ms) *refine_ms = ctx->last_ms_refine;

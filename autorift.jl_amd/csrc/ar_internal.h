@@ -53,6 +53,9 @@ struct ar_context {
     std::map<std::tuple<const void*, int, size_t>, int> occupancy;
 
     double last_ms = 0, last_ms_box = 0, last_ms_track = 0, last_ms_refine = 0;
+    // a pass that returned without waiting (multi-chip levels): its times are read from the
+    // events when asked for
+    bool timing_pending = false, pending_box = false, pending_refine = false;
     int32_t last_path = -1;
 
     std::vector<DevBuf*> all_bufs() {
@@ -64,8 +67,19 @@ struct ar_context {
 // internal entry points shared between translation units
 ar_status ar_upload_masks(ar_context* ctx, const uint8_t* ref_valid, const uint8_t* sec_valid,
                           int64_t N);
+// Pass geometry a caller already knows (an upper bound is enough: the kernels' layouts only need
+// to cover every point, and a wider virtual padding adds invalid zeros that change no result):
+// largest chip extents, radii and padding reach, whether every window lies inside the image.
+struct ar_pass_hint {
+    int64_t cx, cy, rx, ry, px, py;
+    bool fits;
+};
+// With a hint the pass also returns without waiting for the stream (the caller's next readback
+// orders it); its kernel times are resolved by ar_resolve_timing.
+void ar_resolve_timing(ar_context* ctx);
 ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t up,
-                        int32_t measure, float* dx, float* dy, float* corr, uint8_t* srch);
+                        int32_t measure, float* dx, float* dy, float* corr, uint8_t* srch,
+                        const ar_pass_hint* hint = nullptr);
 ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t padx,
                         int32_t** b1, int32_t** b2, int64_t* ld);
 // device bytes a context holds, and giving them all back (serving: idle slots make room)

@@ -9,6 +9,7 @@
 // so the result does not depend on the launch order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -401,6 +402,18 @@ static ar_status mfail(ar_context* ctx, ar_status st, const std::string& m) {
                          std::string(#expr) + ": " + hipGetErrorString(_e));                  \
     } while (0)
 
+// Counts of two flag arrays of n bytes with one readback.
+static ar_status count_flags2(ar_context* ctx, const uint8_t* f0, const uint8_t* f1, int64_t n,
+                              unsigned long long* cnt, unsigned long long* out) {
+    MCHK(hipMemsetAsync(cnt, 0, 16, ctx->stream));
+    count_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(f0, n, cnt);
+    count_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(f1, n, cnt + 1);
+    MCHK(hipGetLastError());
+    MCHK(hipMemcpyAsync(out, cnt, 16, hipMemcpyDeviceToHost, ctx->stream));
+    MCHK(hipStreamSynchronize(ctx->stream));
+    return AR_OK;
+}
+
 static ar_status count_flags(ar_context* ctx, const uint8_t* f, int64_t n,
                              unsigned long long* cnt, unsigned long long* out) {
     MCHK(hipMemsetAsync(cnt, 0, 8, ctx->stream));
@@ -509,18 +522,58 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
     fill_result_kernel<<<gblocks(N), GK, 0, s>>>(rdx, rdy, rc, rcs, rit, N);
     MCHK(hipGetLastError());
 
+    // What the host knows of every pass's geometry without asking the device: the grid's
+    // extent, its largest radius and prior. A level's points (coarse and fine) sit at grid
+    // positions with at most that radius, so when every grid position's window fits the image at
+    // that radius the passes need no padding and their geometry is known here.
+    bool grid_known = true;
+    double fx_lo = 0, fx_hi = 0, fy_lo = 0, fy_hi = 0, prior_max = 0;
+    int64_t r0x = 0, r0y = 0;
+    for (int64_t k = 0; k < N && grid_known; k++) {
+        const double x = grid->x[k], y = grid->y[k];
+        const double ax = std::fabs(grid->dx_prior[k]), ay = std::fabs(grid->dy_prior[k]);
+        if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(ax) || !std::isfinite(ay) ||
+            std::fabs(x) > 1e15 || std::fabs(y) > 1e15 || ax > 1e15 || ay > 1e15) {
+            grid_known = false;
+            break;
+        }
+        const double fx = std::floor(x), fy = std::floor(y);
+        if (k == 0) { fx_lo = fx_hi = fx; fy_lo = fy_hi = fy; }
+        fx_lo = std::min(fx_lo, fx); fx_hi = std::max(fx_hi, fx);
+        fy_lo = std::min(fy_lo, fy); fy_hi = std::max(fy_hi, fy);
+        prior_max = std::max(prior_max, std::max(ax, ay));
+        if (grid->radius_x[k] > 0 && grid->radius_y[k] > 0) {
+            r0x = std::max(r0x, grid->radius_x[k]);
+            r0y = std::max(r0y, grid->radius_y[k]);
+        }
+    }
+
     ar_status st;
     for (int64_t cs = p.chip_size_base; cs <= p.chip_size_max; cs *= 2) {
         if (cs < p.chip_size_min) continue;
-        unsigned long long c = 0;
-        wanted_kernel<<<gblocks(N), GK, 0, s>>>(rcs, N, flags);
-        if ((st = count_flags(ctx, flags, N, cnt, &c))) return st;
-        if (c == 0) break;
+        unsigned long long c2[2] = {0, 0};
         int64_t csx = cs, csy = 2 * (int64_t)std::nearbyint((double)cs * p.chip_aspect / 2.0);
+        wanted_kernel<<<gblocks(N), GK, 0, s>>>(rcs, N, pend);
         level_points_kernel<<<gblocks(N), GK, 0, s>>>(grx, gry, rcs, N, csx, csy,
                                                       p.min_search_radius, lrx, lry, lcx, lcy, flags);
-        if ((st = count_flags(ctx, flags, N, cnt, &c))) return st;
-        if (c == 0) continue;
+        if ((st = count_flags2(ctx, pend, flags, N, cnt, c2))) return st;
+        if (c2[0] == 0) break;
+        if (c2[1] == 0) continue;
+        unsigned long long c = 0;
+        ar_pass_hint hint{};
+        const ar_pass_hint* hp = nullptr;
+        if (grid_known && r0x > 0 && r0y > 0) {
+            const int64_t Rx = std::max(r0x, p.min_search_radius);
+            const int64_t Ry = std::max(r0y, p.min_search_radius);
+            const int64_t hx = csx / 2, hy = csy / 2;
+            const bool fits = fy_lo - hy - Ry >= 1 && fy_hi + hy + Ry - 2 <= ctx->ny &&
+                              fx_lo - hx - Rx >= 1 && fx_hi + hx + Rx - 2 <= ctx->nx;
+            if (fits) {
+                const int64_t q = (int64_t)std::ceil(prior_max);
+                hint = ar_pass_hint{csx, csy, Rx, Ry, hx + Rx + q, hy + Ry + q, true};
+                hp = &hint;
+            }
+        }
 
         // ---- coarse pass
         Win dummy;
@@ -531,10 +584,11 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
             coarse_points_kernel<<<gblocks(M), GK, 0, s>>>(gx, gy, gpdx, gpdy, lrx, lry, nr, nc, stride,
                                                            nrc, ncc, csx, csy, cxp, cyp, cpdx, cpdy,
                                                            crx, cry, ccx, ccy, cs8);
-            if ((st = count_flags(ctx, cs8, M, cnt, &c))) return st;
-            if (c == 0) continue;
+            // (no coarse point searchable: the pass measures nothing and the count below ends
+            // the level)
+            MCHK(hipGetLastError());
             ar_points cp = {cxp, cyp, crx, cry, cpdx, cpdy, ccx, ccy};
-            if ((st = ar_track_pass(ctx, cp, M, 1, p.measure, cdx, cdy, ccr, cs8))) return st;
+            if ((st = ar_track_pass(ctx, cp, M, 1, p.measure, cdx, cdy, ccr, cs8, hp))) return st;
             measured_kernel<<<gblocks(M), GK, 0, s>>>(cdx, M, cmeas);
             if ((st = count_flags(ctx, cmeas, M, cnt, &c))) return st;
             if (c == 0) continue;
@@ -563,7 +617,8 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
 
         // ---- fine pass, rejection, hole filling, merge
         ar_points lp = {gx, gy, lrx, lry, gpdx, gpdy, lcx, lcy};
-        if ((st = ar_track_pass(ctx, lp, N, p.upsampling, p.measure, fdx, fdy, fc, fs))) return st;
+        if ((st = ar_track_pass(ctx, lp, N, p.upsampling, p.measure, fdx, fdy, fc, fs, hp)))
+            return st;
         measured_kernel<<<gblocks(N), GK, 0, s>>>(fdx, N, meas);
         uint8_t* keep = nullptr;
         if ((st = reject(ctx, fdx, fdy, lrx, lry, meas, nr, nc, p.gardner_iterations, nxb, nyb, ka,
@@ -573,11 +628,11 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
         MCHK(hipMemsetAsync(filled, 0, N, s));
         int fwn = (int)p.fill_window;
         int lo = fwn / 2, hi = fwn - 1 - lo, needed = 2 * fwn * fwn / 3;
+        // three passes without counting: a pass with nothing pending changes nothing, so the
+        // reference's early exit is an optimisation only
         for (int pass = 0; pass < 3; pass++) {
             fill_pending_kernel<<<gblocks(N), GK, 0, s>>>(fdx, fdy, nr, nc, lo, hi, needed, px, py,
                                                           pend);
-            if ((st = count_flags(ctx, pend, N, cnt, &c))) return st;
-            if (c == 0) break;
             fill_apply_kernel<<<gblocks(N), GK, 0, s>>>(fdx, fdy, px, py, pend, filled, N);
         }
         merge_kernel<<<gblocks(N), GK, 0, s>>>(rdx, rdy, rc, rcs, rit, fdx, fdy, fc, filled,
