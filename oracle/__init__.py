@@ -15,12 +15,13 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-U8, F32, F64, I16 = 0, 1, 2, 3
-ZNCC, NCC = 0, 1
+U8, F32, F64, I16, C64 = 0, 1, 2, 3, 4
+ZNCC, NCC, COHERENCE = 0, 1, 2
 NUM_DIRECT, NUM_FFT = 0, 1
+ERR_MEASURE, ERR_ROTATION = -2, -3
 
 _DT = {np.dtype(np.uint8): U8, np.dtype(np.float32): F32, np.dtype(np.float64): F64,
-       np.dtype(np.int16): I16}
+       np.dtype(np.int16): I16, np.dtype(np.complex64): C64}
 
 _lib = None
 
@@ -82,7 +83,7 @@ class _Params(C.Structure):
                 ("coarse_buffer", C.c_int64), ("min_coarse_valid_fraction", C.c_double),
                 ("fill_window", C.c_int64), ("upsampling", C.c_int64), ("measure", C.c_int),
                 ("outliers_on", C.c_int), ("gardner", _Gardner), ("nrot", C.c_int),
-                ("angles", C.c_double * 8)]
+                ("angles", C.c_double * 8), ("nmeas", C.c_int), ("measures", C.c_int * 8)]
 
 
 def next_fft_size(n: int) -> int:
@@ -166,9 +167,14 @@ def track(ref, sec, ok_mask, x, y, rx, ry, csx, csy, pdx=0.0, pdy=0.0, up=1, mea
     corr = np.empty(n, np.float32)
     srch = np.empty(n, np.uint8)
     nrot, ang = _angles(rotation)
-    lib().orc_track(_p(ref), _p(sec), _p(ok), _DT[ref.dtype], C.c_int64(ref.shape[0]),
-                    C.c_int64(ref.shape[1]), C.byref(pts), C.c_int64(n), C.c_int64(up), measure,
-                    num_mode, nthreads, nrot, ang, _p(dx), _p(dy), _p(corr), _p(srch))
+    rc = lib().orc_track(_p(ref), _p(sec), _p(ok), _DT[ref.dtype], C.c_int64(ref.shape[0]),
+                         C.c_int64(ref.shape[1]), C.byref(pts), C.c_int64(n), C.c_int64(up),
+                         measure, num_mode, nthreads, nrot, ang, _p(dx), _p(dy), _p(corr),
+                         _p(srch))
+    if rc == ERR_MEASURE:
+        raise ValueError("Coherence needs complex images and ZNCC / NCC real ones")
+    if rc == ERR_ROTATION:
+        raise ValueError("rotation search needs real images")
     r = lambda a: a.reshape(shape, order="F")
     return r(dx), r(dy), r(corr), r(srch).astype(bool)
 
@@ -280,13 +286,43 @@ def fill_holes(dx, dy, w=3):
 def params(chip_size=32, chip_size_min=None, chip_size_max=None, chip_aspect=1.0,
            min_search_radius=6, coarse_stride=4, coarse_buffer=8,
            min_coarse_valid_fraction=0.01, fill_window=3, upsampling=64, measure=ZNCC,
-           outliers=True, g=None, rotation=None):
+           outliers=True, g=None, rotation=None, measures=None):
+    """``measures``: one measure per chip-size level, the last repeating (a similarity
+    tuple), or None for ``measure`` everywhere."""
     cmin = chip_size if chip_size_min is None else chip_size_min
     cmax = 4 * chip_size if chip_size_max is None else chip_size_max
     nrot, ang = _angles(rotation)
+    ms = [] if measures is None else [int(m) for m in measures]
+    assert len(ms) <= 8
     return _Params(chip_size, cmin, cmax, chip_aspect, min_search_radius, coarse_stride,
                    coarse_buffer, min_coarse_valid_fraction, fill_window, upsampling, measure,
-                   int(outliers), g or gardner(), nrot, ang)
+                   int(outliers), g or gardner(), nrot, ang, len(ms),
+                   (C.c_int * 8)(*(ms + [0] * (8 - len(ms)))))
+
+
+def _c64(z):
+    return np.asfortranarray(np.asarray(z, dtype=np.complex64))
+
+
+def ramp_phase(z, mask=None):
+    """ramp_phase (src/preprocess.jl:233-245): (phase_x, phase_y) in radians per pixel."""
+    z = _c64(z)
+    m = _f(np.ones(z.shape, bool) if mask is None else mask, np.uint8)
+    px, py = C.c_double(), C.c_double()
+    lib().orc_ramp_phase(_p(z), _p(m), C.c_int64(z.shape[0]), C.c_int64(z.shape[1]),
+                         C.byref(px), C.byref(py))
+    return px.value, py.value
+
+
+def deramp(z, mask=None, axis="both"):
+    """deramp (src/preprocess.jl:184-231): ComplexF32 image with its linear phase ramp
+    removed."""
+    z = _c64(z)
+    m = _f(np.ones(z.shape, bool) if mask is None else mask, np.uint8)
+    out = np.empty(z.shape, np.complex64, order="F")
+    lib().orc_deramp(_p(z), _p(m), C.c_int64(z.shape[0]), C.c_int64(z.shape[1]),
+                     {"both": 0, "x": 1, "y": 2}[axis], _p(out))
+    return out
 
 
 def rotate_chip(chip, deg):

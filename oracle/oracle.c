@@ -47,6 +47,7 @@ static inline size_t esize(int dt) {
         case ORC_F32: return 4;
         case ORC_F64: return 8;
         case ORC_I16: return 2;
+        case ORC_C64: return 8;
     }
     return 0;
 }
@@ -301,10 +302,12 @@ typedef struct {
 
 static void cws_init(corr_ws* w, int64_t maxch, int64_t maxcw, int64_t maxry, int64_t maxrx) {
     int64_t sh = maxch + 2 * maxry, sw = maxcw + 2 * maxrx;
-    w->chip = (float*)malloc(sizeof(float) * (size_t)(maxch * maxcw + 1));
-    w->S = (double*)malloc(sizeof(double) * (size_t)((sh + 1) * (sw + 1)));
+    /* room for a complex chip (two floats per sample) and a complex sum table (S holds the real
+     * parts, S + (sh + 1)(sw + 1) the imaginary ones) */
+    w->chip = (float*)malloc(sizeof(float) * (size_t)(2 * maxch * maxcw + 1));
+    w->S = (double*)malloc(sizeof(double) * (size_t)(2 * (sh + 1) * (sw + 1)));
     w->S2 = (double*)malloc(sizeof(double) * (size_t)((sh + 1) * (sw + 1)));
-    w->num = (double*)malloc(sizeof(double) * (size_t)(4 * maxry * maxrx + 1));
+    w->num = (double*)malloc(sizeof(double) * (size_t)(8 * maxry * maxrx + 1));
     w->fft.F = 0;
 }
 
@@ -313,9 +316,158 @@ static void cws_free(corr_ws* w) {
     fftws_free(&w->fft);
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* Coherence over ComplexF32 images                                                            */
+/* ------------------------------------------------------------------------------------------ */
+
+/* prepare_chip! for complex input -- src/correlate.jl:480-502: ComplexF64 mean in column-major
+ * order, the chip stored as ComplexF32(z - mean), cnorm = sqrt(sum |z - mean|^2) in Float64
+ * (abs2 = re*re + im*im, two roundings and a third for the sum). */
+static int prepare_cchip(const float* chip, int64_t ld, int64_t ch, int64_t cw, float* dst,
+                         double* cnorm) {
+    double sr = 0.0, si = 0.0;
+    for (int64_t j = 0; j < cw; j++)
+        for (int64_t i = 0; i < ch; i++) {
+            sr += (double)chip[2 * (i + j * ld)];
+            si += (double)chip[2 * (i + j * ld) + 1];
+        }
+    const double n = (double)(ch * cw);
+    const double mr = sr / n, mi = si / n;
+    double sq = 0.0;
+    for (int64_t j = 0; j < cw; j++)
+        for (int64_t i = 0; i < ch; i++) {
+            const double dr = (double)chip[2 * (i + j * ld)] - mr;
+            const double di = (double)chip[2 * (i + j * ld) + 1] - mi;
+            dst[2 * (i + j * ch)] = (float)dr;
+            dst[2 * (i + j * ch) + 1] = (float)di;
+            sq += dr * dr + di * di;
+        }
+    *cnorm = sqrt(sq);
+    return sq > DBL_EPSILON * n;
+}
+
+/* integral_both! for complex input -- src/integral.jl:172-200: Sr / Si the complex running sum,
+ * S2 the sum of |w|^2 (abs2 of a ComplexF32 value is exact product terms, one rounding). */
+static void cintegral_both(const float* a, int64_t ld, int64_t m, int64_t n, double* Sr,
+                           double* Si, double* S2) {
+    int64_t ls = m + 1;
+    for (int64_t j = 0; j <= n; j++) { Sr[j * ls] = 0.0; Si[j * ls] = 0.0; S2[j * ls] = 0.0; }
+    for (int64_t i = 0; i <= m; i++) { Sr[i] = 0.0; Si[i] = 0.0; S2[i] = 0.0; }
+    for (int64_t j = 1; j <= n; j++) {
+        double rr = 0.0, ri = 0.0, runsq = 0.0;
+        for (int64_t i = 1; i <= m; i++) {
+            const double vr = (double)a[2 * ((i - 1) + (j - 1) * ld)];
+            const double vi = (double)a[2 * ((i - 1) + (j - 1) * ld) + 1];
+            rr += vr;
+            ri += vi;
+            runsq += vr * vr + vi * vi;
+            Sr[i + j * ls] = rr + Sr[i + (j - 1) * ls];
+            Si[i + j * ls] = ri + Si[i + (j - 1) * ls];
+            S2[i + j * ls] = runsq + S2[i + (j - 1) * ls];
+        }
+    }
+}
+
+/* _correlate_surface!(::Coherence) -- src/correlate.jl:659-683, with the direct complex
+ * numerator of _cnumerators_direct! (src/correlate.jl:712-730): per shift a ComplexF64
+ * accumulation of conj(T') W in column-major order (conj(c) w = (cr wr + ci wi) + i (cr wi -
+ * ci wr); the Float32 products are exact in Float64, so each component rounds once before it is
+ * added), narrowed to ComplexF32. |num| is Julia's Float32 hypot, which squares and sums in
+ * Float64 and rounds the square root to Float32. wvar = max(s2 - |s1|^2 / n, 0), den = cnorm
+ * sqrt(wvar), surface = den > 0 ? Float32(|num| / den) : 0. */
+static int ccorrelate_ws(corr_ws* w, const float* search, int64_t lds, const float* chip,
+                         int64_t ldc, int64_t ch, int64_t cw, int64_t rx, int64_t ry,
+                         float* surface) {
+    int64_t nr = 2 * ry, nc = 2 * rx;
+    double cnorm;
+    if (!prepare_cchip(chip, ldc, ch, cw, w->chip, &cnorm)) {
+        for (int64_t k = 0; k < nr * nc; k++) surface[k] = 0.0f;
+        return 0;
+    }
+    int64_t sh = ch + nr - 1, sw = cw + nc - 1, ls = sh + 1;
+    double* Sr = w->S;
+    double* Si = w->S + (size_t)((sh + 1) * (sw + 1));
+    cintegral_both(search, lds, sh, sw, Sr, Si, w->S2);
+    const double n = (double)(ch * cw);
+    for (int64_t j = 1; j <= nc; j++)
+        for (int64_t i = 1; i <= nr; i++) {
+            double ar = 0.0, ai = 0.0;
+            for (int64_t jj = 0; jj < cw; jj++)
+                for (int64_t ii = 0; ii < ch; ii++) {
+                    const double cr = (double)w->chip[2 * (ii + jj * ch)];
+                    const double ci = (double)w->chip[2 * (ii + jj * ch) + 1];
+                    const int64_t q = 2 * ((i - 1 + ii) + (j - 1 + jj) * lds);
+                    const double wr = (double)search[q], wi = (double)search[q + 1];
+                    ar += cr * wr + ci * wi;
+                    ai += cr * wi - ci * wr;
+                }
+            const float nr32 = (float)ar, ni32 = (float)ai;
+            const float mag =
+                (float)sqrt((double)nr32 * (double)nr32 + (double)ni32 * (double)ni32);
+            const double s1r = boxsum(Sr, ls, i, j, ch, cw);
+            const double s1i = boxsum(Si, ls, i, j, ch, cw);
+            const double s2 = boxsum(w->S2, ls, i, j, ch, cw);
+            const double a2 = s1r * s1r + s1i * s1i;
+            const double wvar = jmax0(s2 - a2 / n);
+            const double den = cnorm * sqrt(wvar);
+            surface[(i - 1) + (j - 1) * nr] = den > 0 ? (float)((double)mag / den) : 0.0f;
+        }
+    return 1;
+}
+
+/* ramp_phase -- src/preprocess.jl:233-245: the phase of sum conj(z_i) z_(i+1) over valid
+ * neighbour pairs down columns (phase_y) and along rows (phase_x), ComplexF64 sums in
+ * column-major order. */
+static void ramp_sums(const float* z, const uint8_t* m, int64_t nr, int64_t nc, int axis_y,
+                      double* re, double* im) {
+    double ar = 0.0, ai = 0.0;
+    const int64_t di = axis_y ? 1 : 0, dj = axis_y ? 0 : 1;
+    for (int64_t j = 0; j < nc - dj; j++)
+        for (int64_t i = 0; i < nr - di; i++) {
+            const int64_t k = i + j * nr, l = (i + di) + (j + dj) * nr;
+            if (!(m[k] && m[l])) continue;
+            const double zr = (double)z[2 * k], zi = (double)z[2 * k + 1];
+            const double ur = (double)z[2 * l], ui = (double)z[2 * l + 1];
+            ar += zr * ur + zi * ui;
+            ai += zr * ui - zi * ur;
+        }
+    *re = ar;
+    *im = ai;
+}
+
+void orc_ramp_phase(const float* z, const uint8_t* mask, int64_t nr, int64_t nc, double* phase_x,
+                    double* phase_y) {
+    double r, i;
+    ramp_sums(z, mask, nr, nc, 0, &r, &i);
+    *phase_x = atan2(i, r);
+    ramp_sums(z, mask, nr, nc, 1, &r, &i);
+    *phase_y = atan2(i, r);
+}
+
+/* deramp -- src/preprocess.jl:184-231: out = ComplexF32(z * cis(-((i-1) phase_y + (j-1)
+ * phase_x))) with the phases of the axes asked for (the other is zero). */
+void orc_deramp(const float* z, const uint8_t* mask, int64_t nr, int64_t nc, int axis,
+                float* out) {
+    double py = 0.0, px = 0.0, r, im;
+    if (axis != 1) { ramp_sums(z, mask, nr, nc, 1, &r, &im); py = atan2(im, r); }
+    if (axis != 2) { ramp_sums(z, mask, nr, nc, 0, &r, &im); px = atan2(im, r); }
+    for (int64_t j = 0; j < nc; j++)
+        for (int64_t i = 0; i < nr; i++) {
+            const int64_t k = i + j * nr;
+            const double th = -((double)i * py + (double)j * px);
+            const double c = cos(th), s = sin(th);
+            const double zr = (double)z[2 * k], zi = (double)z[2 * k + 1];
+            out[2 * k] = (float)(zr * c - zi * s);
+            out[2 * k + 1] = (float)(zr * s + zi * c);
+        }
+}
+
 static int correlate_ws(corr_ws* w, const void* search, int64_t lds, const void* chip,
                         int64_t ldc, int dt, int cdt, int64_t ch, int64_t cw, int64_t rx,
                         int64_t ry, int measure, int num_mode, float* surface) {
+    if (dt == ORC_C64)
+        return ccorrelate_ws(w, (const float*)search, lds, (const float*)chip, ldc, ch, cw, rx,
+                             ry, surface);
     int64_t nr = 2 * ry, nc = 2 * rx;
     double cnorm;
     int ok = prepare_chip(chip, ldc, cdt, ch, cw, w->chip, &cnorm);
@@ -572,6 +724,9 @@ int orc_track(const void* ref, const void* sec, const uint8_t* ok_mask, int dt, 
     for (int64_t i = 0; i < n; i++) {
         dx[i] = NAN; dy[i] = NAN; corr[i] = NAN; searched[i] = 0;
     }
+    /* Coherence needs complex images (src/correlate.jl:739-748), the real measures real ones */
+    if ((dt == ORC_C64) != (measure == ORC_COHERENCE)) return ORC_ERR_MEASURE;
+    if (dt == ORC_C64 && nrot > 0) return ORC_ERR_ROTATION;
     /* _pass_geometry -- src/track.jl:176-197 */
     int64_t cx = 0, cy = 0, mrx = 0, mry = 0, px = 0, py = 0;
     int fits = 1;
@@ -1339,8 +1494,15 @@ int orc_multichip(const void* ref, const void* sec, const uint8_t* ok_mask, int 
     float* fdx = malloc(sizeof(float) * N); float* fdy = malloc(sizeof(float) * N);
     float* fcorr = malloc(sizeof(float) * N);
     int64_t* filled = malloc(sizeof(int64_t) * N);
-    for (int64_t cs = p->chip_size_base; cs <= p->chip_size_max; cs *= 2) {
-        if (cs < p->chip_size_min) continue;
+    const orc_params* p0 = p;
+    int level = 0;
+    for (int64_t cs = p0->chip_size_base; cs <= p0->chip_size_max; cs *= 2) {
+        if (cs < p0->chip_size_min) continue;
+        /* the level's measure (measure_at): the tuple's entry, the last one repeating */
+        orc_params pl = *p0;
+        if (p0->nmeas > 0) pl.measure = p0->measures[level < p0->nmeas ? level : p0->nmeas - 1];
+        level++;
+        const orc_params* p = &pl;
         int anyw = 0;
         for (int64_t k = 0; k < N; k++) anyw |= (rcs[k] == 0);
         if (!anyw) break;

@@ -19,8 +19,9 @@
 extern "C" {
 #endif
 
-enum { ORC_U8 = 0, ORC_F32 = 1, ORC_F64 = 2, ORC_I16 = 3 };
-enum { ORC_ZNCC = 0, ORC_NCC = 1 };
+/* ORC_C64: ComplexF32, interleaved (re, im) pairs of float. */
+enum { ORC_U8 = 0, ORC_F32 = 1, ORC_F64 = 2, ORC_I16 = 3, ORC_C64 = 4 };
+enum { ORC_ZNCC = 0, ORC_NCC = 1, ORC_COHERENCE = 2 };
 enum { ORC_NUM_DIRECT = 0, ORC_NUM_FFT = 1 };
 
 int64_t orc_next_fft_size(int64_t n);
@@ -57,6 +58,18 @@ int orc_track(const void* ref, const void* sec, const uint8_t* ok_mask, int dtyp
               int64_t nx, const orc_points* pts, int64_t n, int64_t up, int measure,
               int num_mode, int nthreads, int nrot, const double* angles, float* dx, float* dy,
               float* corr, uint8_t* searched);
+/* orc_track returns 0, or ORC_ERR_MEASURE when the measure does not suit the element type
+ * (Coherence needs complex images, ZNCC and NCC real ones), or ORC_ERR_ROTATION for a rotation
+ * search over complex images. */
+enum { ORC_ERR_MEASURE = -2, ORC_ERR_ROTATION = -3 };
+
+/* Complex (SLC) path: ramp_phase and deramp (src/preprocess.jl:184-245) over an nr x nc
+ * ComplexF32 image (interleaved); axis 0 both, 1 x only, 2 y only. */
+void orc_ramp_phase(const float* z, const uint8_t* mask, int64_t nr, int64_t nc, double* phase_x,
+                    double* phase_y);
+void orc_deramp(const float* z, const uint8_t* mask, int64_t nr, int64_t nc, int axis,
+                float* out);
+
 /* _rotate_chip: ch x cw chip (column-major, leading dimension ldc) rotated by deg degrees. */
 void orc_rotate_chip(const void* chip, int64_t ldc, int dtype, int64_t ch, int64_t cw,
                      double deg, float* dst);
@@ -104,6 +117,9 @@ typedef struct {
     orc_gardner gardner;
     int nrot;               /* RotationSearch angles (0: NoRotationSearch) */
     double angles[8];
+    int nmeas;              /* a measure per chip-size level, the last repeating (measure_at,
+                               src/types.jl:842-870); 0: `measure` at every level */
+    int measures[8];
 } orc_params;
 int orc_multichip(const void* ref, const void* sec, const uint8_t* ok_mask, int dtype,
                   int64_t ny, int64_t nx, const orc_points* grid, int64_t nr, int64_t nc,
