@@ -4,10 +4,11 @@ Python host over the C ABI of libautorift_hip.so (include/autorift_hip.h). Names
 reference (src/api.jl, src/points.jl, src/track.jl, src/multichip.jl); Julia's mutating
 `reinit!` / `autorift!` are spelled `reinit_` / `autorift_`.
 """
-from ._lib import (AR_NCC, AR_ZNCC, ArgumentError, AutoRIFTError, DimensionMismatch,
+from ._lib import (AR_COHERENCE, AR_NCC, AR_ZNCC, ArgumentError, AutoRIFTError, DimensionMismatch,
                    NoDeviceError, Params, default_params, lib)
 from .api import (Cache, ImagePair, RotationSearch, angles, autorift, autorift_,
-                  autorift_with_grid, chip_sizes, init, params, reinit_, track)
+                  autorift_with_grid, chip_measures, chip_sizes, init, measure_at, params,
+                  reinit_, track)
 from .core import Context, DisplacementField, MultichipResult
 from .points import (PointSet, chip_bounds, gridpoints, gridpoints_xy, pointset, sanitize_,
                      search_bounds)
@@ -16,10 +17,10 @@ from .geo import autorift_geo, grid_coordinates, pixel_size, to_velocity, years
 from .serve import Server, ServerError
 
 __all__ = [
-    "AR_NCC", "AR_ZNCC", "ArgumentError", "AutoRIFTError", "DimensionMismatch", "NoDeviceError",
+    "AR_COHERENCE", "AR_NCC", "AR_ZNCC", "ArgumentError", "AutoRIFTError", "DimensionMismatch", "NoDeviceError",
     "Params", "default_params", "lib", "Cache", "ImagePair", "autorift", "autorift_",
     "autorift_with_grid", "chip_sizes", "init", "params", "reinit_", "track", "Context",
     "DisplacementField", "MultichipResult", "PointSet", "chip_bounds", "gridpoints",
     "gridpoints_xy", "pointset", "sanitize_", "search_bounds", "Server", "ServerError", "Batch", "RotationSearch", "angles", "autorift_geo", "grid_coordinates",
-    "pixel_size", "to_velocity", "years",
+    "pixel_size", "to_velocity", "years", "chip_measures", "measure_at",
 ]

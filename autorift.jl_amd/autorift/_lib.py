@@ -14,8 +14,8 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libautorift_hip.so")
 
 AR_OK, AR_ERR_INVALID_ARGUMENT, AR_ERR_DIMENSION_MISMATCH = 0, 1, 2
 AR_ERR_OUT_OF_MEMORY, AR_ERR_DEVICE, AR_ERR_UNSUPPORTED, AR_ERR_NO_DEVICE = 3, 4, 5, 6
-AR_U8, AR_F32, AR_F64, AR_I16 = 0, 1, 2, 3
-AR_ZNCC, AR_NCC = 0, 1
+AR_U8, AR_F32, AR_F64, AR_I16, AR_C64 = 0, 1, 2, 3, 4
+AR_ZNCC, AR_NCC, AR_COHERENCE = 0, 1, 2
 AR_PREC_EXACT, AR_PREC_BF16, AR_PREC_FP16 = 0, 1, 2
 AR_BATCH_GRAPH, AR_BATCH_SPLIT_SET = 1, 2
 
@@ -54,7 +54,8 @@ class Params(C.Structure):
         ("coarse_buffer", C.c_int64), ("min_coarse_valid_fraction", C.c_double),
         ("dx_prior", C.c_double), ("dy_prior", C.c_double), ("fill_window", C.c_int64),
         ("precision", C.c_int32), ("rotation_count", C.c_int32),
-        ("rotation_angles", C.c_double * 8),
+        ("rotation_angles", C.c_double * 8), ("deramp_axis", C.c_int32),
+        ("level_measure_count", C.c_int32), ("level_measures", C.c_int32 * 8),
     ]
 
 

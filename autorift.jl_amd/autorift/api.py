@@ -15,8 +15,9 @@ from ._lib import ArgumentError, DimensionMismatch, Params
 from .core import Context, DisplacementField, MultichipResult
 from .points import PointSet, gridpoints
 
-_SIMILARITY = {"zncc": _lib.AR_ZNCC, "ncc": _lib.AR_NCC}
-_PREPROCESS = {"highpass": 1, "none": 0}
+_SIMILARITY = {"zncc": _lib.AR_ZNCC, "ncc": _lib.AR_NCC, "coherence": _lib.AR_COHERENCE}
+_PREPROCESS = {"highpass": 1, "none": 0, "deramp": 2}
+_DERAMP_AXIS = {"both": 0, "x": 1, "y": 2}
 _QUANTIZE = {"uint8": 1, "none": 0}
 _SUBPIXEL = ("pyramid", "none")
 _OUTLIERS = ("gardner", "none")
@@ -88,16 +89,32 @@ def params(similarity="zncc", preprocess="highpass", quantize="uint8", subpixel=
            dy_prior=0.0, coarse_stride=4, coarse_buffer=8, min_coarse_valid_fraction=0.01,
            outliers="gardner", rotation=None, outlier_window=None, outlier_iterations=None,
            min_agree_fraction=None, agree_tolerance=None, mad_scale=None, fill_window=3,
-           threaded=False, progress=False, rng_seed=0, precision="exact") -> Params:
+           threaded=False, progress=False, rng_seed=0, precision="exact",
+           deramp_axis="both") -> Params:
     """Resolve and validate keywords into the flattened parameter set (src/params.jl:306-392).
 
     `precision` (:exact, :bf16, :fp16) is this library's addition: with `quantize = :none` the
     correlation contraction runs on the matrix cores with 16-bit operands and FP32 accumulation;
     UInt8 prepared images always take the exact integer path.
+
+    `similarity` is a measure or a tuple of measures, one per chip-size level with the last
+    repeating (src/types.jl:770-870), e.g. ("coherence", "zncc"). `preprocess = "deramp"` with
+    `deramp_axis` ("both", "x", "y") is Deramp (src/types.jl:383-423), for complex pairs.
     """
     p = _lib.default_params()
     p.precision = _PRECISION[_resolve(_PRECISION, precision, "precision")]
-    p.measure = _SIMILARITY[_resolve(_SIMILARITY, similarity, "similarity")]
+    sims = tuple(similarity) if isinstance(similarity, (tuple, list)) else (similarity,)
+    if not sims:
+        raise ArgumentError("`similarity` must name at least one measure")
+    if len(sims) > 8:
+        raise ArgumentError(f"at most 8 per-level measures, got {len(sims)}")
+    meas = [_SIMILARITY[_resolve(_SIMILARITY, s, "similarity")] for s in sims]
+    p.measure = meas[0]
+    if len(meas) > 1:
+        p.level_measure_count = len(meas)
+        for k, v in enumerate(meas):
+            p.level_measures[k] = v
+    p.deramp_axis = _DERAMP_AXIS[_resolve(_DERAMP_AXIS, deramp_axis, "deramp_axis")]
     pre = _resolve(_PREPROCESS, preprocess, "preprocess")
     p.preprocess = _PREPROCESS[pre]
     if filter_width is not None:
@@ -175,6 +192,10 @@ def params(similarity="zncc", preprocess="highpass", quantize="uint8", subpixel=
         raise ArgumentError(f"`chip_size_max` ({cmax}) must be a power-of-two multiple of "
                             f"`chip_size` ({base}), got a ratio of {ratio}.")
     p.chip_size_base, p.chip_size_min, p.chip_size_max = base, cmin, cmax
+    nlev = len(chip_sizes(p))
+    if len(meas) > nlev:
+        raise ArgumentError(f"{len(meas)} similarity measures for {nlev} chip-size level(s): the "
+                            "extra measures would silently do nothing.")
     if not chip_aspect > 0:
         raise ArgumentError(f"`chip_aspect` must be positive, got {chip_aspect}")
     p.chip_aspect = float(chip_aspect)
@@ -366,3 +387,16 @@ def track(pair, pts: PointSet, p: Params | None = None, subpixel=None, measure=N
     with Context(p, device) as ctx:
         ctx.set_pair(pair.reference, pair.secondary, pair.reference_valid, pair.secondary_valid)
         return ctx.track(pts, upsampling=up, measure=m)
+
+
+def measure_at(p: Params, k: int) -> str:
+    """The measure of chip-size level k (1-based), the tuple's last entry repeating."""
+    names = {v: n for n, v in _SIMILARITY.items()}
+    if p.level_measure_count > 0:
+        return names[p.level_measures[min(k, p.level_measure_count) - 1]]
+    return names[p.measure]
+
+
+def chip_measures(p: Params):
+    """One measure per chip-size level (src/types.jl:852-870)."""
+    return tuple(measure_at(p, k) for k in range(1, len(chip_sizes(p)) + 1))

@@ -14,8 +14,9 @@ from . import _lib
 from ._lib import AR_F32, AR_NCC, AR_U8, AR_ZNCC, Points, check, lib
 
 _DTYPES = {np.dtype(np.uint8): _lib.AR_U8, np.dtype(np.float32): _lib.AR_F32,
-           np.dtype(np.float64): _lib.AR_F64, np.dtype(np.int16): _lib.AR_I16}
-_NP = {_lib.AR_U8: np.uint8, _lib.AR_F32: np.float32}
+           np.dtype(np.float64): _lib.AR_F64, np.dtype(np.int16): _lib.AR_I16,
+           np.dtype(np.complex64): _lib.AR_C64}
+_NP = {_lib.AR_U8: np.uint8, _lib.AR_F32: np.float32, _lib.AR_C64: np.complex64}
 
 
 def _ptr(a):

@@ -25,6 +25,7 @@ __global__ void track_u8_kernel(TrackArgs a);
 __global__ void refine_kernel(TrackArgs a);
 template <int IS_U8>
 __global__ void track_generic_kernel(TrackArgs a);
+__global__ void track_coherence_kernel(TrackArgs a);
 
 }  // namespace ar
 
@@ -423,7 +424,7 @@ extern "C" ar_status ar_get_pair(ar_context* ctx, void* ref, void* sec, uint8_t*
     if (!ctx->has_pair) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "no image pair set");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     int64_t N = ctx->ny * ctx->nx;
-    size_t es = ctx->pdtype == AR_U8 ? 1 : 4;
+    size_t es = ctx->pdtype == AR_U8 ? 1 : (ctx->pdtype == AR_C64 ? 8 : 4);
     if (ref) HIPCHK(ctx, hipMemcpyAsync(ref, ctx->pref.ptr, N * es, hipMemcpyDeviceToHost, ctx->stream));
     if (sec) HIPCHK(ctx, hipMemcpyAsync(sec, ctx->psec.ptr, N * es, hipMemcpyDeviceToHost, ctx->stream));
     if (ref_valid) HIPCHK(ctx, hipMemcpyAsync(ref_valid, ctx->rmask.ptr, N, hipMemcpyDeviceToHost, ctx->stream));
@@ -442,8 +443,21 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     if (up < 1 || up > 64 || (up & (up - 1)) != 0)
         return fail(ctx, AR_ERR_UNSUPPORTED,
                     "upsampling must be a power of two in 1..64 on the device path");
-    if (measure != AR_ZNCC && measure != AR_NCC)
-        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "measure must be ZNCC or NCC");
+    if (measure != AR_ZNCC && measure != AR_NCC && measure != AR_COHERENCE)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "measure must be ZNCC, NCC or Coherence");
+    // Coherence exploits the phase of a complex pair; the real measures need real images
+    // (src/correlate.jl:739-748)
+    const bool cplx = ctx->pdtype == AR_C64;
+    if (measure == AR_COHERENCE && !cplx)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT,
+                    "`Coherence` needs complex input, but the images are real. Coherence exploits "
+                    "the phase of an SLC pair; a real image has none, so there is nothing for it "
+                    "to measure. Use ZNCC for real imagery, or pass the complex data.");
+    if (measure != AR_COHERENCE && cplx)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT,
+                    "ZNCC and NCC need real images; use Coherence for a complex pair");
+    if (cplx && ctx->params.rotation_count > 0)
+        return fail(ctx, AR_ERR_UNSUPPORTED, "rotation search needs real images");
     if (n == 0) return AR_OK;
     ar_status st;
     unsigned long long g[10];
@@ -581,6 +595,9 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         if (st == AR_ERR_OUT_OF_MEMORY) return fail(ctx, st, "out of device memory (box images)");
         if (st) return fail(ctx, st, "mixed-precision set-up failed");
         a.lds_bytes = lds;
+    } else if (cplx) {
+        a.lds_bytes = 0;
+        kern = track_coherence_kernel;
     } else if (!use_u8) {
         a.lds_bytes = 0;
         kern = ctx->pdtype == AR_U8 ? track_generic_kernel<1> : track_generic_kernel<0>;
@@ -626,9 +643,10 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     int64_t nblocks = (int64_t)ctx->ncu * per_cu;
     if (nblocks > n) nblocks = n;
     if (!use_u8 && !use_mp) {
-        GenScratch G = gen_scratch(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
-        a.scratch_per_block = G.total;
-        if ((st = ctx->scratch.reserve(ctx, G.total * nblocks))) return st;
+        const size_t per = cplx ? ar_coherence_scratch(a.max_ch, a.max_cw, a.max_rx, a.max_ry)
+                                : gen_scratch(a.max_ch, a.max_cw, a.max_rx, a.max_ry).total;
+        a.scratch_per_block = per;
+        if ((st = ctx->scratch.reserve(ctx, per * nblocks))) return st;
         a.scratch = (unsigned char*)ctx->scratch.ptr;
     }
     // refinement kernel: one wave per workgroup with its own cascade region

@@ -65,6 +65,11 @@ struct ar_context {
 };
 
 // internal entry points shared between translation units
+// Complex pairs: preparation of one image (isfinite / given mask, Deramp, NoQuantize) and the
+// Coherence tracker's per-workgroup scratch bytes.
+ar_status ar_prepare_complex(ar_context* ctx, const void* img, const uint8_t* mask, int64_t ny,
+                             int64_t nx, void* dst, uint8_t* dmask);
+size_t ar_coherence_scratch(int ch, int cw, int rx, int ry);
 ar_status ar_upload_masks(ar_context* ctx, const uint8_t* ref_valid, const uint8_t* sec_valid,
                           int64_t N);
 // Pass geometry a caller already knows (an upper bound is enough: the kernels' layouts only need

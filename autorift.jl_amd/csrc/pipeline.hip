@@ -549,8 +549,18 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
     }
 
     ar_status st;
+    if (p.level_measure_count < 0 || p.level_measure_count > 8)
+        return mfail(ctx, AR_ERR_INVALID_ARGUMENT, "level_measure_count must be in 0..8");
+    int level = 0;
     for (int64_t cs = p.chip_size_base; cs <= p.chip_size_max; cs *= 2) {
         if (cs < p.chip_size_min) continue;
+        // measure_at (src/types.jl:842-870): the level's entry of the similarity tuple, the last
+        // one repeating
+        const int32_t measure =
+            p.level_measure_count > 0
+                ? p.level_measures[level < p.level_measure_count ? level : p.level_measure_count - 1]
+                : p.measure;
+        level++;
         unsigned long long c2[2] = {0, 0};
         int64_t csx = cs, csy = 2 * (int64_t)std::nearbyint((double)cs * p.chip_aspect / 2.0);
         wanted_kernel<<<gblocks(N), GK, 0, s>>>(rcs, N, pend);
@@ -588,7 +598,7 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
             // the level)
             MCHK(hipGetLastError());
             ar_points cp = {cxp, cyp, crx, cry, cpdx, cpdy, ccx, ccy};
-            if ((st = ar_track_pass(ctx, cp, M, 1, p.measure, cdx, cdy, ccr, cs8, hp))) return st;
+            if ((st = ar_track_pass(ctx, cp, M, 1, measure, cdx, cdy, ccr, cs8, hp))) return st;
             measured_kernel<<<gblocks(M), GK, 0, s>>>(cdx, M, cmeas);
             if ((st = count_flags(ctx, cmeas, M, cnt, &c))) return st;
             if (c == 0) continue;
@@ -617,7 +627,7 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
 
         // ---- fine pass, rejection, hole filling, merge
         ar_points lp = {gx, gy, lrx, lry, gpdx, gpdy, lcx, lcy};
-        if ((st = ar_track_pass(ctx, lp, N, p.upsampling, p.measure, fdx, fdy, fc, fs, hp)))
+        if ((st = ar_track_pass(ctx, lp, N, p.upsampling, measure, fdx, fdy, fc, fs, hp)))
             return st;
         measured_kernel<<<gblocks(N), GK, 0, s>>>(fdx, N, meas);
         uint8_t* keep = nullptr;

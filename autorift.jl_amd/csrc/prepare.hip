@@ -418,6 +418,7 @@ static size_t dt_size(int32_t dt) {
         case AR_I16: return 2;
         case AR_F32: return 4;
         case AR_F64: return 8;
+        case AR_C64: return 8;
     }
     return 0;
 }
@@ -454,15 +455,32 @@ extern "C" ar_status ar_set_images(ar_context* ctx, const void* ref, const void*
                      "new images must match the size the context was built for; build a new "
                      "context (init) for a different image size");
     const ar_params& p = ctx->params;
-    if (p.preprocess != AR_PRE_NONE && p.preprocess != AR_PRE_HIGHPASS)
+    if (p.preprocess != AR_PRE_NONE && p.preprocess != AR_PRE_HIGHPASS &&
+        p.preprocess != AR_PRE_DERAMP)
         return pfail(ctx, AR_ERR_UNSUPPORTED, "pre-filter not available on the device path");
+    // complex input takes Deramp or nothing and NoQuantize; real input takes no Deramp
+    // (src/preprocess.jl:247-260, 473-531)
+    if (dtype == AR_C64 && p.preprocess == AR_PRE_HIGHPASS)
+        return pfail(ctx, AR_ERR_INVALID_ARGUMENT,
+                     "Highpass is an amplitude filter and rejects complex input: it would "
+                     "produce something plausible-looking and meaningless. Use Deramp or no "
+                     "pre-filter for SLC data");
+    if (dtype == AR_C64 && p.quantize != AR_QUANT_NONE)
+        return pfail(ctx, AR_ERR_INVALID_ARGUMENT,
+                     "UInt8 quantization rejects complex input; use quantize = none for SLC data");
+    if (dtype != AR_C64 && p.preprocess == AR_PRE_DERAMP)
+        return pfail(ctx, AR_ERR_INVALID_ARGUMENT,
+                     "Deramp needs complex input: a real image has no phase ramp to remove");
+    if (dtype == AR_C64 && (p.deramp_axis < 0 || p.deramp_axis > 2))
+        return pfail(ctx, AR_ERR_INVALID_ARGUMENT, "Deramp axis must be both, x or y");
     PCHK(hipSetDevice(ctx->device));
     int32_t pdt = p.quantize == AR_QUANT_UINT8 ? AR_U8 : AR_F32;
     if (p.preprocess == AR_PRE_NONE && p.quantize == AR_QUANT_NONE && dtype == AR_U8) pdt = AR_U8;
+    if (dtype == AR_C64) pdt = AR_C64;
     if (ctx->has_pair && (!ref || !sec) && pdt != ctx->pdtype)
         return pfail(ctx, AR_ERR_INVALID_ARGUMENT, "prepared element type changed");
     int64_t N = ny * nx;
-    size_t es = pdt == AR_U8 ? 1 : 4;
+    size_t es = pdt == AR_U8 ? 1 : (pdt == AR_C64 ? 8 : 4);
     ar_status st;
     if (!ctx->has_pair || ctx->ny != ny || ctx->nx != nx || ctx->pdtype != pdt) {
         if ((st = ctx->pref.reserve(ctx, N * es)) || (st = ctx->psec.reserve(ctx, N * es)) ||
@@ -489,6 +507,7 @@ extern "C" ar_status ar_set_images(ar_context* ctx, const void* ref, const void*
             continue;
         }
         switch (dtype) {
+            case AR_C64: st = ar_prepare_complex(ctx, img, m, ny, nx, dst, dm); break;
             case AR_U8: st = prepare_slot<uint8_t>(ctx, img, m, ny, nx, dst, dm); break;
             case AR_I16: st = prepare_slot<int16_t>(ctx, img, m, ny, nx, dst, dm); break;
             case AR_F32: st = prepare_slot<float>(ctx, img, m, ny, nx, dst, dm); break;

@@ -34,9 +34,13 @@ typedef enum {
     AR_ERR_NO_DEVICE = 6           /* no gfx950 device visible */
 } ar_status;
 
-typedef enum { AR_U8 = 0, AR_F32 = 1, AR_F64 = 2, AR_I16 = 3 } ar_dtype;
-typedef enum { AR_ZNCC = 0, AR_NCC = 1 } ar_measure;
-typedef enum { AR_PRE_NONE = 0, AR_PRE_HIGHPASS = 1 } ar_preprocess;
+/* AR_C64: ComplexF32 (single-look-complex radar), interleaved (re, im) float pairs. */
+typedef enum { AR_U8 = 0, AR_F32 = 1, AR_F64 = 2, AR_I16 = 3, AR_C64 = 4 } ar_dtype;
+/* AR_COHERENCE: complex coherence magnitude (src/types.jl:154-200), complex pairs only; ZNCC and
+ * NCC need real pairs. */
+typedef enum { AR_ZNCC = 0, AR_NCC = 1, AR_COHERENCE = 2 } ar_measure;
+/* AR_PRE_DERAMP: Deramp(axis) (src/types.jl:383-423), complex pairs only; Highpass real only. */
+typedef enum { AR_PRE_NONE = 0, AR_PRE_HIGHPASS = 1, AR_PRE_DERAMP = 2 } ar_preprocess;
 typedef enum { AR_QUANT_NONE = 0, AR_QUANT_UINT8 = 1 } ar_quantize;
 /* Arithmetic of the correlation contraction for Float32 prepared images (quantize = :none).
  * EXACT follows the reference's Float64 direct numerator (src/correlate.jl:799-814); BF16 and
@@ -79,6 +83,11 @@ typedef struct {
                                      else the number of chip rotations (at most 8) */
     double rotation_angles[8];    /* the chip rotations in degrees, `about` already subtracted
                                      (angles(m), src/types.jl:577) */
+    int32_t deramp_axis;          /* Deramp axis: 0 both, 1 x only, 2 y only */
+    int32_t level_measure_count;  /* the similarity tuple (src/types.jl:770-870): a measure per
+                                     chip-size level, the last repeating; 0 = `measure` at every
+                                     level (at most 8) */
+    int32_t level_measures[8];
 } ar_params;
 
 /* PointSet fields (src/points.jl:50-74), structure of arrays of equal length. */
