@@ -149,8 +149,11 @@ class Context:
         return r, s, rv.astype(bool), sv.astype(bool)
 
     # -- correlation -----------------------------------------------------------------------------
-    def track(self, pts, upsampling: int = 1, measure: int = AR_ZNCC) -> DisplacementField:
-        """track (src/track.jl:103-160) over every point of `pts`."""
+    def track(self, pts, upsampling: int = 1, measure: int | None = None) -> DisplacementField:
+        """track (src/track.jl:103-160) over every point of `pts`; `measure` defaults to the
+        context's (the first of its similarity measures)."""
+        if measure is None:
+            measure = self.params.measure
         s, keep = _points_struct(pts)
         shape = np.shape(pts.x)
         n = int(np.prod(shape))

@@ -227,6 +227,58 @@ def served_mixed_vs_plain(autorift, count=8, slots=4, seed=7):
     return res
 
 
+def speckle_pair(n, shift, seed):
+    """Circular Gaussian speckle smoothed by two 2x2 box passes (wrap-around), and a copy rolled
+    by `shift` plus a little complex noise: a synthetic single-look-complex pair."""
+    rng = np.random.default_rng(seed)
+    z = (rng.standard_normal((n, n), np.float32) +
+         1j * rng.standard_normal((n, n), np.float32)).astype(np.complex64)
+    for _ in range(2):
+        z = (z + np.roll(z, 1, 0) + np.roll(z, 1, 1) + np.roll(np.roll(z, 1, 0), 1, 1)) / 4
+    z2 = np.roll(z, shift, axis=(0, 1))
+    z2 = z2 + 0.05 * (rng.standard_normal((n, n), np.float32) +
+                      1j * rng.standard_normal((n, n), np.float32))
+    return z.astype(np.complex64), z2.astype(np.complex64)
+
+
+def coherence_block(autorift, steps, n=2048):
+    """Coherence over a complex pair (Deramp, quantize = :none): chip 32, r 20, 64x refinement,
+    grid spacing 16; throughput of the device pass and the oracle on a sample of the points."""
+    z1, z2 = speckle_pair(n, (5, -3), seed=6)
+    xs = np.arange(64.0, n - 63.0, 16.0)
+    pts = autorift.gridpoints_xy(xs, xs, chip_size=32, search_radius=20)
+    p = autorift.params(similarity="coherence", preprocess="deramp", quantize="none")
+    with autorift.Context(p) as ctx:
+        ctx.set_images(z1, z2)
+        ctx.track(pts, upsampling=64)
+        t = []
+        for _ in range(steps):
+            r = ctx.track(pts, upsampling=64)
+            t.append(ctx.last_kernel()[0])
+        pr, ps, pm, _ = ctx.get_pair()
+    ms = float(np.median(t))
+    fin = np.isfinite(r.dx)
+    out = {"pair": f"{n}x{n} ComplexF32 speckle rolled by (5, -3)", "points": len(pts),
+           "chip": 32, "radius": 20, "upsampling": 64, "ms": ms,
+           "points_per_s": int(np.count_nonzero(r.searched)) / (ms / 1000),
+           "median_dx": float(np.median(r.dx[fin])), "median_dy": float(np.median(r.dy[fin])),
+           "expected_dx_dy": [3.0, -5.0]}
+    try:
+        import oracle
+        sub = pts[::12, ::12]
+        t0 = time.perf_counter()
+        oracle.track(pr, ps, pm, sub.x, sub.y, sub.radius_x, sub.radius_y, sub.chip_size_x,
+                     sub.chip_size_y, up=64, measure=oracle.COHERENCE, nthreads=16)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_oracle"] = {"points": len(sub), "threads": 16,
+                             "points_per_s": len(sub) / cpu_s}
+    except Exception as e:   # the CPU comparison is optional
+        out["cpu_oracle"] = f"unavailable: {e}"
+    log(f"coherence: {out['points_per_s']:.0f} points/s over {len(pts)} points, median "
+        f"({out['median_dx']:.3f}, {out['median_dy']:.3f})")
+    return out
+
+
 def mixed_block(autorift, tex, sec, pts, steps):
     """16-bit contraction on the Float32 pair (quantize = :none): throughput of the BF16 and FP16
     matrix-core tracker over the full grid, the exact Float64 path on a sample of it, and the
@@ -429,6 +481,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-pool", type=int, default=6)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-coherence", action="store_true")
     ap.add_argument("--batch-pairs", type=int, default=256,
                     help="pairs of the batched workload (0: skip it)")
     ap.add_argument("--batch-points", type=int, default=1_000_000)
@@ -503,6 +556,10 @@ def main():
         mixed = mixed_block(autorift, tex, sec, pts, min(args.steps, 3))
     del tex, sec
 
+    coherence = None
+    if not args.no_coherence:
+        coherence = coherence_block(autorift, min(args.steps, 3))
+
     batched = None
     if args.batch_pairs > 0:
         batched = batched_block(autorift, min(args.steps, 5), args.batch_pairs, args.batch_points)
@@ -572,6 +629,7 @@ def main():
         "serving": serving,
         "batched": batched,
         "mixed_precision": mixed,
+        "coherence": coherence,
         "served_mixed_vs_plain_fp32": served_vs_plain,
         "prepare_s": prep_s,
     }
