@@ -62,6 +62,7 @@ size_t elem_size(int32_t dtype) {
         case AR_I16: return 2;
         case AR_F32: return 4;
         case AR_F64: return 8;
+        case AR_C64: return 8;
         default: return 0;
     }
 }
@@ -106,7 +107,8 @@ static size_t pair_footprint(int64_t ny, int64_t nx, int32_t dtype, int64_t npts
     const size_t es = elem_size(dtype);
     const size_t raw = 2 * N * es + 2 * N;          // raw pair and masks
     const size_t work = N * (4 + 8);                // Float32 / Float64 preparation planes
-    const size_t prep = 2 * N * 4 + 3 * N;          // prepared pair (Float32 at most), masks
+    const size_t pe = dtype == AR_C64 ? 8 : 4;      // prepared element: ComplexF32 or Float32
+    const size_t prep = 2 * N * pe + 3 * N;         // prepared pair (at most pe bytes), masks
     const size_t box = 2 * N * 8;                   // box images (Float64 for the 16-bit path)
     const size_t copies = 8 * N + 4096;             // int8 copies of the UInt8 pair
     const size_t pts = (size_t)npts * 512;          // point fields, outputs, hand-off, grid planes

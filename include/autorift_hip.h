@@ -127,7 +127,7 @@ ar_status ar_set_pair(ar_context* ctx, const void* ref, const void* sec,
                       int32_t dtype);
 
 /* Copies the prepared pair back (for inspection and tests). Element type is the prepared one:
- * u8 after AR_QUANT_UINT8, f32 otherwise. Any pointer may be NULL. */
+ * u8 after AR_QUANT_UINT8, c64 for a complex pair, f32 otherwise. Any pointer may be NULL. */
 ar_status ar_get_pair(ar_context* ctx, void* ref, void* sec, uint8_t* ref_valid,
                       uint8_t* sec_valid, int32_t* dtype);
 

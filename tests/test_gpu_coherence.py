@@ -171,3 +171,21 @@ def test_type_and_filter_errors():
         ctx.set_images(z1, z2)
         with pytest.raises((autorift.AutoRIFTError, ValueError), match="need real images"):
             ctx.track(pts, upsampling=1, measure=autorift.AR_ZNCC)
+
+
+def test_served_complex_pair_equals_alone():
+    """A complex pair through the serving loop gives the multi-chip result of a context."""
+    z1, z2 = _pair(300, noise=0.2)
+    p = autorift.params(similarity="coherence", preprocess="deramp", quantize="none",
+                        chip_size=16, chip_size_max=32, search_radius=10)
+    print("served coherence pair", flush=True)
+    with autorift.Server(p, slots=2) as srv:
+        t = srv.submit(z1, z2)
+        served, _ = srv.result(t)
+    alone = autorift.autorift(z1, z2, similarity="coherence", preprocess="deramp",
+                              quantize="none", chip_size=16, chip_size_max=32, search_radius=10)
+    for a, b in ((served.dx, alone.dx), (served.dy, alone.dy),
+                 (served.correlation, alone.correlation)):
+        np.testing.assert_array_equal(np.isnan(a), np.isnan(b))
+        np.testing.assert_array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
+    assert np.isfinite(served.dx).mean() > 0.5

@@ -169,3 +169,25 @@ def test_rotation_search_construction():
         autorift.params(rotation=RS(tuple(range(9))))
     with pytest.raises(autorift.ArgumentError):
         autorift.params(rotation="yes")
+
+
+def test_similarity_tuple_maps_onto_levels():
+    """test/complex.jl:239-266: a scalar applies everywhere; a tuple assigns in order with the
+    last entry repeating; more measures than levels and an empty tuple are errors."""
+    ar = autorift
+    p = ar.params(chip_size=32, chip_size_max=128)
+    assert len(ar.chip_sizes(p)) == 3
+    assert ar.chip_measures(p) == ("zncc", "zncc", "zncc")
+    p2 = ar.params(chip_size=32, chip_size_max=128, similarity=("coherence", "zncc"))
+    assert ar.chip_measures(p2) == ("coherence", "zncc", "zncc")
+    assert ar.measure_at(p2, 1) == "coherence" and ar.measure_at(p2, 3) == "zncc"
+    p3 = ar.params(chip_size=32, chip_size_max=128, similarity=("coherence", "ncc", "zncc"))
+    assert ar.chip_measures(p3) == ("coherence", "ncc", "zncc")
+    with pytest.raises(ar.ArgumentError):
+        ar.params(chip_size=32, chip_size_max=32, similarity=("coherence", "zncc"))
+    with pytest.raises(ar.ArgumentError):
+        ar.params(similarity=())
+    with pytest.raises(ar.ArgumentError):
+        ar.params(deramp_axis="z")
+    d = ar.params(preprocess="deramp", deramp_axis="y", quantize="none")
+    assert (d.preprocess, d.deramp_axis) == (2, 2)
