@@ -25,6 +25,7 @@ __global__ void track_u8_kernel(TrackArgs a);
 __global__ void refine_kernel(TrackArgs a);
 template <int IS_U8>
 __global__ void track_generic_kernel(TrackArgs a);
+template <bool LDS>
 __global__ void track_coherence_kernel(TrackArgs a);
 
 }  // namespace ar
@@ -596,8 +597,8 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         if (st) return fail(ctx, st, "mixed-precision set-up failed");
         a.lds_bytes = lds;
     } else if (cplx) {
-        a.lds_bytes = 0;
-        kern = track_coherence_kernel;
+        a.lds_bytes = ar_coherence_lds(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
+        kern = a.lds_bytes ? track_coherence_kernel<true> : track_coherence_kernel<false>;
     } else if (!use_u8) {
         a.lds_bytes = 0;
         kern = ctx->pdtype == AR_U8 ? track_generic_kernel<1> : track_generic_kernel<0>;

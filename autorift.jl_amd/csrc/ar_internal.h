@@ -70,6 +70,7 @@ struct ar_context {
 ar_status ar_prepare_complex(ar_context* ctx, const void* img, const uint8_t* mask, int64_t ny,
                              int64_t nx, void* dst, uint8_t* dmask);
 size_t ar_coherence_scratch(int ch, int cw, int rx, int ry);
+size_t ar_coherence_lds(int ch, int cw, int rx, int ry);
 ar_status ar_upload_masks(ar_context* ctx, const uint8_t* ref_valid, const uint8_t* sec_valid,
                           int64_t N);
 // Pass geometry a caller already knows (an upper bound is enough: the kernels' layouts only need

@@ -137,14 +137,19 @@ __device__ inline float2 load_c(const TrackArgs& a, const void* img, int64_t r, 
 
 }  // namespace
 
+// LDS: the window and the chip sit in dynamic LDS (when they fit) instead of the workgroup's
+// global scratch; the arithmetic and its order are the same.
+template <bool LDS>
 __global__ __launch_bounds__(AR_WG) void track_coherence_kernel(TrackArgs a) {
     __shared__ Best red[AR_NWAVES];
     __shared__ double shd[4];
+    extern __shared__ float2 dyn[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const CohScratch G = coh_scratch(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
     unsigned char* base = a.scratch + (size_t)blockIdx.x * a.scratch_per_block;
-    float2* chip = (float2*)(base + G.off_chip);
-    float2* win = (float2*)(base + G.off_win);
+    const int wmax = (a.max_ch + 2 * a.max_ry - 1) * (a.max_cw + 2 * a.max_rx - 1);
+    float2* chip = LDS ? dyn + wmax : (float2*)(base + G.off_chip);
+    float2* win = LDS ? dyn : (float2*)(base + G.off_win);
     double* Sr = (double*)(base + G.off_sr);
     double* Si = (double*)(base + G.off_si);
     double* S2 = (double*)(base + G.off_s2);
@@ -263,13 +268,31 @@ __global__ __launch_bounds__(AR_WG) void track_coherence_kernel(TrackArgs a) {
     }
 }
 
+template __global__ void track_coherence_kernel<false>(TrackArgs);
+template __global__ void track_coherence_kernel<true>(TrackArgs);
+
 }  // namespace ar
 
-using namespace ar;
+namespace ar {
 
-size_t ar_coherence_scratch(int ch, int cw, int rx, int ry) {
+size_t ar_coherence_scratch_impl(int ch, int cw, int rx, int ry) {
     return coh_scratch(ch, cw, rx, ry).total;
 }
+
+}  // namespace ar
+
+size_t ar_coherence_scratch(int ch, int cw, int rx, int ry) {
+    return ar::ar_coherence_scratch_impl(ch, cw, rx, ry);
+}
+
+// Dynamic LDS bytes of the window and chip of the largest point, or 0 when they do not fit
+// beside the kernel's static LDS.
+size_t ar_coherence_lds(int ch, int cw, int rx, int ry) {
+    const size_t b = ((size_t)(ch + 2 * ry - 1) * (cw + 2 * rx - 1) + (size_t)ch * cw) * 8;
+    return b <= 160 * 1024 - 1024 ? b : 0;
+}
+
+using namespace ar;
 
 // One complex image into dst (ComplexF32) and its validity into dmask: isfinite unless a mask
 // is given, then Deramp when asked for, then NoQuantize (src/preprocess.jl:74-79, 184-231,
