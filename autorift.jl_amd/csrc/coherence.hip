@@ -222,8 +222,9 @@ __global__ __launch_bounds__(AR_WG) void track_coherence_kernel(TrackArgs a) {
         // per shift: the direct complex numerator, |num| (Julia's Float32 hypot: squares and sum
         // in Float64, one rounding to Float32) and the normalisation
         const double n = (double)(ch * cw);
-        for (int j = wave; j < NC; j += AR_NWAVES)
-            for (int i = lane; i < NR; i += 64) {
+        // shifts spread over all 256 lanes (a surface column is shorter than a wave)
+        for (int o = tid; o < NR * NC; o += AR_WG) {
+                const int i = o % NR, j = o / NR;
                 double ar = 0.0, ai = 0.0;
                 for (int jj = 0; jj < cw; jj++) {
                     const float2* wc = win + (j + jj) * winh + i;
