@@ -32,8 +32,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def synthetic_pair(n, seed=3):
-    """Seeded texture and a smoothly warped, noisy copy (float32)."""
+def synthetic_pair(n, seed=3, fields=False):
+    """Seeded texture and a smoothly warped, noisy copy (float32); with `fields`, also the
+    displacement (fx, fy) in pixels: secondary(y, x) = texture(y - fy, x - fx)."""
     from scipy import ndimage
     rng = np.random.default_rng(seed)
     tex = rng.standard_normal((n, n), dtype=np.float32)
@@ -65,6 +66,8 @@ def synthetic_pair(n, seed=3):
         coords = np.stack([I - fy[r0:r1], J - fx[r0:r1]])
         sec[r0:r1] = ndimage.map_coordinates(tex, coords, order=1, mode="nearest")
     sec += np.random.default_rng(seed + 2).normal(0, 2, sec.shape).astype(np.float32)
+    if fields:
+        return tex, sec, fx, fy
     return tex, sec
 
 

@@ -359,3 +359,55 @@ def multichip(ref, sec, ok_mask, x, y, rx, ry, pdx=0.0, pdy=0.0, p=None, num_mod
                         _p(corr), _p(cs), _p(interp))
     r = lambda a: a.reshape(shape, order="F")
     return r(dx), r(dy), r(corr), r(cs), r(interp).astype(bool)
+
+
+def cell_max_radius(radius, stride):
+    """_cell_max_radius! (src/multichip.jl:292-306) over an nr x nc radius grid."""
+    r = _f(radius, np.int64)
+    nr, nc = r.shape
+    out = np.zeros((nr // stride, nc // stride), np.int64, order="F")
+    lib().orc_cell_max_radius(_p(r), C.c_int64(nr), C.c_int64(nc), C.c_int64(stride), _p(out))
+    return out
+
+
+def coarse_fraction_ok(keep, measured, radius_x, min_fraction=0.01):
+    """The coarse pass's coherence gate (src/multichip.jl:265-279)."""
+    k = np.ascontiguousarray(np.ravel(keep, order="F"), np.uint8)
+    m = np.ascontiguousarray(np.ravel(measured, order="F"), np.uint8)
+    r = np.ascontiguousarray(np.ravel(radius_x, order="F"), np.int64)
+    lib().orc_coarse_fraction_ok.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                             C.c_double]
+    return bool(lib().orc_coarse_fraction_ok(_p(k), _p(m), _p(r), k.size, float(min_fraction)))
+
+
+def coarse_pass(ref, sec, ok_mask, x, y, rx, ry, csx, csy, pdx=0.0, pdy=0.0, p=None,
+                num_mode=NUM_DIRECT):
+    """_coarse_pass (src/multichip.jl:224-288) over a gridded level point set: the fine-pass
+    mask, or None when the level has nothing coherent."""
+    p = p or params()
+    ref = _f(ref)
+    sec = _f(sec, ref.dtype)
+    ok = _f(ok_mask, np.uint8)
+    shape = np.shape(x)
+    b = lambda v, t: np.broadcast_to(np.asarray(v, dtype=t), shape)
+    pts, keep = _points(b(x, np.float64), b(y, np.float64), b(rx, np.int64), b(ry, np.int64),
+                        b(pdx, np.float64), b(pdy, np.float64), b(csx, np.int64),
+                        b(csy, np.int64))
+    mask = np.zeros(shape, np.uint8, order="F")
+    r = lib().orc_coarse_pass(_p(ref), _p(sec), _p(ok), _DT[ref.dtype], C.c_int64(ref.shape[0]),
+                              C.c_int64(ref.shape[1]), C.byref(pts), C.c_int64(shape[0]),
+                              C.c_int64(shape[1]), C.c_int64(csx), C.c_int64(csy), C.byref(p),
+                              num_mode, 1, _p(mask))
+    return mask.astype(bool) if r else None
+
+
+def merge_level(result, level, filled, chip_size):
+    """_merge_level! (src/multichip.jl:401-422). ``result`` = (dx, dy, corr, chip_size,
+    interpolated) arrays, updated in place (Fortran order); ``level`` = (dx, dy, corr)."""
+    rdx, rdy, rc, rcs, rint = result
+    fdx, fdy, fc = (np.asfortranarray(a, np.float32) for a in level)
+    f = np.ascontiguousarray(filled, np.int64)
+    scratch = np.zeros(rdx.size, np.uint8)
+    lib().orc_merge_level(_p(rdx), _p(rdy), _p(rc), _p(rcs), _p(rint), _p(fdx), _p(fdy), _p(fc),
+                          _p(f), C.c_int64(f.size), C.c_int64(rdx.size), C.c_int64(chip_size),
+                          _p(scratch))

@@ -1407,11 +1407,47 @@ static void keep_from_filter(const orc_params* p, const orc_gardner* g, const fl
     else memcpy(keep, valid, (size_t)(nr * nc));
 }
 
+/* _cell_max_radius! -- src/multichip.jl:292-306: the largest radius over each coarse cell, rows
+ * (i - stride/2):(i + stride - 1 - stride/2) clipped to the grid (left-biased, like the sliding
+ * windows). out is (nr / stride) x (nc / stride), column-major. */
+void orc_cell_max_radius(const int64_t* radius, int64_t nr, int64_t nc, int64_t stride,
+                         int64_t* out) {
+    int64_t nrc = nr / stride, ncc = nc / stride;
+    int64_t lo = stride / 2, hi = stride - 1 - lo;
+    for (int64_t jo = 1; jo <= ncc; jo++)
+        for (int64_t io = 1; io <= nrc; io++) {
+            int64_t i = io * stride, j = jo * stride;
+            int64_t m = 0;
+            for (int64_t jj = (j - lo > 1 ? j - lo : 1); jj <= (j + hi < nc ? j + hi : nc); jj++)
+                for (int64_t ii = (i - lo > 1 ? i - lo : 1); ii <= (i + hi < nr ? i + hi : nr); ii++) {
+                    int64_t v = radius[(ii - 1) + (jj - 1) * nr];
+                    if (v > m) m = v;
+                }
+            out[(io - 1) + (jo - 1) * nrc] = m;
+        }
+}
+
+/* The coarse pass's coherence gate -- src/multichip.jl:265-279: over the points with a positive
+ * x radius that were measured, the kept fraction must not fall below min_fraction; no such point
+ * at all also fails. Returns 1 when the level goes on. */
+int orc_coarse_fraction_ok(const uint8_t* keep, const uint8_t* measured, const int64_t* radius_x,
+                           int64_t n, double min_fraction) {
+    int64_t denom = 0, numer = 0;
+    for (int64_t k = 0; k < n; k++) {
+        if (!(radius_x[k] > 0)) continue;
+        if (!measured[k]) continue;
+        denom++;
+        if (keep[k]) numer++;
+    }
+    if (denom == 0) return 0;
+    return !((double)numer / (double)denom < min_fraction);
+}
+
 /* _coarse_pass -- src/multichip.jl:224-288. Returns 1 with mask filled, 0 for "nothing". */
-static int coarse_pass(const void* ref, const void* sec, const uint8_t* ok, int dt, int64_t ny,
-                       int64_t nx, const orc_points* lv, int64_t nr, int64_t nc, int64_t csx,
-                       int64_t csy, const orc_params* p, int num_mode, int nthreads,
-                       uint8_t* mask) {
+int orc_coarse_pass(const void* ref, const void* sec, const uint8_t* ok, int dt, int64_t ny,
+                    int64_t nx, const orc_points* lv, int64_t nr, int64_t nc, int64_t csx,
+                    int64_t csy, const orc_params* p, int num_mode, int nthreads,
+                    uint8_t* mask) {
     int64_t stride = p->coarse_stride;
     int64_t nrc = nr / stride, ncc = nc / stride;
     orc_gardner filt = p->gardner;
@@ -1423,22 +1459,16 @@ static int coarse_pass(const void* ref, const void* sec, const uint8_t* ok, int 
     double* pdx = malloc(sizeof(double) * M); double* pdy = malloc(sizeof(double) * M);
     int64_t* rx = malloc(sizeof(int64_t) * M); int64_t* ry = malloc(sizeof(int64_t) * M);
     int64_t* cx = malloc(sizeof(int64_t) * M); int64_t* cy = malloc(sizeof(int64_t) * M);
-    int64_t lo = stride / 2, hi = stride - 1 - lo;
     for (int64_t jo = 1; jo <= ncc; jo++)
         for (int64_t io = 1; io <= nrc; io++) {
             int64_t i = io * stride, j = jo * stride;
             int64_t src = (i - 1) + (j - 1) * nr, dst = (io - 1) + (jo - 1) * nrc;
             x[dst] = lv->x[src]; y[dst] = lv->y[src];
             pdx[dst] = lv->dx_prior[src]; pdy[dst] = lv->dy_prior[src];
-            int64_t mx = 0, my = 0;
-            for (int64_t jj = (j - lo > 1 ? j - lo : 1); jj <= (j + hi < nc ? j + hi : nc); jj++)
-                for (int64_t ii = (i - lo > 1 ? i - lo : 1); ii <= (i + hi < nr ? i + hi : nr); ii++) {
-                    int64_t k = (ii - 1) + (jj - 1) * nr;
-                    if (lv->radius_x[k] > mx) mx = lv->radius_x[k];
-                    if (lv->radius_y[k] > my) my = lv->radius_y[k];
-                }
-            rx[dst] = mx; ry[dst] = my; cx[dst] = csx; cy[dst] = csy;
+            cx[dst] = csx; cy[dst] = csy;
         }
+    orc_cell_max_radius(lv->radius_x, nr, nc, stride, rx);
+    orc_cell_max_radius(lv->radius_y, nr, nc, stride, ry);
     int result = 0;
     float *cdx = NULL, *cdy = NULL, *ccorr = NULL, *gm = NULL;
     uint8_t *srch = NULL, *meas = NULL, *keep = NULL, *grown = NULL;
@@ -1455,15 +1485,7 @@ static int coarse_pass(const void* ref, const void* sec, const uint8_t* ok, int 
     if (!anym) goto done;
     keep_from_filter(p, &filt, cdx, cdy, rx, ry, meas, nrc, ncc, keep);
     for (int64_t k = 0; k < M; k++) if (!meas[k]) keep[k] = 0;
-    int64_t denom = 0, numer = 0;
-    for (int64_t k = 0; k < M; k++) {
-        if (!(rx[k] > 0)) continue;
-        if (!meas[k]) continue;
-        denom++;
-        if (keep[k]) numer++;
-    }
-    if (denom == 0) goto done;
-    if ((double)numer / (double)denom < p->min_coarse_valid_fraction) goto done;
+    if (!orc_coarse_fraction_ok(keep, meas, rx, M, p->min_coarse_valid_fraction)) goto done;
     grown = malloc(M);
     orc_dilate_within(keep, nrc, ncc, (double)p->coarse_buffer, grown);
     float* gf = malloc(sizeof(float) * M);
@@ -1477,6 +1499,24 @@ done:
     free(x); free(y); free(pdx); free(pdy); free(rx); free(ry); free(cx); free(cy);
     free(cdx); free(cdy); free(ccorr); free(srch); free(meas); free(keep); free(grown); free(gm);
     return result;
+}
+
+/* _merge_level! -- src/multichip.jl:401-422: the level's value goes wherever no finer level owns
+ * the point (chip_size 0) and the level measured or filled it; interpolated marks the filled
+ * indices. wasfilled is n bytes of scratch. */
+void orc_merge_level(float* rdx, float* rdy, float* rcorr, uint16_t* rcs, uint8_t* rinterp,
+                     const float* fdx, const float* fdy, const float* fcorr,
+                     const int64_t* filled, int64_t nfilled, int64_t n, int64_t chip_size,
+                     uint8_t* wasfilled) {
+    memset(wasfilled, 0, (size_t)n);
+    for (int64_t k = 0; k < nfilled; k++) wasfilled[filled[k]] = 1;
+    for (int64_t k = 0; k < n; k++) {
+        if (rcs[k] != 0) continue;
+        if (isnan(fdx[k])) continue;
+        rdx[k] = fdx[k]; rdy[k] = fdy[k]; rcorr[k] = fcorr[k];
+        rcs[k] = (uint16_t)chip_size;
+        rinterp[k] = wasfilled[k];
+    }
 }
 
 int orc_multichip(const void* ref, const void* sec, const uint8_t* ok_mask, int dt, int64_t ny,
@@ -1515,7 +1555,7 @@ int orc_multichip(const void* ref, const void* sec, const uint8_t* ok_mask, int 
         }
         if (sanitize(rx, ry, N, p->min_search_radius) == 0) continue;
         orc_points lv = {grid->x, grid->y, rx, ry, grid->dx_prior, grid->dy_prior, cx, cy};
-        if (!coarse_pass(ref, sec, ok_mask, dt, ny, nx, &lv, nr, nc, csx, csy, p, num_mode,
+        if (!orc_coarse_pass(ref, sec, ok_mask, dt, ny, nx, &lv, nr, nc, csx, csy, p, num_mode,
                          nthreads, cmask))
             continue;
         for (int64_t k = 0; k < N; k++)
@@ -1529,16 +1569,8 @@ int orc_multichip(const void* ref, const void* sec, const uint8_t* ok_mask, int 
         for (int64_t k = 0; k < N; k++)
             if (!keep[k]) { fdx[k] = NAN; fdy[k] = NAN; fcorr[k] = NAN; }
         int64_t nf = orc_fill_holes(fdx, fdy, nr, nc, p->fill_window, filled);
-        /* _merge_level! -- src/multichip.jl:401-422 */
-        memset(wasfilled, 0, N);
-        for (int64_t k = 0; k < nf; k++) wasfilled[filled[k]] = 1;
-        for (int64_t k = 0; k < N; k++) {
-            if (rcs[k] != 0) continue;
-            if (isnan(fdx[k])) continue;
-            rdx[k] = fdx[k]; rdy[k] = fdy[k]; rcorr[k] = fcorr[k];
-            rcs[k] = (uint16_t)cs;
-            rinterp[k] = wasfilled[k];
-        }
+        orc_merge_level(rdx, rdy, rcorr, rcs, rinterp, fdx, fdy, fcorr, filled, nf, N, cs,
+                        wasfilled);
     }
     free(rx); free(ry); free(cx); free(cy); free(cmask); free(meas); free(keep); free(srch);
     free(wasfilled); free(fdx); free(fdy); free(fcorr); free(filled);

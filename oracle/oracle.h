@@ -121,6 +121,19 @@ typedef struct {
                                src/types.jl:842-870); 0: `measure` at every level */
     int measures[8];
 } orc_params;
+/* Steps of the multi-chip loop, exported so the tests can pin each one on its own. */
+void orc_cell_max_radius(const int64_t* radius, int64_t nr, int64_t nc, int64_t stride,
+                         int64_t* out);
+int orc_coarse_fraction_ok(const uint8_t* keep, const uint8_t* measured, const int64_t* radius_x,
+                           int64_t n, double min_fraction);
+int orc_coarse_pass(const void* ref, const void* sec, const uint8_t* ok, int dt, int64_t ny,
+                    int64_t nx, const orc_points* lv, int64_t nr, int64_t nc, int64_t csx,
+                    int64_t csy, const orc_params* p, int num_mode, int nthreads,
+                    uint8_t* mask);
+void orc_merge_level(float* rdx, float* rdy, float* rcorr, uint16_t* rcs, uint8_t* rinterp,
+                     const float* fdx, const float* fdy, const float* fcorr,
+                     const int64_t* filled, int64_t nfilled, int64_t n, int64_t chip_size,
+                     uint8_t* wasfilled);
 int orc_multichip(const void* ref, const void* sec, const uint8_t* ok_mask, int dtype,
                   int64_t ny, int64_t nx, const orc_points* grid, int64_t nr, int64_t nc,
                   const orc_params* p, int num_mode, int nthreads, float* dx, float* dy,
