@@ -27,6 +27,11 @@ template <int IS_U8>
 __global__ void track_generic_kernel(TrackArgs a);
 template <bool LDS>
 __global__ void track_coherence_kernel(TrackArgs a);
+// Defined in track_f64.hip.
+template <int IS_U8, bool LDS_T>
+__global__ void f64_prep_kernel(TrackArgs a, F64Stage S);
+template <int IS_U8>
+__global__ void track_f64_kernel(TrackArgs a, F64Stage S, int pitch, const double* stage);
 
 }  // namespace ar
 
@@ -203,6 +208,32 @@ static int box_images_mode() {
     if (std::strcmp(e, "always") == 0) return 1;
     if (std::strcmp(e, "never") == 0) return -1;
     return 0;
+}
+
+// AR_F64_DIRECT=never sends the passes the direct Float64 tracker would take to the one-kernel
+// generic tracker instead (the same results; kept for comparisons). Unset or anything else: the
+// direct tracker wherever its window fits in LDS.
+static bool f64_direct_enabled() {
+    const char* e = std::getenv("AR_F64_DIRECT");
+    return !(e && std::strcmp(e, "never") == 0);
+}
+
+// Staging bytes the direct Float64 tracker uses per chunk of points.
+static const size_t kF64StageBytes = (size_t)2 << 30;
+
+// The direct Float64 tracker's launch shape for a pass: window LDS bytes, column pitch and
+// threads per workgroup (one lane per 8-row tile of shifts); false when it does not apply.
+static bool f64_shape(const TrackArgs& a, size_t* lds, int* pitch, int* threads) {
+    const int p = f64_pitch(a.max_ch, a.max_ry);
+    const int winw = a.max_cw + 2 * a.max_rx - 1;
+    const size_t wl = (size_t)winw * p * 4;
+    const int tiles = (2 * a.max_ry + 7) / 8 * (2 * a.max_rx);
+    const int t = (tiles + 63) / 64 * 64;
+    const size_t surf = (size_t)(2 * a.max_ry) * (2 * a.max_rx) * 4;   // reuses the window's LDS
+    *lds = wl > surf ? wl : surf;
+    *pitch = p;
+    *threads = t;
+    return *lds <= kLdsLimit && t <= 1024;
 }
 
 // Four int8 copies of a UInt8 image (value ^ 0x80), copy k shifted k bytes later
@@ -435,6 +466,65 @@ extern "C" ar_status ar_get_pair(ar_context* ctx, void* ref, void* sec, uint8_t*
     return AR_OK;
 }
 
+// The direct Float64 tracker over the pass's points (records the pass start event): chunks of
+// points whose staging slots fit kF64StageBytes, each a preparation launch (persistent, one
+// point per workgroup at a time, per-block summed-area tables) and a tracking launch (one
+// workgroup per point).
+static ar_status f64_launch(ar_context* ctx, const TrackArgs& a) {
+    ar_status st;
+    size_t wl = 0;
+    int pitch = 0, threads = 0;
+    f64_shape(a, &wl, &pitch, &threads);
+    const bool u8 = ctx->pdtype == AR_U8;
+    const int winh = a.max_ch + 2 * a.max_ry - 1, winw = a.max_cw + 2 * a.max_rx - 1;
+    const size_t td = (size_t)(winh + 1) * (winw + 1);
+    // the preparation kernel's summed-area table in LDS when it fits next to the chip
+    const size_t lchip = ((size_t)a.max_ch * a.max_cw * 4 + 15) / 16 * 16;
+    const bool ldst = lchip + td * 8 <= kLdsLimit;
+    void (*fprep)(TrackArgs, F64Stage) =
+        ldst ? (u8 ? f64_prep_kernel<1, true> : f64_prep_kernel<0, true>)
+             : (u8 ? f64_prep_kernel<1, false> : f64_prep_kernel<0, false>);
+    const void* kprep = (const void*)fprep;
+    const void* ktrack = u8 ? (const void*)track_f64_kernel<1> : (const void*)track_f64_kernel<0>;
+    const F64Slot sl = f64_slot(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
+    int64_t chunk = (int64_t)(kF64StageBytes / sl.bytes);
+    if (chunk < 1) chunk = 1;
+    if (chunk > a.n) chunk = a.n;
+    const size_t lprep = ldst ? lchip + td * 8 : lchip;
+    int pa = 0, pb = 0;
+    if ((st = kernel_occupancy(ctx, kprep, F64_PREP_WG, lprep, &pa))) return st;
+    if ((st = kernel_occupancy(ctx, ktrack, threads, wl, &pb))) return st;
+    if (pa < 1 || pb < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "tracker kernel does not fit on a CU");
+    int64_t nprep = (int64_t)ctx->ncu * pa;
+    if (nprep > chunk) nprep = chunk;
+    const size_t tbytes = ldst ? 0 : (size_t)nprep * td * 8;
+    if ((st = ctx->scratch.reserve(ctx, (size_t)chunk * sl.bytes + tbytes))) return st;
+    F64Stage S;
+    S.slots = (unsigned char*)ctx->scratch.ptr;
+    S.slot_bytes = sl.bytes;
+    S.off_chip = sl.off_chip;
+    S.off_b1 = sl.off_b1;
+    S.off_b2 = sl.off_b2;
+    S.chp = sl.chp;
+    S.tables = (double*)((unsigned char*)ctx->scratch.ptr + (size_t)chunk * sl.bytes);
+    S.table_doubles = td;
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int64_t p0 = 0; p0 < a.n; p0 += chunk) {
+        S.p0 = p0;
+        S.cnt = a.n - p0 < chunk ? a.n - p0 : chunk;
+        const int64_t gp = nprep < S.cnt ? nprep : S.cnt;
+        hipLaunchKernelGGL(fprep, dim3((unsigned)gp), dim3(F64_PREP_WG), lprep, ctx->stream, a,
+                           S);
+        HIPCHK(ctx, hipGetLastError());
+        const int64_t gt = 8 * ((S.cnt + 7) / 8);
+        hipLaunchKernelGGL(u8 ? track_f64_kernel<1> : track_f64_kernel<0>, dim3((unsigned)gt),
+                           dim3(threads), wl, ctx->stream, a, S, pitch,
+                           (const double*)ctx->scratch.ptr);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    return AR_OK;
+}
+
 // The pass on device-resident points. All pointers device pointers.
 ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t up,
                         int32_t measure, float* dx, float* dy, float* corr, uint8_t* srch,
@@ -541,6 +631,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     }
     // the u8 kernel's integer identities hold in int32 for chip areas up to 128 x 128, and its
     // uint16 column sums of W for chip heights up to 256
+    bool use_f64 = false;   // direct Float64 tracker (track_f64.hip)
     bool use_u8 = nrot == 0 && ctx->pdtype == AR_U8 && pow2 &&
                   (int64_t)a.max_ch * a.max_cw <= 16384 && a.max_ch <= 256;
     if (use_u8) {
@@ -602,6 +693,9 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     } else if (!use_u8) {
         a.lds_bytes = 0;
         kern = ctx->pdtype == AR_U8 ? track_generic_kernel<1> : track_generic_kernel<0>;
+        size_t wl = 0;
+        int pitch = 0, threads = 0;
+        use_f64 = nrot == 0 && f64_direct_enabled() && f64_shape(a, &wl, &pitch, &threads);
     }
     bool box_timed = mp_box_timed;
     const int bmode = box_images_mode();
@@ -639,11 +733,13 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     a.rpatch = (float*)ctx->refine.ptr;
     a.rmeta = (int32_t*)((char*)ctx->refine.ptr + (size_t)n * 128);
     int per_cu = 0;
-    if ((st = kernel_occupancy(ctx, (const void*)kern, AR_WG, a.lds_bytes, &per_cu))) return st;
-    if (per_cu < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "tracker kernel does not fit on a CU");
+    if (!use_f64) {
+        if ((st = kernel_occupancy(ctx, (const void*)kern, AR_WG, a.lds_bytes, &per_cu))) return st;
+        if (per_cu < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "tracker kernel does not fit on a CU");
+    }
     int64_t nblocks = (int64_t)ctx->ncu * per_cu;
     if (nblocks > n) nblocks = n;
-    if (!use_u8 && !use_mp) {
+    if (!use_u8 && !use_mp && !use_f64) {
         const size_t per = cplx ? ar_coherence_scratch(a.max_ch, a.max_cw, a.max_rx, a.max_ry)
                                 : gen_scratch(a.max_ch, a.max_cw, a.max_rx, a.max_ry).total;
         a.scratch_per_block = per;
@@ -666,9 +762,13 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         point_prep_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(a);
         HIPCHK(ctx, hipGetLastError());
     }
-    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(AR_WG), a.lds_bytes, ctx->stream, a);
-    HIPCHK(ctx, hipGetLastError());
+    if (use_f64) {
+        if ((st = f64_launch(ctx, a))) return st;
+    } else {
+        HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(AR_WG), a.lds_bytes, ctx->stream, a);
+        HIPCHK(ctx, hipGetLastError());
+    }
     HIPCHK(ctx, hipEventRecord(ctx->evm, ctx->stream));
     if (up > 1) {
         hipLaunchKernelGGL(refine_kernel, dim3((unsigned)rblocks), dim3(64), rlds, ctx->stream, a);
@@ -678,7 +778,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     ctx->timing_pending = true;
     ctx->pending_box = box_timed;
     ctx->pending_refine = up > 1;
-    ctx->last_path = use_u8 ? 0 : (use_mp ? 2 : 1);
+    ctx->last_path = use_u8 ? 0 : (use_mp ? 2 : (use_f64 ? 3 : 1));
     if (hint) return AR_OK;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ar_resolve_timing(ctx);

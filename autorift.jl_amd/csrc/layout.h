@@ -1,6 +1,7 @@
 // LDS and scratch layouts of the tracker kernels, shared by the kernels and the host dispatcher.
 #pragma once
 #include <stddef.h>
+#include <stdint.h>
 
 
 namespace ar {
@@ -144,5 +145,48 @@ __host__ __device__ inline MpLayout mp_layout(int max_ch, int max_cw, int max_rx
     if (L.total < post) L.total = post;   // the post-argmax region may cover the 16-bit chip
     return L;
 }
+
+// ---- direct Float64 tracker (track_f64.hip)
+
+// Slot of one point in the staging buffer (byte offsets; sizes from the pass maxima).
+struct F64Slot {
+    size_t off_chip, off_b1, off_b2, bytes;
+    int chp;   // chip rows rounded up to a multiple of 8 (rows past ch are zero)
+};
+
+__host__ __device__ inline F64Slot f64_slot(int max_ch, int max_cw, int max_rx, int max_ry) {
+    F64Slot s;
+    s.chp = (max_ch + 7) / 8 * 8;
+    const size_t shifts = (size_t)(2 * max_rx) * (2 * max_ry);
+    s.off_chip = 64;   // header: status (int32), then cnorm (double) at 8
+    s.off_b1 = s.off_chip + (size_t)s.chp * max_cw * 8;
+    s.off_b2 = s.off_b1 + shifts * 8;
+    s.bytes = (s.off_b2 + shifts * 8 + 255) / 256 * 256;
+    return s;
+}
+
+// LDS column pitch of the window in track_f64_kernel (floats): row r of a column at r + 1, so
+// that the 8 rows a lane reads at a time start on a 16-byte boundary; every row a tile reads
+// (8 ceil(NR / 8) + chp - 1 of them) lies inside the column, and the pitch is 4 mod 8 floats so
+// that neighbouring columns start on different LDS banks.
+__host__ __device__ inline int f64_pitch(int max_ch, int max_ry) {
+    const int nrp = (2 * max_ry + 7) / 8 * 8, chp = (max_ch + 7) / 8 * 8;
+    int p = nrp + chp + 8;
+    p = (p + 3) / 4 * 4;
+    if (p % 8 == 0) p += 4;
+    return p;
+}
+
+#define F64_PREP_WG 128
+
+struct F64Stage {
+    unsigned char* slots;      // chunk's slots, slot.bytes apart
+    size_t slot_bytes;
+    size_t off_chip, off_b1, off_b2;
+    int chp;
+    int64_t p0, cnt;           // chunk: points p0 .. p0 + cnt - 1
+    double* tables;            // per-block summed-area table (when it does not fit in LDS)
+    size_t table_doubles;      // doubles per table ((winh + 1) x (winw + 1) of the maxima)
+};
 
 }  // namespace ar

@@ -1,0 +1,413 @@
+// Direct Float64 numerator for Float32 images (and UInt8 chips whose area is not a power of two):
+// the reference's arithmetic in its own order, fast.
+//
+// A point's correlation is split over two kernels that run back to back over a chunk of points,
+// with a staging slot per point between them:
+//
+//  * f64_prep_kernel (two waves per point): the chip's Float64 mean (src/correlate.jl:437-441),
+//    the mean-removed chip Float32(x - mean) written as Float64 for the second kernel, the chip
+//    energy (sequential in one lane, as the reference sums it), and the window's summed-area
+//    tables in the reference's recurrence (src/integral.jl:141-170), from which the box sums of
+//    W and W^2 at every shift (src/integral.jl:244-247) go into the slot.
+//
+//  * track_f64_kernel (one lane per 8-row tile of shifts): the direct numerator of every shift
+//    (src/correlate.jl:799-814) as eight independent Float64 FMA chains per lane. Each chain
+//    adds its products in the reference's order -- chip column jj outer, chip row ii inner -- and
+//    a product of two Float32 values is exact in Float64, so fma(c, w, acc) is the reference's
+//    `acc += c * w`. The window sits in LDS as Float32; the chip values come from the slot through
+//    scalar loads (they are uniform over the workgroup) and enter the FMAs as scalar operands.
+//    The normalisation (src/correlate.jl:591-630), argmax (src/peak.jl:32-61) and the 5x5 patch
+//    hand-off to the refinement kernel follow.
+//
+// The sum of the chip (the mean's numerator) is the only reduction whose order could matter in
+// parallel. It is taken in parallel only when every partial sum of the reference's sequential loop
+// is exactly representable, so that every order gives the same bits: all values are multiples
+// of 2^emin (the lowest set bit over the chip) and |partial sums| <= n 2^(emax+1) < 2^(emin+53).
+// UInt8 chips always qualify; otherwise one lane sums in the reference's order.
+#include "layout.h"
+#include "track_common.h"
+
+namespace ar {
+
+// A chunk's point k -> the block that handles it: the blocks of each XCD (dispatched round-robin
+// over the 8 XCDs) take one contiguous eighth of the chunk, so overlapping windows of neighbouring
+// points are read through the same L2. Returns -1 for the padding blocks of the grid.
+__device__ inline int64_t f64_point_of_block(int64_t cnt) {
+    const int64_t b = blockIdx.x, per = (cnt + 7) / 8;
+    const int64_t k = (b % 8) * per + b / 8;
+    return (b / 8 < per && k < cnt) ? k : -1;
+}
+
+__device__ inline float f64_pix(const TrackArgs& a, int is_u8, int64_t r, int64_t c) {
+    const int64_t k = pix(a, r, c);
+    if (k < 0) return 0.0f;
+    return is_u8 ? (float)((const uint8_t*)a.ref)[k] : ((const float*)a.ref)[k];
+}
+
+// Lowest set bit exponent (value = m 2^e with m odd) and the exponent of the leading bit of a
+// finite nonzero float.
+__device__ inline void f32_bits(float x, int& elow, int& ehigh) {
+    const uint32_t u = __float_as_uint(x) & 0x7fffffffu;
+    const int be = (int)(u >> 23);
+    const uint32_t m = u & 0x7fffffu;
+    if (be == 0) {   // subnormal
+        elow = -149 + __builtin_ctz(m);
+        ehigh = -149 + 31 - __builtin_clz(m);
+    } else {
+        elow = be - 150 + __builtin_ctz(m | 0x800000u);
+        ehigh = be - 127;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// kernel 1: chip statistics, mean-removed chip, box sums of every shift
+// ---------------------------------------------------------------------------------------------
+
+// Column j's running sums of W (SQ = 0) or of W^2 (SQ = 1) into table column j + 1, rows 1..winh
+// (the first loop of src/integral.jl:157-168), then S[i, j + 1] = run(i, j) + S[i, j] along each
+// row. One wave does both phases (no workgroup barrier between them).
+template <int IS_U8, int SQ>
+__device__ inline void f64_table(const TrackArgs& b, const Geom& g, int winh, int winw,
+                                 double* T, int lane) {
+    const int ls = winh + 1;
+    const bool inside = g.wr0 - b.pady >= 1 && g.wc0 - b.padx >= 1 &&
+                        g.wr0 - b.pady + winh - 1 <= b.ny && g.wc0 - b.padx + winw - 1 <= b.nx;
+    const int64_t base = (g.wr0 - b.pady - 1) + (g.wc0 - b.padx - 1) * b.ny;
+    for (int j = lane; j <= winw; j += 64) T[j * ls] = 0.0;
+    for (int j = lane; j < winw; j += 64) {
+        double run = 0.0;
+        for (int i0 = 0; i0 < winh; i0 += 16) {
+            float v[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                const int r = i0 + u;
+                if (r >= winh) v[u] = 0.0f;
+                else if (inside) {
+                    const int64_t q = base + r + (int64_t)j * b.ny;
+                    v[u] = IS_U8 ? (float)((const uint8_t*)b.ref)[q] : ((const float*)b.ref)[q];
+                } else {
+                    v[u] = f64_pix(b, IS_U8, g.wr0 + r, g.wc0 + j);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                if (i0 + u < winh) {
+                    const double x = (double)v[u];
+                    run += SQ ? x * x : x;
+                    T[(i0 + u + 1) + (j + 1) * ls] = run;
+                }
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    for (int i = lane + 1; i <= winh; i += 64) {
+        double s = 0.0;   // S[i, 1] = 0
+        int j = 1;
+        for (; j + 8 <= winw + 1; j += 8) {
+            double r[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) r[u] = T[i + (j + u) * ls];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                s = r[u] + s;
+                T[i + (j + u) * ls] = s;
+            }
+        }
+        for (; j <= winw; j++) {
+            s = T[i + j * ls] + s;
+            T[i + j * ls] = s;
+        }
+    }
+    for (int i = lane; i <= winh; i += 64) T[i] = 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+}
+
+// The chip energy over elements [e0, e1) (src/correlate.jl:442-447, sequential), continuing sq.
+__device__ inline double f64_energy(const float* chip, int e0, int e1, double mean, double sq) {
+    int e = e0;
+    for (; e + 8 <= e1; e += 8) {
+        double d[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const double t = (double)chip[e + u] - mean;
+            d[u] = t * t;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) sq += d[u];
+    }
+    for (; e < e1; e++) {
+        const double t = (double)chip[e] - mean;
+        sq += t * t;
+    }
+    return sq;
+}
+
+// One point per workgroup at a time (persistent grid). LDS: the chip (Float32), then -- when
+// LDS_T -- one summed-area table, built for W, read for the box sums of W, then rebuilt for W^2;
+// otherwise the table is the block's own region of S.tables.
+template <int IS_U8, bool LDS_T>
+__global__ __launch_bounds__(F64_PREP_WG) void f64_prep_kernel(TrackArgs a, F64Stage S) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* chip = (float*)smem;
+    __shared__ double red_d[2];
+    __shared__ int red_i[4];
+    __shared__ double stat[2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double* T = LDS_T ? (double*)(smem + (size_t)(a.max_ch * a.max_cw * 4 + 15) / 16 * 16)
+                      : S.tables + (size_t)blockIdx.x * S.table_doubles;
+    for (int64_t k = blockIdx.x; k < S.cnt; k += gridDim.x) {
+        const int64_t p = S.p0 + k;
+        TrackArgs b = a;
+        if (a.pairs) use_pair(b, p);
+        unsigned char* slot = S.slots + (size_t)k * S.slot_bytes;
+        const Geom g = point_geom(b, p);
+        bool go = g.status != 0;
+        if (go) {
+            // _any_valid (src/track.jl:416-421); any_valid() assumes a 256-thread workgroup
+            int found = 0;
+            if (b.ok == nullptr) {
+                const int64_t r0 = g.cr0 - b.pady - 1, c0 = g.cc0 - b.padx - 1;
+                found = r0 + g.ch - 1 >= 0 && r0 < b.ny && c0 + g.cw - 1 >= 0 && c0 < b.nx;
+            } else {
+                for (int e = tid; e < g.ch * g.cw; e += F64_PREP_WG) {
+                    const int64_t q = pix(b, g.cr0 + e % g.ch, g.cc0 + e / g.ch);
+                    if (q >= 0 && b.ok[q]) found = 1;
+                }
+            }
+            go = __syncthreads_or(found) != 0;
+        }
+        if (!go) {
+            write_skip(a, p, false);
+            if (tid == 0) *(int32_t*)slot = 0;
+            __syncthreads();
+            continue;
+        }
+        const int ch = g.ch, cw = g.cw, n = ch * cw;
+        const int NR = 2 * g.ry, NC = 2 * g.rx;
+        const int winh = ch + NR - 1, winw = cw + NC - 1, ls = winh + 1;
+        // the chip (from the secondary), Float32 in LDS, with the bit span of its values
+        int elow = 1 << 20, ehigh = -(1 << 20);
+        for (int e = tid; e < n; e += F64_PREP_WG) {
+            const int64_t q = pix(b, g.cr0 + e % ch, g.cc0 + e / ch);
+            float v = 0.0f;
+            if (q >= 0) v = IS_U8 ? (float)((const uint8_t*)b.sec)[q] : ((const float*)b.sec)[q];
+            chip[e] = v;
+            if (v != 0.0f) {
+                int lo, hi;
+                f32_bits(v, lo, hi);
+                elow = min(elow, lo);
+                ehigh = max(ehigh, hi);
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            elow = min(elow, __shfl_xor(elow, o));
+            ehigh = max(ehigh, __shfl_xor(ehigh, o));
+        }
+        if (lane == 0) {
+            red_i[wave] = elow;
+            red_i[2 + wave] = ehigh;
+        }
+        __syncthreads();
+        elow = min(red_i[0], red_i[1]);
+        ehigh = max(red_i[2], red_i[3]);
+        // ceil(log2 n) + ehigh + 1 <= elow + 53: every partial sum exact, any order the same
+        const int lg = 32 - __builtin_clz((unsigned)(n - 1) | 1u);
+        const bool exact = ehigh < elow || lg + ehigh + 1 <= elow + 53;
+        if (exact) {
+            double s = 0.0;
+            for (int e = tid; e < n; e += F64_PREP_WG) s += (double)chip[e];
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+            if (lane == 0) red_d[wave] = s;
+            __syncthreads();
+            if (tid == 0) stat[0] = (red_d[0] + red_d[1]) / (double)n;
+        } else if (tid == 0) {
+            // src/correlate.jl:437-440: column-major, left to right
+            double s = 0.0;
+            for (int e = 0; e < n; e++) s += (double)chip[e];
+            stat[0] = s / (double)n;
+        }
+        __syncthreads();
+        const double mean = stat[0];
+        // Float32(x - mean), as Float64, column pitch chp (rows past ch zero)
+        double* cd = (double*)(slot + S.off_chip);
+        for (int e = tid; e < S.chp * cw; e += F64_PREP_WG) {
+            const int i = e % S.chp, j = e / S.chp;
+            cd[e] = i < ch ? (double)(float)((double)chip[i + j * ch] - mean) : 0.0;
+        }
+        // the table of W, then of W^2 (integral_both!, src/integral.jl:141-170), each read for its
+        // box sums (src/integral.jl:244-247): ((S[i+h, j+w] - S[i, j+w]) - S[i+h, j]) + S[i, j];
+        // meanwhile one lane of wave 1 sums the chip energy, half per table
+        double sq = 0.0;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            if (wave == 0) {
+                if (q == 0) f64_table<IS_U8, 0>(b, g, winh, winw, T, lane);
+                else f64_table<IS_U8, 1>(b, g, winh, winw, T, lane);
+            } else if (lane == 0) {
+                sq = f64_energy(chip, q == 0 ? 0 : n / 2, q == 0 ? n / 2 : n, mean, sq);
+            }
+            __syncthreads();
+            double* bq = (double*)(slot + (q == 0 ? S.off_b1 : S.off_b2));
+            for (int e = tid; e < NR * NC; e += F64_PREP_WG) {
+                const int i = e % NR + 1, j = e / NR + 1;
+                bq[e] = T[(i + ch - 1) + (j + cw - 1) * ls] - T[(i - 1) + (j + cw - 1) * ls] -
+                        T[(i + ch - 1) + (j - 1) * ls] + T[(i - 1) + (j - 1) * ls];
+            }
+            __syncthreads();
+        }
+        // ok = sq > eps(Float64) * ch * cw (src/correlate.jl:452)
+        if (wave == 1 && lane == 0) {
+            const bool ok = sq > 2.220446049250313e-16 * (double)ch * (double)cw;
+            *(int32_t*)slot = ok ? 2 : 1;
+            *(double*)(slot + 8) = sqrt(sq);
+            stat[1] = ok ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        if (stat[1] == 0.0) write_skip(a, p, true);   // searched, no measurement
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// kernel 2: numerators, normalisation, argmax, patch
+// ---------------------------------------------------------------------------------------------
+
+template <int IS_U8>
+__global__ __launch_bounds__(1024) void track_f64_kernel(TrackArgs a, F64Stage S, int pitch,
+                                                         const double* __restrict__ stage) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* W = (float*)smem;
+    __shared__ Best red[16];
+    const int64_t k = f64_point_of_block(S.cnt);
+    if (k < 0) return;
+    const int64_t p = S.p0 + k;
+    const unsigned char* slot = S.slots + (size_t)k * S.slot_bytes;
+    if (*(const int32_t*)slot != 2) return;   // skipped or degenerate: written by kernel 1
+    TrackArgs b = a;
+    if (a.pairs) use_pair(b, p);
+    const Geom g = point_geom(b, p);
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int ch = g.ch, cw = g.cw, NR = 2 * g.ry, NC = 2 * g.rx;
+    const int winh = ch + NR - 1, winw = cw + NC - 1;
+    const int chp = (ch + 7) / 8 * 8;
+    // the window, Float32, row r of column c at W[c * pitch + 1 + r]; zeros below it
+    const bool inside = g.wr0 - b.pady >= 1 && g.wc0 - b.padx >= 1 &&
+                        g.wr0 - b.pady + winh - 1 <= b.ny && g.wc0 - b.padx + winw - 1 <= b.nx;
+    const int64_t base = (g.wr0 - b.pady - 1) + (g.wc0 - b.padx - 1) * b.ny;
+    for (int e = tid; e < winw * pitch; e += nthr) {
+        const int c = e / pitch, r = e % pitch - 1;
+        float v = 0.0f;
+        if (r >= 0 && r < winh) {
+            if (inside) {
+                const int64_t q = base + r + (int64_t)c * b.ny;
+                v = IS_U8 ? (float)((const uint8_t*)b.ref)[q] : ((const float*)b.ref)[q];
+            } else {
+                v = f64_pix(b, IS_U8, g.wr0 + r, g.wc0 + c);
+            }
+        }
+        W[e] = v;
+    }
+    __syncthreads();
+    // this lane's tile: shift rows i0 .. i0 + 7 of shift column j
+    const int ntr = (NR + 7) / 8;
+    const int t = tid < ntr * NC ? tid : 0;   // idle lanes repeat tile 0 and drop it
+    const int i0 = 8 * (t % ntr), j = t / ntr;
+    // the chip, chp x cw Float64 (column-major), read through scalar loads: every lane reads the
+    // same values, so they are scalar operands of the FMAs
+    const double* __restrict__ cd = stage + ((size_t)k * S.slot_bytes + S.off_chip) / 8;
+    double acc[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) acc[r] = 0.0;
+    const int groups = chp / 8;   // groups of 8 chip rows per chip column (rows past ch are 0)
+    const int total = groups * cw;
+    double c[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) c[u] = cd[u];
+    double w[8];
+    for (int q = 0; q < total; q++) {
+        const int jj = q / groups, ii = 8 * (q - jj * groups);
+        const int qn = q + 1 < total ? q + 1 : q;
+        const int jn = qn / groups;
+        const double* cnp = cd + (size_t)jn * S.chp + 8 * (qn - jn * groups);
+        const float* wc = W + (j + jj) * pitch + i0 + ii;   // wc[1 + m] = W[i0 + ii + m, j + jj]
+        if (ii == 0) {
+            const float4 u0 = *(const float4*)wc;
+            const float4 u1 = *(const float4*)(wc + 4);
+            w[0] = u0.y; w[1] = u0.z; w[2] = u0.w;
+            w[3] = u1.x; w[4] = u1.y; w[5] = u1.z; w[6] = u1.w;
+        }
+        // rows i0 + ii + 7 .. i0 + ii + 14 of the window column, and the next group's chip rows
+        const float4 u0 = *(const float4*)(wc + 8);
+        const float4 u1 = *(const float4*)(wc + 12);
+        const float nx[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+        double cn[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) cn[u] = cnp[u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            // step ii + u: slot (u + 7) % 8 takes W[i0 + ii + u + 7]
+            w[(u + 7) & 7] = (double)nx[u];
+#pragma unroll
+            for (int r = 0; r < 8; r++) acc[r] = __builtin_fma(c[u], w[(u + r) & 7], acc[r]);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) c[u] = cn[u];
+    }
+    __syncthreads();   // every lane is past the window: its LDS now holds the surface
+    float* surf = (float*)smem;   // surf[i + j * NR]
+    const double cnorm = *(const double*)(slot + 8);
+    const double* b1 = (const double*)(slot + S.off_b1);
+    const double* b2 = (const double*)(slot + S.off_b2);
+    const double nn = (double)(ch * cw);
+    Best best = best_empty();
+    if (tid < ntr * NC) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const int i = i0 + r;
+            if (i < NR) {
+                // src/correlate.jl:591-600 (ZNCC) and 619-628 (NCC)
+                double den;
+                if (a.measure == 1) {
+                    const double wsq = b2[i + j * NR];
+                    den = cnorm * sqrt(wsq < 0.0 ? 0.0 : wsq);
+                } else {
+                    const double s1 = b1[i + j * NR], s2 = b2[i + j * NR];
+                    const double wv = s2 - s1 * s1 / nn;
+                    den = cnorm * sqrt(wv < 0.0 ? 0.0 : wv);
+                }
+                const float v = den > 0 ? (float)(acc[r] / den) : 0.0f;
+                surf[i + j * NR] = v;
+                const int idx = i * NC + j;
+                if (better(v, idx, best)) {
+                    best.v = v;
+                    best.idx = idx;
+                }
+            }
+        }
+    }
+    // argmax under peak_index's rule over the workgroup
+    best = wave_best(best);
+    const int nw = (nthr + 63) >> 6;
+    if ((tid & 63) == 0) red[tid >> 6] = best;
+    __syncthreads();
+    best = red[0];
+    for (int w = 1; w < nw; w++) best = best_merge(best, red[w]);
+    if (best.idx == 0x7fffffff) best.idx = 0;   // all NaN: (1, 1)
+    const int pi = best.idx / NC + 1, pj = best.idx % NC + 1;
+    const float pval = surf[(pi - 1) + (pj - 1) * NR];
+    int pi0 = 1, pj0 = 1;
+    const bool refine = a.up > 1 && patch_origin(pi, pj, NR, NC, pi0, pj0);
+    float pv = 0.0f;
+    if (refine && tid < 25) pv = surf[(pi0 - 1 + tid / 5) + (pj0 - 1 + tid % 5) * NR];
+    finish_point(a, p, g, pi, pj, pval, pi0, pj0, refine, pv);
+}
+
+template __global__ void f64_prep_kernel<0, false>(TrackArgs, F64Stage);
+template __global__ void f64_prep_kernel<1, false>(TrackArgs, F64Stage);
+template __global__ void f64_prep_kernel<0, true>(TrackArgs, F64Stage);
+template __global__ void f64_prep_kernel<1, true>(TrackArgs, F64Stage);
+template __global__ void track_f64_kernel<0>(TrackArgs, F64Stage, int, const double*);
+template __global__ void track_f64_kernel<1>(TrackArgs, F64Stage, int, const double*);
+
+}  // namespace ar

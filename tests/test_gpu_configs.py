@@ -1,0 +1,264 @@
+"""GPU parity on the benchmark's own workloads: the default pipeline on the 512^2 pair, the
+reference's FFT arithmetic, the 10000^2 multi-chip search (a crop bit-exact against the oracle,
+the full scene against its known displacement field), batched points of many pairs, and a
+served mix that holds a 10000^2 pair.
+
+Bars: bit-exact against the oracle's direct numerator (the device's arithmetic); against the
+oracle's FFT numerator (the reference's arithmetic, FFTW in Float32), equal integer peaks except
+where the exact surface's best two values lie within 1e-5, and subpixel offsets within 1e-3 px
+except where an upsampled near-tie moves the peak by one 1/64 step.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from bench import cfg1_pair, synthetic_pair
+
+torch = pytest.importorskip("torch")   # loads its HIP runtime before the library's first call
+
+pytestmark = pytest.mark.gpu
+
+autorift = pytest.importorskip("autorift")
+
+
+def _bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+def _same_fields(got, exp, what):
+    for name, g, e in zip(("dx", "dy", "correlation"), got, exp):
+        np.testing.assert_array_equal(_bits(g), _bits(e), err_msg=f"{what}: {name}")
+
+
+def _oracle_prepare(img, quantize=True):
+    ok = np.isfinite(img)
+    hp, vm = oracle.highpass_filtered(img, ok, 5)
+    return (oracle.to_uint8(hp, vm) if quantize else hp), vm
+
+
+def _cfg1_grid():
+    xs = np.arange(40.0, 473.0, 16.0)
+    return autorift.gridpoints_xy(xs, xs, chip_size=32, search_radius=25)
+
+
+# --------------------------------------------------------------------------------------------
+# cfg1: highpass(5) -> UInt8 -> multi-chip levels 32 / 64 / 128 over the 784-point grid
+# --------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("quantize", ["uint8", "none"])
+def test_cfg1_default_pipeline_matches_oracle(quantize):
+    a, b = cfg1_pair(1)
+    grid = _cfg1_grid()
+    p = autorift.params(quantize=quantize)
+    print(f"cfg1 default pipeline, quantize {quantize}", flush=True)
+    with autorift.Context(p) as ctx:
+        ctx.set_images(a, b)
+        r8, s8, rv, sv = ctx.get_pair()
+        got = ctx.run_multichip(grid)
+    q = quantize == "uint8"
+    er, erv = _oracle_prepare(a, q)
+    es, esv = _oracle_prepare(b, q)
+    np.testing.assert_array_equal(rv, erv)
+    np.testing.assert_array_equal(sv, esv)
+    raw = lambda v: np.ascontiguousarray(v).view(np.uint8)   # bytes: Float32 compared bitwise
+    np.testing.assert_array_equal(raw(r8), raw(er))
+    np.testing.assert_array_equal(raw(s8), raw(es))
+    e = oracle.multichip(er, es, erv & esv, grid.x, grid.y, grid.radius_x, grid.radius_y,
+                         p=oracle.params(chip_size=32), nthreads=16)
+    _same_fields((got.dx, got.dy, got.correlation), e[:3], "cfg1 multichip")
+    np.testing.assert_array_equal(got.chip_size, e[3])
+    np.testing.assert_array_equal(got.interpolated, e[4])
+    ok = np.isfinite(got.dx)
+    assert ok.mean() > 0.95
+    # outputs are secondary -> reference offsets: (-3.30, +1.70)
+    assert abs(np.median(got.dx[ok]) + 3.30) < 0.03 and abs(np.median(got.dy[ok]) - 1.70) < 0.03
+
+
+def _top2_gap(ref, sec, ok, grid, r=25, cs=32):
+    """Gap between the largest and second-largest value of each point's exact surface."""
+    gaps = np.full(len(grid), np.inf)
+    x = np.ravel(grid.x, order="F")
+    y = np.ravel(grid.y, order="F")
+    h = cs // 2
+    pad = h + r + 2   # windows that leave the image read zeros (src/track.jl:212-220)
+    ref = np.pad(ref, pad)
+    sec = np.pad(sec, pad)
+    for k in range(len(grid)):
+        cx, cy = int(np.floor(x[k])) + pad, int(np.floor(y[k])) + pad
+        chip = sec[cy - h - 1:cy + h - 1, cx - h - 1:cx + h - 1]
+        win = ref[cy - h - r - 1:cy + h + r - 2, cx - h - r - 1:cx + h + r - 2]
+        s, good = oracle.correlate(win, chip, r)
+        if good:
+            v = np.sort(s.ravel())
+            gaps[k] = v[-1] - v[-2]
+    return gaps.reshape(grid.shape, order="F")
+
+
+@pytest.mark.parametrize("quantize", ["uint8", "none"])
+def test_cfg1_against_fft_numerator(quantize):
+    """The reference computes its numerators by FFT in Float32 (src/correlate.jl:816-848); the
+    device computes them exactly. Integer peaks must agree except at near-ties; subpixel offsets
+    within 1e-3 px except one upsampled step at an upsampled near-tie."""
+    a, b = cfg1_pair(1)
+    grid = _cfg1_grid()
+    p = autorift.params(quantize=quantize)
+    with autorift.Context(p) as ctx:
+        ctx.set_images(a, b)
+        ref, sec, rv, sv = ctx.get_pair()
+        g1 = ctx.track(grid, upsampling=1)
+        g64 = ctx.track(grid, upsampling=64)
+    ok = rv & sv
+    args = (ref, sec, ok, grid.x, grid.y, 25, 25, 32, 32)
+    f1 = oracle.track(*args, up=1, num_mode=oracle.NUM_FFT, nthreads=16)
+    f64 = oracle.track(*args, up=64, num_mode=oracle.NUM_FFT, nthreads=16)
+    meas = np.isfinite(g1.dx)
+    np.testing.assert_array_equal(meas, np.isfinite(f1[0]))
+    gap = _top2_gap(ref, sec, ok, grid)
+    differ = meas & ((g1.dx != f1[0]) | (g1.dy != f1[1]))
+    near = gap < 1e-5
+    print(f"cfg1 {quantize}: {meas.sum()} points, near-ties (gap < 1e-5) {int((near & meas).sum())},"
+          f" integer peaks differing {int(differ.sum())}", flush=True)
+    assert not (differ & ~near).any()
+    err = np.hypot(g64.dx - f64[0], g64.dy - f64[1])[meas]
+    print(f"cfg1 {quantize}: subpixel |d| > 1e-3 px at {int((err > 1e-3).sum())} of {err.size}, "
+          f"max {err.max():.4g} px", flush=True)
+    assert (err <= 1e-3).mean() >= 0.99
+    assert err.max() <= 1 / 64 * np.sqrt(2) + 1e-6
+    np.testing.assert_allclose(g1.correlation[meas], f1[2][meas], rtol=0, atol=1e-5)
+
+
+# --------------------------------------------------------------------------------------------
+# cfg2: 10000^2 texture warped by a smooth field, spacing 24, chip sizes 16..128, r 25
+# --------------------------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def cfg2_scene():
+    print("cfg2 scene: generating the 10000^2 pair", flush=True)
+    return synthetic_pair(10000, fields=True)
+
+
+def _cfg2_params(quantize):
+    return autorift.params(chip_size=16, chip_size_max=128, grid_spacing=24,
+                           search_radius=25, quantize=quantize)
+
+
+@pytest.mark.parametrize("quantize", ["uint8", "none"])
+def test_cfg2_crop_matches_oracle(cfg2_scene, quantize):
+    tex, sec, _, _ = cfg2_scene
+    r0, c0, n = 4000, 5200, 1024
+    a = np.ascontiguousarray(tex[r0:r0 + n, c0:c0 + n])
+    b = np.ascontiguousarray(sec[r0:r0 + n, c0:c0 + n])
+    p = _cfg2_params(quantize)
+    grid = autorift.api._build_grid(a.shape, p)
+    print(f"cfg2 crop {n}^2 quantize {quantize}: {len(grid)} points", flush=True)
+    with autorift.Context(p) as ctx:
+        ctx.set_images(a, b)
+        got = ctx.run_multichip(grid)
+    q = quantize == "uint8"
+    er, erv = _oracle_prepare(a, q)
+    es, esv = _oracle_prepare(b, q)
+    e = oracle.multichip(er, es, erv & esv, grid.x, grid.y, grid.radius_x, grid.radius_y,
+                         p=oracle.params(chip_size=16, chip_size_max=128), nthreads=16)
+    _same_fields((got.dx, got.dy, got.correlation), e[:3], "cfg2 crop")
+    np.testing.assert_array_equal(got.chip_size, e[3])
+    np.testing.assert_array_equal(got.interpolated, e[4])
+    assert np.isfinite(got.dx).mean() > 0.9
+
+
+@pytest.mark.parametrize("quantize", ["uint8", "none"])
+def test_cfg2_full_scene_recovers_field(cfg2_scene, quantize):
+    tex, sec, fx, fy = cfg2_scene
+    p = _cfg2_params(quantize)
+    grid = autorift.api._build_grid(tex.shape, p)
+    print(f"cfg2 full scene quantize {quantize}: {len(grid)} points", flush=True)
+    with autorift.Context(p) as ctx:
+        ctx.set_images(tex, sec)
+        got = ctx.run_multichip(grid)
+    ok = np.isfinite(got.dx)
+    # the field at each point's chip (secondary) position; outputs are secondary -> reference
+    iy = np.clip(np.floor(grid.y).astype(int) - 1, 0, tex.shape[0] - 1)
+    ix = np.clip(np.floor(grid.x).astype(int) - 1, 0, tex.shape[1] - 1)
+    ex, ey = -fx[iy, ix], -fy[iy, ix]
+    err = np.hypot(got.dx - ex, got.dy - ey)[ok]
+    levels = {int(c): int((got.chip_size == c).sum()) for c in np.unique(got.chip_size)}
+    print(f"cfg2 {quantize}: measured {ok.mean():.4f}, median error {np.median(err):.3f} px, "
+          f"95th {np.percentile(err, 95):.3f} px, levels {levels}", flush=True)
+    assert ok.mean() > 0.95
+    assert np.median(err) < 0.1 and np.percentile(err, 95) < 0.5
+
+
+# --------------------------------------------------------------------------------------------
+# cfg3: many pairs, chips {16, 32, 48, 64}, radii 8..40, scattered fractional centres
+# --------------------------------------------------------------------------------------------
+
+def test_cfg3_batched_sample_matches_oracle():
+    rng = np.random.default_rng(5)
+    npairs, per = 16, 130
+    pairs, xs, ys, cs, rs, pr = [], [], [], [], [], []
+    for k in range(npairs):
+        n = int(rng.integers(1024, 1537))
+        a, b = cfg1_pair(500 + k, n=n, shift=tuple(rng.uniform(-10, 10, 2)))
+        pairs.append((a, b))
+        xs.append(1 + rng.random(per) * (n - 1))
+        ys.append(1 + rng.random(per) * (n - 1))
+        cs.append(rng.choice([16, 32, 48, 64], per))
+        rs.append(rng.integers(8, 41, per))
+        pr.append(np.full(per, k, np.int32))
+    x, y = np.concatenate(xs), np.concatenate(ys)
+    c, r = np.concatenate(cs).astype(np.int64), np.concatenate(rs).astype(np.int64)
+    pair = np.concatenate(pr)
+    ps = autorift.pointset(x, y, search_radius_x=r, search_radius_y=r, chip_size_x=c,
+                           chip_size_y=c)
+    print(f"cfg3: {len(ps)} points over {npairs} pairs", flush=True)
+    with autorift.Batch(autorift.params()) as bt:
+        for a, b in pairs:
+            bt.add_pair(a, b)
+        got = bt.track(ps, pair, upsampling=64)
+    same = np.zeros(len(ps), bool)
+    for k, (a, b) in enumerate(pairs):
+        sel = np.nonzero(pair == k)[0]
+        er, erv = _oracle_prepare(a)
+        es, esv = _oracle_prepare(b)
+        e = oracle.track(er, es, erv & esv, x[sel], y[sel], r[sel], r[sel], c[sel], c[sel],
+                         up=64, nthreads=16)
+        np.testing.assert_array_equal(got.searched[sel], e[3])
+        ok = np.isfinite(e[0])
+        np.testing.assert_array_equal(np.isfinite(got.dx[sel]), ok)
+        same[sel] = ((_bits(got.dx[sel]) == _bits(e[0])) & (_bits(got.dy[sel]) == _bits(e[1])) &
+                     (_bits(got.correlation[sel]) == _bits(e[2])))
+    for chip in (16, 32, 48, 64):
+        m = (c == chip) & np.isfinite(got.dx)
+        print(f"cfg3 chip {chip}: {m.sum()} measured, bit-equal {same[m].mean():.4f}", flush=True)
+        if chip == 48:
+            assert same[m].mean() >= 0.999
+        else:
+            assert same[m].all()
+
+
+# --------------------------------------------------------------------------------------------
+# cfg5: a served mix holding a 10000^2 pair
+# --------------------------------------------------------------------------------------------
+
+def test_cfg5_served_mix_with_a_full_size_pair(cfg2_scene):
+    tex, sec, _, _ = cfg2_scene
+    big = (np.asfortranarray(tex), np.asfortranarray(sec))
+    small = [cfg1_pair(700 + k, n=n) for k, n in enumerate((512, 900, 1536, 640))]
+    pairs = [small[0], big, small[1], small[2], small[3]]
+    p = autorift.params()
+    alone = []
+    print("cfg5 mix: alone", flush=True)
+    with autorift.Context(p) as ctx:
+        for a, b in pairs:
+            ctx.set_images(a, b)
+            alone.append(ctx.run_multichip(autorift.api._build_grid(a.shape, p)))
+    print("cfg5 mix: served", flush=True)
+    with autorift.Server(p, slots=3) as srv:
+        tickets = [srv.submit(a, b) for a, b in pairs]
+        got = [srv.result(t)[0] for t in tickets]
+        st = srv.stats()
+    assert st["completed"] == len(pairs) and st["peak_running"] >= 2
+    for k, (r, e) in enumerate(zip(got, alone)):
+        _same_fields((r.dx, r.dy, r.correlation), (e.dx, e.dy, e.correlation), f"pair {k}")
+        np.testing.assert_array_equal(r.chip_size, e.chip_size)
+        np.testing.assert_array_equal(r.interpolated, e.interpolated)
+    assert got[1].dx.shape == autorift.api._build_grid(tex.shape, p).shape

@@ -126,13 +126,13 @@ def test_track_mixed_geometry(ctx, dtype):
     _compare(ctx, ref, sec, ok, pts, 64, measure=oracle.NCC)
 
 
-def test_track_nonpow2_chip_u8_uses_generic(ctx):
+def test_track_nonpow2_chip_u8_uses_direct_f64(ctx):
     ref, sec = _scene(160, np.uint8, seed=5)
     ok = np.ones(ref.shape, bool)
     xs = np.arange(60, 110, 16, dtype=float)
     pts = autorift.gridpoints_xy(xs, xs, chip_size=48, search_radius=12)
     _compare(ctx, ref, sec, ok, pts, 64)
-    assert ctx.last_kernel()[1] == 1
+    assert ctx.last_kernel()[1] == 3
     pts = autorift.gridpoints_xy(xs, xs, chip_size=32, search_radius=12)
     _compare(ctx, ref, sec, ok, pts, 64)
     assert ctx.last_kernel()[1] == 0
