@@ -28,7 +28,7 @@ __global__ void track_generic_kernel(TrackArgs a);
 template <bool LDS>
 __global__ void track_coherence_kernel(TrackArgs a);
 // Defined in track_f64.hip.
-template <int IS_U8, bool LDS_T>
+template <int IS_U8, int MODE>
 __global__ void f64_prep_kernel(TrackArgs a, F64Stage S);
 template <int IS_U8>
 __global__ void track_f64_kernel(TrackArgs a, F64Stage S, int pitch, const double* stage);
@@ -363,6 +363,11 @@ void ar_context_destroy(ar_context* ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     for (DevBuf* b : ctx->all_bufs()) b->release();
+    if (ctx->stream2) {
+        hipStreamSynchronize(ctx->stream2);
+        for (int k = 0; k < 5; k++) hipEventDestroy(ctx->evf[k]);
+        hipStreamDestroy(ctx->stream2);
+    }
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
     if (ctx->evm) hipEventDestroy(ctx->evm);
@@ -466,10 +471,12 @@ extern "C" ar_status ar_get_pair(ar_context* ctx, void* ref, void* sec, uint8_t*
     return AR_OK;
 }
 
-// The direct Float64 tracker over the pass's points (records the pass start event): chunks of
-// points whose staging slots fit kF64StageBytes, each a preparation launch (persistent, one
-// point per workgroup at a time, per-block summed-area tables) and a tracking launch (one
-// workgroup per point).
+// The direct Float64 tracker over the pass's points (records the pass start event). The points go
+// in chunks whose staging slots fit half of kF64StageBytes; each chunk is a preparation launch
+// (persistent, one point per workgroup at a time) and a tracking launch (one workgroup per point).
+// With more than one chunk the preparations run on the context's second stream, one chunk ahead
+// of the tracking launches, over two alternating halves of the staging buffer; events order the
+// two streams.
 static ar_status f64_launch(ar_context* ctx, const TrackArgs& a) {
     ar_status st;
     size_t wl = 0;
@@ -478,49 +485,74 @@ static ar_status f64_launch(ar_context* ctx, const TrackArgs& a) {
     const bool u8 = ctx->pdtype == AR_U8;
     const int winh = a.max_ch + 2 * a.max_ry - 1, winw = a.max_cw + 2 * a.max_rx - 1;
     const size_t td = (size_t)(winh + 1) * (winw + 1);
-    // the preparation kernel's summed-area table in LDS when it fits next to the chip
+    // the preparation kernel's LDS: chip, then the window (mode 2) and the table (modes 1, 2)
     const size_t lchip = ((size_t)a.max_ch * a.max_cw * 4 + 15) / 16 * 16;
-    const bool ldst = lchip + td * 8 <= kLdsLimit;
-    void (*fprep)(TrackArgs, F64Stage) =
-        ldst ? (u8 ? f64_prep_kernel<1, true> : f64_prep_kernel<0, true>)
-             : (u8 ? f64_prep_kernel<1, false> : f64_prep_kernel<0, false>);
-    const void* kprep = (const void*)fprep;
-    const void* ktrack = u8 ? (const void*)track_f64_kernel<1> : (const void*)track_f64_kernel<0>;
+    const size_t lwin = ((size_t)winh * winw * 4 + 15) / 16 * 16;
+    const int mode = lchip + lwin + td * 8 <= kLdsLimit ? 2 : (lchip + td * 8 <= kLdsLimit ? 1 : 0);
+    void (*fprep)(TrackArgs, F64Stage);
+    if (mode == 2) fprep = u8 ? f64_prep_kernel<1, 2> : f64_prep_kernel<0, 2>;
+    else if (mode == 1) fprep = u8 ? f64_prep_kernel<1, 1> : f64_prep_kernel<0, 1>;
+    else fprep = u8 ? f64_prep_kernel<1, 0> : f64_prep_kernel<0, 0>;
+    const size_t lprep = lchip + (mode == 2 ? lwin : 0) + (mode >= 1 ? td * 8 : 0);
+    void (*ftrack)(TrackArgs, F64Stage, int, const double*) =
+        u8 ? track_f64_kernel<1> : track_f64_kernel<0>;
     const F64Slot sl = f64_slot(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
-    int64_t chunk = (int64_t)(kF64StageBytes / sl.bytes);
+    int64_t chunk = (int64_t)(kF64StageBytes / 2 / sl.bytes);
     if (chunk < 1) chunk = 1;
+    const bool overlap = a.n > chunk;
     if (chunk > a.n) chunk = a.n;
-    const size_t lprep = ldst ? lchip + td * 8 : lchip;
     int pa = 0, pb = 0;
-    if ((st = kernel_occupancy(ctx, kprep, F64_PREP_WG, lprep, &pa))) return st;
-    if ((st = kernel_occupancy(ctx, ktrack, threads, wl, &pb))) return st;
+    if ((st = kernel_occupancy(ctx, (const void*)fprep, F64_PREP_WG, lprep, &pa))) return st;
+    if ((st = kernel_occupancy(ctx, (const void*)ftrack, threads, wl, &pb))) return st;
     if (pa < 1 || pb < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "tracker kernel does not fit on a CU");
-    int64_t nprep = (int64_t)ctx->ncu * pa;
+    // running beside the tracker, the preparation keeps to one workgroup per CU
+    int64_t nprep = (int64_t)ctx->ncu * (overlap ? 1 : pa);
     if (nprep > chunk) nprep = chunk;
-    const size_t tbytes = ldst ? 0 : (size_t)nprep * td * 8;
-    if ((st = ctx->scratch.reserve(ctx, (size_t)chunk * sl.bytes + tbytes))) return st;
+    const int nbuf = overlap ? 2 : 1;
+    const size_t half = (size_t)chunk * sl.bytes;
+    const size_t tbytes = mode >= 1 ? 0 : (size_t)nbuf * nprep * td * 8;
+    if ((st = ctx->scratch.reserve(ctx, nbuf * half + tbytes))) return st;
+    if (overlap && !ctx->stream2) {
+        HIPCHK(ctx, hipStreamCreate(&ctx->stream2));
+        for (int k = 0; k < 5; k++)
+            HIPCHK(ctx, hipEventCreateWithFlags(&ctx->evf[k], hipEventDisableTiming));
+    }
+    unsigned char* base = (unsigned char*)ctx->scratch.ptr;
     F64Stage S;
-    S.slots = (unsigned char*)ctx->scratch.ptr;
     S.slot_bytes = sl.bytes;
     S.off_chip = sl.off_chip;
     S.off_b1 = sl.off_b1;
     S.off_b2 = sl.off_b2;
     S.chp = sl.chp;
-    S.tables = (double*)((unsigned char*)ctx->scratch.ptr + (size_t)chunk * sl.bytes);
     S.table_doubles = td;
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    for (int64_t p0 = 0; p0 < a.n; p0 += chunk) {
+    hipStream_t ps = overlap ? ctx->stream2 : ctx->stream;
+    if (overlap) {
+        // evf[0]: everything before the pass; evf[1 + b]: preparation into half b done;
+        // evf[3 + b]: tracking from half b done
+        HIPCHK(ctx, hipEventRecord(ctx->evf[0], ctx->stream));
+        HIPCHK(ctx, hipStreamWaitEvent(ctx->stream2, ctx->evf[0], 0));
+    }
+    int64_t c = 0;
+    for (int64_t p0 = 0; p0 < a.n; p0 += chunk, c++) {
+        const int h = (int)(c & 1) % nbuf;
         S.p0 = p0;
         S.cnt = a.n - p0 < chunk ? a.n - p0 : chunk;
+        S.slots = base + (size_t)h * half;
+        S.tables = (double*)(base + nbuf * half + (size_t)h * (tbytes / nbuf));
+        if (overlap && c >= 2) HIPCHK(ctx, hipStreamWaitEvent(ctx->stream2, ctx->evf[3 + h], 0));
         const int64_t gp = nprep < S.cnt ? nprep : S.cnt;
-        hipLaunchKernelGGL(fprep, dim3((unsigned)gp), dim3(F64_PREP_WG), lprep, ctx->stream, a,
-                           S);
+        hipLaunchKernelGGL(fprep, dim3((unsigned)gp), dim3(F64_PREP_WG), lprep, ps, a, S);
         HIPCHK(ctx, hipGetLastError());
+        if (overlap) {
+            HIPCHK(ctx, hipEventRecord(ctx->evf[1 + h], ctx->stream2));
+            HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->evf[1 + h], 0));
+        }
         const int64_t gt = 8 * ((S.cnt + 7) / 8);
-        hipLaunchKernelGGL(u8 ? track_f64_kernel<1> : track_f64_kernel<0>, dim3((unsigned)gt),
-                           dim3(threads), wl, ctx->stream, a, S, pitch,
-                           (const double*)ctx->scratch.ptr);
+        hipLaunchKernelGGL(ftrack, dim3((unsigned)gt), dim3(threads), wl, ctx->stream, a, S, pitch,
+                           (const double*)S.slots);
         HIPCHK(ctx, hipGetLastError());
+        if (overlap) HIPCHK(ctx, hipEventRecord(ctx->evf[3 + h], ctx->stream));
     }
     return AR_OK;
 }

@@ -25,6 +25,10 @@ struct ar_context {
     int ncu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr, evb = nullptr;
+    // the direct Float64 tracker's second stream (preparation of the next chunk of points) and
+    // the events that order it with `stream`; made on first use
+    hipStream_t stream2 = nullptr;
+    hipEvent_t evf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     std::string err;
 
     // prepared pair (what the tracker correlates), column-major ny x nx

@@ -177,7 +177,7 @@ __host__ __device__ inline int f64_pitch(int max_ch, int max_ry) {
     return p;
 }
 
-#define F64_PREP_WG 128
+#define F64_PREP_WG 256
 
 struct F64Stage {
     unsigned char* slots;      // chunk's slots, slot.bytes apart

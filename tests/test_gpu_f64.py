@@ -119,3 +119,28 @@ def test_direct_equals_generic_and_large_window_fallback(ctx):
     print("direct f64 fallback", flush=True)
     ref2, sec2 = shifted_pair(300, (3, -2), seed=14, dtype=np.float32)
     _check(ctx, ref2, sec2, ok, big, 64, 1)
+
+
+def test_direct_chunks_on_two_streams_equal_generic(ctx):
+    """20000 points at chip 64, radius 40 (about 135 KB of staging each): several chunks, prepared
+    on the second stream while the previous chunk is tracked. Every bit equals the one-kernel
+    generic tracker (itself bit-exact against the oracle above)."""
+    ref, sec = shifted_pair(1200, (1.25, -2.75), seed=15, dtype=np.float32)
+    ok = np.ones(ref.shape, bool)
+    rng = np.random.default_rng(8)
+    m = 20000
+    pts = autorift.pointset(rng.uniform(80, 1120, m), rng.uniform(80, 1120, m),
+                            search_radius=40, chip_size=64)
+    ctx.set_pair(ref, sec, ok, ok)
+    print("direct f64 chunks", flush=True)
+    a = ctx.track(pts, upsampling=64)
+    assert ctx.last_kernel()[1] == 3
+    os.environ["AR_F64_DIRECT"] = "never"
+    try:
+        b = ctx.track(pts, upsampling=64)
+    finally:
+        del os.environ["AR_F64_DIRECT"]
+    np.testing.assert_array_equal(a.searched, b.searched)
+    for u, v in zip((a.dx, a.dy, a.correlation), (b.dx, b.dy, b.correlation)):
+        np.testing.assert_array_equal(_bits(u), _bits(v))
+    assert np.isfinite(a.dx).mean() > 0.99
