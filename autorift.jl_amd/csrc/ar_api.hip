@@ -471,88 +471,105 @@ extern "C" ar_status ar_get_pair(ar_context* ctx, void* ref, void* sec, uint8_t*
     return AR_OK;
 }
 
-// The direct Float64 tracker over the pass's points (records the pass start event). The points go
-// in chunks whose staging slots fit half of kF64StageBytes; each chunk is a preparation launch
-// (persistent, one point per workgroup at a time) and a tracking launch (one workgroup per point).
-// With more than one chunk the preparations run on the context's second stream, one chunk ahead
-// of the tracking launches, over two alternating halves of the staging buffer; events order the
-// two streams.
-static ar_status f64_launch(ar_context* ctx, const TrackArgs& a) {
+// Launch plan of the direct Float64 tracker over a.n points (a.order, when set, lists them):
+// kernels, LDS sizes, chunking, and the staging buffer, reserved here so that f64_enqueue makes no
+// allocation and no runtime query (it may run under graph capture). The points go in chunks whose
+// staging slots fit half of kF64StageBytes; each chunk is a preparation launch (persistent, one
+// point per workgroup at a time) and a tracking launch (one workgroup per point). With `overlap`
+// and more than one chunk the preparations run on the context's second stream, one chunk ahead of
+// the tracking launches, over two alternating halves of the staging buffer; events order the two
+// streams.
+struct F64Plan {
+    void (*fprep)(TrackArgs, F64Stage);
+    void (*ftrack)(TrackArgs, F64Stage, int, const double*);
+    size_t lprep, wl;
+    int pitch, threads;
+    int64_t chunk, nprep;
+    bool overlap;
+    int nbuf;
+    size_t half, tbytes;
+    F64Slot sl;
+    size_t td;
+};
+
+static ar_status f64_plan(ar_context* ctx, const TrackArgs& a, bool u8, bool allow_overlap,
+                          F64Plan* P) {
     ar_status st;
-    size_t wl = 0;
-    int pitch = 0, threads = 0;
-    f64_shape(a, &wl, &pitch, &threads);
-    const bool u8 = ctx->pdtype == AR_U8;
+    f64_shape(a, &P->wl, &P->pitch, &P->threads);
     const int winh = a.max_ch + 2 * a.max_ry - 1, winw = a.max_cw + 2 * a.max_rx - 1;
-    const size_t td = (size_t)(winh + 1) * (winw + 1);
+    P->td = (size_t)(winh + 1) * (winw + 1);
     // the preparation kernel's LDS: chip, then the window (mode 2) and the table (modes 1, 2)
     const size_t lchip = ((size_t)a.max_ch * a.max_cw * 4 + 15) / 16 * 16;
     const size_t lwin = ((size_t)winh * winw * 4 + 15) / 16 * 16;
-    const int mode = lchip + lwin + td * 8 <= kLdsLimit ? 2 : (lchip + td * 8 <= kLdsLimit ? 1 : 0);
-    void (*fprep)(TrackArgs, F64Stage);
-    if (mode == 2) fprep = u8 ? f64_prep_kernel<1, 2> : f64_prep_kernel<0, 2>;
-    else if (mode == 1) fprep = u8 ? f64_prep_kernel<1, 1> : f64_prep_kernel<0, 1>;
-    else fprep = u8 ? f64_prep_kernel<1, 0> : f64_prep_kernel<0, 0>;
-    const size_t lprep = lchip + (mode == 2 ? lwin : 0) + (mode >= 1 ? td * 8 : 0);
-    void (*ftrack)(TrackArgs, F64Stage, int, const double*) =
-        u8 ? track_f64_kernel<1> : track_f64_kernel<0>;
-    const F64Slot sl = f64_slot(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
-    int64_t chunk = (int64_t)(kF64StageBytes / 2 / sl.bytes);
+    const size_t td8 = P->td * 8;
+    const int mode = lchip + lwin + td8 <= kLdsLimit ? 2 : (lchip + td8 <= kLdsLimit ? 1 : 0);
+    if (mode == 2) P->fprep = u8 ? f64_prep_kernel<1, 2> : f64_prep_kernel<0, 2>;
+    else if (mode == 1) P->fprep = u8 ? f64_prep_kernel<1, 1> : f64_prep_kernel<0, 1>;
+    else P->fprep = u8 ? f64_prep_kernel<1, 0> : f64_prep_kernel<0, 0>;
+    P->lprep = lchip + (mode == 2 ? lwin : 0) + (mode >= 1 ? td8 : 0);
+    P->ftrack = u8 ? track_f64_kernel<1> : track_f64_kernel<0>;
+    P->sl = f64_slot(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
+    int64_t chunk = (int64_t)(kF64StageBytes / 2 / P->sl.bytes);
     if (chunk < 1) chunk = 1;
-    const bool overlap = a.n > chunk;
+    P->overlap = allow_overlap && a.n > chunk;
     if (chunk > a.n) chunk = a.n;
+    P->chunk = chunk;
     int pa = 0, pb = 0;
-    if ((st = kernel_occupancy(ctx, (const void*)fprep, F64_PREP_WG, lprep, &pa))) return st;
-    if ((st = kernel_occupancy(ctx, (const void*)ftrack, threads, wl, &pb))) return st;
+    if ((st = kernel_occupancy(ctx, (const void*)P->fprep, F64_PREP_WG, P->lprep, &pa))) return st;
+    if ((st = kernel_occupancy(ctx, (const void*)P->ftrack, P->threads, P->wl, &pb))) return st;
     if (pa < 1 || pb < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "tracker kernel does not fit on a CU");
     // running beside the tracker, the preparation keeps to one workgroup per CU
-    int64_t nprep = (int64_t)ctx->ncu * (overlap ? 1 : pa);
-    if (nprep > chunk) nprep = chunk;
-    const int nbuf = overlap ? 2 : 1;
-    const size_t half = (size_t)chunk * sl.bytes;
-    const size_t tbytes = mode >= 1 ? 0 : (size_t)nbuf * nprep * td * 8;
-    if ((st = ctx->scratch.reserve(ctx, nbuf * half + tbytes))) return st;
-    if (overlap && !ctx->stream2) {
+    P->nprep = (int64_t)ctx->ncu * (P->overlap ? 1 : pa);
+    if (P->nprep > chunk) P->nprep = chunk;
+    P->nbuf = P->overlap ? 2 : 1;
+    P->half = (size_t)chunk * P->sl.bytes;
+    P->tbytes = mode >= 1 ? 0 : (size_t)P->nbuf * P->nprep * td8;
+    if ((st = ctx->scratch.reserve(ctx, P->nbuf * P->half + P->tbytes))) return st;
+    if (P->overlap && !ctx->stream2) {
         HIPCHK(ctx, hipStreamCreate(&ctx->stream2));
         for (int k = 0; k < 5; k++)
             HIPCHK(ctx, hipEventCreateWithFlags(&ctx->evf[k], hipEventDisableTiming));
     }
+    return AR_OK;
+}
+
+static ar_status f64_enqueue(ar_context* ctx, const TrackArgs& a, const F64Plan& P) {
     unsigned char* base = (unsigned char*)ctx->scratch.ptr;
     F64Stage S;
-    S.slot_bytes = sl.bytes;
-    S.off_chip = sl.off_chip;
-    S.off_b1 = sl.off_b1;
-    S.off_b2 = sl.off_b2;
-    S.chp = sl.chp;
-    S.table_doubles = td;
-    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    hipStream_t ps = overlap ? ctx->stream2 : ctx->stream;
-    if (overlap) {
+    S.slot_bytes = P.sl.bytes;
+    S.off_chip = P.sl.off_chip;
+    S.off_b1 = P.sl.off_b1;
+    S.off_b2 = P.sl.off_b2;
+    S.chp = P.sl.chp;
+    S.table_doubles = P.td;
+    hipStream_t ps = P.overlap ? ctx->stream2 : ctx->stream;
+    if (P.overlap) {
         // evf[0]: everything before the pass; evf[1 + b]: preparation into half b done;
         // evf[3 + b]: tracking from half b done
         HIPCHK(ctx, hipEventRecord(ctx->evf[0], ctx->stream));
         HIPCHK(ctx, hipStreamWaitEvent(ctx->stream2, ctx->evf[0], 0));
     }
     int64_t c = 0;
-    for (int64_t p0 = 0; p0 < a.n; p0 += chunk, c++) {
-        const int h = (int)(c & 1) % nbuf;
+    for (int64_t p0 = 0; p0 < a.n; p0 += P.chunk, c++) {
+        const int h = (int)(c & 1) % P.nbuf;
         S.p0 = p0;
-        S.cnt = a.n - p0 < chunk ? a.n - p0 : chunk;
-        S.slots = base + (size_t)h * half;
-        S.tables = (double*)(base + nbuf * half + (size_t)h * (tbytes / nbuf));
-        if (overlap && c >= 2) HIPCHK(ctx, hipStreamWaitEvent(ctx->stream2, ctx->evf[3 + h], 0));
-        const int64_t gp = nprep < S.cnt ? nprep : S.cnt;
-        hipLaunchKernelGGL(fprep, dim3((unsigned)gp), dim3(F64_PREP_WG), lprep, ps, a, S);
+        S.cnt = a.n - p0 < P.chunk ? a.n - p0 : P.chunk;
+        S.slots = base + (size_t)h * P.half;
+        S.tables = (double*)(base + P.nbuf * P.half + (size_t)h * (P.tbytes / P.nbuf));
+        if (P.overlap && c >= 2)
+            HIPCHK(ctx, hipStreamWaitEvent(ctx->stream2, ctx->evf[3 + h], 0));
+        const int64_t gp = P.nprep < S.cnt ? P.nprep : S.cnt;
+        hipLaunchKernelGGL(P.fprep, dim3((unsigned)gp), dim3(F64_PREP_WG), P.lprep, ps, a, S);
         HIPCHK(ctx, hipGetLastError());
-        if (overlap) {
+        if (P.overlap) {
             HIPCHK(ctx, hipEventRecord(ctx->evf[1 + h], ctx->stream2));
             HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->evf[1 + h], 0));
         }
         const int64_t gt = 8 * ((S.cnt + 7) / 8);
-        hipLaunchKernelGGL(ftrack, dim3((unsigned)gt), dim3(threads), wl, ctx->stream, a, S, pitch,
-                           (const double*)S.slots);
+        hipLaunchKernelGGL(P.ftrack, dim3((unsigned)gt), dim3(P.threads), P.wl, ctx->stream, a, S,
+                           P.pitch, (const double*)S.slots);
         HIPCHK(ctx, hipGetLastError());
-        if (overlap) HIPCHK(ctx, hipEventRecord(ctx->evf[3 + h], ctx->stream));
+        if (P.overlap) HIPCHK(ctx, hipEventRecord(ctx->evf[3 + h], ctx->stream));
     }
     return AR_OK;
 }
@@ -795,7 +812,10 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         HIPCHK(ctx, hipGetLastError());
     }
     if (use_f64) {
-        if ((st = f64_launch(ctx, a))) return st;
+        F64Plan P;
+        if ((st = f64_plan(ctx, a, ctx->pdtype == AR_U8, true, &P))) return st;
+        HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        if ((st = f64_enqueue(ctx, a, P))) return st;
     } else {
         HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(AR_WG), a.lds_bytes, ctx->stream, a);
@@ -1013,13 +1033,28 @@ struct ar_batch {
     GraphKey gkey;
     bool has_graph = false;
     int graph_launches = 0, graph_split = 0;
+    // device buffers the captured pass addresses; a replay needs every one where it was (a later
+    // pass that grew one of them moved it)
+    const void* gbufs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    void graph_bufs(const void** out) const {
+        out[0] = desc.ptr; out[1] = lists.ptr; out[2] = counts.ptr; out[3] = ctx->refine.ptr;
+        out[4] = ctx->prec.ptr; out[5] = ctx->scratch.ptr;
+    }
+    bool graph_bufs_same() const {
+        const void* b[6];
+        graph_bufs(b);
+        for (int k = 0; k < 6; k++)
+            if (b[k] != gbufs[k]) return false;
+        return true;
+    }
     double last_ms = 0;
     int last_split = 0, last_launches = 0, last_replayed = 0;
 };
 
 // Radius classes of a batched pass: class 0 max(rx, ry) <= R1, class 1 <= R2 (R2 > R1, or 0:
-// no such class), class 2 the rest; each class list in any order. One atomic per wave and class
-// (ballot, then each lane's rank among the wave's lanes of its class).
+// no such class), class 2 the rest, and class 3 every searchable point whose chip area is not a
+// power of two (the direct Float64 tracker takes those); each class list in any order. One atomic
+// per wave and class (ballot, then each lane's rank among the wave's lanes of its class).
 __global__ __launch_bounds__(256) void batch_classify_kernel(ar_points pts, int64_t n, int R1,
                                                              int R2, int32_t* lists,
                                                              uint32_t* counts) {
@@ -1032,9 +1067,13 @@ __global__ __launch_bounds__(256) void batch_classify_kernel(ar_points pts, int6
         if (i < n) {
             const int64_t r = max(pts.radius_x[i], pts.radius_y[i]);
             c = r <= R1 ? 0 : ((R2 > 0 && r <= R2) ? 1 : 2);
+            // a searchable chip whose area is not a power of two: the direct Float64 tracker
+            const int64_t area = (2 * (pts.chip_size_y[i] / 2)) * (2 * (pts.chip_size_x[i] / 2));
+            if (pts.radius_x[i] > 0 && pts.radius_y[i] > 0 && area > 0 && (area & (area - 1)) != 0)
+                c = 3;
         }
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < 4; k++) {
             const uint64_t m = __ballot(c == k);
             if (m == 0) continue;
             const int leader = __ffsll((long long)m) - 1;
@@ -1140,7 +1179,7 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     key.up = up;
     key.measure = measure;
     key.split = split;
-    if (replay && B->has_graph && B->gkey == key) {
+    if (replay && B->has_graph && B->gkey == key && B->graph_bufs_same()) {
         // the captured pass over the same buffers (contents may differ only in values)
         HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         HIPCHK(ctx, hipGraphLaunch(B->gexec, ctx->stream));
@@ -1178,7 +1217,7 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     unsigned long long g[10];
     for (int k = 0; k < 10; k++) g[k] = gs[AR_GEOM_SLOT * k];
     const int64_t cx = g[0], cy = g[1], rx = g[2], ry = g[3], px = g[4], py = g[5];
-    const bool fits = g[8] != 0;
+    const bool fits = g[8] != 0, pow2 = g[9] != 0;
     if (cx == 0) {
         skip_all_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dx, dy, corr, srch, n);
         HIPCHK(ctx, hipGetLastError());
@@ -1210,7 +1249,7 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     a.rmeta = (int32_t*)((char*)ctx->refine.ptr + (size_t)n * 128);
     if ((st = ctx->prec.reserve(ctx, (size_t)n * AR_PREC * 4))) return st;
     a.prec = (const int32_t*)ctx->prec.ptr;
-    if ((st = B->lists.reserve(ctx, (size_t)n * 12))) return st;
+    if ((st = B->lists.reserve(ctx, (size_t)n * 16))) return st;
     int32_t* lists = (int32_t*)B->lists.ptr;
     // the tuner: time each candidate split once per pass shape, keep the fastest
     const int rmax = (int)(rx > ry ? rx : ry);
@@ -1266,15 +1305,29 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
         if (rblocks > n) rblocks = n;
     }
     // class sizes (the class lists are rebuilt inside the pass, so a replay needs the same points)
-    int64_t m[3] = {n, 0, 0};
-    if (R1 > 0) {
-        batch_zero_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 3);
+    int64_t m[4] = {n, 0, 0, 0};
+    const bool classify = R1 > 0 || !pow2;
+    if (classify) {
+        batch_zero_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 4);
         batch_classify_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, R1, R2, lists, cnt);
         HIPCHK(ctx, hipGetLastError());
-        uint32_t c3[3] = {0, 0, 0};
-        HIPCHK(ctx, hipMemcpyAsync(c3, cnt, 12, hipMemcpyDeviceToHost, ctx->stream));
+        uint32_t c4[4] = {0, 0, 0, 0};
+        HIPCHK(ctx, hipMemcpyAsync(c4, cnt, 16, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        for (int k = 0; k < 3; k++) m[k] = c3[k];
+        for (int k = 0; k < 4; k++) m[k] = c4[k];
+    }
+    // the non-power-of-two chips' launches are planned (occupancy, staging) before any capture
+    F64Plan fp;
+    TrackArgs af = a;
+    af.order = lists + 3 * n;
+    af.n = m[3];
+    if (m[3] > 0) {
+        size_t wl = 0;
+        int pitch = 0, threads = 0;
+        if (!f64_shape(af, &wl, &pitch, &threads))
+            return fail(ctx, AR_ERR_UNSUPPORTED,
+                        "chip or search window too large for the batched Float64 tracker");
+        if ((st = f64_plan(ctx, af, true, false, &fp))) return st;
     }
     const bool capture = replay;
     if (capture) {
@@ -1290,8 +1343,8 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     B->last_launches = 0;
     // ---- the pass: point records, class lists, the tracker per class, the refinement
     point_prep_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(a);
-    if (R1 > 0) {
-        batch_zero_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 3);
+    if (classify) {
+        batch_zero_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 4);
         batch_classify_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, R1, R2, lists, cnt);
     } else {
         batch_iota_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(lists, n);
@@ -1301,6 +1354,10 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     ar_status lst = AR_OK;
     for (int k = 0; k < 3 && lst == AR_OK; k++)
         if (m[k] > 0) lst = batch_launch(B, a, lists + k * n, m[k], caps[k]);
+    if (lst == AR_OK && m[3] > 0) {
+        lst = f64_enqueue(ctx, af, fp);
+        B->last_launches += 2 * (int)((m[3] + fp.chunk - 1) / fp.chunk);
+    }
     if (lst == AR_OK && up > 1)
         hipLaunchKernelGGL(refine_kernel, dim3((unsigned)rblocks), dim3(64), rlds, ctx->stream, a);
     if (capture) {
@@ -1319,6 +1376,7 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
         B->graph = gr;
         B->gexec = ex;
         B->gkey = key;
+        B->graph_bufs(B->gbufs);
         B->has_graph = true;
         B->graph_launches = B->last_launches;
         B->graph_split = split;

@@ -262,7 +262,7 @@ __global__ __launch_bounds__(F64_PREP_WG) void f64_prep_kernel(TrackArgs a, F64S
     double* T = MODE >= 1 ? (double*)(smem + lchip + lwin)
                           : S.tables + (size_t)blockIdx.x * S.table_doubles;
     for (int64_t k = blockIdx.x; k < S.cnt; k += gridDim.x) {
-        const int64_t p = S.p0 + k;
+        const int64_t p = a.order ? (int64_t)a.order[S.p0 + k] : S.p0 + k;
         TrackArgs b = a;
         if (a.pairs) use_pair(b, p);
         unsigned char* slot = S.slots + (size_t)k * S.slot_bytes;
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(1024) void track_f64_kernel(TrackArgs a, F64Stage S
     __shared__ Best red[16];
     const int64_t k = f64_point_of_block(S.cnt);
     if (k < 0) return;
-    const int64_t p = S.p0 + k;
+    const int64_t p = a.order ? (int64_t)a.order[S.p0 + k] : S.p0 + k;
     const unsigned char* slot = S.slots + (size_t)k * S.slot_bytes;
     if (*(const int32_t*)slot != 2) return;   // skipped or degenerate: written by kernel 1
     TrackArgs b = a;

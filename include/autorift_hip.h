@@ -182,9 +182,9 @@ const char* ar_version(void);
  * A batch keeps many prepared UInt8 pairs resident (default pipeline: the batch's params must
  * quantise to UInt8) and runs one tracking pass over points of all of them, each point naming
  * its pair: track (src/track.jl:103-160) over every point, with uneven chip sizes and radii.
- * For power-of-two chip areas the results equal a pass over each pair alone, bit for bit; for
- * other even chip areas the numerator is the exact correlation of the unrounded mean-removed
- * chip (the reference rounds it to Float32 first, src/correlate.jl:448-450). */
+ * The results equal a pass over each pair alone, bit for bit: power-of-two chip areas take the
+ * integer matrix-core tracker, other chip areas the direct Float64 tracker (the reference's
+ * Float32(x - mean) chip and Float64 sums, src/correlate.jl:423-453, 799-814). */
 typedef struct ar_batch ar_batch;
 #define AR_BATCH_GRAPH 1      /* capture the pass in a HIP graph, replay it for the same buffers */
 #define AR_BATCH_SPLIT_SET 2  /* bits 8..15 (R1) and 16..23 (R2) give the radius classes

@@ -1,10 +1,9 @@
 """GPU tests of the batched pass (ar_batch): points of many pairs, uneven chip sizes and radii,
 in one tracking pass.
 
-For power-of-two chip areas every point's result must equal a pass over its own pair alone, bit
-for bit (the same exact integer arithmetic; the pass's shared padding only adds invalid zeros).
-Chip areas that are not powers of two correlate the unrounded mean-removed chip: integer peaks
-and values are held to the exact Float64 path within a stated tolerance.
+Every point's result must equal a pass over its own pair alone, bit for bit: power-of-two chip
+areas run the same exact integer arithmetic, other chip areas the same direct Float64 tracker, and
+the pass's shared padding only adds invalid zeros.
 """
 import numpy as np
 import pytest
@@ -118,21 +117,40 @@ def test_batch_splits_and_graph_replay():
             check(f"replay {rep}")
 
 
-def test_batch_non_power_of_two_chips():
-    """Chip 48 (area 2304): the exact rational correlation; peaks agree with the Float64 path."""
-    pairs, ps, pair = _scene(3, chips=(48,), n_per=40)
-    print("batch chip 48", flush=True)
-    exp = _alone(pairs, ps, pair, 1)      # generic Float64 path (reference arithmetic)
+@pytest.mark.parametrize("up", [1, 64])
+def test_batch_non_power_of_two_chips(up):
+    """Chip 48 (area 2304) mixed with power-of-two chips: the direct Float64 tracker inside the
+    batched pass, bit-equal to each pair alone, also when the pass is captured and replayed."""
+    pairs, ps, pair = _scene(3, chips=(16, 48, 32, 24), n_per=50)
+    print(f"batch chip 48 up{up}", flush=True)
+    exp = _alone(pairs, ps, pair, up)
     with autorift.Batch(autorift.params()) as bt:
         for a, b in pairs:
             bt.add_pair(a, b)
-        got = bt.track(ps, pair, upsampling=1)
+        got = bt.track(ps, pair, upsampling=up)
+        n = len(ps)
+        dev = torch.device("cuda:0")
+        flds = [torch.from_numpy(np.ascontiguousarray(np.ravel(getattr(ps, f), order="F"))).to(dev)
+                for f in ("x", "y", "radius_x", "radius_y", "dx_prior", "dy_prior",
+                          "chip_size_x", "chip_size_y")]
+        pr = torch.from_numpy(pair.astype(np.int32)).to(dev)
+        out = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(3)]
+        out.append(torch.empty(n, dtype=torch.uint8, device=dev))
+        fp = [t.data_ptr() for t in flds]
+        op = [t.data_ptr() for t in out]
+        for rep in range(3):
+            for t in out:
+                t.zero_()
+            torch.cuda.synchronize()
+            bt.track_device(fp, pr.data_ptr(), n, up, 0, op, graph=True, split=24)
+            assert bt.info()["replayed"] == (rep > 0)
+            torch.cuda.synchronize()
+            for t, e in zip(out[:3], exp[:3]):
+                np.testing.assert_array_equal(_bits(t.cpu().numpy()), _bits(e))
     np.testing.assert_array_equal(got.searched, exp[3])
-    fin = np.isfinite(exp[0])
-    assert np.array_equal(np.isfinite(got.dx), fin)
-    same = (got.dx[fin] == exp[0][fin]) & (got.dy[fin] == exp[1][fin])
-    assert same.mean() >= 0.99
-    np.testing.assert_allclose(got.correlation[fin][same], exp[2][fin][same], rtol=0, atol=2e-6)
+    for g, e in zip((got.dx, got.dy, got.correlation), exp[:3]):
+        np.testing.assert_array_equal(_bits(g), _bits(e))
+    assert np.isfinite(got.dx[ps.chip_size_x == 48]).mean() > 0.5
 
 
 def test_batch_errors():

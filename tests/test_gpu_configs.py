@@ -229,10 +229,7 @@ def test_cfg3_batched_sample_matches_oracle():
     for chip in (16, 32, 48, 64):
         m = (c == chip) & np.isfinite(got.dx)
         print(f"cfg3 chip {chip}: {m.sum()} measured, bit-equal {same[m].mean():.4f}", flush=True)
-        if chip == 48:
-            assert same[m].mean() >= 0.999
-        else:
-            assert same[m].all()
+        assert m.sum() > 100 and same[m].all()
 
 
 # --------------------------------------------------------------------------------------------
