@@ -87,6 +87,9 @@ SIGNATURES = [
                                      C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int32,
                                      C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_float)]),
+    ("ar_correlate_point_noalloc", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+      C.c_int64, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_float)]),
     ("ar_last_kernel", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     ("ar_last_timing", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)]),

@@ -254,6 +254,16 @@ __global__ void refx_kernel(const uint8_t* src, uint8_t* dst, int64_t n, int64_t
 static std::mutex g_attr_mu;
 static std::set<const void*> g_attr_set;
 
+static ar_status lds_allowance(ar_context* ctx, const void* kern) {
+    std::lock_guard<std::mutex> lk(g_attr_mu);
+    if (!g_attr_set.count(kern)) {
+        HIPCHK(ctx, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)kLdsLimit));
+        g_attr_set.insert(kern);
+    }
+    return AR_OK;
+}
+
 static ar_status kernel_occupancy(ar_context* ctx, const void* kern, int threads, size_t lds,
                                   int* per_cu) {
     const auto key = std::make_tuple(kern, threads, lds);
@@ -262,14 +272,7 @@ static ar_status kernel_occupancy(ar_context* ctx, const void* kern, int threads
         *per_cu = it->second;
         return AR_OK;
     }
-    {
-        std::lock_guard<std::mutex> lk(g_attr_mu);
-        if (!g_attr_set.count(kern)) {
-            HIPCHK(ctx, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                            (int)kLdsLimit));
-            g_attr_set.insert(kern);
-        }
-    }
+    if (ar_status st = lds_allowance(ctx, kern)) return st;
     if (lds > kLdsLimit) {
         *per_cu = 0;
         return AR_OK;
@@ -492,9 +495,8 @@ struct F64Plan {
     size_t td;
 };
 
-static ar_status f64_plan(ar_context* ctx, const TrackArgs& a, bool u8, bool allow_overlap,
-                          F64Plan* P) {
-    ar_status st;
+// Kernels, LDS sizes and slot layout of the plan (no runtime call).
+static void f64_plan_shape(const TrackArgs& a, bool u8, F64Plan* P) {
     f64_shape(a, &P->wl, &P->pitch, &P->threads);
     const int winh = a.max_ch + 2 * a.max_ry - 1, winw = a.max_cw + 2 * a.max_rx - 1;
     P->td = (size_t)(winh + 1) * (winw + 1);
@@ -509,6 +511,14 @@ static ar_status f64_plan(ar_context* ctx, const TrackArgs& a, bool u8, bool all
     P->lprep = lchip + (mode == 2 ? lwin : 0) + (mode >= 1 ? td8 : 0);
     P->ftrack = u8 ? track_f64_kernel<1> : track_f64_kernel<0>;
     P->sl = f64_slot(a.max_ch, a.max_cw, a.max_rx, a.max_ry);
+    P->tbytes = mode >= 1 ? 0 : td8;   // one block's table; scaled by the launch plan
+}
+
+static ar_status f64_plan(ar_context* ctx, const TrackArgs& a, bool u8, bool allow_overlap,
+                          F64Plan* P) {
+    ar_status st;
+    f64_plan_shape(a, u8, P);
+    const size_t table1 = P->tbytes;
     int64_t chunk = (int64_t)(kF64StageBytes / 2 / P->sl.bytes);
     if (chunk < 1) chunk = 1;
     P->overlap = allow_overlap && a.n > chunk;
@@ -523,7 +533,7 @@ static ar_status f64_plan(ar_context* ctx, const TrackArgs& a, bool u8, bool all
     if (P->nprep > chunk) P->nprep = chunk;
     P->nbuf = P->overlap ? 2 : 1;
     P->half = (size_t)chunk * P->sl.bytes;
-    P->tbytes = mode >= 1 ? 0 : (size_t)P->nbuf * P->nprep * td8;
+    P->tbytes = (size_t)P->nbuf * P->nprep * table1;
     if ((st = ctx->scratch.reserve(ctx, P->nbuf * P->half + P->tbytes))) return st;
     if (P->overlap && !ctx->stream2) {
         HIPCHK(ctx, hipStreamCreate(&ctx->stream2));
@@ -533,8 +543,8 @@ static ar_status f64_plan(ar_context* ctx, const TrackArgs& a, bool u8, bool all
     return AR_OK;
 }
 
-static ar_status f64_enqueue(ar_context* ctx, const TrackArgs& a, const F64Plan& P) {
-    unsigned char* base = (unsigned char*)ctx->scratch.ptr;
+static ar_status f64_enqueue(ar_context* ctx, const TrackArgs& a, const F64Plan& P,
+                             unsigned char* base) {
     F64Stage S;
     S.slot_bytes = P.sl.bytes;
     S.off_chip = P.sl.off_chip;
@@ -815,7 +825,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         F64Plan P;
         if ((st = f64_plan(ctx, a, ctx->pdtype == AR_U8, true, &P))) return st;
         HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        if ((st = f64_enqueue(ctx, a, P))) return st;
+        if ((st = f64_enqueue(ctx, a, P, (unsigned char*)ctx->scratch.ptr))) return st;
     } else {
         HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(AR_WG), a.lds_bytes, ctx->stream, a);
@@ -987,6 +997,139 @@ extern "C" ar_status ar_correlate_point(ar_context* ctx, const void* search, con
     c->ny = sny; c->nx = snx; c->pdtype = sdt; c->all_valid = sall; c->has_pair = had;
     c->ref_gen++;   // the restored pair is not the one the int8 copies were made from
     return st;
+}
+
+// ---------------------------------------------------------------------------------------------
+// ar_correlate_point_noalloc: ar_correlate_point without any device allocation
+// ---------------------------------------------------------------------------------------------
+
+// The device memory of ar_correlate_point_noalloc: the window, the chip's secondary, the point,
+// its outputs, the refinement hand-off and the direct Float64 tracker's staging slot (and its
+// summed-area table when that does not fit in LDS). One array per device, for the life of the
+// process. Host code never uses the name: the device address comes from hipGetSymbolAddress.
+#define AR_NOALLOC_BYTES ((size_t)1 << 20)
+static __device__ __align__(256) unsigned char scratch[AR_NOALLOC_BYTES];
+
+static std::mutex g_noalloc_mu;
+static unsigned char* g_noalloc_ptr[64];   // per device ordinal, nullptr until looked up
+
+// The array's address on the context's device (current), looked up once per device.
+static ar_status noalloc_scratch(ar_context* ctx, unsigned char** out) {
+    if (ctx->device < 0 || ctx->device >= 64)
+        return fail(ctx, AR_ERR_UNSUPPORTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[ctx->device]) {
+        void* ptr = nullptr;
+        HIPCHK(ctx, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(ctx, AR_ERR_DEVICE, "hipGetSymbolAddress returned no address");
+        g_noalloc_ptr[ctx->device] = (unsigned char*)ptr;
+    }
+    *out = g_noalloc_ptr[ctx->device];
+    return AR_OK;
+}
+
+static size_t up256(size_t x) { return (x + 255) / 256 * 256; }
+
+extern "C" ar_status ar_correlate_point_noalloc(ar_context* ctx, const void* search,
+                                                const void* chip, int32_t dtype, int64_t ch,
+                                                int64_t cw, int64_t rx, int64_t ry,
+                                                int64_t upsampling, int32_t measure, double* dx,
+                                                double* dy, float* corr) {
+    if (!ctx) return fail(nullptr, AR_ERR_INVALID_ARGUMENT, "context is NULL");
+    if (!search || !chip || !dx || !dy || !corr)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    if (ch <= 0 || cw <= 0 || (ch & 1) || (cw & 1) || rx <= 0 || ry <= 0)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "chip extents must be even and radii positive");
+    if (dtype != AR_U8 && dtype != AR_F32)
+        return fail(ctx, AR_ERR_UNSUPPORTED, "dtype must be UInt8 or Float32");
+    if (upsampling < 1 || upsampling > 64 || (upsampling & (upsampling - 1)) != 0)
+        return fail(ctx, AR_ERR_UNSUPPORTED,
+                    "upsampling must be a power of two in 1..64 on the device path");
+    if (measure != AR_ZNCC && measure != AR_NCC)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "measure must be ZNCC or NCC");
+    if (ch > 512 || cw > 512 || rx > 256 || ry > 256)
+        return fail(ctx, AR_ERR_UNSUPPORTED, "chip size above 512 or search radius above 256");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    // the window is the reference image and the chip sits at offset (ry, rx) of an equally sized
+    // secondary, as in ar_correlate_point: one point with zero prior reproduces correlate!
+    const int64_t wh = ch + 2 * ry - 1, ww = cw + 2 * rx - 1;
+    const size_t es = dtype == AR_U8 ? 1 : 4;
+    TrackArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.max_ch = (int32_t)ch;
+    a.max_cw = (int32_t)cw;
+    a.max_rx = (int32_t)rx;
+    a.max_ry = (int32_t)ry;
+    a.n = 1;
+    a.up = (int32_t)upsampling;
+    a.measure = measure;
+    a.ny = wh;
+    a.nx = ww;
+    size_t wl = 0;
+    int pitch = 0, threads = 0;
+    if (!f64_shape(a, &wl, &pitch, &threads))
+        return fail(ctx, AR_ERR_UNSUPPORTED, "chip or search window too large for the device path");
+    F64Plan P;
+    f64_plan_shape(a, dtype == AR_U8, &P);
+    P.chunk = 1;
+    P.overlap = false;
+    P.nbuf = 1;
+    P.nprep = 1;
+    P.half = up256(P.sl.bytes);
+    // regions of the array
+    const size_t img = up256((size_t)wh * ww * es);
+    const size_t o_ref = 0, o_sec = o_ref + img, o_pts = o_sec + img;
+    const size_t o_out = o_pts + 256, o_ref_h = o_out + 256, o_slot = o_ref_h + 256;
+    const size_t need = o_slot + P.half + up256(P.tbytes);
+    if (need > AR_NOALLOC_BYTES)
+        return fail(ctx, AR_ERR_UNSUPPORTED,
+                    "chip " + std::to_string(ch) + "x" + std::to_string(cw) + " at radius " +
+                        std::to_string(ry) + "x" + std::to_string(rx) + " needs " +
+                        std::to_string(need) + " bytes of the no-alloc scratch, which holds " +
+                        std::to_string(AR_NOALLOC_BYTES));
+    unsigned char* base = nullptr;
+    ar_status st;
+    if ((st = noalloc_scratch(ctx, &base))) return st;
+    if ((st = lds_allowance(ctx, (const void*)P.fprep))) return st;
+    if ((st = lds_allowance(ctx, (const void*)P.ftrack))) return st;
+    hipStream_t s = ctx->stream;
+    // inputs: the window, then the chip into a zeroed secondary (a fill kernel, not a memset)
+    HIPCHK(ctx, hipMemcpyAsync(base + o_ref, search, (size_t)wh * ww * es, hipMemcpyHostToDevice, s));
+    fill_u8_kernel<<<grid_for(wh * ww * (int64_t)es), 256, 0, s>>>(base + o_sec, 0,
+                                                                   wh * ww * (int64_t)es);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpy2DAsync(base + o_sec + (ry + rx * wh) * es, wh * es, chip, ch * es,
+                                 ch * es, cw, hipMemcpyHostToDevice, s));
+    struct {
+        double x, y, pdx, pdy;
+        int64_t rx, ry, cx, cy;
+    } pt = {(double)(1 + cw / 2 + rx), (double)(1 + ch / 2 + ry), 0.0, 0.0, rx, ry, cw, ch};
+    HIPCHK(ctx, hipMemcpyAsync(base + o_pts, &pt, sizeof(pt), hipMemcpyHostToDevice, s));
+    double* dpts = (double*)(base + o_pts);
+    int64_t* ipts = (int64_t*)(base + o_pts + 32);
+    a.ref = base + o_ref;
+    a.sec = base + o_sec;
+    a.ok = nullptr;
+    a.x = dpts; a.y = dpts + 1; a.dx_prior = dpts + 2; a.dy_prior = dpts + 3;
+    a.radius_x = ipts; a.radius_y = ipts + 1; a.chip_size_x = ipts + 2; a.chip_size_y = ipts + 3;
+    float* dout = (float*)(base + o_out);
+    a.dx = dout; a.dy = dout + 1; a.corr = dout + 2;
+    a.searched = (uint8_t*)(dout + 3);
+    a.rpatch = (float*)(base + o_ref_h);
+    a.rmeta = (int32_t*)(base + o_ref_h + 128);
+    if ((st = f64_enqueue(ctx, a, P, base + o_slot))) return st;
+    if (upsampling > 1) {
+        const size_t rlds = (size_t)cascade_floats((int)upsampling) * 4;
+        hipLaunchKernelGGL(refine_kernel, dim3(1), dim3(64), rlds, s, a);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    float o[3];
+    HIPCHK(ctx, hipMemcpyAsync(o, dout, sizeof(o), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    *dx = o[0];
+    *dy = o[1];
+    *corr = o[2];
+    return AR_OK;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1355,7 +1498,7 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     for (int k = 0; k < 3 && lst == AR_OK; k++)
         if (m[k] > 0) lst = batch_launch(B, a, lists + k * n, m[k], caps[k]);
     if (lst == AR_OK && m[3] > 0) {
-        lst = f64_enqueue(ctx, af, fp);
+        lst = f64_enqueue(ctx, af, fp, (unsigned char*)ctx->scratch.ptr);
         B->last_launches += 2 * (int)((m[3] + fp.chunk - 1) / fp.chunk);
     }
     if (lst == AR_OK && up > 1)

@@ -159,6 +159,18 @@ ar_status ar_correlate_point(ar_context* ctx, const void* search, const void* ch
                              int64_t upsampling, int32_t measure, double* dx, double* dy,
                              float* corr);
 
+/* As ar_correlate_point, allocating no device memory: no counterpart in the reference. For callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The window, chip, staging and outputs live in a library-owned 1 MiB device array
+ * (one per GPU, kept until the process ends); a chip and window too large for it are refused
+ * with AR_ERR_UNSUPPORTED. Every call on one GPU uses that GPU's array: calls that overlap on the
+ * same GPU (from different threads or contexts) race and must be serialized by the caller. Uses
+ * the context's device and stream only; the context's image pair is left untouched. */
+ar_status ar_correlate_point_noalloc(ar_context* ctx, const void* search, const void* chip,
+                                     int32_t dtype, int64_t ch, int64_t cw, int64_t rx,
+                                     int64_t ry, int64_t upsampling, int32_t measure, double* dx,
+                                     double* dy, float* corr);
+
 /* Device timing of the last ar_track / ar_track_device pass, in milliseconds (HIP events on
  * the context's stream: correlation kernel plus refinement kernel), and the correlation path it
  * took: 0 u8 matrix-core path, 1 generic Float64, 2 BF16 / FP16 matrix-core path. */
