@@ -40,7 +40,7 @@ using namespace ar;
 static std::mutex g_err_mu;
 static std::string g_err;   // errors with no context (create)
 
-static const size_t kLdsLimit = 160 * 1024 - 2048;   // leave room for static shared arrays
+static const size_t kLdsLimit = AR_LDS_LIMIT;
 
 static ar_status fail(ar_context* ctx, ar_status st, const std::string& msg) {
     if (ctx) {
@@ -792,6 +792,14 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     a.rpatch = (float*)ctx->refine.ptr;
     a.rmeta = (int32_t*)((char*)ctx->refine.ptr + (size_t)n * 128);
     int per_cu = 0;
+    if (cplx && a.lds_bytes) {
+        // the Coherence tracker's LDS instance, unless its window and chip leave no room on a CU
+        if ((st = kernel_occupancy(ctx, (const void*)kern, AR_WG, a.lds_bytes, &per_cu))) return st;
+        if (per_cu < 1) {
+            kern = track_coherence_kernel<false>;
+            a.lds_bytes = 0;
+        }
+    }
     if (!use_f64) {
         if ((st = kernel_occupancy(ctx, (const void*)kern, AR_WG, a.lds_bytes, &per_cu))) return st;
         if (per_cu < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "tracker kernel does not fit on a CU");

@@ -11,6 +11,10 @@
 
 struct ar_context;
 
+// Dynamic LDS any one kernel of the library may take: a CU's 160 KiB less room for the kernels'
+// static shared arrays. Every dispatcher sizes its layouts against this one figure.
+#define AR_LDS_LIMIT ((size_t)160 * 1024 - 2048)
+
 // A growable device allocation owned by a context.
 struct DevBuf {
     void* ptr = nullptr;

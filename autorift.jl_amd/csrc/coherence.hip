@@ -290,7 +290,7 @@ size_t ar_coherence_scratch(int ch, int cw, int rx, int ry) {
 // beside the kernel's static LDS.
 size_t ar_coherence_lds(int ch, int cw, int rx, int ry) {
     const size_t b = ((size_t)(ch + 2 * ry - 1) * (cw + 2 * rx - 1) + (size_t)ch * cw) * 8;
-    return b <= 160 * 1024 - 1024 ? b : 0;
+    return b <= AR_LDS_LIMIT ? b : 0;
 }
 
 using namespace ar;

@@ -551,6 +551,20 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
     ar_status st;
     if (p.level_measure_count < 0 || p.level_measure_count > 8)
         return mfail(ctx, AR_ERR_INVALID_ARGUMENT, "level_measure_count must be in 0..8");
+    {
+        // _check_measures (src/types.jl:899-906): a tuple longer than the level list would
+        // leave measures that never run
+        int64_t nlevels = 0;
+        for (int64_t cs = p.chip_size_base; cs > 0 && cs <= p.chip_size_max; cs *= 2)
+            if (cs >= p.chip_size_min) nlevels++;
+        if (p.level_measure_count > nlevels)
+            return mfail(ctx, AR_ERR_INVALID_ARGUMENT,
+                         "`similarity` names " + std::to_string(p.level_measure_count) +
+                             " measures but there are only " + std::to_string(nlevels) +
+                             " chip-size levels. Widen the `chip_size_min`/`chip_size_max` range, "
+                             "or name fewer measures — the last one applies to every remaining "
+                             "level.");
+    }
     int level = 0;
     for (int64_t cs = p.chip_size_base; cs <= p.chip_size_max; cs *= 2) {
         if (cs < p.chip_size_min) continue;

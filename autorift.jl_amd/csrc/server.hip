@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -26,6 +27,7 @@
 #include <vector>
 
 #include "ar_internal.h"
+#include "layout.h"
 
 namespace {
 
@@ -100,9 +102,13 @@ struct ar_server {
 
 // Device bytes a context needs for one pair: raw upload and preparation work images, the
 // prepared pair and masks, box images (two int32 planes), the four byte-shifted int8 copies of
-// each prepared image, per-point buffers and the multi-chip grid planes, plus a fixed margin.
-// Pessimistic on purpose: the estimate only decides admission.
-static size_t pair_footprint(int64_t ny, int64_t nx, int32_t dtype, int64_t npts) {
+// each prepared image, per-point buffers and the multi-chip grid planes, the tracker scratch of the
+// largest chip-size level the pair can need (the direct Float64 tracker's staging for Float32
+// pairs with exact arithmetic or chip areas that are not a power of two, the generic tracker's
+// per-workgroup regions for a rotation search, the Coherence tracker's for complex pairs), plus a
+// fixed margin.
+static size_t pair_footprint(const ar_params& p, int64_t ny, int64_t nx, int32_t dtype,
+                             int64_t npts) {
     const size_t N = (size_t)ny * (size_t)nx;
     const size_t es = elem_size(dtype);
     const size_t raw = 2 * N * es + 2 * N;          // raw pair and masks
@@ -112,7 +118,25 @@ static size_t pair_footprint(int64_t ny, int64_t nx, int32_t dtype, int64_t npts
     const size_t box = 2 * N * 8;                   // box images (Float64 for the 16-bit path)
     const size_t copies = 8 * N + 4096;             // int8 copies of the UInt8 pair
     const size_t pts = (size_t)npts * 512;          // point fields, outputs, hand-off, grid planes
-    return raw + work + prep + box + copies + pts + ((size_t)64 << 20);
+    // the largest level: chip_size_max (its rows by the aspect), the largest radius
+    const int cx = (int)(p.chip_size_max < 512 ? p.chip_size_max : 512);
+    const int cy = 2 * (int)std::llround((double)cx * p.chip_aspect / 2.0);
+    const int r = (int)(p.search_radius_x > p.search_radius_y ? p.search_radius_x
+                                                              : p.search_radius_y);
+    const int rr = r < 256 ? r : 256;
+    const bool pow2 = cx > 0 && cy > 0 && ((int64_t)cx * cy & ((int64_t)cx * cy - 1)) == 0;
+    size_t tracker = 0;
+    if (dtype == AR_C64) {
+        tracker = (size_t)2048 * ar_coherence_scratch(cy, cx, rr, rr);
+    } else if (p.rotation_count > 0) {
+        tracker = (size_t)2048 * ar::gen_scratch(cy, cx, rr, rr).total;
+    } else if ((p.quantize == AR_QUANT_NONE && p.precision == AR_PREC_EXACT) || !pow2) {
+        const size_t slot = ar::f64_slot(cy, cx, rr, rr).bytes;
+        const size_t half = (size_t)1 << 30;        // half of the tracker's staging
+        const size_t want = (size_t)npts * slot;
+        tracker = want <= half ? want : 2 * half;
+    }
+    return raw + work + prep + box + copies + pts + tracker + ((size_t)64 << 20);
 }
 
 // Host copy split over up to 8 threads for large buffers: one thread's memcpy reaches only a
@@ -147,7 +171,7 @@ static void worker(ar_server* S, int s) {
                 size_t others = 0;
                 for (size_t t = 0; t < S->held.size(); t++)
                     if ((int)t != s) others += S->held[t];
-                const size_t need = h->footprint > S->held[s] ? h->footprint : S->held[s];
+                size_t need = h->footprint > S->held[s] ? h->footprint : S->held[s];
                 if (others + need > (size_t)S->budget) {
                     // idle slots give their buffers back, then check again
                     for (size_t t = 0; t < S->held.size(); t++)
@@ -158,6 +182,13 @@ static void worker(ar_server* S, int s) {
                     others = 0;
                     for (size_t t = 0; t < S->held.size(); t++)
                         if ((int)t != s) others += S->held[t];
+                    // this slot's own buffers from a larger earlier pair may be what is in the
+                    // way: it gives them back too and asks for the pair's footprint alone
+                    if (others + need > (size_t)S->budget && S->held[s] > h->footprint) {
+                        ar_context_release(ctx);
+                        S->held[s] = 0;
+                        need = h->footprint;
+                    }
                 }
                 if (others + need <= (size_t)S->budget) {
                     r = h;
@@ -295,7 +326,7 @@ ar_status ar_server_submit(ar_server* S, const void* ref, const void* sec, const
     r->dtype = dtype;
     const size_t N = (size_t)ny * nx;
     const int64_t n = nr * nc;
-    r->footprint = pair_footprint(ny, nx, dtype, n);
+    r->footprint = pair_footprint(S->params, ny, nx, dtype, n);
     {
         std::lock_guard<std::mutex> lk(S->mu);
         if (r->footprint > (size_t)S->budget) {

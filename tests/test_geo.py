@@ -1,6 +1,6 @@
 """Map output conventions (ext/AutoRIFTRastersExt.jl, ext/AutoRIFTDimensionalDataExt.jl): signs,
 north-up / south-up storage, dt spellings, grid coordinates. CPU only: results are built by hand.
-The end-to-end run on the GPU is in test_gpu_geo.py."""
+The end-to-end run on the GPU (velocities from a tracked pair) is in test_gpu_app.py."""
 import datetime
 
 import numpy as np

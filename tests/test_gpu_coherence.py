@@ -32,10 +32,13 @@ def _ctx(**kw):
     return autorift.Context(autorift.params(**kw))
 
 
-@pytest.mark.parametrize("cs,r,up", [(16, 6, 1), (32, 12, 16), (24, 9, 64)])
+@pytest.mark.parametrize("cs,r,up", [(16, 6, 1), (32, 12, 16), (24, 9, 64), (32, 54, 16),
+                                     (64, 64, 4)])
 def test_track_matches_oracle(cs, r, up):
     """Interior, edge (virtual zero padding) and masked points; a flat chip is degenerate."""
-    n = 180
+    # (32, 54): window and chip within 1 KiB of the LDS limit -- the global-scratch instance
+    # when the LDS one cannot be placed; (64, 64): a window far beyond LDS
+    n = 180 if r < 50 else 360
     z1, z2 = _pair(n)
     z2[70:110, 70:110] = 3 - 2j          # flat: searched, no measurement
     m = np.ones(z1.shape, bool)
@@ -49,15 +52,16 @@ def test_track_matches_oracle(cs, r, up):
         pr, ps, pm, _ = ctx.get_pair()
     np.testing.assert_array_equal(pr, z1)
     exp = oracle.track(pr, ps, m, pts.x, pts.y, pts.radius_x, pts.radius_y, pts.chip_size_x,
-                       pts.chip_size_y, up=up, measure=oracle.COHERENCE)
+                       pts.chip_size_y, up=up, measure=oracle.COHERENCE, nthreads=16)
     np.testing.assert_array_equal(got.searched, exp[3])
     for a, b in zip((got.dx, got.dy, got.correlation), exp[:3]):
         np.testing.assert_array_equal(np.isnan(a), np.isnan(b))
         np.testing.assert_array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
     fin = np.isfinite(got.dx)
     assert fin.sum() >= 10
-    centre = (pts.x == 90.0) & (pts.y == 90.0)
-    assert got.searched[centre].all() and np.isnan(got.dx[centre]).all()
+    if cs <= 40:   # the chip at (90, 90) lies inside the flat patch: degenerate
+        centre = (pts.x == 90.0) & (pts.y == 90.0)
+        assert got.searched[centre].all() and np.isnan(got.dx[centre]).all()
 
 
 def test_deramp_matches_oracle():

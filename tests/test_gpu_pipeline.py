@@ -132,3 +132,19 @@ def test_multichip_matches_oracle(case):
     np.testing.assert_array_equal(g.chip_size, e[3])
     np.testing.assert_array_equal(g.interpolated, e[4])
     assert g.nmeasured() > 0.5 * g.dx.size
+
+
+def test_multichip_refuses_more_measures_than_levels():
+    """_check_measures (src/types.jl:899-906) on the device path: a caller of the C ABI whose
+    similarity tuple is longer than the chip-size levels gets the reference's ArgumentError."""
+    p = _params(chip_size_base=32, chip_size_min=32, chip_size_max=64, preprocess=0, quantize=0)
+    p.level_measure_count = 3
+    for k in range(3):
+        p.level_measures[k] = 0
+    ref, sec = shifted_pair(200, (1, 1), seed=3, dtype=np.uint8)
+    xs = np.arange(80.0, 121.0, 20.0)
+    grid = autorift.gridpoints_xy(xs, xs, chip_size=64, search_radius=10)
+    with autorift.Context(p) as ctx:
+        ctx.set_pair(ref, sec)
+        with pytest.raises(autorift.ArgumentError, match="names 3 measures but there are only 2"):
+            ctx.run_multichip(grid)

@@ -107,3 +107,20 @@ def test_served_pairs_with_masks():
         for t, e in zip(tickets, alone):
             r, _ = srv.result(t)
             _same(r, e)
+
+
+def test_one_slot_gives_back_its_own_buffers():
+    """One slot whose buffers from a larger pair exceed a small budget: it releases them and runs
+    the next, smaller pair instead of waiting on itself (a pytest timeout would catch a hang)."""
+    p = autorift.params()
+    big, small = cfg1_pair(300, n=1400), cfg1_pair(301, n=384)
+    with autorift.Context(p) as ctx:
+        ctx.set_images(*small)
+        exp = ctx.run_multichip(autorift.api._build_grid(small[0].shape, p))
+    print("serve one slot, own buffers", flush=True)
+    with autorift.Server(p, slots=1, hbm_budget=200 << 20) as srv:
+        r0, _ = srv.result(srv.submit(*big))
+        r1, _ = srv.result(srv.submit(*small))
+        assert srv.stats()["completed"] == 2
+    assert r0.nmeasured() > 0
+    _same(r1, exp)
