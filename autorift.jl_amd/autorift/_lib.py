@@ -107,6 +107,10 @@ SIGNATURES = [
                                         C.c_void_p, C.c_void_p, C.c_int32]),
     ("ar_batch_info", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     ("ar_batch_last_error", C.c_char_p, [C.c_void_p]),
+    ("ar_batch_set_exact", C.c_int, [C.c_void_p, C.c_int32]),
+    ("ar_tune_cache_path", C.c_int64, [C.c_char_p, C.c_char_p, C.c_int64]),
+    ("ar_tune_cache_lookup", C.c_int32, [C.c_char_p, C.c_char_p, C.POINTER(C.c_int32)]),
+    ("ar_tune_cache_record", C.c_int32, [C.c_char_p, C.c_char_p, C.c_int32]),
     ("ar_server_create", C.c_int, [C.POINTER(Params), C.c_int32, C.c_int32, C.c_int64,
                                    C.POINTER(C.c_void_p)]),
     ("ar_server_destroy", None, [C.c_void_p]),

@@ -16,15 +16,27 @@ from .core import _DTYPES, DisplacementField, _fortran, _points_struct, _ptr
 
 
 class Batch:
-    """Resident UInt8-prepared pairs on `device` (the parameters must quantise to UInt8)."""
+    """Resident UInt8-prepared pairs on `device` (the parameters must quantise to UInt8).
 
-    def __init__(self, p: Params | None = None, device: int = 0):
+    Chips whose area is a power of two give results bit-equal to a pass over their pair alone.
+    Other chip areas take the integer matrix-core tracker with the unrounded mean-removed chip
+    (integer peaks agree with the exact pass to the batched form's tolerance), or, with
+    `exact=True`, the direct Float64 tracker, bit-equal as well."""
+
+    def __init__(self, p: Params | None = None, device: int = 0, exact: bool = False):
         self.params = p if p is not None else _params()
         self._h = C.c_void_p()
         st = lib().ar_batch_create(C.byref(self.params), int(device), C.byref(self._h))
         if st != _lib.AR_OK:
             _lib.check(st, None)
         self.shapes = []
+        self.set_exact(exact)
+
+    def set_exact(self, exact: bool):
+        """Chips whose area is not a power of two: the direct Float64 tracker (True) or the
+        integer tracker with the unrounded mean-removed chip (False)."""
+        self.exact = bool(exact)
+        self._check(lib().ar_batch_set_exact(self._h, int(self.exact)))
 
     def close(self):
         if self._h:
@@ -122,3 +134,21 @@ class Batch:
         lib().ar_batch_info(self._h, C.byref(ms), v)
         return {"ms": ms.value, "pairs": v[0], "split": (v[1] & 0xff, v[1] >> 8),
                 "launches": v[2], "replayed": bool(v[3])}
+
+
+def tune_cache_path(device: str) -> str | None:
+    """The tuning cache file for a device name such as "gfx950-256cu" (None when caching is off
+    or no cache directory is usable)."""
+    buf = C.create_string_buffer(4096)
+    n = lib().ar_tune_cache_path(device.encode(), buf, len(buf))
+    return buf.value.decode() if n > 0 else None
+
+
+def tune_cache_lookup(device: str, key: str) -> int | None:
+    v = C.c_int32()
+    return v.value if lib().ar_tune_cache_lookup(device.encode(), key.encode(), C.byref(v)) else None
+
+
+def tune_cache_record(device: str, key: str, value: int) -> bool:
+    """Records key = value; False when the write was dropped (it never raises)."""
+    return bool(lib().ar_tune_cache_record(device.encode(), key.encode(), int(value)))

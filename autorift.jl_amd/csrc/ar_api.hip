@@ -1163,10 +1163,10 @@ struct BatchPair {
 struct GraphKey {
     const void* p[11];
     int64_t n, up;
-    int32_t measure, split;
+    int32_t measure, split, exact;
     bool operator==(const GraphKey& o) const {
         return std::memcmp(p, o.p, sizeof(p)) == 0 && n == o.n && up == o.up &&
-               measure == o.measure && split == o.split;
+               measure == o.measure && split == o.split && exact == o.exact;
     }
 };
 
@@ -1177,8 +1177,12 @@ struct ar_batch {
     std::vector<BatchPair*> pairs;
     DevBuf desc, lists, counts, pidx;   // descriptor table, class lists and counts, pair indices
     bool desc_dirty = true;
-    // radius split chosen by the tuner, per pass shape (max ch, cw, rx, ry, points / 4096)
-    std::map<std::tuple<int, int, int, int, int64_t>, int> tuned;
+    // chips whose area is not a power of two go to the direct Float64 tracker
+    bool exact = false;
+    // radius split chosen by the tuner, per pass shape (max ch, cw, rx, ry, points / 4096,
+    // exact); also recorded in the tuning cache under the device's name
+    std::map<std::tuple<int, int, int, int, int64_t, int>, int> tuned;
+    std::string tune_device;            // "<arch>-<CUs>cu"
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     GraphKey gkey;
@@ -1203,11 +1207,12 @@ struct ar_batch {
 };
 
 // Radius classes of a batched pass: class 0 max(rx, ry) <= R1, class 1 <= R2 (R2 > R1, or 0:
-// no such class), class 2 the rest, and class 3 every searchable point whose chip area is not a
-// power of two (the direct Float64 tracker takes those); each class list in any order. One atomic
+// no such class), class 2 the rest, and, with `exact`, class 3 every searchable point whose chip
+// area is not a power of two (the direct Float64 tracker takes those); each class list in any
+// order. One atomic
 // per wave and class (ballot, then each lane's rank among the wave's lanes of its class).
 __global__ __launch_bounds__(256) void batch_classify_kernel(ar_points pts, int64_t n, int R1,
-                                                             int R2, int32_t* lists,
+                                                             int R2, int exact, int32_t* lists,
                                                              uint32_t* counts) {
     const int lane = threadIdx.x & 63;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -1220,7 +1225,8 @@ __global__ __launch_bounds__(256) void batch_classify_kernel(ar_points pts, int6
             c = r <= R1 ? 0 : ((R2 > 0 && r <= R2) ? 1 : 2);
             // a searchable chip whose area is not a power of two: the direct Float64 tracker
             const int64_t area = (2 * (pts.chip_size_y[i] / 2)) * (2 * (pts.chip_size_x[i] / 2));
-            if (pts.radius_x[i] > 0 && pts.radius_y[i] > 0 && area > 0 && (area & (area - 1)) != 0)
+            if (exact && pts.radius_x[i] > 0 && pts.radius_y[i] > 0 && area > 0 &&
+                (area & (area - 1)) != 0)
                 c = 3;
         }
 #pragma unroll
@@ -1303,6 +1309,9 @@ static ar_status batch_launch(ar_batch* B, TrackArgs a, const int32_t* list, int
     return AR_OK;
 }
 
+// A radius split read back from the tuning cache: R1 | R2 << 8 with both in 0..255.
+static bool batch_split_ok(int32_t s) { return s >= 0 && s < (1 << 16); }
+
 // The batched pass on device-resident points, pair indices and outputs. split: radius threshold
 // of the small class (0: one launch over every point), or -1: the tuner's choice.
 static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpair, int64_t n,
@@ -1330,6 +1339,7 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     key.up = up;
     key.measure = measure;
     key.split = split;
+    key.exact = B->exact;
     if (replay && B->has_graph && B->gkey == key && B->graph_bufs_same()) {
         // the captured pass over the same buffers (contents may differ only in values)
         HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -1405,10 +1415,18 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     // the tuner: time each candidate split once per pass shape, keep the fastest
     const int rmax = (int)(rx > ry ? rx : ry);
     if (split < 0) {
-        auto tk = std::make_tuple(a.max_ch, a.max_cw, a.max_rx, a.max_ry, n >> 12);
+        auto tk = std::make_tuple(a.max_ch, a.max_cw, a.max_rx, a.max_ry, n >> 12, (int)B->exact);
         auto it = B->tuned.find(tk);
+        char ckey[96];
+        std::snprintf(ckey, sizeof ckey, "batch c%dx%d r%dx%d n%lld e%d", a.max_ch, a.max_cw,
+                      a.max_rx, a.max_ry, (long long)(n >> 12), (int)B->exact);
+        int32_t cached = 0;
         if (it != B->tuned.end()) {
             split = it->second;
+        } else if (ar_tune_cache_lookup(B->tune_device.c_str(), ckey, &cached) &&
+                   batch_split_ok(cached)) {
+            B->tuned[tk] = cached;
+            split = cached;
         } else {
             int best = 0;
             double best_ms = 1e30;
@@ -1426,6 +1444,7 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
             }
             B->tuned[tk] = best;
             split = best;
+            ar_tune_cache_record(B->tune_device.c_str(), ckey, best);   // dropped if it fails
         }
         key.split = split;
     }
@@ -1457,10 +1476,11 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     }
     // class sizes (the class lists are rebuilt inside the pass, so a replay needs the same points)
     int64_t m[4] = {n, 0, 0, 0};
-    const bool classify = R1 > 0 || !pow2;
+    const bool classify = R1 > 0 || (!pow2 && B->exact);
     if (classify) {
         batch_zero_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 4);
-        batch_classify_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, R1, R2, lists, cnt);
+        batch_classify_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, R1, R2, B->exact,
+                                                                    lists, cnt);
         HIPCHK(ctx, hipGetLastError());
         uint32_t c4[4] = {0, 0, 0, 0};
         HIPCHK(ctx, hipMemcpyAsync(c4, cnt, 16, hipMemcpyDeviceToHost, ctx->stream));
@@ -1496,7 +1516,8 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     point_prep_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(a);
     if (classify) {
         batch_zero_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 4);
-        batch_classify_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, R1, R2, lists, cnt);
+        batch_classify_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(dp, n, R1, R2, B->exact,
+                                                                    lists, cnt);
     } else {
         batch_iota_kernel<<<grid_for(n), 256, 0, ctx->stream>>>(lists, n);
     }
@@ -1566,7 +1587,14 @@ ar_status ar_batch_create(const ar_params* p, int32_t device, ar_batch** out) {
     if (st) return st;
     ar_batch* B = new ar_batch();
     B->ctx = c;
+    B->tune_device = "gfx950-" + std::to_string(c->ncu) + "cu";
     *out = B;
+    return AR_OK;
+}
+
+ar_status ar_batch_set_exact(ar_batch* B, int32_t exact) {
+    if (!B) return fail(nullptr, AR_ERR_INVALID_ARGUMENT, "batch is NULL");
+    B->exact = exact != 0;
     return AR_OK;
 }
 

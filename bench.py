@@ -112,15 +112,16 @@ def pair_latency(autorift, npairs, warm=2):
             "expected_dx_dy": [-3.30, 1.70]}
 
 
-def serve_pool(count, seed=6, lo=512, hi=10000):
-    """Pairs for the serving sweep: side log-uniform in [lo, hi]; blurred noise (sigma 1.5) scaled
-    to 0..255, the secondary translated by a random subpixel shift (|d| <= 15 px, bilinear)
-    plus N(0, 2^2) noise; Float32."""
+def serve_pool(count, seed=6, lo=512, hi=10000, sides=()):
+    """Pairs for the serving sweep: `count` sides log-uniform in [lo, hi], then the given `sides`;
+    blurred noise (sigma 1.5) scaled to 0..255, the secondary translated by a random subpixel
+    shift (|d| <= 15 px, bilinear) plus N(0, 2^2) noise; Float32."""
     from scipy import ndimage
     rng = np.random.default_rng(seed)
     out = []
-    for _ in range(count):
-        n = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+    todo = [None] * count + list(sides)
+    for side in todo:
+        n = side or int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
         a = ndimage.gaussian_filter(rng.standard_normal((n, n), dtype=np.float32), 1.5,
                                     truncate=4.0, mode="mirror")
         a -= a.min()
@@ -133,12 +134,14 @@ def serve_pool(count, seed=6, lo=512, hi=10000):
     return out
 
 
-def serving_sweep(autorift, rates, nreq, slots, pool_size, seed=6):
-    """Poisson arrivals of pairs from the pool at each rate into one Server; per rate: p50 / p99
-    latency (submission to result), pairs/s, and the server's peak concurrency."""
+def serving_sweep(autorift, rates, nreq_max, slots, pool_size, seed=6):
+    """Poisson arrivals of pairs from the pool (pool_size random sides and one 10000^2 pair) at
+    each rate into one Server; 3 s of arrivals per rate, between 10 and nreq_max requests; per
+    rate: p50 / p99 latency (submission to result), pairs/s, and the server's peak
+    concurrency."""
     import threading
     t0 = time.time()
-    pool = serve_pool(pool_size, seed)
+    pool = serve_pool(pool_size, seed, sides=(10000,))
     log(f"serving pool: sides {[a.shape[0] for a, _, _ in pool]} in {time.time() - t0:.1f}s")
     p = autorift.params()
     rng = np.random.default_rng(seed + 1)
@@ -147,6 +150,7 @@ def serving_sweep(autorift, rates, nreq, slots, pool_size, seed=6):
         for a, b, _ in pool[:2]:          # warm the slots' buffers and kernels
             srv.result(srv.submit(a, b))
         for rate in rates:
+            nreq = int(min(nreq_max, max(10, round(3 * rate))))
             order = rng.integers(0, len(pool), nreq)
             gaps = rng.exponential(1.0 / rate, nreq)
             tickets = [None] * nreq
@@ -282,29 +286,53 @@ def coherence_block(autorift, steps, n=2048):
     return out
 
 
+FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 vector (and matrix), TFLOP/s
+BF16_PEAK_TFLOPS = 2500.0        # MI355X dense BF16 / FP16 matrix, TFLOP/s
+
+
 def mixed_block(autorift, tex, sec, pts, steps):
-    """16-bit contraction on the Float32 pair (quantize = :none): throughput of the BF16 and FP16
-    matrix-core tracker over the full grid, the exact Float64 path on a sample of it, and the
-    agreement on the 512^2 shifted pair."""
+    """The Float32 pair (quantize = :none) over the full grid: the exact direct Float64 tracker
+    (the plain Float32 drop-in, bit-exact) and the BF16 / FP16 matrix-core tracker, each with the
+    roofline of its arithmetic; the one-kernel generic Float64 tracker on every 7th grid row and
+    column for comparison; agreement on the 512^2 shifted pair."""
+    import os as _os
     out = {}
-    sample = pts[::7, ::7]           # every 7th grid row and column
-    for prec in ("bf16", "fp16", "exact"):
-        ctx = autorift.Context(autorift.params(quantize="none", precision=prec))
+    flop = FLOP_PER_POINT
+    for prec in ("bf16", "fp16", "exact", "generic"):
+        ctx = autorift.Context(autorift.params(quantize="none",
+                                               precision="exact" if prec == "generic" else prec))
         ctx.set_images(tex, sec)
-        use = pts if prec != "exact" else sample
-        ctx.track(use, upsampling=64)                 # warm-up (box images, kernels)
-        t = []
-        for _ in range(steps if prec != "exact" else 1):
-            r = ctx.track(use, upsampling=64)
-            box_ms = ctx.last_timing()[0]          # the pass's Float64 box images (mixed path)
-            t.append(ctx.last_kernel()[0] + box_ms)
-        ms = float(np.median(t))
-        out[prec] = {"points": len(use), "ms": ms,
-                     "points_per_s": int(np.count_nonzero(r.searched)) / (ms / 1000),
-                     "path": ctx.last_kernel()[1]}
+        use = pts if prec != "generic" else pts[::7, ::7]
+        if prec == "generic":
+            _os.environ["AR_F64_DIRECT"] = "never"
+        try:
+            ctx.track(use, upsampling=64)                 # warm-up (box images, kernels)
+            t, tt = [], []
+            for _ in range(steps if prec != "generic" else 1):
+                r = ctx.track(use, upsampling=64)
+                box_ms, track_ms, _ = ctx.last_timing()   # box: Float64 box images (mixed path)
+                t.append(ctx.last_kernel()[0] + box_ms)
+                tt.append(track_ms)
+        finally:
+            _os.environ.pop("AR_F64_DIRECT", None)
+        ms, ms_t = float(np.median(t)), float(np.median(tt))
+        n_s = int(np.count_nonzero(r.searched))
+        out[prec] = {"points": len(use), "ms": ms, "track_ms": ms_t,
+                     "points_per_s": n_s / (ms / 1000), "path": ctx.last_kernel()[1]}
+        peak = FP64_PEAK_TFLOPS if prec in ("exact", "generic") else BF16_PEAK_TFLOPS
+        ach = n_s * flop / (ms_t / 1000) / 1e12
+        # the tracker phase: HIP events on the pass's stream around the correlation launches
+        # (for the direct Float64 tracker: the first chunk's preparation, then every
+        # track_f64_kernel launch; later chunks' preparation overlaps on a second stream)
+        out[prec]["roofline"] = {"bound": "fp64" if peak == FP64_PEAK_TFLOPS else "mfma",
+                                 "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                                 "frac": ach / peak, "scope": "tracker phase"}
         ctx.close()
-        log(f"mixed {prec}: {out[prec]['points_per_s']:.0f} points/s over {len(use)} points")
+        log(f"mixed {prec}: {out[prec]['points_per_s']:.0f} points/s over {len(use)} points "
+            f"({ach:.1f} TFLOP/s)")
     out["speedup_bf16_vs_exact"] = out["bf16"]["points_per_s"] / out["exact"]["points_per_s"]
+    out["speedup_exact_vs_generic"] = (out["exact"]["points_per_s"] /
+                                       out["generic"]["points_per_s"])
     # agreement on the 512^2 pair shifted by (3.30, -1.70)
     a, b = cfg1_pair(1)
     xs = np.arange(40.0, 473.0, 16.0)
@@ -427,7 +455,32 @@ def batched_block(autorift, steps, npairs=256, npoints=1_000_000, seed=5):
     ey = tr[:, 1] + lin[:, 1, 0] * (x - 1 - cxx) + lin[:, 1, 1] * (y - 1 - cyy)
     err = np.hypot(dx[ok] - ex[ok], dy[ok] - ey[ok])
     flop = 2.0 * (2 * r) * (2 * r) * cs.astype(np.float64) ** 2
+    # chips of area 2304 (48 x 48): the default pass correlates the unrounded mean-removed chip;
+    # the exact pass (direct Float64 tracker, bit-equal to the oracle) is timed and compared
+    def grab():
+        torch.cuda.synchronize()
+        return out[0].cpu().numpy(), out[1].cpu().numpy()
+    bt.track_device(fp, pr.data_ptr(), npoints, 1, 0, op, split=split)
+    d1 = grab()
+    bt.set_exact(True)
+    bt.track_device(fp, pr.data_ptr(), npoints, 1, 0, op, split=split)
+    e1 = grab()
+    bt.track_device(fp, pr.data_ptr(), npoints, 64, 0, op, split=split)
+    exact_ms = bt.info()["ms"]
+    e64 = grab()
     bt.close()
+    m48 = (cs == 48) & np.isfinite(d1[0]) & np.isfinite(e1[0])
+    peaks = float(np.mean((d1[0][m48] == e1[0][m48]) & (d1[1][m48] == e1[1][m48])))
+    b48 = (cs == 48) & ok & np.isfinite(e64[0])
+    off = np.hypot(dx[b48] - e64[0][b48], dy[b48] - e64[1][b48])
+    chip48 = {"points": int(m48.sum()), "integer_peak_agreement": peaks,
+              "subpixel_within_0_05px": float(np.mean(off <= 0.05)),
+              "subpixel_offset_p999_px": float(np.quantile(off, 0.999)),
+              "subpixel_offset_max_px": float(off.max()),
+              "exact_pass_ms": exact_ms,
+              "exact_points_per_s": searched / (exact_ms / 1000)}
+    log(f"batched chip 48: integer peaks equal {peaks:.5f} over {m48.sum()} points, subpixel "
+        f"within 0.05 px {chip48['subpixel_within_0_05px']:.5f}; exact pass {exact_ms:.1f} ms")
     ms_med = float(np.median(ms))
     res = {"pairs": npairs, "points": npoints, "searched": searched,
            "measured": int(ok.sum()), "ms_per_pass": ms_med, "wall_ms_per_pass": wall * 1000,
@@ -436,7 +489,7 @@ def batched_block(autorift, steps, npairs=256, npoints=1_000_000, seed=5):
            "radius_classes": list(split), "launches": info["launches"],
            "graph_replayed": info["replayed"],
            "median_error_px": float(np.median(err)), "p95_error_px": float(np.percentile(err, 95)),
-           "generate_s": gen_s, "tune_s": tune_s}
+           "generate_s": gen_s, "tune_s": tune_s, "chip48_vs_exact": chip48}
     log(f"batched: {npairs} pairs, {npoints} points, {ms_med:.1f} ms/pass "
         f"({res['points_per_s'] / 1e6:.1f} M points/s, {res['alg_tflop_per_s']:.0f} TFLOP/s "
         f"algorithmic), split {split}, launches {info['launches']}, median error "
@@ -467,6 +520,81 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": nbytes}), flush=True)
 
 
+REF_FINE_POINTS_PER_S = 5060.0   # 900 points / 177.86 ms (min), one Apple M2 Max thread, Julia
+                                 # 1.12: "track/fine c32 r25 1024x1024" in the reference's
+                                 # benchmark/results/baseline.json:35-38
+REF_THREADED_POINTS_PER_S = 36320.0   # 900 points / 24.78 ms (min) on 8 M2 Max threads:
+                                      # "track/threaded c32 r25 1024x1024", baseline.json:197-199
+
+
+def host_cpus():
+    """(threads this process may use, nproc, CPU model): the affinity mask, capped by the cgroup
+    CPU quota when one is set."""
+    import subprocess
+    aff = len(os.sched_getaffinity(0))
+    use = aff
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if quota != "max":
+            use = max(1, min(aff, int(float(quota) / float(period))))
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        for ln in subprocess.run(["lscpu"], capture_output=True, text=True,
+                                 timeout=10).stdout.splitlines():
+            if ln.startswith("Model name:"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return use, os.cpu_count(), model
+
+
+def cpu_baseline(ref8, sec8, pts, n, npts, sample):
+    """The oracle's tracker (FFT numerator, as the reference uses at chip 32 r 25) on the same
+    UInt8 pair: serial on one thread and threaded on every usable core, each the minimum of 5
+    timed runs after a warm-up, on evenly spaced samples of the grid sized for about 2 s a run."""
+    import oracle
+    threads, nproc, model = host_cpus()
+    ok = np.ones((n, n), bool)
+    names = ("x", "y", "radius_x", "radius_y", "chip_size_x", "chip_size_y")
+    full = {f: np.ravel(getattr(pts, f), order="F") for f in names}
+
+    def run(k, nth, reps=5):
+        sel = np.linspace(0, npts - 1, k).astype(np.int64)
+        f = {key: v[sel] for key, v in full.items()}
+        t = []
+        for _ in range(reps + 1):                  # the first run is the warm-up
+            tc = time.perf_counter()
+            oracle.track(ref8, sec8, ok, f["x"], f["y"], f["radius_x"], f["radius_y"],
+                         f["chip_size_x"], f["chip_size_y"], up=64, num_mode=oracle.NUM_FFT,
+                         nthreads=nth)
+            t.append(time.perf_counter() - tc)
+        return k / min(t[1:]), sum(t)
+
+    k1 = min(sample or 1000, npts)
+    serial, s1 = run(k1, 1)
+    kt = min(sample or int(min(npts, max(1000, serial * threads * 2))), npts)
+    threaded, s2 = run(kt, threads)
+    cpu = {"value": threaded, "unit": "points/s", "cores": threads, "kind": "port",
+           "sample": f"{kt} of the {npts} grid points (threaded) and {k1} (serial), evenly "
+                     f"spaced; same UInt8 images; FFT numerator; min of 5 runs after a warm-up "
+                     f"({s1 + s2:.1f} s in all)",
+           "serial_points_per_s": serial, "threaded_points_per_s": threaded,
+           "per_thread_points_per_s": threaded / threads, "nproc": nproc, "cpu_model": model,
+           "reference_serial_points_per_s": REF_FINE_POINTS_PER_S,
+           "reference_threaded_points_per_s_8_threads": REF_THREADED_POINTS_PER_S,
+           "reference_source": "benchmark/results/baseline.json:35,197 (Apple M2 Max)",
+           "reference_projected_points_per_s": REF_FINE_POINTS_PER_S * threads,
+           "projection": "the reference's one-thread rate times the cores used here (ideal "
+                         "scaling; an estimate of the reference on this host, not a run)"}
+    log(f"cpu baseline: serial {serial:.0f} points/s, {threads} threads {threaded:.0f} points/s "
+        f"({threaded / threads:.0f} per thread; the reference's fine pass: "
+        f"{REF_FINE_POINTS_PER_S:.0f} points/s on one M2 thread); {model}, nproc {nproc}")
+    return cpu
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -475,15 +603,18 @@ def main():
     ap.add_argument("--size", type=int, default=10000)
     ap.add_argument("--spacing", type=int, default=24)
     ap.add_argument("--cpu-sample", type=int, default=0,
-                    help="points in the CPU baseline sample (0: every grid point, ~20 s)")
+                    help="points per CPU baseline run (0: ~1000 serial, ~2 s threaded)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-pairs", type=int, default=20)
-    ap.add_argument("--serve-rates", default="2,10,50",
+    ap.add_argument("--serve-rates", default="1,2,5,10,20,50,100,200",
                     help="Poisson arrival rates (pairs/s) of the serving sweep; empty: skip")
-    ap.add_argument("--serve-requests", type=int, default=30)
+    ap.add_argument("--serve-requests", type=int, default=60,
+                    help="most requests per rate (3 s of arrivals, at least 10)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-pool", type=int, default=6)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-served-compare", action="store_true",
+                    help="skip the served BF16 against plain Float32 comparison")
     ap.add_argument("--no-coherence", action="store_true")
     ap.add_argument("--batch-pairs", type=int, default=256,
                     help="pairs of the batched workload (0: skip it)")
@@ -568,7 +699,7 @@ def main():
         batched = batched_block(autorift, min(args.steps, 5), args.batch_pairs, args.batch_points)
 
     served_vs_plain = None
-    if not args.no_mixed:
+    if not args.no_mixed and not args.no_served_compare:
         served_vs_plain = served_mixed_vs_plain(autorift)
 
     serving = None
@@ -586,23 +717,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        import oracle
-        threads = min(16, os.cpu_count() or 1)
-        k = min(args.cpu_sample or npts, npts)
-        sel = np.linspace(0, npts - 1, k).astype(np.int64)
-        flat = {f: np.ravel(getattr(pts, f), order="F")[sel]
-                for f in ("x", "y", "radius_x", "radius_y", "dx_prior", "dy_prior",
-                          "chip_size_x", "chip_size_y")}
-        ok = np.ones((n, n), bool)
-        tc = time.perf_counter()
-        oracle.track(ref8, sec8, ok, flat["x"], flat["y"], flat["radius_x"], flat["radius_y"],
-                     flat["chip_size_x"], flat["chip_size_y"], up=64,
-                     num_mode=oracle.NUM_FFT, nthreads=threads)
-        el = time.perf_counter() - tc
-        cpu = {"value": k / el, "unit": "points/s", "cores": threads, "kind": "port",
-               "sample": f"{k} of the {npts} grid points (evenly spaced), same images, FFT "
-                         f"numerator, OpenMP dynamic schedule, {el:.1f} s"}
-        log(f"cpu baseline {k / el:.0f} points/s on {threads} threads")
+        cpu = cpu_baseline(ref8, sec8, pts, n, npts, args.cpu_sample)
 
     # the counter summary covers the default workload only
     traffic = profiled_traffic() if (n, args.spacing) == (10000, 24) else None

@@ -194,9 +194,10 @@ const char* ar_version(void);
  * A batch keeps many prepared UInt8 pairs resident (default pipeline: the batch's params must
  * quantise to UInt8) and runs one tracking pass over points of all of them, each point naming
  * its pair: track (src/track.jl:103-160) over every point, with uneven chip sizes and radii.
- * The results equal a pass over each pair alone, bit for bit: power-of-two chip areas take the
- * integer matrix-core tracker, other chip areas the direct Float64 tracker (the reference's
- * Float32(x - mean) chip and Float64 sums, src/correlate.jl:423-453, 799-814). */
+ * Power-of-two chip areas take the integer matrix-core tracker and equal a pass over each pair
+ * alone, bit for bit; other chip areas take the same tracker with the unrounded mean-removed chip,
+ * or, after ar_batch_set_exact(b, 1), the direct Float64 tracker (the reference's
+ * Float32(x - mean) chip and Float64 sums, src/correlate.jl:423-453, 799-814), bit-equal too. */
 typedef struct ar_batch ar_batch;
 #define AR_BATCH_GRAPH 1      /* capture the pass in a HIP graph, replay it for the same buffers */
 #define AR_BATCH_SPLIT_SET 2  /* bits 8..15 (R1) and 16..23 (R2) give the radius classes
@@ -226,6 +227,29 @@ ar_status ar_batch_track_device(ar_batch* b, const ar_points* dev_pts, const int
  * replayed (0/1). */
 ar_status ar_batch_info(const ar_batch* b, double* ms, int32_t* info);
 const char* ar_batch_last_error(const ar_batch* b);
+/* Chips whose area is not a power of two: exact = 1 sends them to the direct Float64 tracker
+ * (every result bit-equal to a pass over its pair alone); exact = 0 (the default) keeps them on
+ * the integer matrix-core tracker with the unrounded mean-removed chip, (n x - sum) / n, in place
+ * of the reference's Float32(x - mean) chip: integer peaks agree with the exact pass to the
+ * tolerance stated for the batched form, subpixel offsets to a small fraction of a pixel.
+ * Power-of-two areas are bit-equal either way. */
+ar_status ar_batch_set_exact(ar_batch* b, int32_t exact);
+
+/* ---- Tuning cache: the batched pass's tuner records its choice of radius classes per pass
+ * shape in a small per-device file, as the reference keeps its FFTW wisdom
+ * (src/plans.jl:133-257). Directory: $AR_TUNE_CACHE_DIR, else $XDG_CACHE_HOME/autorift_hip,
+ * else $HOME/.cache/autorift_hip, resolved on every call; AR_TUNE_CACHE=off disables it. The
+ * file is named for `device` (e.g. "gfx950-256cu") and the library version. These calls never
+ * fail their caller: a missing, unreadable or corrupt file reads as empty, and a write that
+ * cannot be made (read-only directory, full disk) is dropped. Writes go to a temporary renamed
+ * over the file. */
+/* Path of the cache file for `device` (copied to buf, NUL-terminated, when len > 0); returns its
+ * length, or 0 when caching is off or no directory is usable. */
+int64_t ar_tune_cache_path(const char* device, char* buf, int64_t len);
+/* 1 and *value when the file holds `key`, else 0. */
+int32_t ar_tune_cache_lookup(const char* device, const char* key, int32_t* value);
+/* Records key = value; 1 when the file was written, 0 when the write was dropped. */
+int32_t ar_tune_cache_record(const char* device, const char* key, int32_t value);
 
 /* ---- Serving (no counterpart in the reference, which correlates one pair per call).
  * A server owns `slots` contexts, each with its own worker thread and HIP stream. Pairs of any

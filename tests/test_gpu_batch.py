@@ -1,9 +1,10 @@
 """GPU tests of the batched pass (ar_batch): points of many pairs, uneven chip sizes and radii,
 in one tracking pass.
 
-Every point's result must equal a pass over its own pair alone, bit for bit: power-of-two chip
-areas run the same exact integer arithmetic, other chip areas the same direct Float64 tracker, and
-the pass's shared padding only adds invalid zeros.
+Power-of-two chip areas must equal a pass over each pair alone, bit for bit (the same exact integer
+arithmetic; the pass's shared padding only adds invalid zeros). Other chip areas do too when the
+batch is exact (the same direct Float64 tracker); by default they take the integer tracker with
+the unrounded mean-removed chip and are held to a tolerance.
 """
 import numpy as np
 import pytest
@@ -119,12 +120,13 @@ def test_batch_splits_and_graph_replay():
 
 @pytest.mark.parametrize("up", [1, 64])
 def test_batch_non_power_of_two_chips(up):
-    """Chip 48 (area 2304) mixed with power-of-two chips: the direct Float64 tracker inside the
-    batched pass, bit-equal to each pair alone, also when the pass is captured and replayed."""
+    """Chip 48 (area 2304) mixed with power-of-two chips in an exact batch: the direct Float64
+    tracker inside the batched pass, bit-equal to each pair alone, also when the pass is captured
+    and replayed."""
     pairs, ps, pair = _scene(3, chips=(16, 48, 32, 24), n_per=50)
-    print(f"batch chip 48 up{up}", flush=True)
+    print(f"batch chip 48 exact up{up}", flush=True)
     exp = _alone(pairs, ps, pair, up)
-    with autorift.Batch(autorift.params()) as bt:
+    with autorift.Batch(autorift.params(), exact=True) as bt:
         for a, b in pairs:
             bt.add_pair(a, b)
         got = bt.track(ps, pair, upsampling=up)
@@ -151,6 +153,41 @@ def test_batch_non_power_of_two_chips(up):
     for g, e in zip((got.dx, got.dy, got.correlation), exp[:3]):
         np.testing.assert_array_equal(_bits(g), _bits(e))
     assert np.isfinite(got.dx[ps.chip_size_x == 48]).mean() > 0.5
+
+
+def test_batch_non_power_of_two_chips_default():
+    """The default batch: chips 24 and 48 on the integer tracker (with the power-of-two chips),
+    integer peaks equal to the exact pass, subpixel offsets within 0.05 px; switching the batch
+    to exact and back gives each mode's own results."""
+    pairs, ps, pair = _scene(3, chips=(16, 48, 32, 24), n_per=50)
+    print("batch chip 48 default", flush=True)
+    exp1 = _alone(pairs, ps, pair, 1)
+    exp64 = _alone(pairs, ps, pair, 64)
+    with autorift.Batch(autorift.params()) as bt:
+        for a, b in pairs:
+            bt.add_pair(a, b)
+        g1 = bt.track(ps, pair, upsampling=1)
+        g64 = bt.track(ps, pair, upsampling=64)
+        bt.set_exact(True)
+        x64 = bt.track(ps, pair, upsampling=64)
+        bt.set_exact(False)
+        d64 = bt.track(ps, pair, upsampling=64)
+    for f, e in zip(("dx", "dy", "correlation"), exp64[:3]):
+        np.testing.assert_array_equal(_bits(getattr(x64, f)), _bits(e))
+        np.testing.assert_array_equal(_bits(getattr(d64, f)), _bits(getattr(g64, f)))
+    np.testing.assert_array_equal(g64.searched, exp64[3])
+    area = ps.chip_size_x * ps.chip_size_y
+    p2 = (area & (area - 1)) == 0          # chips 16 and 32; 24 and 48 are not powers of two
+    for f, e in zip(("dx", "dy", "correlation"), exp64[:3]):
+        np.testing.assert_array_equal(_bits(getattr(g64, f)[p2]), _bits(e[p2]))
+    m = ~p2 & np.isfinite(exp1[0])
+    assert np.array_equal(np.isfinite(g1.dx[~p2]), np.isfinite(exp1[0][~p2]))
+    assert ((g1.dx[m] == exp1[0][m]) & (g1.dy[m] == exp1[1][m])).all()
+    ok = ~p2 & np.isfinite(exp64[0])
+    off = np.hypot(g64.dx[ok] - exp64[0][ok], g64.dy[ok] - exp64[1][ok])
+    print(f"chips 24, 48 default: {ok.sum()} points, max subpixel offset {off.max():.2e} px",
+          flush=True)
+    assert off.max() <= 0.05
 
 
 def test_batch_errors():

@@ -210,7 +210,7 @@ def test_cfg3_batched_sample_matches_oracle():
     ps = autorift.pointset(x, y, search_radius_x=r, search_radius_y=r, chip_size_x=c,
                            chip_size_y=c)
     print(f"cfg3: {len(ps)} points over {npairs} pairs", flush=True)
-    with autorift.Batch(autorift.params()) as bt:
+    with autorift.Batch(autorift.params(), exact=True) as bt:
         for a, b in pairs:
             bt.add_pair(a, b)
         got = bt.track(ps, pair, upsampling=64)
@@ -230,6 +230,56 @@ def test_cfg3_batched_sample_matches_oracle():
         m = (c == chip) & np.isfinite(got.dx)
         print(f"cfg3 chip {chip}: {m.sum()} measured, bit-equal {same[m].mean():.4f}", flush=True)
         assert m.sum() > 100 and same[m].all()
+
+
+def test_cfg3_default_batch_against_exact():
+    """The default batched pass against the exact one (bit-equal to the oracle above) on 32 000
+    points of the cfg3 kind: power-of-two chips bit-equal; chip 48 (unrounded mean-removed chip)
+    with integer peaks equal at >= 99.9 % and subpixel offsets within 0.05 px (the batched form's
+    tolerance), printed with the worst offset seen."""
+    rng = np.random.default_rng(15)
+    npairs, per = 16, 2000
+    pairs, xs, ys, cs, rs, pr = [], [], [], [], [], []
+    for k in range(npairs):
+        n = int(rng.integers(1024, 2049))
+        a, b = cfg1_pair(900 + k, n=n, shift=tuple(rng.uniform(-10, 10, 2)))
+        pairs.append((a, b))
+        xs.append(1 + rng.random(per) * (n - 1))
+        ys.append(1 + rng.random(per) * (n - 1))
+        cs.append(rng.choice([16, 32, 48, 64], per, p=[0.1, 0.1, 0.7, 0.1]))
+        rs.append(rng.integers(8, 41, per))
+        pr.append(np.full(per, k, np.int32))
+    x, y = np.concatenate(xs), np.concatenate(ys)
+    c, r = np.concatenate(cs).astype(np.int64), np.concatenate(rs).astype(np.int64)
+    pair = np.concatenate(pr)
+    ps = autorift.pointset(x, y, search_radius_x=r, search_radius_y=r, chip_size_x=c,
+                           chip_size_y=c)
+    res = {}
+    for exact in (True, False):
+        print(f"cfg3 default vs exact: exact={exact}, {len(ps)} points", flush=True)
+        with autorift.Batch(autorift.params(), exact=exact) as bt:
+            for a, b in pairs:
+                bt.add_pair(a, b)
+            res[exact] = (bt.track(ps, pair, upsampling=1), bt.track(ps, pair, upsampling=64))
+    (e1, e64), (g1, g64) = res[True], res[False]
+    np.testing.assert_array_equal(g64.searched, e64.searched)
+    p2 = c != 48
+    for g, e in ((g1, e1), (g64, e64)):
+        for f in ("dx", "dy", "correlation"):
+            np.testing.assert_array_equal(_bits(getattr(g, f)[p2]), _bits(getattr(e, f)[p2]))
+    m = (c == 48) & np.isfinite(e1.dx) & np.isfinite(g1.dx)
+    peaks = (g1.dx[m] == e1.dx[m]) & (g1.dy[m] == e1.dy[m])
+    both = (c == 48) & np.isfinite(e64.dx) & np.isfinite(g64.dx)
+    off = np.hypot(g64.dx[both] - e64.dx[both], g64.dy[both] - e64.dy[both])
+    ident = np.mean((_bits(g64.dx[both]) == _bits(e64.dx[both])) &
+                    (_bits(g64.dy[both]) == _bits(e64.dy[both])))
+    print(f"cfg3 chip 48 default: {m.sum()} points, integer peaks equal {peaks.mean():.5f}, "
+          f"subpixel offset max {off.max():.2e} px (p99.9 {np.quantile(off, 0.999):.2e}), "
+          f"bit-equal {ident:.4f}", flush=True)
+    assert m.sum() > 5000
+    assert np.array_equal(np.isfinite(g1.dx[c == 48]), np.isfinite(e1.dx[c == 48]))
+    assert peaks.mean() >= 0.999
+    assert np.mean(off <= 0.05) >= 0.999
 
 
 # --------------------------------------------------------------------------------------------
