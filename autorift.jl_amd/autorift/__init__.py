@@ -6,7 +6,7 @@ reference (src/api.jl, src/points.jl, src/track.jl, src/multichip.jl); Julia's m
 """
 from ._lib import (AR_COHERENCE, AR_NCC, AR_ZNCC, ArgumentError, AutoRIFTError, DimensionMismatch,
                    NoDeviceError, Params, default_params, lib)
-from .api import (Cache, ImagePair, RotationSearch, angles, autorift, autorift_,
+from .api import (Cache, Highpass, ImagePair, RotationSearch, Wallis, angles, autorift, autorift_,
                   autorift_with_grid, chip_measures, chip_sizes, init, measure_at, params,
                   reinit_, track)
 from .core import Context, DisplacementField, MultichipResult
@@ -22,5 +22,5 @@ __all__ = [
     "autorift_with_grid", "chip_sizes", "init", "params", "reinit_", "track", "Context",
     "DisplacementField", "MultichipResult", "PointSet", "chip_bounds", "gridpoints",
     "gridpoints_xy", "pointset", "sanitize_", "search_bounds", "Server", "ServerError", "Batch", "RotationSearch", "angles", "autorift_geo", "grid_coordinates",
-    "pixel_size", "to_velocity", "years", "chip_measures", "measure_at",
+    "pixel_size", "to_velocity", "years", "chip_measures", "measure_at", "Highpass", "Wallis",
 ]

@@ -56,6 +56,7 @@ class Params(C.Structure):
         ("precision", C.c_int32), ("rotation_count", C.c_int32),
         ("rotation_angles", C.c_double * 8), ("deramp_axis", C.c_int32),
         ("level_measure_count", C.c_int32), ("level_measures", C.c_int32 * 8),
+        ("min_std", C.c_double),
     ]
 
 

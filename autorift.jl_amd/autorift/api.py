@@ -16,7 +16,7 @@ from .core import Context, DisplacementField, MultichipResult
 from .points import PointSet, gridpoints
 
 _SIMILARITY = {"zncc": _lib.AR_ZNCC, "ncc": _lib.AR_NCC, "coherence": _lib.AR_COHERENCE}
-_PREPROCESS = {"highpass": 1, "none": 0, "deramp": 2}
+_PREPROCESS = {"highpass": 1, "none": 0, "deramp": 2, "wallis": 3}
 _DERAMP_AXIS = {"both": 0, "x": 1, "y": 2}
 _QUANTIZE = {"uint8": 1, "none": 0}
 _SUBPIXEL = ("pyramid", "none")
@@ -49,6 +49,42 @@ class RotationSearch:
 
     def __repr__(self):
         return f"RotationSearch({self.angles}, about={self.about})"
+
+
+def _check_filter_width(width, who):
+    """src/types.jl:587-594: a window filter's width is odd and at least 3."""
+    w = int(width)
+    if w < 3:
+        raise ArgumentError(f"{who} `width` must be >= 3, got {w}")
+    if w % 2 == 0:
+        raise ArgumentError(f"{who} `width` must be odd so the filter has a well-defined centre, "
+                            f"got {w}")
+    return w
+
+
+class Highpass:
+    """Highpass(; width = 5) (src/types.jl:286-297): subtract the local box mean."""
+
+    def __init__(self, width=5):
+        self.width = _check_filter_width(width, "Highpass")
+
+    def __repr__(self):
+        return f"Highpass(width={self.width})"
+
+
+class Wallis:
+    """Wallis(; width = 5, min_std = 0.0) (src/types.jl:299-322): subtract the local mean and
+    divide by the local standard deviation, floored at `min_std` (0 disables the floor)."""
+
+    def __init__(self, width=5, min_std=0.0):
+        ms = float(min_std)
+        if not ms >= 0:
+            raise ArgumentError(f"`min_std` must be >= 0, got {min_std}")
+        self.width = _check_filter_width(width, "Wallis")
+        self.min_std = ms
+
+    def __repr__(self):
+        return f"Wallis(width={self.width}, min_std={self.min_std})"
 
 
 def angles(m: RotationSearch):
@@ -115,12 +151,18 @@ def params(similarity="zncc", preprocess="highpass", quantize="uint8", subpixel=
         for k, v in enumerate(meas):
             p.level_measures[k] = v
     p.deramp_axis = _DERAMP_AXIS[_resolve(_DERAMP_AXIS, deramp_axis, "deramp_axis")]
-    pre = _resolve(_PREPROCESS, preprocess, "preprocess")
-    p.preprocess = _PREPROCESS[pre]
-    if filter_width is not None:
-        if int(filter_width) < 1:
-            raise ArgumentError(f"`filter_width` must be >= 1, got {filter_width}")
-        p.filter_width = int(filter_width)
+    if isinstance(preprocess, (Highpass, Wallis)):
+        # a method instance carries its own width (src/params.jl:90)
+        p.preprocess = _PREPROCESS["wallis" if isinstance(preprocess, Wallis) else "highpass"]
+        p.filter_width = preprocess.width
+        if isinstance(preprocess, Wallis):
+            p.min_std = preprocess.min_std
+    else:
+        pre = _resolve(_PREPROCESS, preprocess, "preprocess")
+        p.preprocess = _PREPROCESS[pre]
+        # windowless methods ignore filter_width (src/params.jl:91-103)
+        if filter_width is not None and pre in ("highpass", "wallis"):
+            p.filter_width = _check_filter_width(filter_width, pre.capitalize())
     p.quantize = _QUANTIZE[_resolve(_QUANTIZE, quantize, "quantize")]
     sub = _resolve({k: 0 for k in _SUBPIXEL}, subpixel, "subpixel")
     if sub == "none":

@@ -149,6 +149,97 @@ __global__ void box_rowpass_highpass_kernel(const T* img, const uint8_t* mask, i
     }
 }
 
+// Row pass producing Wallis's two intermediate planes (src/preprocess.jl:333-351, 416-438): the
+// masked box mean (as _masked_boxmean returns it) and the Float32 squared deviation about it,
+// dev2 = mask & isfinite(mean) ? (Float32(img) - mean)^2 : NaN. Same summation as the highpass
+// row pass.
+template <typename T, bool NANAWARE>
+__global__ void box_rowpass_wallis_dev_kernel(const T* img, const uint8_t* mask, int64_t nr,
+                                              int64_t nc, int lx, int rx, int ly, int ry,
+                                              const double* colsum, const int32_t* colcnt,
+                                              float* mean_out, float* dev2) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nr) return;
+    int64_t I = i + 1;
+    int64_t ci = (I + ry < nr ? I + ry : nr) - (I - ly > 1 ? I - ly : 1) + 1;
+    double s = 0.0;
+    int32_t c = 0;
+    int64_t b1 = rx + 1 < nc ? rx + 1 : nc;
+    for (int64_t j = 1; j <= b1; j++) {
+        s += colsum[i + (j - 1) * nr];
+        if (NANAWARE) c += colcnt[i + (j - 1) * nr];
+    }
+    for (int64_t j = 1; j <= nc; j++) {
+        const int64_t k = i + (j - 1) * nr;
+        float mean;
+        if (NANAWARE) {
+            mean = c > 0 ? (float)(s / (double)c) : NAN;
+        } else {
+            int64_t cj = (j + rx < nc ? j + rx : nc) - (j - lx > 1 ? j - lx : 1) + 1;
+            mean = (float)(s / (double)(ci * cj));
+        }
+        mean_out[k] = mean;
+        if (mask[k] && isfinite(mean)) {
+            const float d = as_f32(img, k) - mean;
+            dev2[k] = d * d;
+        } else {
+            dev2[k] = NAN;
+        }
+        int64_t oj = j - lx, jj = j + rx + 1;
+        if (1 <= oj && oj <= nc) {
+            s -= colsum[i + (oj - 1) * nr];
+            if (NANAWARE) c -= colcnt[i + (oj - 1) * nr];
+        }
+        if (jj <= nc) {
+            s += colsum[i + (jj - 1) * nr];
+            if (NANAWARE) c += colcnt[i + (jj - 1) * nr];
+        }
+    }
+}
+
+// Final Wallis row pass: the NaN-aware box mean of dev2 (windowmean(dev2; hasnan = gaps): with no
+// NaN in dev2 its dense path adds the same values in the same order and divides by the same
+// count, so the NaN-aware form gives the same result either way), sd = sqrt(max(msd, 0)) with
+// NaN kept, then out = (Float32(img) - mean) / max(sd, floor_sd), NaN where the mask is false,
+// mean or sd is not finite, or the divisor is 0. The Float32 square root and quotient are formed
+// in Float64 and rounded once, which gives the correctly rounded Float32 result.
+template <typename T>
+__global__ void box_rowpass_wallis_out_kernel(const T* img, const uint8_t* mask, int64_t nr,
+                                              int64_t nc, int lx, int rx, const double* colsum,
+                                              const int32_t* colcnt, const float* mean_in,
+                                              float floor_sd, float* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nr) return;
+    double s = 0.0;
+    int32_t c = 0;
+    int64_t b1 = rx + 1 < nc ? rx + 1 : nc;
+    for (int64_t j = 1; j <= b1; j++) {
+        s += colsum[i + (j - 1) * nr];
+        c += colcnt[i + (j - 1) * nr];
+    }
+    for (int64_t j = 1; j <= nc; j++) {
+        const int64_t k = i + (j - 1) * nr;
+        const float msd = c > 0 ? (float)(s / (double)c) : NAN;
+        const float sd = isnan(msd) ? NAN : (float)sqrt((double)(msd < 0.0f ? 0.0f : msd));
+        const float mean = mean_in[k];
+        float o = NAN;
+        if (mask[k] && isfinite(mean) && isfinite(sd)) {
+            const float sdiv = sd < floor_sd ? floor_sd : sd;
+            if (sdiv > 0.0f) o = (float)((double)(as_f32(img, k) - mean) / (double)sdiv);
+        }
+        out[k] = o;
+        int64_t oj = j - lx, jj = j + rx + 1;
+        if (1 <= oj && oj <= nc) {
+            s -= colsum[i + (oj - 1) * nr];
+            c -= colcnt[i + (oj - 1) * nr];
+        }
+        if (jj <= nc) {
+            s += colsum[i + (jj - 1) * nr];
+            c += colcnt[i + (jj - 1) * nr];
+        }
+    }
+}
+
 // _erode_mask (src/preprocess.jl:447-460) and _filtered (275-288): a pixel stays valid only if
 // every in-bounds pixel of its (clamped) w-window is valid; non-finite filtered values become 0
 // and invalid.
@@ -353,9 +444,12 @@ static ar_status prepare_one(ar_context* ctx, const T* raw, const uint8_t* mask,
     ar_status st;
     const uint8_t* curmask = mask;
     float* hp = nullptr;
-    if (p.preprocess == AR_PRE_HIGHPASS) {
+    if (p.preprocess == AR_PRE_HIGHPASS || p.preprocess == AR_PRE_WALLIS) {
+        const bool wallis = p.preprocess == AR_PRE_WALLIS;
         int w = p.filter_width;
         if (w < 1) return pfail(ctx, AR_ERR_INVALID_ARGUMENT, "filter width must be >= 1");
+        if (wallis && !(p.min_std >= 0.0))
+            return pfail(ctx, AR_ERR_INVALID_ARGUMENT, "Wallis `min_std` must be >= 0");
         if ((st = ctx->flags.reserve(ctx, 64))) return st;
         int f0[2] = {1, 0};
         PCHK(hipMemcpyAsync(ctx->flags.ptr, f0, 8, hipMemcpyHostToDevice, ctx->stream));
@@ -369,7 +463,7 @@ static ar_status prepare_one(ar_context* ctx, const T* raw, const uint8_t* mask,
         int wx = (int)(w < nx ? w : nx), wy = (int)(w < ny ? w : ny);
         int lx = wx / 2, ly = wy / 2, rx = wx - 1 - lx, ry = wy - 1 - ly;
         if ((st = ctx->work_f64.reserve(ctx, N * 8 + N * 4))) return st;
-        if ((st = ctx->work_f32.reserve(ctx, N * 4))) return st;
+        if ((st = ctx->work_f32.reserve(ctx, N * 4 * (wallis ? 3 : 1)))) return st;
         double* colsum = (double*)ctx->work_f64.ptr;
         int32_t* colcnt = (int32_t*)(colsum + N);
         hp = (float*)ctx->work_f32.ptr;
@@ -377,7 +471,23 @@ static ar_status prepare_one(ar_context* ctx, const T* raw, const uint8_t* mask,
         int64_t nthr = ((ny + nb - 1) / nb) * nx;
         int gb = (int)((nthr + 255) / 256);
         int gr = (int)((ny + 63) / 64);
-        if (allmask && !anynan) {
+        if (wallis) {
+            // mean and squared deviations, then the box mean of the deviations and the output
+            float* mean = hp + N;
+            float* dev2 = hp + 2 * N;
+            if (allmask && !anynan) {
+                box_colpass_kernel<T, false, false><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
+                box_rowpass_wallis_dev_kernel<T, false><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, mean, dev2);
+            } else if (allmask) {
+                box_colpass_kernel<T, false, true><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
+                box_rowpass_wallis_dev_kernel<T, true><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, mean, dev2);
+            } else {
+                box_colpass_kernel<T, true, true><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
+                box_rowpass_wallis_dev_kernel<T, true><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, mean, dev2);
+            }
+            box_colpass_kernel<float, false, true><<<gb, 256, 0, ctx->stream>>>(dev2, mask, ny, nx, ly, ry, colsum, colcnt);
+            box_rowpass_wallis_out_kernel<T><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, colsum, colcnt, mean, (float)p.min_std, hp);
+        } else if (allmask && !anynan) {
             box_colpass_kernel<T, false, false><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
             box_rowpass_highpass_kernel<T, false><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, hp);
         } else if (allmask) {
@@ -456,13 +566,14 @@ extern "C" ar_status ar_set_images(ar_context* ctx, const void* ref, const void*
                      "context (init) for a different image size");
     const ar_params& p = ctx->params;
     if (p.preprocess != AR_PRE_NONE && p.preprocess != AR_PRE_HIGHPASS &&
-        p.preprocess != AR_PRE_DERAMP)
+        p.preprocess != AR_PRE_WALLIS && p.preprocess != AR_PRE_DERAMP)
         return pfail(ctx, AR_ERR_UNSUPPORTED, "pre-filter not available on the device path");
     // complex input takes Deramp or nothing and NoQuantize; real input takes no Deramp
     // (src/preprocess.jl:247-260, 473-531)
-    if (dtype == AR_C64 && p.preprocess == AR_PRE_HIGHPASS)
+    if (dtype == AR_C64 && (p.preprocess == AR_PRE_HIGHPASS || p.preprocess == AR_PRE_WALLIS))
         return pfail(ctx, AR_ERR_INVALID_ARGUMENT,
-                     "Highpass is an amplitude filter and rejects complex input: it would "
+                     std::string(p.preprocess == AR_PRE_WALLIS ? "Wallis" : "Highpass") +
+                     " is an amplitude filter and rejects complex input: it would "
                      "produce something plausible-looking and meaningless. Use Deramp or no "
                      "pre-filter for SLC data");
     if (dtype == AR_C64 && p.quantize != AR_QUANT_NONE)

@@ -39,8 +39,9 @@ typedef enum { AR_U8 = 0, AR_F32 = 1, AR_F64 = 2, AR_I16 = 3, AR_C64 = 4 } ar_dt
 /* AR_COHERENCE: complex coherence magnitude (src/types.jl:154-200), complex pairs only; ZNCC and
  * NCC need real pairs. */
 typedef enum { AR_ZNCC = 0, AR_NCC = 1, AR_COHERENCE = 2 } ar_measure;
-/* AR_PRE_DERAMP: Deramp(axis) (src/types.jl:383-423), complex pairs only; Highpass real only. */
-typedef enum { AR_PRE_NONE = 0, AR_PRE_HIGHPASS = 1, AR_PRE_DERAMP = 2 } ar_preprocess;
+/* AR_PRE_DERAMP: Deramp(axis) (src/types.jl:383-423), complex pairs only; Highpass and Wallis
+ * (src/types.jl:299-322, src/preprocess.jl:333-351) real only. */
+typedef enum { AR_PRE_NONE = 0, AR_PRE_HIGHPASS = 1, AR_PRE_DERAMP = 2, AR_PRE_WALLIS = 3 } ar_preprocess;
 typedef enum { AR_QUANT_NONE = 0, AR_QUANT_UINT8 = 1 } ar_quantize;
 /* Arithmetic of the correlation contraction for Float32 prepared images (quantize = :none).
  * EXACT follows the reference's Float64 direct numerator (src/correlate.jl:799-814); BF16 and
@@ -53,7 +54,7 @@ typedef enum { AR_PREC_EXACT = 0, AR_PREC_BF16 = 1, AR_PREC_FP16 = 2 } ar_precis
  * Built by the host from its resolved parameter set; the library never parses strings. */
 typedef struct {
     int32_t preprocess;           /* ar_preprocess, default AR_PRE_HIGHPASS */
-    int32_t filter_width;         /* Highpass width, default 5 */
+    int32_t filter_width;         /* Highpass / Wallis width, default 5 */
     int32_t quantize;             /* ar_quantize, default AR_QUANT_UINT8 */
     int32_t measure;              /* ar_measure, default AR_ZNCC */
     int64_t upsampling;           /* PyramidRefine(64); 1 means NoRefine */
@@ -88,6 +89,7 @@ typedef struct {
                                      chip-size level, the last repeating; 0 = `measure` at every
                                      level (at most 8) */
     int32_t level_measures[8];
+    double min_std;               /* Wallis divisor floor (min_std), default 0.0 */
 } ar_params;
 
 /* PointSet fields (src/points.jl:50-74), structure of arrays of equal length. */

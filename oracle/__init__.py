@@ -207,6 +207,19 @@ def highpass_filtered(img, mask, w=5):
     return out, vm.astype(bool)
 
 
+def wallis_filtered(img, mask, w=5, min_std=0.0):
+    """preprocess(img, mask, Wallis(w, min_std)) (src/preprocess.jl:158-159, 333-351, 416-438):
+    (filtered Float32 image, eroded validity)."""
+    img = _f(img)
+    mask = _f(mask, np.uint8)
+    out = np.empty(img.shape, np.float32, order="F")
+    vm = np.empty(img.shape, np.uint8, order="F")
+    lib().orc_wallis_filtered(_p(img), _p(mask), _DT[img.dtype], C.c_int64(img.shape[0]),
+                              C.c_int64(img.shape[1]), C.c_int64(w), C.c_double(min_std),
+                              _p(out), _p(vm))
+    return out, vm.astype(bool)
+
+
 def to_uint8(img, mask):
     img = _f(img, np.float32)
     mask = _f(mask, np.uint8)

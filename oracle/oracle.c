@@ -1043,17 +1043,16 @@ void orc_window_extremum(const float* A, int64_t nr, int64_t nc, int64_t wx, int
 }
 
 /* ------------------------------------------------------------------------------------------ */
-/* pre-filter -- src/preprocess.jl:275-288 (_filtered), 304-312 (highpass),                  */
-/* 371-399 (_masked_boxmean), 447-460 (_erode_mask)                                           */
+/* pre-filters -- src/preprocess.jl:275-288 (_filtered), 304-312 (highpass), 333-351 (wallis), */
+/* 371-399 (_masked_boxmean), 416-438 (_masked_boxstd), 447-460 (_erode_mask)                 */
 /* ------------------------------------------------------------------------------------------ */
 
-void orc_highpass_filtered(const void* img, const uint8_t* mask, int dt, int64_t nr, int64_t nc,
-                           int64_t w, float* out, uint8_t* vmask) {
+/* _masked_boxmean -- src/preprocess.jl:371-399: the box mean of the valid pixels. With every
+ * pixel marked valid the image itself is averaged (windowmean! decides the NaN-aware path from
+ * the data); otherwise a Float32 copy with NaN where the mask is false. */
+static void masked_boxmean(const void* img, const uint8_t* mask, int dt, int64_t nr, int64_t nc,
+                           int64_t wx, int64_t wy, int allmask, float* out) {
     int64_t N = nr * nc;
-    int64_t wx = w < nc ? w : nc, wy = w < nr ? w : nr;
-    int allmask = 1;
-    for (int64_t k = 0; k < N; k++)
-        if (!mask[k]) { allmask = 0; break; }
     if (allmask) {
         int gaps = 0;
         for (int64_t k = 0; k < N; k++)
@@ -1066,9 +1065,13 @@ void orc_highpass_filtered(const void* img, const uint8_t* mask, int dt, int64_t
         windowmean_masked(masked, ORC_F32, nr, nc, wx, wy, out);
         free(masked);
     }
-    for (int64_t k = 0; k < N; k++)
-        out[k] = (mask[k] && isfinite(out[k])) ? ldf(img, dt, k) - out[k] : NAN;
-    /* _erode_mask */
+}
+
+/* _filtered -- src/preprocess.jl:275-288, with _erode_mask 447-460: the mask eroded by the
+ * filter width, then non-finite outputs become 0 and invalid. */
+static void filtered_finish(float* out, const uint8_t* mask, int allmask, int64_t nr, int64_t nc,
+                            int64_t w, uint8_t* vmask) {
+    int64_t N = nr * nc;
     if (allmask) {
         memcpy(vmask, mask, (size_t)N);
     } else {
@@ -1081,6 +1084,65 @@ void orc_highpass_filtered(const void* img, const uint8_t* mask, int dt, int64_t
     }
     for (int64_t k = 0; k < N; k++)
         if (!isfinite(out[k])) { out[k] = 0.0f; vmask[k] = 0; }
+}
+
+static int all_set(const uint8_t* mask, int64_t N) {
+    for (int64_t k = 0; k < N; k++)
+        if (!mask[k]) return 0;
+    return 1;
+}
+
+/* preprocess(img, mask, ::Highpass) -- src/preprocess.jl:155-156, highpass 304-312 */
+void orc_highpass_filtered(const void* img, const uint8_t* mask, int dt, int64_t nr, int64_t nc,
+                           int64_t w, float* out, uint8_t* vmask) {
+    int64_t N = nr * nc;
+    int64_t wx = w < nc ? w : nc, wy = w < nr ? w : nr;
+    int allmask = all_set(mask, N);
+    masked_boxmean(img, mask, dt, nr, nc, wx, wy, allmask, out);
+    for (int64_t k = 0; k < N; k++)
+        out[k] = (mask[k] && isfinite(out[k])) ? ldf(img, dt, k) - out[k] : NAN;
+    filtered_finish(out, mask, allmask, nr, nc, w, vmask);
+}
+
+/* preprocess(img, mask, ::Wallis) -- src/preprocess.jl:158-159; wallis 333-351; _masked_boxstd
+ * 416-438: the population standard deviation about the measured local mean, from the box mean
+ * (NaN-aware when any squared deviation is NaN) of Float32 squared deviations; out =
+ * (Float32(x) - mean) / max(sd, Float32(min_std)), NaN where the mask is false, the mean or sd is
+ * not finite, or the divisor is 0. Julia's max propagates NaN, and max(x, 0f0) of a negative x
+ * (a running sum's rounding) is 0. */
+void orc_wallis_filtered(const void* img, const uint8_t* mask, int dt, int64_t nr, int64_t nc,
+                         int64_t w, double min_std, float* out, uint8_t* vmask) {
+    int64_t N = nr * nc;
+    int64_t wx = w < nc ? w : nc, wy = w < nr ? w : nr;
+    int allmask = all_set(mask, N);
+    float* mean = (float*)malloc(sizeof(float) * (size_t)N);
+    float* dev2 = (float*)malloc(sizeof(float) * (size_t)N);
+    float* msd = (float*)malloc(sizeof(float) * (size_t)N);
+    masked_boxmean(img, mask, dt, nr, nc, wx, wy, allmask, mean);
+    int gaps = 0;
+    for (int64_t k = 0; k < N; k++) {
+        if (mask[k] && isfinite(mean[k])) {
+            float d = ldf(img, dt, k) - mean[k];
+            float v = d * d;
+            dev2[k] = v;
+            gaps |= isnan(v);
+        } else {
+            dev2[k] = NAN;
+            gaps = 1;
+        }
+    }
+    if (gaps) windowmean_masked(dev2, ORC_F32, nr, nc, wx, wy, msd);
+    else windowmean_dense(dev2, ORC_F32, nr, nc, wx, wy, msd);
+    const float floor_sd = (float)min_std;
+    for (int64_t k = 0; k < N; k++) {
+        float m = msd[k];
+        float sd = isnan(m) ? NAN : sqrtf(m < 0.0f ? 0.0f : m);
+        if (!mask[k] || !isfinite(mean[k]) || !isfinite(sd)) { out[k] = NAN; continue; }
+        float sdiv = sd < floor_sd ? floor_sd : sd;
+        out[k] = sdiv > 0 ? (ldf(img, dt, k) - mean[k]) / sdiv : NAN;
+    }
+    free(mean); free(dev2); free(msd);
+    filtered_finish(out, mask, allmask, nr, nc, w, vmask);
 }
 
 /* _to_uint8 -- src/preprocess.jl:546-592 */

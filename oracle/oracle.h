@@ -81,6 +81,8 @@ void orc_window_extremum(const float* a, int64_t nr, int64_t nc, int64_t wx, int
                          int is_max, float* out);
 void orc_highpass_filtered(const void* img, const uint8_t* mask, int dtype, int64_t nr,
                            int64_t nc, int64_t w, float* out, uint8_t* out_mask);
+void orc_wallis_filtered(const void* img, const uint8_t* mask, int dtype, int64_t nr, int64_t nc,
+                         int64_t w, double min_std, float* out, uint8_t* vmask);
 void orc_to_uint8(const float* img, const uint8_t* mask, int64_t nr, int64_t nc, uint8_t* out);
 void orc_noquantize(const float* img, int64_t nr, int64_t nc, float* out);
 

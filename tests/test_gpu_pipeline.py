@@ -67,6 +67,63 @@ def test_prepare_matches_oracle(kind, quant):
         np.testing.assert_array_equal(r.view(np.uint32), np.asarray(er, np.float32).view(np.uint32))
 
 
+@pytest.mark.parametrize("kind", ["u8", "i16", "f32", "f64", "f32_nan", "f32_mask"])
+@pytest.mark.parametrize("w,min_std", [(5, 0.0), (7, 2.0), (21, 0.0)])
+def test_wallis_prepare_matches_oracle(kind, w, min_std):
+    """Wallis (src/preprocess.jl:158-159, 333-351, 416-438) on the device, bit-exact against the
+    oracle's restatement, both quantisers."""
+    img, mask = _images(kind)
+    img2, _ = _images(kind, seed=1)
+    m = np.isfinite(img.astype(np.float64)) if mask is None else mask
+    er, evm = oracle.wallis_filtered(img, m, w, min_std)
+    print(f"wallis {kind} w {w} min_std {min_std}", flush=True)
+    for quant in (True, False):
+        p = _params(quantize=1 if quant else 0, preprocess=3, filter_width=w, min_std=min_std)
+        with autorift.Context(p) as ctx:
+            ctx.set_images(img, img2, mask, mask)
+            r, _, rv, _ = ctx.get_pair()
+        np.testing.assert_array_equal(rv, evm)
+        if quant:
+            np.testing.assert_array_equal(r, oracle.to_uint8(er, evm))
+        else:
+            np.testing.assert_array_equal(r.view(np.uint32), er.view(np.uint32))
+    # (scattered NaNs erode most of the image at width 21: a pixel needs its whole window valid)
+    assert evm.sum() > (0 if kind == "f32_nan" else evm.size // 2)
+
+
+def test_wallis_multichip_matches_oracle():
+    """The default search over a Wallis-prepared pair (width 7): device preparation and the
+    multi-chip pass, bit-exact against the oracle's Wallis + quantiser + multichip."""
+    a, b = shifted_pair(320, (2.25, -1.5), seed=8)
+    a, b = a * np.float32(255), b * np.float32(255)
+    p = autorift.params(preprocess=autorift.Wallis(width=7, min_std=0.5), chip_size=32)
+    xs = np.arange(72.0, 249.0, 16.0)
+    grid = autorift.gridpoints_xy(xs, xs, chip_size=32, search_radius=25)
+    print("wallis multichip", flush=True)
+    with autorift.Context(p) as ctx:
+        ctx.set_images(a, b)
+        got = ctx.run_multichip(grid)
+    ok = np.isfinite(a)
+    er, erv = oracle.wallis_filtered(a, ok, 7, 0.5)
+    es, esv = oracle.wallis_filtered(b, np.isfinite(b), 7, 0.5)
+    e = oracle.multichip(oracle.to_uint8(er, erv), oracle.to_uint8(es, esv), erv & esv, grid.x,
+                         grid.y, grid.radius_x, grid.radius_y, p=oracle.params(chip_size=32),
+                         nthreads=16)
+    for g, x in zip((got.dx, got.dy, got.correlation), e[:3]):
+        np.testing.assert_array_equal(np.asarray(g, np.float32).view(np.uint32),
+                                      np.asarray(x, np.float32).view(np.uint32))
+    np.testing.assert_array_equal(got.chip_size, e[3])
+    assert np.isfinite(got.dx).mean() > 0.9
+
+
+def test_wallis_rejects_complex_input():
+    z = (np.ones((64, 64)) + 1j * np.ones((64, 64))).astype(np.complex64)
+    p = autorift.params(similarity="coherence", preprocess="wallis", quantize="none")
+    with autorift.Context(p) as ctx:
+        with pytest.raises(autorift.ArgumentError, match="Wallis"):
+            ctx.set_images(z, z)
+
+
 def test_prepare_reinit_keeps_other_slot():
     a, _ = _images("f32", seed=2)
     b, _ = _images("f32", seed=3)

@@ -65,6 +65,29 @@ def test_params_defaults_and_validation():
             autorift.params(**bad)
 
 
+def test_preprocess_methods_and_widths():
+    """Highpass / Wallis as symbols with filter_width or as method instances; widths follow
+    _check_filter_width (src/types.jl:587-594); windowless methods ignore filter_width
+    (src/params.jl:91-103)."""
+    p = autorift.params(preprocess="wallis", filter_width=21)
+    assert (p.preprocess, p.filter_width, p.min_std) == (3, 21, 0.0)
+    p = autorift.params(preprocess=autorift.Wallis(width=7, min_std=0.25))
+    assert (p.preprocess, p.filter_width, p.min_std) == (3, 7, 0.25)
+    p = autorift.params(preprocess=autorift.Highpass(width=9))
+    assert (p.preprocess, p.filter_width) == (1, 9)
+    assert autorift.params(preprocess="none", filter_width=4).preprocess == 0
+    assert autorift.params().min_std == 0.0 and autorift.params().filter_width == 5
+    for bad in (dict(preprocess="wallis", filter_width=4), dict(preprocess="wallis", filter_width=1),
+                dict(preprocess="highpass", filter_width=6), dict(preprocess="sobel")):
+        with pytest.raises(ArgumentError):
+            autorift.params(**bad)
+    for bad in (dict(width=4), dict(width=1), dict(min_std=-0.1), dict(min_std=float("nan"))):
+        with pytest.raises(ArgumentError):
+            autorift.Wallis(**bad)
+    with pytest.raises(ArgumentError):
+        autorift.Highpass(width=2)
+
+
 def test_pointset_and_grid_geometry():
     pts = autorift.pointset([10.0, 50.5, 200.0], [20.0, 60.0, 30.25], search_radius=25,
                             chip_size=32)
