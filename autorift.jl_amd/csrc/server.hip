@@ -112,7 +112,8 @@ static size_t pair_footprint(const ar_params& p, int64_t ny, int64_t nx, int32_t
     const size_t N = (size_t)ny * (size_t)nx;
     const size_t es = elem_size(dtype);
     const size_t raw = 2 * N * es + 2 * N;          // raw pair and masks
-    const size_t work = N * (4 + 8);                // Float32 / Float64 preparation planes
+    // Float64 column sums and counts, Float32 planes (Wallis: output, mean, squared deviations)
+    const size_t work = N * (8 + 4) + N * 4 * (p.preprocess == AR_PRE_WALLIS ? 3 : 1);
     const size_t pe = dtype == AR_C64 ? 8 : 4;      // prepared element: ComplexF32 or Float32
     const size_t prep = 2 * N * pe + 3 * N;         // prepared pair (at most pe bytes), masks
     const size_t box = 2 * N * 8;                   // box images (Float64 for the 16-bit path)
