@@ -371,6 +371,7 @@ void ar_context_destroy(ar_context* ctx) {
         for (int k = 0; k < 5; k++) hipEventDestroy(ctx->evf[k]);
         hipStreamDestroy(ctx->stream2);
     }
+    if (ctx->pin) hipHostFree(ctx->pin);
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
     if (ctx->evm) hipEventDestroy(ctx->evm);
@@ -380,6 +381,26 @@ void ar_context_destroy(ar_context* ctx) {
 }
 
 }  // extern "C"
+
+ar_status ar_pinned(ar_context* ctx, size_t n, void** out) {
+    if (n > ctx->pin_cap) {
+        if (ctx->pin) {
+            if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+                return fail(ctx, AR_ERR_DEVICE, "stream synchronisation failed");
+            hipHostFree(ctx->pin);
+            ctx->pin = nullptr;
+            ctx->pin_cap = 0;
+        }
+        const size_t cap = (n + 65535) & ~(size_t)65535;
+        if (hipHostMalloc(&ctx->pin, cap, hipHostMallocDefault) != hipSuccess) {
+            ctx->pin = nullptr;
+            return fail(ctx, AR_ERR_OUT_OF_MEMORY, "page-locked staging allocation failed");
+        }
+        ctx->pin_cap = cap;
+    }
+    *out = ctx->pin;
+    return AR_OK;
+}
 
 // Device bytes the context holds.
 size_t ar_context_bytes(ar_context* ctx) {
@@ -400,6 +421,10 @@ void ar_context_release(ar_context* ctx) {
 }
 
 // Upload masks (or derive them) and combine; sets ctx->all_valid.
+__global__ void set_flag_kernel(int* f, int v) {
+    if (threadIdx.x == 0) *f = v;
+}
+
 ar_status ar_upload_masks(ar_context* ctx, const uint8_t* ref_valid, const uint8_t* sec_valid,
                           int64_t N) {
     ar_status st;
@@ -410,16 +435,16 @@ ar_status ar_upload_masks(ar_context* ctx, const uint8_t* ref_valid, const uint8
     // rmask/smask by the preparation step when the caller passed NULL).
     if (ref_valid) HIPCHK(ctx, hipMemcpyAsync(ctx->rmask.ptr, ref_valid, N, hipMemcpyHostToDevice, ctx->stream));
     if (sec_valid) HIPCHK(ctx, hipMemcpyAsync(ctx->smask.ptr, sec_valid, N, hipMemcpyHostToDevice, ctx->stream));
-    int one = 1;
-    HIPCHK(ctx, hipMemcpyAsync(ctx->flags.ptr, &one, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    set_flag_kernel<<<1, 64, 0, ctx->stream>>>((int*)ctx->flags.ptr, 1);
     combine_masks_kernel<<<grid_for(N), 256, 0, ctx->stream>>>(
         (const uint8_t*)ctx->rmask.ptr, (const uint8_t*)ctx->smask.ptr, (uint8_t*)ctx->okmask.ptr, N,
         (int*)ctx->flags.ptr);
     HIPCHK(ctx, hipGetLastError());
-    int all = 0;
-    HIPCHK(ctx, hipMemcpyAsync(&all, ctx->flags.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    void* pin = nullptr;
+    if ((st = ar_pinned(ctx, 256, &pin))) return st;
+    HIPCHK(ctx, hipMemcpyAsync(pin, ctx->flags.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->all_valid = all != 0;
+    ctx->all_valid = *(const int*)pin != 0;
     return AR_OK;
 }
 

@@ -53,6 +53,10 @@ struct ar_context {
     DevBuf box;                 // box images of W and W^2 for a pass with one chip size
     DevBuf refx;                // int8 copies (value ^ 0x80) of the UInt8 pair, for the matrix cores
     DevBuf prec;                // per-point geometry records of the current pass (u8 path)
+    DevBuf mc;                  // multi-chip driver planes (grid, level, coarse and result planes)
+    // page-locked host staging for the multi-chip driver's uploads, results and readbacks
+    void* pin = nullptr;
+    size_t pin_cap = 0;
     uint64_t ref_gen = 0, refx_gen = ~0ULL;   // versions of the pair and of the copies made from it
     float absmax = 0.0f;        // largest |value| of the prepared Float32 pair (FP16 operand scale)
     uint64_t absmax_gen = ~0ULL;
@@ -68,7 +72,8 @@ struct ar_context {
 
     std::vector<DevBuf*> all_bufs() {
         return {&pref, &psec, &rmask, &smask, &okmask, &raw, &rawmask, &work_f32, &work_f64,
-                &flags, &geom, &pts, &out, &scratch, &grid_a, &grid_b, &refine, &box, &refx, &prec};
+                &flags, &geom, &pts, &out, &scratch, &grid_a, &grid_b, &refine, &box, &refx, &prec,
+                &mc};
     }
 };
 
@@ -96,6 +101,8 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
                         const ar_pass_hint* hint = nullptr);
 ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t padx,
                         int32_t** b1, int32_t** b2, int64_t* ld);
+// Page-locked host staging of at least n bytes (grow-only; the stream is idle when it grows).
+ar_status ar_pinned(ar_context* ctx, size_t n, void** out);
 // device bytes a context holds, and giving them all back (serving: idle slots make room)
 size_t ar_context_bytes(ar_context* ctx);
 void ar_context_release(ar_context* ctx);

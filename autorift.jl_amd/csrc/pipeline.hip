@@ -106,7 +106,93 @@ __device__ inline float masked(const float* a, const uint8_t* keep, int64_t k) {
     return keep[k] ? a[k] : qnan();
 }
 
-// Stage 1 (agreement counts, src/window.jl:594-619) on values masked by keep_in.
+__device__ inline void isort(float* b, int n) {
+    for (int k = 1; k < n; k++) {
+        float v = b[k];
+        int m = k - 1;
+        while (m >= 0 && b[m] > v) { b[m + 1] = b[m]; m--; }
+        b[m + 1] = v;
+    }
+}
+
+__device__ inline float smedian(const float* b, int n) {
+    return (n & 1) ? b[(n + 1) / 2 - 1] : (b[n / 2 - 1] + b[n / 2]) / 2;
+}
+
+#define MAXWIN 81   // window^2 for windows up to 9
+
+// ---- small windows (side <= WM, WM * WM <= 32) held in registers ------------------------------
+// The reference's stable insertion sort (src/window.jl:547-558) puts value i at the position
+// rank_i = #{j : v_j < v_i, or v_j == v_i and j before i}; selecting by that rank gives the same
+// element, bit for bit (equal floats differ only as -0 / +0, and the earlier one keeps its
+// place). All compares run unrolled on registers instead of a sort through private memory.
+
+// Gathers the window around (i, j) in the reference's column-major order into slot
+// (jj - j0) * WM + (ii - i0); ok marks the non-NaN (kept) values. Returns their count.
+template <int WM, bool KEEP>
+__device__ inline int gather_small(const float* a, const uint8_t* kin, int64_t i, int64_t j,
+                                   int64_t nr, int64_t nc, Win w, float (&v)[WM * WM],
+                                   uint32_t& ok) {
+    ok = 0;
+    const int64_t j0 = j - w.lx, i0 = i - w.ly;
+#pragma unroll
+    for (int cj = 0; cj < WM; cj++)
+#pragma unroll
+        for (int ci = 0; ci < WM; ci++) {
+            const int slot = cj * WM + ci;
+            const int64_t jj = j0 + cj, ii = i0 + ci;
+            float x = qnan();
+            if (cj <= w.lx + w.rx && ci <= w.ly + w.ry && jj >= 0 && jj < nc && ii >= 0 &&
+                ii < nr) {
+                const int64_t k = ii + jj * nr;
+                x = KEEP ? masked(a, kin, k) : a[k];
+            }
+            v[slot] = x;
+            if (!isnan(x)) ok |= 1u << slot;
+        }
+    return __builtin_popcount(ok);
+}
+
+// The median of the kept values in stable-sort order (src/window.jl:561-564): the value at
+// position (n - 1) / 2, and for even n the Float32 mean of positions n / 2 - 1 and n / 2.
+template <int WM>
+__device__ inline float median_small(const float (&v)[WM * WM], uint32_t ok, int n) {
+    const int k1 = (n - 1) / 2, k2 = n / 2;
+    float a = qnan(), b = qnan();
+#pragma unroll
+    for (int q = 0; q < WM * WM; q++) {
+        int r = 0;
+#pragma unroll
+        for (int t = 0; t < WM * WM; t++) {
+            const bool before = v[t] < v[q] || (t < q && v[t] == v[q]);
+            r += (((ok >> t) & 1u) && before) ? 1 : 0;
+        }
+        const bool live = (ok >> q) & 1u;
+        if (live && r == k1) a = v[q];
+        if (live && r == k2) b = v[q];
+    }
+    return (n & 1) ? a : (a + b) / 2;
+}
+
+// _medmad_of! (src/window.jl:535-545) on a small window; false when every value is NaN.
+template <int WM>
+__device__ inline bool medmad_small(const float* a, const uint8_t* kin, int64_t i, int64_t j,
+                                    int64_t nr, int64_t nc, Win w, float& med, float& mad) {
+    float v[WM * WM];
+    uint32_t ok;
+    const int n = gather_small<WM, true>(a, kin, i, j, nr, nc, w, v, ok);
+    if (n == 0) return false;
+    const float m = median_small<WM>(v, ok, n);
+#pragma unroll
+    for (int q = 0; q < WM * WM; q++) v[q] = fabsf(v[q] - m);
+    med = m;
+    mad = median_small<WM>(v, ok, n);
+    return true;
+}
+
+// Stage 1 (agreement counts, src/window.jl:594-619) on values masked by keep_in. WM > 0: windows
+// of side at most WM gathered into registers; WM == 0: any window.
+template <int WM>
 __global__ void gardner_stage1_kernel(const float* nx, const float* ny, const uint8_t* kin,
                                       uint8_t* kout, int64_t nr, int64_t nc, Win w, float tol,
                                       float needed) {
@@ -115,6 +201,27 @@ __global__ void gardner_stage1_kernel(const float* nx, const float* ny, const ui
     int64_t j = k / nr, i = k - j * nr;
     float cx = masked(nx, kin, k), cy = masked(ny, kin, k);
     float ax = 0.0f, ay = 0.0f;
+    if (WM > 0) {
+        constexpr int S = WM > 0 ? WM : 1;
+        float v[S * S];
+        uint32_t ok;
+        if (!isnan(cx)) {
+            gather_small<S, true>(nx, kin, i, j, nr, nc, w, v, ok);
+            int c = 0;
+#pragma unroll
+            for (int q = 0; q < S * S; q++) c += (((ok >> q) & 1u) && fabsf(v[q] - cx) < tol) ? 1 : 0;
+            ax = (float)c;
+        }
+        if (!isnan(cy)) {
+            gather_small<S, true>(ny, kin, i, j, nr, nc, w, v, ok);
+            int c = 0;
+#pragma unroll
+            for (int q = 0; q < S * S; q++) c += (((ok >> q) & 1u) && fabsf(v[q] - cy) < tol) ? 1 : 0;
+            ay = (float)c;
+        }
+        kout[k] = (ax >= needed && ay >= needed) ? 1 : 0;
+        return;
+    }
     int64_t j0 = j - w.lx > 0 ? j - w.lx : 0, j1 = j + w.rx < nc - 1 ? j + w.rx : nc - 1;
     int64_t i0 = i - w.ly > 0 ? i - w.ly : 0, i1 = i + w.ry < nr - 1 ? i + w.ry : nr - 1;
     if (!isnan(cx)) {
@@ -138,20 +245,115 @@ __global__ void gardner_stage1_kernel(const float* nx, const float* ny, const ui
     kout[k] = (ax >= needed && ay >= needed) ? 1 : 0;
 }
 
-__device__ inline void isort(float* b, int n) {
-    for (int k = 1; k < n; k++) {
-        float v = b[k];
-        int m = k - 1;
-        while (m >= 0 && b[m] > v) { b[m + 1] = b[m]; m--; }
-        b[m + 1] = v;
+// ---- 5 x 5 windows, one lane per window slot --------------------------------------------------
+// Half a wave (32 lanes, 25 used) per grid point: lane s holds slot s = (jj - j0) * 5 + (ii - i0)
+// of the window in the reference's column-major order, NaN where the window is clamped or the
+// value is masked. Ranks follow the stable insertion sort as above (slot t before s wins a tie).
+#define W5_PTS 8   // grid points per 256-thread workgroup
+
+struct Slot5 {
+    int64_t k;       // the point's linear index (or -1 past the end)
+    int64_t kk;      // this lane's window element (or -1)
+    int s, half;     // slot 0..31 (25..31 unused), half-wave
+};
+
+__device__ inline Slot5 slot5(int64_t n, int64_t nr, int64_t nc, Win w) {
+    Slot5 q;
+    const int lane = threadIdx.x & 63;
+    q.half = lane >> 5;
+    q.s = lane & 31;
+    const int64_t k = (int64_t)blockIdx.x * W5_PTS + (threadIdx.x >> 5);
+    q.k = k < n ? k : -1;
+    q.kk = -1;
+    if (q.k >= 0 && q.s < 25) {
+        const int64_t j = k / nr, i = k - j * nr;
+        const int cj = q.s / 5, ci = q.s - 5 * cj;
+        const int64_t jj = j - w.lx + cj, ii = i - w.ly + ci;
+        if (cj <= w.lx + w.rx && ci <= w.ly + w.ry && jj >= 0 && jj < nc && ii >= 0 && ii < nr)
+            q.kk = ii + jj * nr;
     }
+    return q;
 }
 
-__device__ inline float smedian(const float* b, int n) {
-    return (n & 1) ? b[(n + 1) / 2 - 1] : (b[n / 2 - 1] + b[n / 2]) / 2;
+// value of slot t of this lane's half-wave
+__device__ inline float slot_val(float v, int t, int half) {
+    const float a = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), t));
+    const float b = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32 + t));
+    return half ? b : a;
 }
 
-#define MAXWIN 81   // window^2 for windows up to 9
+// stable-sort rank of this lane's value among the non-NaN values of its half-wave
+__device__ inline int rank5(float v, const Slot5& q) {
+    int r = 0;
+#pragma unroll
+    for (int t = 0; t < 25; t++) {
+        const float x = slot_val(v, t, q.half);
+        r += (t < q.s ? x <= v : x < v) ? 1 : 0;
+    }
+    return r;
+}
+
+// the value at sorted positions k1 and k2 of the half-wave (every lane gets them)
+__device__ inline float median5(float v, const Slot5& q, int n) {
+    const int r = rank5(v, q);
+    const bool live = !isnan(v);
+    const int k1 = (n - 1) / 2, k2 = n / 2;
+    const uint64_t b1 = __ballot(live && r == k1), b2 = __ballot(live && r == k2);
+    const uint32_t h1 = (uint32_t)(b1 >> (32 * q.half)), h2 = (uint32_t)(b2 >> (32 * q.half));
+    const float a = __shfl(v, (q.half << 5) + (__ffs(h1) - 1));
+    const float b = __shfl(v, (q.half << 5) + (__ffs(h2) - 1));
+    return (n & 1) ? a : (a + b) / 2;
+}
+
+// median and MAD of one plane's window; n = 0 leaves both NaN
+__device__ inline void medmad5(float v, const Slot5& q, float& med, float& mad) {
+    const uint64_t bl = __ballot(!isnan(v));
+    const int n = __popc((uint32_t)(bl >> (32 * q.half)));
+    med = qnan();
+    mad = qnan();
+    if (n == 0) return;
+    const float m = median5(v, q, n);
+    med = m;
+    mad = median5(fabsf(v - m), q, n);
+}
+
+// Stage 1 on 5 x 5 windows: the agreement count is a ballot.
+__global__ __launch_bounds__(256) void gardner_stage1_w5_kernel(const float* nx, const float* ny,
+                                                                const uint8_t* kin, uint8_t* kout,
+                                                                int64_t nr, int64_t nc, Win w,
+                                                                float tol, float needed) {
+    const Slot5 q = slot5(nr * nc, nr, nc, w);
+    const float vx = q.kk >= 0 ? masked(nx, kin, q.kk) : qnan();
+    const float vy = q.kk >= 0 ? masked(ny, kin, q.kk) : qnan();
+    const float cx = q.k >= 0 ? masked(nx, kin, q.k) : qnan();
+    const float cy = q.k >= 0 ? masked(ny, kin, q.k) : qnan();
+    const uint64_t bx = __ballot(fabsf(vx - cx) < tol), by = __ballot(fabsf(vy - cy) < tol);
+    const float ax = isnan(cx) ? 0.0f : (float)__popc((uint32_t)(bx >> (32 * q.half)));
+    const float ay = isnan(cy) ? 0.0f : (float)__popc((uint32_t)(by >> (32 * q.half)));
+    if (q.k >= 0 && q.s == 0) kout[q.k] = (ax >= needed && ay >= needed) ? 1 : 0;
+}
+
+// Stage 2 on 5 x 5 windows (src/outliers.jl:140-157).
+__global__ __launch_bounds__(256) void gardner_stage2_w5_kernel(
+    const float* nx, const float* ny, const uint8_t* kin, uint8_t* kout, const int64_t* rxa,
+    const int64_t* rya, int64_t nr, int64_t nc, Win w, double scale, int64_t up) {
+    const Slot5 q = slot5(nr * nc, nr, nc, w);
+    const float vx = q.kk >= 0 ? masked(nx, kin, q.kk) : qnan();
+    const float vy = q.kk >= 0 ? masked(ny, kin, q.kk) : qnan();
+    float medx, madx, medy, mady;
+    medmad5(vx, q, medx, madx);
+    medmad5(vy, q, medy, mady);
+    if (q.k < 0 || q.s != 0) return;
+    const int64_t k = q.k;
+    if (!kin[k]) { kout[k] = 0; return; }
+    const int64_t rx = rxa[k], ry = rya[k];
+    double floor_x = rx > 0 ? 2.0 / (double)(up * rx) : INFINITY;
+    double floor_y = ry > 0 ? 2.0 / (double)(up * ry) : INFINITY;
+    double tx = scale * (double)madx, ty = scale * (double)mady;
+    double tolx = (isnan(tx) || isnan(floor_x)) ? NAN : (tx < floor_x ? floor_x : tx);
+    double toly = (isnan(ty) || isnan(floor_y)) ? NAN : (ty < floor_y ? floor_y : ty);
+    kout[k] = ((double)fabsf(nx[k] - medx) <= tolx && (double)fabsf(ny[k] - medy) <= toly) ? 1 : 0;
+}
 
 // median and MAD of the non-NaN masked values around (i, j) (src/window.jl:513-563)
 __device__ inline bool medmad(const float* a, const uint8_t* kin, int64_t i, int64_t j,
@@ -177,6 +379,8 @@ __device__ inline bool medmad(const float* a, const uint8_t* kin, int64_t i, int
 }
 
 // Stage 2 (median-deviation test with the quantisation floor), src/outliers.jl:140-157
+// WM > 0: windows of side at most WM in registers; WM == 0: any window up to MAXWIN values.
+template <int WM>
 __global__ void gardner_stage2_kernel(const float* nx, const float* ny, const uint8_t* kin,
                                       uint8_t* kout, const int64_t* rxa, const int64_t* rya,
                                       int64_t nr, int64_t nc, Win w, double scale, int64_t up) {
@@ -185,8 +389,14 @@ __global__ void gardner_stage2_kernel(const float* nx, const float* ny, const ui
     if (!kin[k]) { kout[k] = 0; return; }
     int64_t j = k / nr, i = k - j * nr;
     float medx = qnan(), madx = qnan(), medy = qnan(), mady = qnan();
-    medmad(nx, kin, i, j, nr, nc, w, medx, madx);
-    medmad(ny, kin, i, j, nr, nc, w, medy, mady);
+    if (WM > 0) {
+        constexpr int S = WM > 0 ? WM : 1;
+        medmad_small<S>(nx, kin, i, j, nr, nc, w, medx, madx);
+        medmad_small<S>(ny, kin, i, j, nr, nc, w, medy, mady);
+    } else {
+        medmad(nx, kin, i, j, nr, nc, w, medx, madx);
+        medmad(ny, kin, i, j, nr, nc, w, medy, mady);
+    }
     int64_t rx = rxa[k], ry = rya[k];
     double floor_x = rx > 0 ? 2.0 / (double)(up * rx) : INFINITY;
     double floor_y = ry > 0 ? 2.0 / (double)(up * ry) : INFINITY;
@@ -359,6 +569,10 @@ __global__ void merge_kernel(float* rdx, float* rdy, float* rcorr, uint16_t* rcs
     rdx[k] = dx[k]; rdy[k] = dy[k]; rcorr[k] = corr[k]; rcs[k] = cs; rit[k] = filled[k];
 }
 
+__global__ void zero_u64_kernel(unsigned long long* c, int k) {
+    if ((int)threadIdx.x < k) c[threadIdx.x] = 0ull;
+}
+
 __global__ void wanted_kernel(const uint16_t* rcs, int64_t n, uint8_t* w) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) w[k] = rcs[k] == 0 ? 1 : 0;
@@ -370,20 +584,17 @@ using namespace ar;
 
 namespace {
 
-struct Alloc {
-    ar_context* ctx;
-    std::vector<void*> ptrs;
-    hipError_t err = hipSuccess;
+// Carves the driver's planes out of one grow-only context buffer: a first pass (base == nullptr)
+// only adds up the sizes, the second hands out 256-byte-aligned pointers. No allocation per call.
+struct Carve {
+    char* base = nullptr;
+    size_t off = 0;
     template <typename T>
     T* get(int64_t n) {
-        void* p = nullptr;
-        hipError_t e = hipMalloc(&p, (size_t)(n > 0 ? n : 1) * sizeof(T));
-        if (e != hipSuccess) { err = e; return nullptr; }
-        ptrs.push_back(p);
-        return (T*)p;
-    }
-    ~Alloc() {
-        for (void* p : ptrs) (void)hipFree(p);
+        const size_t b = (((size_t)(n > 0 ? n : 1) * sizeof(T)) + 255) & ~(size_t)255;
+        T* p = base ? (T*)(base + off) : nullptr;
+        off += b;
+        return p;
     }
 };
 
@@ -402,26 +613,33 @@ static ar_status mfail(ar_context* ctx, ar_status st, const std::string& m) {
                          std::string(#expr) + ": " + hipGetErrorString(_e));                  \
     } while (0)
 
-// Counts of two flag arrays of n bytes with one readback.
-static ar_status count_flags2(ar_context* ctx, const uint8_t* f0, const uint8_t* f1, int64_t n,
-                              unsigned long long* cnt, unsigned long long* out) {
-    MCHK(hipMemsetAsync(cnt, 0, 16, ctx->stream));
-    count_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(f0, n, cnt);
-    count_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(f1, n, cnt + 1);
-    MCHK(hipGetLastError());
-    MCHK(hipMemcpyAsync(out, cnt, 16, hipMemcpyDeviceToHost, ctx->stream));
+// Readback of the first `words` counters into page-locked staging (rb), then into out.
+static ar_status read_counts(ar_context* ctx, const unsigned long long* cnt, int words,
+                             unsigned long long* rb, unsigned long long* out) {
+    MCHK(hipMemcpyAsync(rb, cnt, 8 * words, hipMemcpyDeviceToHost, ctx->stream));
     MCHK(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < words; k++) out[k] = rb[k];
     return AR_OK;
 }
 
+// Counts of two flag arrays of n bytes with one readback.
+static ar_status count_flags2(ar_context* ctx, const uint8_t* f0, const uint8_t* f1, int64_t n,
+                              unsigned long long* cnt, unsigned long long* rb,
+                              unsigned long long* out) {
+    zero_u64_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 2);
+    count_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(f0, n, cnt);
+    count_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(f1, n, cnt + 1);
+    MCHK(hipGetLastError());
+    return read_counts(ctx, cnt, 2, rb, out);
+}
+
 static ar_status count_flags(ar_context* ctx, const uint8_t* f, int64_t n,
-                             unsigned long long* cnt, unsigned long long* out) {
-    MCHK(hipMemsetAsync(cnt, 0, 8, ctx->stream));
+                             unsigned long long* cnt, unsigned long long* rb,
+                             unsigned long long* out) {
+    zero_u64_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 1);
     count_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(f, n, cnt);
     MCHK(hipGetLastError());
-    MCHK(hipMemcpyAsync(out, cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
-    MCHK(hipStreamSynchronize(ctx->stream));
-    return AR_OK;
+    return read_counts(ctx, cnt, 1, rb, out);
 }
 
 // reject_outliers with the Gardner filter (or the measured mask when outliers are off).
@@ -448,14 +666,22 @@ static ar_status reject(ar_context* ctx, const float* dx, const float* dy, const
     float tol = (float)p.agree_tolerance;
     uint8_t *kin = ka, *kout = kb;
     for (int64_t it = 0; it < iterations; it++) {
-        gardner_stage1_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(nxb, nyb, kin, kout, nr, nc, win,
-                                                                  tol, needed);
+        if (wx <= 5 && wy <= 5)
+            gardner_stage1_w5_kernel<<<(unsigned)((n + W5_PTS - 1) / W5_PTS), 256, 0, ctx->stream>>>(
+                nxb, nyb, kin, kout, nr, nc, win, tol, needed);
+        else
+            gardner_stage1_kernel<0><<<gblocks(n), GK, 0, ctx->stream>>>(nxb, nyb, kin, kout, nr,
+                                                                         nc, win, tol, needed);
         uint8_t* t = kin; kin = kout; kout = t;
     }
     int64_t n2 = iterations - 1 > 1 ? iterations - 1 : 1;
     for (int64_t it = 0; it < n2; it++) {
-        gardner_stage2_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(
-            nxb, nyb, kin, kout, rx, ry, nr, nc, win, p.mad_scale, p.upsampling);
+        if (wx <= 5 && wy <= 5)
+            gardner_stage2_w5_kernel<<<(unsigned)((n + W5_PTS - 1) / W5_PTS), 256, 0, ctx->stream>>>(
+                nxb, nyb, kin, kout, rx, ry, nr, nc, win, p.mad_scale, p.upsampling);
+        else
+            gardner_stage2_kernel<0><<<gblocks(n), GK, 0, ctx->stream>>>(
+                nxb, nyb, kin, kout, rx, ry, nr, nc, win, p.mad_scale, p.upsampling);
         uint8_t* t = kin; kin = kout; kout = t;
     }
     MCHK(hipGetLastError());
@@ -480,45 +706,70 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
     MCHK(hipSetDevice(ctx->device));
     const int64_t N = nr * nc;
     hipStream_t s = ctx->stream;
-    Alloc A{ctx};
-    double* gx = A.get<double>(N); double* gy = A.get<double>(N);
-    double* gpdx = A.get<double>(N); double* gpdy = A.get<double>(N);
-    int64_t* grx = A.get<int64_t>(N); int64_t* gry = A.get<int64_t>(N);
-    int64_t* lrx = A.get<int64_t>(N); int64_t* lry = A.get<int64_t>(N);
-    int64_t* lcx = A.get<int64_t>(N); int64_t* lcy = A.get<int64_t>(N);
-    float* rdx = A.get<float>(N); float* rdy = A.get<float>(N); float* rc = A.get<float>(N);
-    uint16_t* rcs = A.get<uint16_t>(N); uint8_t* rit = A.get<uint8_t>(N);
-    float* fdx = A.get<float>(N); float* fdy = A.get<float>(N); float* fc = A.get<float>(N);
-    uint8_t* fs = A.get<uint8_t>(N); uint8_t* meas = A.get<uint8_t>(N);
-    uint8_t* ka = A.get<uint8_t>(N); uint8_t* kb = A.get<uint8_t>(N);
-    float* nxb = A.get<float>(N); float* nyb = A.get<float>(N);
-    float* px = A.get<float>(N); float* py = A.get<float>(N);
-    uint8_t* pend = A.get<uint8_t>(N); uint8_t* filled = A.get<uint8_t>(N);
-    uint8_t* flags = A.get<uint8_t>(N);
-    unsigned long long* cnt = A.get<unsigned long long>(4);
-    // coarse grid buffers
+    // planes: the grid's six input fields and the five result planes each contiguous (one copy
+    // in, one copy out through page-locked staging), then the level and coarse-grid planes
     int64_t stride = p.coarse_stride;
     int64_t nrc = nr / stride, ncc = nc / stride, M = nrc * ncc;
-    double* cxp = A.get<double>(M); double* cyp = A.get<double>(M);
-    double* cpdx = A.get<double>(M); double* cpdy = A.get<double>(M);
-    int64_t* crx = A.get<int64_t>(M); int64_t* cry = A.get<int64_t>(M);
-    int64_t* ccx = A.get<int64_t>(M); int64_t* ccy = A.get<int64_t>(M);
-    float* cdx = A.get<float>(M); float* cdy = A.get<float>(M); float* ccr = A.get<float>(M);
-    uint8_t* cs8 = A.get<uint8_t>(M); uint8_t* cmeas = A.get<uint8_t>(M);
-    uint8_t* cka = A.get<uint8_t>(M); uint8_t* ckb = A.get<uint8_t>(M);
-    float* cnx = A.get<float>(M); float* cny = A.get<float>(M);
-    double* d2 = A.get<double>(M); int64_t* ev = A.get<int64_t>(M);
-    double* ez = A.get<double>(M + nrc); double* edd = A.get<double>(M);
-    uint8_t* grown = A.get<uint8_t>(M);
-    if (A.err != hipSuccess)
-        return mfail(ctx, AR_ERR_OUT_OF_MEMORY, std::string("multichip buffers: ") + hipGetErrorString(A.err));
-
-    MCHK(hipMemcpyAsync(gx, grid->x, 8 * N, hipMemcpyHostToDevice, s));
-    MCHK(hipMemcpyAsync(gy, grid->y, 8 * N, hipMemcpyHostToDevice, s));
-    MCHK(hipMemcpyAsync(gpdx, grid->dx_prior, 8 * N, hipMemcpyHostToDevice, s));
-    MCHK(hipMemcpyAsync(gpdy, grid->dy_prior, 8 * N, hipMemcpyHostToDevice, s));
-    MCHK(hipMemcpyAsync(grx, grid->radius_x, 8 * N, hipMemcpyHostToDevice, s));
-    MCHK(hipMemcpyAsync(gry, grid->radius_y, 8 * N, hipMemcpyHostToDevice, s));
+    double *gx, *gy, *gpdx, *gpdy;
+    int64_t *grx, *gry, *lrx, *lry, *lcx, *lcy;
+    float *rdx, *rdy, *rc, *fdx, *fdy, *fc, *nxb, *nyb, *px, *py;
+    uint16_t* rcs;
+    uint8_t *rit, *fs, *meas, *ka, *kb, *pend, *filled, *flags;
+    unsigned long long* cnt;
+    double *cxp, *cyp, *cpdx, *cpdy, *d2, *ez, *edd;
+    int64_t *crx, *cry, *ccx, *ccy, *ev;
+    float *cdx, *cdy, *ccr, *cnx, *cny;
+    uint8_t *cs8, *cmeas, *cka, *ckb, *grown;
+    char *gin, *res;
+    auto carve = [&](Carve& C) {
+        gin = C.get<char>(48 * N);
+        gx = (double*)gin; gy = gx + N; gpdx = gy + N; gpdy = gpdx + N;
+        grx = (int64_t*)(gpdy + N); gry = grx + N;
+        res = C.get<char>(15 * N);
+        rdx = (float*)res; rdy = rdx + N; rc = rdy + N;
+        rcs = (uint16_t*)(rc + N); rit = (uint8_t*)(rcs + N);
+        lrx = C.get<int64_t>(N); lry = C.get<int64_t>(N);
+        lcx = C.get<int64_t>(N); lcy = C.get<int64_t>(N);
+        fdx = C.get<float>(N); fdy = C.get<float>(N); fc = C.get<float>(N);
+        fs = C.get<uint8_t>(N); meas = C.get<uint8_t>(N);
+        ka = C.get<uint8_t>(N); kb = C.get<uint8_t>(N);
+        nxb = C.get<float>(N); nyb = C.get<float>(N);
+        px = C.get<float>(N); py = C.get<float>(N);
+        pend = C.get<uint8_t>(N); filled = C.get<uint8_t>(N);
+        flags = C.get<uint8_t>(N);
+        cnt = C.get<unsigned long long>(4);
+        cxp = C.get<double>(M); cyp = C.get<double>(M);
+        cpdx = C.get<double>(M); cpdy = C.get<double>(M);
+        crx = C.get<int64_t>(M); cry = C.get<int64_t>(M);
+        ccx = C.get<int64_t>(M); ccy = C.get<int64_t>(M);
+        cdx = C.get<float>(M); cdy = C.get<float>(M); ccr = C.get<float>(M);
+        cs8 = C.get<uint8_t>(M); cmeas = C.get<uint8_t>(M);
+        cka = C.get<uint8_t>(M); ckb = C.get<uint8_t>(M);
+        cnx = C.get<float>(M); cny = C.get<float>(M);
+        d2 = C.get<double>(M); ev = C.get<int64_t>(M);
+        ez = C.get<double>(M + nrc); edd = C.get<double>(M);
+        grown = C.get<uint8_t>(M);
+    };
+    ar_status st;
+    Carve dry;
+    carve(dry);
+    if ((st = ctx->mc.reserve(ctx, dry.off))) return st;
+    Carve C;
+    C.base = (char*)ctx->mc.ptr;
+    carve(C);
+    // page-locked staging: the grid fields on the way in, the results on the way out, and the
+    // count readbacks (after both)
+    void* pinv = nullptr;
+    const size_t rb_off = ((size_t)48 * N + 255) & ~(size_t)255;
+    if ((st = ar_pinned(ctx, rb_off + 256, &pinv))) return st;
+    char* pin = (char*)pinv;
+    unsigned long long* rb = (unsigned long long*)(pin + rb_off);
+    {
+        const void* src[6] = {grid->x, grid->y, grid->dx_prior, grid->dy_prior, grid->radius_x,
+                              grid->radius_y};
+        for (int f = 0; f < 6; f++) std::memcpy(pin + (size_t)f * 8 * N, src[f], 8 * (size_t)N);
+        MCHK(hipMemcpyAsync(gin, pin, 48 * (size_t)N, hipMemcpyHostToDevice, s));
+    }
     fill_result_kernel<<<gblocks(N), GK, 0, s>>>(rdx, rdy, rc, rcs, rit, N);
     MCHK(hipGetLastError());
 
@@ -548,7 +799,6 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
         }
     }
 
-    ar_status st;
     if (p.level_measure_count < 0 || p.level_measure_count > 8)
         return mfail(ctx, AR_ERR_INVALID_ARGUMENT, "level_measure_count must be in 0..8");
     {
@@ -580,7 +830,7 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
         wanted_kernel<<<gblocks(N), GK, 0, s>>>(rcs, N, pend);
         level_points_kernel<<<gblocks(N), GK, 0, s>>>(grx, gry, rcs, N, csx, csy,
                                                       p.min_search_radius, lrx, lry, lcx, lcy, flags);
-        if ((st = count_flags2(ctx, pend, flags, N, cnt, c2))) return st;
+        if ((st = count_flags2(ctx, pend, flags, N, cnt, rb, c2))) return st;
         if (c2[0] == 0) break;
         if (c2[1] == 0) continue;
         unsigned long long c = 0;
@@ -614,7 +864,7 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
             ar_points cp = {cxp, cyp, crx, cry, cpdx, cpdy, ccx, ccy};
             if ((st = ar_track_pass(ctx, cp, M, 1, measure, cdx, cdy, ccr, cs8, hp))) return st;
             measured_kernel<<<gblocks(M), GK, 0, s>>>(cdx, M, cmeas);
-            if ((st = count_flags(ctx, cmeas, M, cnt, &c))) return st;
+            if ((st = count_flags(ctx, cmeas, M, cnt, rb, &c))) return st;
             if (c == 0) continue;
             uint8_t* keep = nullptr;
             int64_t iters = p.gardner_iterations - 1 > 1 ? p.gardner_iterations - 1 : 1;
@@ -622,12 +872,11 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
                              &keep)))
                 return st;
             and_kernel<<<gblocks(M), GK, 0, s>>>(keep, cmeas, M);
-            MCHK(hipMemsetAsync(cnt, 0, 16, s));
+            zero_u64_kernel<<<1, 64, 0, s>>>(cnt, 2);
             fraction_kernel<<<gblocks(M), GK, 0, s>>>(crx, cmeas, keep, M, cnt);
             unsigned long long fr[2];
             MCHK(hipGetLastError());
-            MCHK(hipMemcpyAsync(fr, cnt, 16, hipMemcpyDeviceToHost, s));
-            MCHK(hipStreamSynchronize(s));
+            if ((st = read_counts(ctx, cnt, 2, rb, fr))) return st;
             if (fr[0] == 0) continue;
             if ((double)fr[1] / (double)fr[0] < p.min_coarse_valid_fraction) continue;
             double r2 = (double)p.coarse_buffer * (double)p.coarse_buffer;
@@ -635,7 +884,7 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
             edt_row_dilate_kernel<<<gblocks(nrc), GK, 0, s>>>(d2, nrc, ncc, r2, ev, ez, edd, grown);
             nearest_gate_kernel<<<gblocks(N), GK, 0, s>>>(grown, nrc, ncc, nr, nc, lrx, lry, flags);
             MCHK(hipGetLastError());
-            if ((st = count_flags(ctx, flags, N, cnt, &c))) return st;
+            if ((st = count_flags(ctx, flags, N, cnt, rb, &c))) return st;
             if (c == 0) continue;
         }
 
@@ -663,11 +912,12 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
                                                (uint16_t)cs, N);
         MCHK(hipGetLastError());
     }
-    MCHK(hipMemcpyAsync(dx, rdx, 4 * N, hipMemcpyDeviceToHost, s));
-    MCHK(hipMemcpyAsync(dy, rdy, 4 * N, hipMemcpyDeviceToHost, s));
-    MCHK(hipMemcpyAsync(corr, rc, 4 * N, hipMemcpyDeviceToHost, s));
-    MCHK(hipMemcpyAsync(chip_size, rcs, 2 * N, hipMemcpyDeviceToHost, s));
-    MCHK(hipMemcpyAsync(interpolated, rit, N, hipMemcpyDeviceToHost, s));
+    MCHK(hipMemcpyAsync(pin, res, 15 * (size_t)N, hipMemcpyDeviceToHost, s));
     MCHK(hipStreamSynchronize(s));
+    std::memcpy(dx, pin, 4 * (size_t)N);
+    std::memcpy(dy, pin + 4 * (size_t)N, 4 * (size_t)N);
+    std::memcpy(corr, pin + 8 * (size_t)N, 4 * (size_t)N);
+    std::memcpy(chip_size, pin + 12 * (size_t)N, 2 * (size_t)N);
+    std::memcpy(interpolated, pin + 14 * (size_t)N, (size_t)N);
     return AR_OK;
 }

@@ -10,9 +10,11 @@
 // a pixel only if (v - lo) * scale lies within ~1e-12 of a rounding boundary.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/autorift_hip.h"
@@ -33,11 +35,142 @@ __device__ inline double boxval(const T* img, const uint8_t* mask, int64_t k) {
     return as_f64(img, k);
 }
 
+// ---- exact-sum certificate -------------------------------------------------------------------
+// The reference's box mean is a chain of Float64 running sums (down each column of a band, then
+// along each row). When every value that enters them is a multiple of 2^kmin and below 2^(emax+1)
+// in magnitude, every partial sum of at most (wx + 1)(wy + 1) of them is a multiple of 2^kmin
+// below 2^(lg + emax + 1), lg = ceil(log2((wx + 1)(wy + 1))); if that is at most 2^(kmin + 53),
+// each partial sum is exactly representable, no addition rounds, and any summation order gives
+// the reference's values. stats = {kmin, emax, non-finite seen} over the finite nonzero summed
+// values; with no such value every sum is 0.
+struct Cert {
+    const int* stats;   // nullptr: no certificate (the sequential kernels always run)
+    int wx, wy;
+};
+
+__device__ inline bool cert_holds(const Cert& c) {
+    if (!c.stats) return false;
+    if (c.stats[2]) return false;
+    const int kmin = c.stats[0], emax = c.stats[1];
+    if (kmin == INT_MAX) return true;
+    const int64_t cnt = (int64_t)(c.wx + 1) * (int64_t)(c.wy + 1);
+    int lg = 0;
+    while (((int64_t)1 << lg) < cnt) lg++;
+    return lg + emax + 1 <= kmin + 53;
+}
+
+__global__ void stats_init_kernel(int* stats) {
+    if (threadIdx.x == 0) {
+        stats[0] = INT_MAX;
+        stats[1] = INT_MIN;
+        stats[2] = 0;
+        stats[3] = 0;   // some pixel's mask is false
+    }
+}
+
+// kmin (exponent of the lowest set bit) and emax (floor(log2|v|)) of the values the masked box
+// mean sums: Float32(img) where the mask is set; NaN is skipped by the sums, infinities fail the
+// certificate.
+template <typename T>
+__global__ __launch_bounds__(256) void exact_stats_kernel(const T* img, const uint8_t* mask,
+                                                          int64_t n, int* stats) {
+    int kmin = INT_MAX, emax = INT_MIN, bad = 0, gap = 0;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        if (!mask[k]) { gap = 1; continue; }
+        const double v = (double)as_f32(img, k);
+        if (isnan(v) || v == 0.0) continue;
+        if (isinf(v)) { bad = 1; continue; }
+        const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+        const int E = (int)((b >> 52) & 0x7ff);
+        const unsigned long long S = (b & 0xfffffffffffffull) | (1ull << 52);
+        const int e = E - 1023;
+        const int lo = e - 52 + __builtin_ctzll(S);
+        kmin = lo < kmin ? lo : kmin;
+        emax = e > emax ? e : emax;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const int a = __shfl_xor(kmin, o), b2 = __shfl_xor(emax, o), c = __shfl_xor(bad, o);
+        const int g = __shfl_xor(gap, o);
+        kmin = a < kmin ? a : kmin;
+        emax = b2 > emax ? b2 : emax;
+        bad |= c;
+        gap |= g;
+    }
+    // one set of atomics per workgroup
+    __shared__ int red[4][4];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][wv] = kmin;
+        red[1][wv] = emax;
+        red[2][wv] = bad;
+        red[3][wv] = gap;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
+            kmin = red[0][q] < kmin ? red[0][q] : kmin;
+            emax = red[1][q] > emax ? red[1][q] : emax;
+            bad |= red[2][q];
+            gap |= red[3][q];
+        }
+        if (kmin != INT_MAX) atomicMin(stats + 0, kmin);
+        if (emax != INT_MIN) atomicMax(stats + 1, emax);
+        if (bad) atomicOr(stats + 2, 1);
+        if (gap) atomicOr(stats + 3, 1);
+    }
+}
+
+// Certified box mean, column sums: thread per pixel, the NaN-aware sum and count of the valid
+// Float32 values in its column window (the masked form of src/window.jl:299-325; under the
+// certificate equal to the banded running sums).
+template <typename T>
+__global__ __launch_bounds__(256) void box_col_direct_kernel(const T* img, const uint8_t* mask,
+                                                             int64_t nr, int64_t nc, int ly,
+                                                             int ry, Cert cert, double* colsum,
+                                                             int32_t* colcnt) {
+    if (!cert_holds(cert)) return;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nr * nc) return;
+    const int64_t j = k / nr, i = k - j * nr;
+    const int64_t a0 = i - ly > 0 ? i - ly : 0, a1 = i + ry < nr - 1 ? i + ry : nr - 1;
+    double s = 0.0;
+    int32_t c = 0;
+    for (int64_t ii = a0; ii <= a1; ii++) {
+        const double v = boxval<T, true>(img, mask, ii + j * nr);
+        if (!isnan(v)) { s += v; c += 1; }
+    }
+    colsum[k] = s;
+    colcnt[k] = c;
+}
+
+// Certified box mean, row sums fused with highpass (src/preprocess.jl:304-312): thread per pixel.
+template <typename T>
+__global__ __launch_bounds__(256) void box_row_direct_highpass_kernel(
+    const T* img, const uint8_t* mask, int64_t nr, int64_t nc, int lx, int rx, Cert cert,
+    const double* colsum, const int32_t* colcnt, float* out) {
+    if (!cert_holds(cert)) return;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nr * nc) return;
+    const int64_t j = k / nr, i = k - j * nr;
+    const int64_t b0 = j - lx > 0 ? j - lx : 0, b1 = j + rx < nc - 1 ? j + rx : nc - 1;
+    double s = 0.0;
+    int32_t c = 0;
+    for (int64_t jj = b0; jj <= b1; jj++) {
+        s += colsum[i + jj * nr];
+        c += colcnt[i + jj * nr];
+    }
+    const float mean = c > 0 ? (float)(s / (double)c) : NAN;
+    out[k] = (mask[k] && isfinite(mean)) ? as_f32(img, k) - mean : NAN;
+}
+
 // Column pass of one 16-row band (src/window.jl:299-325 masked, 397-409 dense).
 // Thread per (band, column). colsum/colcnt hold, for every row i, the band's running state at i.
 template <typename T, bool MASKED, bool NANAWARE>
 __global__ void box_colpass_kernel(const T* img, const uint8_t* mask, int64_t nr, int64_t nc,
-                                   int ly, int ry, double* colsum, int32_t* colcnt) {
+                                   int ly, int ry, double* colsum, int32_t* colcnt,
+                                   Cert cert = Cert{nullptr, 0, 0}) {
+    if (cert_holds(cert)) return;   // the certified direct kernels ran instead
     int64_t nb = nr < 16 ? nr : 16;
     int64_t nbands = (nr + nb - 1) / nb;
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -91,7 +224,8 @@ template <typename T, bool NANAWARE>
 __global__ void box_rowpass_highpass_kernel(const T* img, const uint8_t* mask, int64_t nr,
                                             int64_t nc, int lx, int rx, int ly, int ry,
                                             const double* colsum, const int32_t* colcnt,
-                                            float* out) {
+                                            float* out, Cert cert = Cert{nullptr, 0, 0}) {
+    if (cert_holds(cert)) return;   // the certified direct kernels ran instead
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // 0-based row
     if (i >= nr) return;
     int64_t I = i + 1;
@@ -244,9 +378,11 @@ __global__ void box_rowpass_wallis_out_kernel(const T* img, const uint8_t* mask,
 // every in-bounds pixel of its (clamped) w-window is valid; non-finite filtered values become 0
 // and invalid.
 __global__ void erode_finalize_kernel(float* out, const uint8_t* mask, uint8_t* vmask, int64_t nr,
-                                      int64_t nc, int lx, int rx, int ly, int ry, int allmask) {
+                                      int64_t nc, int lx, int rx, int ly, int ry, int allmask,
+                                      const int* stats = nullptr) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nr * nc) return;
+    if (stats) allmask = stats[3] == 0;   // every mask bit set (the device's own statistics)
     int64_t j = k / nr, i = k - j * nr;
     uint8_t v = mask[k];
     if (!allmask) {
@@ -287,9 +423,19 @@ __global__ void finite_mask_t_kernel(const T* img, uint8_t* out, int64_t n) {
 // Quantiser statistics, deterministic two-level reduction: per-block partials over a fixed
 // block-cyclic assignment, then one block combining them in index order.
 #define QBLOCKS 1024
+// The quantiser's parameters, formed on the device (no host round trip): mean, Bessel standard
+// deviation, lo = mean - 3 sd, scale = 255 / (6 sd), and the mode: 0 nothing valid, 1 sd == 0,
+// 2 map (src/preprocess.jl:546-592).
+struct QParams {
+    double mean, lo, scale;
+    long long n;
+    int mode;
+};
+
 template <typename T>
 __global__ void q_partial_kernel(const T* img, const uint8_t* mask, int64_t n, int pass,
-                                 double mean, double* part_s, long long* part_n) {
+                                 const QParams* qp, double* part_s, long long* part_n) {
+    const double mean = pass == 1 ? qp->mean : 0.0;
     __shared__ double ss[256];
     __shared__ long long sn[256];
     double s = 0.0;
@@ -320,7 +466,7 @@ __global__ void q_partial_kernel(const T* img, const uint8_t* mask, int64_t n, i
 }
 
 __global__ void q_final_kernel(const double* part_s, const long long* part_n, int nparts,
-                               double* out_s, long long* out_n) {
+                               int pass, QParams* qp) {
     __shared__ double ss[256];
     __shared__ long long sn[256];
     double s = 0.0;
@@ -336,13 +482,26 @@ __global__ void q_final_kernel(const double* part_s, const long long* part_n, in
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) { *out_s = ss[0]; *out_n = sn[0]; }
+    if (threadIdx.x == 0) {
+        if (pass == 0) {
+            qp->n = sn[0];
+            qp->mean = ss[0] / (double)sn[0];
+        } else {
+            const long long cnt = qp->n;
+            const double sd = cnt > 1 ? sqrt(ss[0] / (double)(cnt - 1)) : 0.0;
+            qp->mode = cnt == 0 ? 0 : (sd == 0 ? 1 : 2);
+            qp->lo = qp->mean - 3 * sd;
+            qp->scale = 255.0 / (6 * sd);
+        }
+    }
 }
 
 // round(UInt8, clamp((v - lo) * scale, 0, 255)), ties to even; invalid -> 0; flat -> 0x80.
 template <typename T>
-__global__ void q_map_kernel(const T* img, const uint8_t* mask, int64_t n, int mode, double lo,
-                             double scale, uint8_t* out) {
+__global__ void q_map_kernel(const T* img, const uint8_t* mask, int64_t n, const QParams* qp,
+                             uint8_t* out) {
+    const int mode = qp->mode;
+    const double lo = qp->lo, scale = qp->scale;
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         if (mode == 0) { out[k] = 0; continue; }                       // nothing valid
@@ -397,39 +556,15 @@ template <typename T>
 static ar_status quantize_u8(ar_context* ctx, const T* img, const uint8_t* mask, int64_t N,
                              uint8_t* out) {
     ar_status st;
-    if ((st = ctx->grid_a.reserve(ctx, QBLOCKS * 16 + 64))) return st;
+    if ((st = ctx->grid_a.reserve(ctx, QBLOCKS * 16 + 256))) return st;
     double* ps = (double*)ctx->grid_a.ptr;
     long long* pn = (long long*)(ps + QBLOCKS);
-    double* rs = (double*)(pn + QBLOCKS);
-    long long* rn = (long long*)(rs + 1);
-    q_partial_kernel<T><<<QBLOCKS, 256, 0, ctx->stream>>>(img, mask, N, 0, 0.0, ps, pn);
-    q_final_kernel<<<1, 256, 0, ctx->stream>>>(ps, pn, QBLOCKS, rs, rn);
-    double s;
-    long long n;
-    PCHK(hipGetLastError());
-    PCHK(hipMemcpyAsync(&s, rs, 8, hipMemcpyDeviceToHost, ctx->stream));
-    PCHK(hipMemcpyAsync(&n, rn, 8, hipMemcpyDeviceToHost, ctx->stream));
-    PCHK(hipStreamSynchronize(ctx->stream));
-    int mode = 2;
-    double lo = 0, scale = 0;
-    if (n == 0) {
-        mode = 0;
-    } else {
-        double mean = s / (double)n;
-        q_partial_kernel<T><<<QBLOCKS, 256, 0, ctx->stream>>>(img, mask, N, 1, mean, ps, pn);
-        q_final_kernel<<<1, 256, 0, ctx->stream>>>(ps, pn, QBLOCKS, rs, rn);
-        double ss;
-        PCHK(hipGetLastError());
-        PCHK(hipMemcpyAsync(&ss, rs, 8, hipMemcpyDeviceToHost, ctx->stream));
-        PCHK(hipStreamSynchronize(ctx->stream));
-        double sd = n > 1 ? std::sqrt(ss / (double)(n - 1)) : 0.0;
-        if (sd == 0) mode = 1;
-        else {
-            lo = mean - 3 * sd;
-            scale = 255.0 / (6 * sd);
-        }
+    QParams* qp = (QParams*)(pn + QBLOCKS);
+    for (int pass = 0; pass < 2; pass++) {
+        q_partial_kernel<T><<<QBLOCKS, 256, 0, ctx->stream>>>(img, mask, N, pass, qp, ps, pn);
+        q_final_kernel<<<1, 256, 0, ctx->stream>>>(ps, pn, QBLOCKS, pass, qp);
     }
-    q_map_kernel<T><<<gsz(N), 256, 0, ctx->stream>>>(img, mask, N, mode, lo, scale, out);
+    q_map_kernel<T><<<gsz(N), 256, 0, ctx->stream>>>(img, mask, N, qp, out);
     PCHK(hipGetLastError());
     return AR_OK;
 }
@@ -451,15 +586,24 @@ static ar_status prepare_one(ar_context* ctx, const T* raw, const uint8_t* mask,
         if (wallis && !(p.min_std >= 0.0))
             return pfail(ctx, AR_ERR_INVALID_ARGUMENT, "Wallis `min_std` must be >= 0");
         if ((st = ctx->flags.reserve(ctx, 64))) return st;
-        int f0[2] = {1, 0};
-        PCHK(hipMemcpyAsync(ctx->flags.ptr, f0, 8, hipMemcpyHostToDevice, ctx->stream));
-        mask_stats_kernel<T><<<gsz(N) > 4096 ? 4096 : gsz(N), 256, 0, ctx->stream>>>(
-            raw, mask, N, (int*)ctx->flags.ptr);
-        int f[2];
-        PCHK(hipGetLastError());
-        PCHK(hipMemcpyAsync(f, ctx->flags.ptr, 8, hipMemcpyDeviceToHost, ctx->stream));
-        PCHK(hipStreamSynchronize(ctx->stream));
-        bool allmask = f[0] != 0, anynan = f[1] != 0;
+        // Highpass of a non-Float64 image needs no host decision: Float32(x) == x for every such
+        // element, so the masked NaN-aware box mean equals the reference's dense and unmasked
+        // forms wherever those apply, and the erosion of an all-valid mask is the mask itself.
+        // Float64 images (the masked copy rounds them) and Wallis read the mask statistics back.
+        const bool certified = !wallis && !std::is_same<T, double>::value;
+        bool allmask = false, anynan = true;
+        if (!certified) {
+            int f0[2] = {1, 0};
+            PCHK(hipMemcpyAsync(ctx->flags.ptr, f0, 8, hipMemcpyHostToDevice, ctx->stream));
+            mask_stats_kernel<T><<<gsz(N) > 4096 ? 4096 : gsz(N), 256, 0, ctx->stream>>>(
+                raw, mask, N, (int*)ctx->flags.ptr);
+            int f[2];
+            PCHK(hipGetLastError());
+            PCHK(hipMemcpyAsync(f, ctx->flags.ptr, 8, hipMemcpyDeviceToHost, ctx->stream));
+            PCHK(hipStreamSynchronize(ctx->stream));
+            allmask = f[0] != 0;
+            anynan = f[1] != 0;
+        }
         int wx = (int)(w < nx ? w : nx), wy = (int)(w < ny ? w : ny);
         int lx = wx / 2, ly = wy / 2, rx = wx - 1 - lx, ry = wy - 1 - ly;
         if ((st = ctx->work_f64.reserve(ctx, N * 8 + N * 4))) return st;
@@ -487,6 +631,21 @@ static ar_status prepare_one(ar_context* ctx, const T* raw, const uint8_t* mask,
             }
             box_colpass_kernel<float, false, true><<<gb, 256, 0, ctx->stream>>>(dev2, mask, ny, nx, ly, ry, colsum, colcnt);
             box_rowpass_wallis_out_kernel<T><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, colsum, colcnt, mean, (float)p.min_std, hp);
+        } else if (certified) {
+            // exact-sum certificate on the device; the direct kernels run when it holds, the
+            // reference's sequential running sums otherwise (each kernel checks it and exits)
+            int* stats = (int*)ctx->flags.ptr;
+            stats_init_kernel<<<1, 64, 0, ctx->stream>>>(stats);
+            // about 16 values per thread, at most 1024 workgroups
+            const int64_t sb = (N + 4095) / 4096;
+            exact_stats_kernel<T><<<(unsigned)(sb < 1 ? 1 : (sb > 1024 ? 1024 : sb)), 256, 0,
+                                    ctx->stream>>>(raw, mask, N, stats);
+            const Cert cert{stats, wx, wy};
+            const unsigned gp = (unsigned)((N + 255) / 256);
+            box_col_direct_kernel<T><<<gp, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, cert, colsum, colcnt);
+            box_colpass_kernel<T, true, true><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt, cert);
+            box_row_direct_highpass_kernel<T><<<gp, 256, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, cert, colsum, colcnt, hp);
+            box_rowpass_highpass_kernel<T, true><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, hp, cert);
         } else if (allmask && !anynan) {
             box_colpass_kernel<T, false, false><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
             box_rowpass_highpass_kernel<T, false><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, hp);
@@ -500,7 +659,8 @@ static ar_status prepare_one(ar_context* ctx, const T* raw, const uint8_t* mask,
         PCHK(hipGetLastError());
         // erosion uses the full filter width w (clamped to the image) like windowmin(f, w)
         erode_finalize_kernel<<<(unsigned)((N + 255) / 256), 256, 0, ctx->stream>>>(
-            hp, mask, dmask, ny, nx, lx, rx, ly, ry, allmask ? 1 : 0);
+            hp, mask, dmask, ny, nx, lx, rx, ly, ry, allmask ? 1 : 0,
+            certified ? (const int*)ctx->flags.ptr : nullptr);
         PCHK(hipGetLastError());
         curmask = dmask;
     } else {
@@ -625,7 +785,7 @@ extern "C" ar_status ar_set_images(ar_context* ctx, const void* ref, const void*
             default: st = prepare_slot<double>(ctx, img, m, ny, nx, dst, dm); break;
         }
         if (st) return st;
-        PCHK(hipStreamSynchronize(ctx->stream));
+        // (no wait: the next slot's upload into the same raw buffer is ordered on the stream)
     }
     ctx->ny = ny;
     ctx->nx = nx;

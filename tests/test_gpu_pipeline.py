@@ -43,10 +43,21 @@ def _images(kind, n=(96, 130), seed=0):
         m[:, :7] = False
         m[50:58, 60:90] = False
         return img, m
+    if kind == "f32_tiny":
+        # values 2^-40 apart from their neighbours' scale: the box sums round, so the
+        # preparation must take the reference's sequential running sums
+        img[rng.random(n) < 0.05] = np.float32(3e-12)
+        img[40:44, 20:30] *= np.float32(1 + 2 ** -20)
+        return img, None
+    if kind == "f32_inf":
+        img[30, 40] = np.inf
+        img[70, 100] = -np.inf
+        return img, None
     return img, None
 
 
-@pytest.mark.parametrize("kind", ["u8", "i16", "f32", "f64", "f32_nan", "f32_mask"])
+@pytest.mark.parametrize("kind", ["u8", "i16", "f32", "f64", "f32_nan", "f32_mask", "f32_tiny",
+                                  "f32_inf"])
 @pytest.mark.parametrize("quant", [True, False])
 def test_prepare_matches_oracle(kind, quant):
     img, mask = _images(kind)
