@@ -32,22 +32,28 @@ __global__ void fill_result_kernel(float* dx, float* dy, float* corr, uint16_t* 
     dx[k] = qnan(); dy[k] = qnan(); corr[k] = qnan(); cs[k] = 0; it[k] = 0;
 }
 
-// count of flags[k] != 0 into *out (atomic; counts are exact integers)
-__global__ void count_kernel(const uint8_t* f, int64_t n, unsigned long long* out) {
+// counts of f0[k] != 0 and f1[k] != 0 into out[0], out[1]
+__global__ void count2_kernel(const uint8_t* f0, const uint8_t* f1, int64_t n,
+                              unsigned long long* out) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int v = k < n && f[k] ? 1 : 0;
-    unsigned long long c = __syncthreads_count(v);
-    if (threadIdx.x == 0 && c) atomicAdd(out, c);
+    const unsigned long long a = __syncthreads_count(k < n && f0[k] ? 1 : 0);
+    const unsigned long long b = __syncthreads_count(k < n && f1[k] ? 1 : 0);
+    if (threadIdx.x == 0) {
+        if (a) atomicAdd(out, a);
+        if (b) atomicAdd(out + 1, b);
+    }
 }
 
 // _level_points + sanitize! (src/multichip.jl:193-213, src/points.jl:332-346)
+// Also writes wanted = chip_size == 0 (src/multichip.jl:118-136).
 __global__ void level_points_kernel(const int64_t* grx, const int64_t* gry, const uint16_t* rcs,
                                     int64_t n, int64_t csx, int64_t csy, int64_t minr,
                                     int64_t* rx, int64_t* ry, int64_t* cx, int64_t* cy,
-                                    uint8_t* srch) {
+                                    uint8_t* srch, uint8_t* want) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     bool wanted = rcs[k] == 0;
+    want[k] = wanted ? 1 : 0;
     int64_t a = wanted ? grx[k] : 0, b = wanted ? gry[k] : 0;
     if (a <= 0 || b <= 0) { a = 0; b = 0; }
     else { a = a < minr ? minr : a; b = b < minr ? minr : b; }
@@ -61,8 +67,10 @@ __global__ void coarse_points_kernel(const double* x, const double* y, const dou
                                      int64_t nr, int64_t nc, int64_t stride, int64_t nrc,
                                      int64_t ncc, int64_t csx, int64_t csy, double* ox, double* oy,
                                      double* opdx, double* opdy, int64_t* orx, int64_t* ory,
-                                     int64_t* ocx, int64_t* ocy, uint8_t* srch) {
+                                     int64_t* ocx, int64_t* ocy, uint8_t* srch,
+                                     unsigned long long* frac) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) { frac[0] = 0ull; frac[1] = 0ull; }   // the level's valid-fraction counters
     if (t >= nrc * ncc) return;
     int64_t jo = t / nrc, io = t - jo * nrc;
     int64_t i = (io + 1) * stride, j = (jo + 1) * stride;   // 1-based grid indices
@@ -88,14 +96,17 @@ __global__ void measured_kernel(const float* dx, int64_t n, uint8_t* m) {
 }
 
 // nx = Float32(dx / r) (Float32 arithmetic; NaN where r == 0), src/outliers.jl:96-102
+// The valid mask is the measured points (!isnan(dx)), also written to meas.
 __global__ void normalise_kernel(const float* dx, const float* dy, const int64_t* rx,
-                                 const int64_t* ry, const uint8_t* valid, int64_t n, float* nx,
+                                 const int64_t* ry, uint8_t* meas, int64_t n, float* nx,
                                  float* ny, uint8_t* keep) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
+    const uint8_t v = isnan(dx[k]) ? 0 : 1;
+    meas[k] = v;
     nx[k] = rx[k] > 0 ? dx[k] / (float)rx[k] : qnan();
     ny[k] = ry[k] > 0 ? dy[k] / (float)ry[k] : qnan();
-    keep[k] = valid[k] ? 1 : 0;
+    keep[k] = v;
 }
 
 struct Win {
@@ -406,17 +417,17 @@ __global__ void gardner_stage2_kernel(const float* nx, const float* ny, const ui
     kout[k] = ((double)fabsf(nx[k] - medx) <= tolx && (double)fabsf(ny[k] - medy) <= toly) ? 1 : 0;
 }
 
-__global__ void and_kernel(uint8_t* a, const uint8_t* b, int64_t n) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) a[k] = (a[k] && b[k]) ? 1 : 0;
-}
-
-// numer/denom of the valid-fraction gate (src/multichip.jl:265-276)
-__global__ void fraction_kernel(const int64_t* rx, const uint8_t* meas, const uint8_t* keep,
+// keep &= measured (src/multichip.jl:259-262), then numer/denom of the valid-fraction gate
+// (265-276)
+__global__ void fraction_kernel(const int64_t* rx, const uint8_t* meas, uint8_t* keep,
                                 int64_t n, unsigned long long* out) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int d = 0, u = 0;
-    if (k < n && rx[k] > 0 && meas[k]) { d = 1; u = keep[k] ? 1 : 0; }
+    if (k < n) {
+        const uint8_t kp = (keep[k] && meas[k]) ? 1 : 0;
+        keep[k] = kp;
+        if (rx[k] > 0 && meas[k]) { d = 1; u = kp; }
+    }
     unsigned long long cd = __syncthreads_count(d), cu = __syncthreads_count(u);
     if (threadIdx.x == 0) {
         if (cd) atomicAdd(&out[0], cd);
@@ -501,10 +512,21 @@ __global__ void edt_row_dilate_kernel(const double* d2, int64_t nr, int64_t nc, 
 
 // resample(grown, (nr, nc), Nearest()) .> 0.5 (src/resample.jl:56-71), then zero the radii
 // outside the coarse gate (src/multichip.jl:171-176).
+// frac = {denom, numer} of the valid-fraction gate: with denom == 0 or numer / denom below
+// min_fraction the coarse pass returns `nothing` (src/multichip.jl:251-276) and the level has no
+// searchable point, which leaves every result as the reference's skipped level does.
 __global__ void nearest_gate_kernel(const uint8_t* grown, int64_t sr, int64_t sc, int64_t dr,
-                                    int64_t dc, int64_t* rx, int64_t* ry, uint8_t* srch) {
+                                    int64_t dc, int64_t* rx, int64_t* ry, uint8_t* srch,
+                                    const unsigned long long* frac, double min_fraction) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= dr * dc) return;
+    const unsigned long long den = frac[0], num = frac[1];
+    if (den == 0 || (double)num / (double)den < min_fraction) {
+        rx[k] = 0;
+        ry[k] = 0;
+        srch[k] = 0;
+        return;
+    }
     int64_t j = k / dr, i = k - j * dr;
     double ys = (double)sr / (double)dr, xs = (double)sc / (double)dc;
     int64_t sj = (int64_t)floor(((double)(j + 1) - 0.5) * xs) + 1;
@@ -516,10 +538,13 @@ __global__ void nearest_gate_kernel(const uint8_t* grown, int64_t sr, int64_t sc
     srch[k] = (rx[k] > 0 && ry[k] > 0) ? 1 : 0;
 }
 
+// _reject_and_fill! (src/multichip.jl:318-330): rejected points become NaN; clears `filled`.
 __global__ void reject_apply_kernel(const uint8_t* keep, float* dx, float* dy, float* corr,
-                                    int64_t n) {
+                                    uint8_t* filled, int64_t n) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n && !keep[k]) { dx[k] = qnan(); dy[k] = qnan(); corr[k] = qnan(); }
+    if (k >= n) return;
+    filled[k] = 0;
+    if (!keep[k]) { dx[k] = qnan(); dy[k] = qnan(); corr[k] = qnan(); }
 }
 
 // _fill_holes! phase one (src/multichip.jl:347-390): pending medians for NaN points with at
@@ -573,11 +598,6 @@ __global__ void zero_u64_kernel(unsigned long long* c, int k) {
     if ((int)threadIdx.x < k) c[threadIdx.x] = 0ull;
 }
 
-__global__ void wanted_kernel(const uint16_t* rcs, int64_t n, uint8_t* w) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) w[k] = rcs[k] == 0 ? 1 : 0;
-}
-
 }  // namespace ar
 
 using namespace ar;
@@ -627,31 +647,23 @@ static ar_status count_flags2(ar_context* ctx, const uint8_t* f0, const uint8_t*
                               unsigned long long* cnt, unsigned long long* rb,
                               unsigned long long* out) {
     zero_u64_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 2);
-    count_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(f0, n, cnt);
-    count_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(f1, n, cnt + 1);
+    count2_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(f0, f1, n, cnt);
     MCHK(hipGetLastError());
     return read_counts(ctx, cnt, 2, rb, out);
 }
 
-static ar_status count_flags(ar_context* ctx, const uint8_t* f, int64_t n,
-                             unsigned long long* cnt, unsigned long long* rb,
-                             unsigned long long* out) {
-    zero_u64_kernel<<<1, 64, 0, ctx->stream>>>(cnt, 1);
-    count_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(f, n, cnt);
-    MCHK(hipGetLastError());
-    return read_counts(ctx, cnt, 1, rb, out);
-}
-
-// reject_outliers with the Gardner filter (or the measured mask when outliers are off).
+// reject_outliers with the Gardner filter (or the measured mask when outliers are off) over the
+// measured points (!isnan(dx)), which are also written to meas.
 static ar_status reject(ar_context* ctx, const float* dx, const float* dy, const int64_t* rx,
-                        const int64_t* ry, const uint8_t* valid, int64_t nr, int64_t nc,
+                        const int64_t* ry, uint8_t* meas, int64_t nr, int64_t nc,
                         int64_t iterations, float* nxb, float* nyb, uint8_t* ka, uint8_t* kb,
                         uint8_t** keep_out) {
     const ar_params& p = ctx->params;
     int64_t n = nr * nc;
     if (!p.outliers) {
-        MCHK(hipMemcpyAsync(ka, valid, n, hipMemcpyDeviceToDevice, ctx->stream));
-        *keep_out = ka;
+        measured_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(dx, n, meas);
+        MCHK(hipGetLastError());
+        *keep_out = meas;
         return AR_OK;
     }
     int64_t w = p.gardner_window;
@@ -661,7 +673,7 @@ static ar_status reject(ar_context* ctx, const float* dx, const float* dy, const
     win.ly = (int)(wy / 2);
     win.rx = (int)(wx - 1 - win.lx);
     win.ry = (int)(wy - 1 - win.ly);
-    normalise_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(dx, dy, rx, ry, valid, n, nxb, nyb, ka);
+    normalise_kernel<<<gblocks(n), GK, 0, ctx->stream>>>(dx, dy, rx, ry, meas, n, nxb, nyb, ka);
     float needed = (float)(p.min_agree_fraction * (double)(w * w));
     float tol = (float)p.agree_tolerance;
     uint8_t *kin = ka, *kout = kb;
@@ -827,13 +839,12 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
         level++;
         unsigned long long c2[2] = {0, 0};
         int64_t csx = cs, csy = 2 * (int64_t)std::nearbyint((double)cs * p.chip_aspect / 2.0);
-        wanted_kernel<<<gblocks(N), GK, 0, s>>>(rcs, N, pend);
         level_points_kernel<<<gblocks(N), GK, 0, s>>>(grx, gry, rcs, N, csx, csy,
-                                                      p.min_search_radius, lrx, lry, lcx, lcy, flags);
+                                                      p.min_search_radius, lrx, lry, lcx, lcy, flags,
+                                                      pend);
         if ((st = count_flags2(ctx, pend, flags, N, cnt, rb, c2))) return st;
         if (c2[0] == 0) break;
         if (c2[1] == 0) continue;
-        unsigned long long c = 0;
         ar_pass_hint hint{};
         const ar_pass_hint* hp = nullptr;
         if (grid_known && r0x > 0 && r0y > 0) {
@@ -855,50 +866,39 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
         int64_t fw = p.outliers ? p.gardner_window : 0;
         bool gate_all = nrc < fw || ncc < fw;
         if (!gate_all) {
+            // Every decision of the coarse pass that ends the level early (nothing searchable or
+            // measured, too small a coherent fraction, an empty gate) is made on the device: the
+            // gate then leaves no searchable point, and the fine pass, rejection, filling and
+            // merge change nothing, as for the reference's skipped level.
             coarse_points_kernel<<<gblocks(M), GK, 0, s>>>(gx, gy, gpdx, gpdy, lrx, lry, nr, nc, stride,
                                                            nrc, ncc, csx, csy, cxp, cyp, cpdx, cpdy,
-                                                           crx, cry, ccx, ccy, cs8);
-            // (no coarse point searchable: the pass measures nothing and the count below ends
-            // the level)
+                                                           crx, cry, ccx, ccy, cs8, cnt + 2);
             MCHK(hipGetLastError());
             ar_points cp = {cxp, cyp, crx, cry, cpdx, cpdy, ccx, ccy};
             if ((st = ar_track_pass(ctx, cp, M, 1, measure, cdx, cdy, ccr, cs8, hp))) return st;
-            measured_kernel<<<gblocks(M), GK, 0, s>>>(cdx, M, cmeas);
-            if ((st = count_flags(ctx, cmeas, M, cnt, rb, &c))) return st;
-            if (c == 0) continue;
             uint8_t* keep = nullptr;
             int64_t iters = p.gardner_iterations - 1 > 1 ? p.gardner_iterations - 1 : 1;
             if ((st = reject(ctx, cdx, cdy, crx, cry, cmeas, nrc, ncc, iters, cnx, cny, cka, ckb,
                              &keep)))
                 return st;
-            and_kernel<<<gblocks(M), GK, 0, s>>>(keep, cmeas, M);
-            zero_u64_kernel<<<1, 64, 0, s>>>(cnt, 2);
-            fraction_kernel<<<gblocks(M), GK, 0, s>>>(crx, cmeas, keep, M, cnt);
-            unsigned long long fr[2];
-            MCHK(hipGetLastError());
-            if ((st = read_counts(ctx, cnt, 2, rb, fr))) return st;
-            if (fr[0] == 0) continue;
-            if ((double)fr[1] / (double)fr[0] < p.min_coarse_valid_fraction) continue;
+            fraction_kernel<<<gblocks(M), GK, 0, s>>>(crx, cmeas, keep, M, cnt + 2);
             double r2 = (double)p.coarse_buffer * (double)p.coarse_buffer;
             edt_col_kernel<<<gblocks(ncc), GK, 0, s>>>(keep, nrc, ncc, d2);
             edt_row_dilate_kernel<<<gblocks(nrc), GK, 0, s>>>(d2, nrc, ncc, r2, ev, ez, edd, grown);
-            nearest_gate_kernel<<<gblocks(N), GK, 0, s>>>(grown, nrc, ncc, nr, nc, lrx, lry, flags);
+            nearest_gate_kernel<<<gblocks(N), GK, 0, s>>>(grown, nrc, ncc, nr, nc, lrx, lry, flags,
+                                                          cnt + 2, p.min_coarse_valid_fraction);
             MCHK(hipGetLastError());
-            if ((st = count_flags(ctx, flags, N, cnt, rb, &c))) return st;
-            if (c == 0) continue;
         }
 
         // ---- fine pass, rejection, hole filling, merge
         ar_points lp = {gx, gy, lrx, lry, gpdx, gpdy, lcx, lcy};
         if ((st = ar_track_pass(ctx, lp, N, p.upsampling, measure, fdx, fdy, fc, fs, hp)))
             return st;
-        measured_kernel<<<gblocks(N), GK, 0, s>>>(fdx, N, meas);
         uint8_t* keep = nullptr;
         if ((st = reject(ctx, fdx, fdy, lrx, lry, meas, nr, nc, p.gardner_iterations, nxb, nyb, ka,
                          kb, &keep)))
             return st;
-        reject_apply_kernel<<<gblocks(N), GK, 0, s>>>(keep, fdx, fdy, fc, N);
-        MCHK(hipMemsetAsync(filled, 0, N, s));
+        reject_apply_kernel<<<gblocks(N), GK, 0, s>>>(keep, fdx, fdy, fc, filled, N);
         int fwn = (int)p.fill_window;
         int lo = fwn / 2, hi = fwn - 1 - lo, needed = 2 * fwn * fwn / 3;
         // three passes without counting: a pass with nothing pending changes nothing, so the

@@ -88,10 +88,11 @@ def cfg1_pair(seed, n=512, shift=(3.30, -1.70)):
 
 def pair_latency(autorift, npairs, warm=2):
     """Default pipeline on 512^2 pairs (high-pass, UInt8, multi-chip search over the 784-point
-    grid 40:16:472), one context reused across pairs as init/reinit!/autorift! do."""
+    grid 40:16:472), one context reused across pairs as init/reinit!/autorift! do. The images are
+    held column-major, as a Julia caller holds them (the ABI's layout: no host transposition)."""
     xs = np.arange(40.0, 473.0, 16.0)
     grid = autorift.gridpoints_xy(xs, xs, chip_size=32, search_radius=25)
-    pairs = [cfg1_pair(100 + k) for k in range(npairs + warm)]
+    pairs = [tuple(np.asfortranarray(x) for x in cfg1_pair(100 + k)) for k in range(npairs + warm)]
     ctx = autorift.Context(autorift.params())
     lat = []
     meas = []

@@ -156,9 +156,26 @@ def _grid(n, spacing, margin):
     return autorift.gridpoints_xy(xs, xs, chip_size=32, search_radius=25)
 
 
-@pytest.mark.parametrize("case", ["shift_u8", "shift_f32_masked", "warp_u8_levels"])
+@pytest.mark.parametrize("case", ["shift_u8", "shift_f32_masked", "warp_u8_levels", "noise_gate",
+                                  "all_masked"])
 def test_multichip_matches_oracle(case):
-    if case == "shift_u8":
+    if case == "noise_gate":
+        # uncorrelated images and a strict coherence gate: levels end in the coarse pass
+        rng = np.random.default_rng(11)
+        ref = rng.integers(0, 256, (300, 300)).astype(np.uint8)
+        sec = rng.integers(0, 256, (300, 300)).astype(np.uint8)
+        ok = np.ones(ref.shape, bool)
+        grid = _grid(300, 6, 60)
+        p = _params(preprocess=0, quantize=0, chip_size_base=16, chip_size_min=16,
+                    chip_size_max=64, min_coarse_valid_fraction=0.5)
+        op = oracle.params(chip_size=16, chip_size_max=64, min_coarse_valid_fraction=0.5)
+    elif case == "all_masked":
+        ref, sec = shifted_pair(300, (1.0, 1.0), seed=9, dtype=np.uint8)
+        ok = np.zeros(ref.shape, bool)
+        grid = _grid(300, 6, 60)
+        p = _params(preprocess=0, quantize=0)
+        op = oracle.params(chip_size=32)
+    elif case == "shift_u8":
         ref, sec = shifted_pair(400, (4.4, -3.2), seed=5, dtype=np.uint8)
         ok = np.ones(ref.shape, bool)
         grid = _grid(400, 8, 90)
@@ -199,7 +216,12 @@ def test_multichip_matches_oracle(case):
         np.testing.assert_array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
     np.testing.assert_array_equal(g.chip_size, e[3])
     np.testing.assert_array_equal(g.interpolated, e[4])
-    assert g.nmeasured() > 0.5 * g.dx.size
+    if case == "all_masked":
+        assert g.nmeasured() == 0
+    elif case == "noise_gate":
+        print(f"noise_gate: {g.nmeasured()} of {g.dx.size} measured", flush=True)
+    else:
+        assert g.nmeasured() > 0.5 * g.dx.size
 
 
 def test_multichip_refuses_more_measures_than_levels():
