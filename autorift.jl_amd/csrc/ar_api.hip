@@ -20,7 +20,7 @@
 namespace ar {
 
 // Defined in track.hip.
-template <bool BATCH, int OCC>
+template <bool BATCH, int OCC, bool SG>
 __global__ void track_u8_kernel(TrackArgs a);
 __global__ void refine_kernel(TrackArgs a);
 template <int IS_U8>
@@ -726,7 +726,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     }
     void (*kern)(TrackArgs);
     if (use_u8) {
-        kern = track_u8_kernel<false, 5>;
+        kern = track_u8_kernel<false, 5, false>;
         // the int8 copy of the reference the matrix cores read (made once per prepared image)
         const int64_t N = ctx->ny * ctx->nx;
         const int64_t stride = (N + 1024 + 255) / 256 * 256;   // room for a window column's tail
@@ -1201,6 +1201,7 @@ struct ar_batch {
     ar_context* ctx = nullptr;          // preparation, stream, pass scratch, error text
     std::vector<BatchPair*> pairs;
     DevBuf desc, lists, counts, pidx;   // descriptor table, class lists and counts, pair indices
+    DevBuf sbuf;                        // global box sums of the largest radius class
     bool desc_dirty = true;
     // chips whose area is not a power of two go to the direct Float64 tracker
     bool exact = false;
@@ -1215,15 +1216,15 @@ struct ar_batch {
     int graph_launches = 0, graph_split = 0;
     // device buffers the captured pass addresses; a replay needs every one where it was (a later
     // pass that grew one of them moved it)
-    const void* gbufs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const void* gbufs[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     void graph_bufs(const void** out) const {
         out[0] = desc.ptr; out[1] = lists.ptr; out[2] = counts.ptr; out[3] = ctx->refine.ptr;
-        out[4] = ctx->prec.ptr; out[5] = ctx->scratch.ptr;
+        out[4] = ctx->prec.ptr; out[5] = ctx->scratch.ptr; out[6] = sbuf.ptr;
     }
     bool graph_bufs_same() const {
-        const void* b[6];
+        const void* b[7];
         graph_bufs(b);
-        for (int k = 0; k < 6; k++)
+        for (int k = 0; k < 7; k++)
             if (b[k] != gbufs[k]) return false;
         return true;
     }
@@ -1309,25 +1310,61 @@ static ar_status batch_sync_desc(ar_batch* B) {
     return AR_OK;
 }
 
+// Kernel and LDS of one class launch of the batched tracker (largest chip, the class's radius
+// cap). A class keeps its box sums S1 / S2 in global scratch when that admits more workgroups
+// per CU than holding them in LDS; launches whose LDS leaves room for at most two workgroups per
+// CU take the instance with twice the registers (no spills).
+struct BatchKern {
+    void (*kern)(TrackArgs);
+    size_t lds;
+    bool sg;
+    int64_t sg_ints;   // int32 per workgroup slot (S1 and S2)
+};
+static BatchKern batch_kernel(const TrackArgs& a, int rcap) {
+    const int rx = a.max_rx < rcap ? a.max_rx : rcap, ry = a.max_ry < rcap ? a.max_ry : rcap;
+    BatchKern k;
+    k.lds = u8_layout(a.max_ch, a.max_cw, rx, ry, false).total;
+    k.sg = false;
+    k.sg_ints = 0;
+    // workgroups per CU the LDS admits (at most 4: the register budget of the 4-wave instance)
+    auto fit = [](size_t l) { const size_t f = kLdsLimit / (l > 0 ? l : 1); return f < 4 ? f : 4; };
+    const size_t l2 = u8_layout(a.max_ch, a.max_cw, rx, ry, false, true).total;
+    if (fit(l2) > fit(k.lds)) {
+        k.lds = l2;
+        k.sg = true;
+        k.sg_ints = 2 * (int64_t)(2 * rx) * (2 * ry);
+    }
+    const bool wide = k.lds > (160 * 1024) / 3;
+    if (k.sg) k.kern = wide ? track_u8_kernel<true, 2, true> : track_u8_kernel<true, 4, true>;
+    else k.kern = wide ? track_u8_kernel<true, 2, false> : track_u8_kernel<true, 4, false>;
+    return k;
+}
+
 // One class launch of the batched tracker: the points in `list` (count m), LDS carved for the
 // class's largest radius.
 static ar_status batch_launch(ar_batch* B, TrackArgs a, const int32_t* list, int64_t m, int rcap) {
     ar_context* ctx = B->ctx;
     if (m == 0) return AR_OK;
+    const BatchKern bk = batch_kernel(a, rcap);
     a.order = list;
     a.n = m;
     a.max_rx = a.max_rx < rcap ? a.max_rx : rcap;
     a.max_ry = a.max_ry < rcap ? a.max_ry : rcap;
-    a.lds_bytes = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, false).total;
-    // launches whose LDS leaves room for at most two workgroups per CU take the instance with
-    // twice the registers (no spills)
-    auto kern = a.lds_bytes > (160 * 1024) / 3 ? track_u8_kernel<true, 2> : track_u8_kernel<true, 4>;
+    a.lds_bytes = bk.lds;
     int per_cu = 0;
-    if (ar_status st = kernel_occupancy(ctx, (const void*)kern, AR_WG, a.lds_bytes, &per_cu))
+    if (ar_status st = kernel_occupancy(ctx, (const void*)bk.kern, AR_WG, a.lds_bytes, &per_cu))
         return st;
     if (per_cu < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "batched tracker does not fit on a CU");
     int64_t nb = (int64_t)ctx->ncu * per_cu;
     if (nb > m) nb = m;
+    if (bk.sg) {
+        // reserved before any capture (batch_pass sizes it for every class)
+        if (B->sbuf.cap < (size_t)nb * bk.sg_ints * 4)
+            return fail(ctx, AR_ERR_DEVICE, "batched tracker box-sum scratch not reserved");
+        a.sg = (int32_t*)B->sbuf.ptr;
+        a.sg_stride = bk.sg_ints;
+    }
+    auto kern = bk.kern;
     hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(AR_WG), a.lds_bytes, ctx->stream, a);
     HIPCHK(ctx, hipGetLastError());
     B->last_launches++;
@@ -1455,8 +1492,9 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
         } else {
             int best = 0;
             double best_ms = 1e30;
-            // one launch; two classes split at 16, 24 or 32; three classes
-            const int cands[6] = {0, 16, 24, 32, 16 | (28 << 8), 20 | (32 << 8)};
+            // one launch; two classes split at 16, 20, 24, 28 or 32; three classes
+            const int cands[11] = {0, 16, 20, 24, 28, 32, 12 | (24 << 8), 16 | (28 << 8),
+                                   16 | (32 << 8), 20 | (32 << 8), 24 | (32 << 8)};
             for (int c : cands) {
                 const int c1 = c & 0xff, c2 = c >> 8;
                 if (c != 0 && (c1 >= rmax || (c2 > 0 && c2 >= rmax))) continue;
@@ -1482,13 +1520,16 @@ static ar_status batch_pass(ar_batch* B, const ar_points& dp, const int32_t* dpa
     {
         int pc = 0;
         const int caps0[3] = {R1 > 0 ? R1 : 1 << 30, R2 > 0 ? R2 : 1 << 30, 1 << 30};
+        size_t sbytes = 0;
         for (int k = 0; k < 3; k++) {
-            const int rc = caps0[k];
-            const size_t l = u8_layout(a.max_ch, a.max_cw, a.max_rx < rc ? a.max_rx : rc,
-                                       a.max_ry < rc ? a.max_ry : rc, false).total;
-            auto kk = l > (160 * 1024) / 3 ? track_u8_kernel<true, 2> : track_u8_kernel<true, 4>;
-            if ((st = kernel_occupancy(ctx, (const void*)kk, AR_WG, l, &pc))) return st;
+            const BatchKern bk = batch_kernel(a, caps0[k]);
+            if ((st = kernel_occupancy(ctx, (const void*)bk.kern, AR_WG, bk.lds, &pc))) return st;
+            if (bk.sg) {
+                const size_t need = (size_t)ctx->ncu * pc * bk.sg_ints * 4;
+                if (need > sbytes) sbytes = need;
+            }
         }
+        if (sbytes && (st = B->sbuf.reserve(ctx, sbytes))) return st;
     }
     const size_t rlds = (size_t)cascade_floats((int)up) * 4;
     int64_t rblocks = 0;
@@ -1642,6 +1683,7 @@ void ar_batch_destroy(ar_batch* B) {
     B->lists.release();
     B->counts.release();
     B->pidx.release();
+    B->sbuf.release();
     ar_context_destroy(B->ctx);
     delete B;
 }

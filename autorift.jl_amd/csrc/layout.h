@@ -34,8 +34,10 @@ __host__ __device__ inline bool u8_one_batch(int max_ch, int max_rx, int max_ry)
 // Regions: window, chip, S1, S2 (int32 per shift), then the per-shift int32 accumulators -- in
 // the window's bytes (running on into the chip's) when they fit there and a wave holds one batch
 // -- and, when the pass builds its box sums per point (box == false), the column sums of W / W^2.
+// sglobal: the box sums S1 / S2 live in a global scratch slot per workgroup instead of LDS (the
+// batched tracker's largest radius class, whose LDS would otherwise admit one workgroup per CU).
 __host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx, int max_ry,
-                                             bool box) {
+                                             bool box, bool sglobal = false) {
     U8Layout L;
     int nti = (2 * max_ry + 15) / 16, ntj = (2 * max_rx + 15) / 16;
     int nrb = (max_ch + 15 + 15) / 16;   // 16-row blocks of the band, ceil((ch + 15) / 16)
@@ -52,8 +54,8 @@ __host__ __device__ inline U8Layout u8_layout(int max_ch, int max_cw, int max_rx
     L.off_w = 0;
     L.off_t = (size_t)winw * L.ldw;
     L.off_s1 = rup((int)(L.off_t + (size_t)(max_cw + 2) * L.cpad), 16);
-    L.off_s2 = L.off_s1 + shifts;
-    L.off_tmp = L.off_s2 + shifts;
+    L.off_s2 = L.off_s1 + (sglobal ? 0 : shifts);
+    L.off_tmp = L.off_s2 + (sglobal ? 0 : shifts);
     L.acc_in_w = shifts <= L.off_s1 && u8_one_batch(max_ch, max_rx, max_ry);
     L.acc_cols_t = 0;
     if (L.acc_in_w && shifts > L.off_t) L.acc_cols_t = (int)((shifts - L.off_t + L.cpad - 1) / L.cpad);

@@ -174,19 +174,20 @@ __device__ inline void mfma_batch(const int8_t* Wl, const int8_t* Tp, int ldw, i
 // OCC: waves per SIMD the registers are budgeted for (5: 96 VGPRs, five workgroups per CU at
 // chip 32 r 25, some spills; 4: 128; 2: 256, for launches whose LDS allows at most two
 // workgroups per CU, with no spills).
-template <bool BATCH, int OCC>
+// SG: the box sums S1 / S2 in the workgroup's global scratch slot (a.sg) instead of LDS.
+template <bool BATCH, int OCC, bool SG>
 __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(OCC))) void track_u8_kernel(
     TrackArgs a0) {
     TrackArgs a = a0;
     extern __shared__ __align__(16) unsigned char smem[];
-    const U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, a.box1 != nullptr);
+    const U8Layout L = u8_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, a.box1 != nullptr, SG);
     // one set of reduction slots per call site (see block_sum2_u32 / block_best1)
     __shared__ Best red_screen[AR_NWAVES], red_exact[AR_NWAVES];
     __shared__ uint32_t red_chip[2 * AR_NWAVES];
     int8_t* Wl = (int8_t*)(smem + L.off_w);
     int8_t* Tp = (int8_t*)(smem + L.off_t);
-    int32_t* S1 = (int32_t*)(smem + L.off_s1);
-    int32_t* S2 = (int32_t*)(smem + L.off_s2);
+    int32_t* S1 = SG ? a.sg + (int64_t)blockIdx.x * a.sg_stride : (int32_t*)(smem + L.off_s1);
+    int32_t* S2 = SG ? S1 + a.sg_stride / 2 : (int32_t*)(smem + L.off_s2);
     int32_t* colS = (int32_t*)(smem + L.off_tmp);
     // the wave index in a scalar register: loops over a wave's share are scalar loops
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -649,9 +650,11 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     }
 }
 
-template __global__ void track_u8_kernel<false, 5>(TrackArgs);
-template __global__ void track_u8_kernel<true, 4>(TrackArgs);
-template __global__ void track_u8_kernel<true, 2>(TrackArgs);
+template __global__ void track_u8_kernel<false, 5, false>(TrackArgs);
+template __global__ void track_u8_kernel<true, 4, false>(TrackArgs);
+template __global__ void track_u8_kernel<true, 2, false>(TrackArgs);
+template __global__ void track_u8_kernel<true, 4, true>(TrackArgs);
+template __global__ void track_u8_kernel<true, 2, true>(TrackArgs);
 
 // ---------------------------------------------------------------------------------------------
 // generic path (FP64, reference order)

@@ -60,6 +60,10 @@ struct TrackArgs {
     const int32_t* box1;
     const int32_t* box2;
     int64_t box_ld;
+    // batched tracker with the box sums in global memory: S1 then S2 of workgroup b at
+    // sg + b * sg_stride (sg_stride / 2 int32 each)
+    int32_t* sg;
+    int64_t sg_stride;
     // mixed-precision path: Float64 box images of W and W^2 (same indexing as box1 / box2) and
     // the power-of-two operand scales (window, chip) with the factor that undoes them
     const double* fbox1;
