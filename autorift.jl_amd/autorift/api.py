@@ -23,7 +23,7 @@ _SUBPIXEL = ("pyramid", "none")
 _OUTLIERS = ("gardner", "none")
 # arithmetic of the correlation contraction for Float32 prepared images (no reference
 # counterpart: the reference always correlates in Float32 / Float64)
-_PRECISION = {"exact": 0, "bf16": 1, "fp16": 2}
+_PRECISION = {"exact": 0, "bf16": 1, "fp16": 2, "fft": 3}
 
 
 class RotationSearch:

@@ -23,6 +23,7 @@ namespace ar {
 template <bool BATCH, int OCC, bool SG>
 __global__ void track_u8_kernel(TrackArgs a);
 __global__ void refine_kernel(TrackArgs a);
+__global__ void track_fft_kernel(TrackArgs a);
 template <int IS_U8>
 __global__ void track_generic_kernel(TrackArgs a);
 template <bool LDS>
@@ -751,14 +752,38 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
             a.refx_stride = stride;
         }
     }
+    const int prec = ctx->params.precision;
+    if (prec < AR_PREC_EXACT || prec > AR_PREC_FFT)
+        return fail(ctx, AR_ERR_INVALID_ARGUMENT, "precision must be EXACT, BF16, FP16 or FFT");
+    // FFT requested: the reference's dispatch (src/correlate.jl:785-797) on the pass's largest
+    // point; the transform kernel carves its LDS for one chip size per pass and takes transforms
+    // up to 128 per side, the exact direct tracker everything else
+    bool use_fft = !use_u8 && nrot == 0 && !cplx && ctx->pdtype == AR_F32 && prec == AR_PREC_FFT &&
+                   uniform &&
+                   (double)(2 * a.max_ry) * (2 * a.max_rx) * a.max_ch * a.max_cw > 20000.0;
+    bool mp_box_timed = false;
+    if (use_fft) {
+        size_t lds = 0;
+        HIPCHK(ctx, hipEventRecord(ctx->evb, ctx->stream));   // the box images count in the pass
+        st = ar_fft_setup(ctx, a, &kern, &lds);
+        if (st == AR_ERR_UNSUPPORTED) {
+            use_fft = false;
+        } else if (st == AR_ERR_OUT_OF_MEMORY) {
+            return fail(ctx, st, "out of device memory (box images)");
+        } else if (st) {
+            return fail(ctx, st, "FFT set-up failed");
+        } else {
+            mp_box_timed = true;
+            a.lds_bytes = lds;
+        }
+    }
     // Float32 images with a 16-bit contraction requested: the mixed-precision kernel, which
     // carves its LDS for one chip size per pass
     const bool use_mp = !use_u8 && nrot == 0 && ctx->pdtype == AR_F32 &&
-                        ctx->params.precision != AR_PREC_EXACT;
-    bool mp_box_timed = false;
-    if (use_mp) {
-        if (ctx->params.precision != AR_PREC_BF16 && ctx->params.precision != AR_PREC_FP16)
-            return fail(ctx, AR_ERR_INVALID_ARGUMENT, "precision must be EXACT, BF16 or FP16");
+                        (prec == AR_PREC_BF16 || prec == AR_PREC_FP16);
+    if (use_fft) {
+        // set up above
+    } else if (use_mp) {
         if (!uniform)
             return fail(ctx, AR_ERR_UNSUPPORTED,
                         "the BF16 / FP16 tracker needs one chip size per pass");
@@ -782,6 +807,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         use_f64 = nrot == 0 && f64_direct_enabled() && f64_shape(a, &wl, &pitch, &threads);
     }
     bool box_timed = mp_box_timed;
+    const int kwg = use_fft ? AR_FFT_WG : AR_WG;   // threads per workgroup of the tracker kernel
     const int bmode = box_images_mode();
     // Box images of W and W^2 replace the per-point box sums when every point has the same chip,
     // the points' shifts cover at least a quarter of the image's chip positions and there are at
@@ -826,12 +852,12 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         }
     }
     if (!use_f64) {
-        if ((st = kernel_occupancy(ctx, (const void*)kern, AR_WG, a.lds_bytes, &per_cu))) return st;
+        if ((st = kernel_occupancy(ctx, (const void*)kern, kwg, a.lds_bytes, &per_cu))) return st;
         if (per_cu < 1) return fail(ctx, AR_ERR_UNSUPPORTED, "tracker kernel does not fit on a CU");
     }
     int64_t nblocks = (int64_t)ctx->ncu * per_cu;
     if (nblocks > n) nblocks = n;
-    if (!use_u8 && !use_mp && !use_f64) {
+    if (!use_u8 && !use_mp && !use_fft && !use_f64) {
         const size_t per = cplx ? ar_coherence_scratch(a.max_ch, a.max_cw, a.max_rx, a.max_ry)
                                 : gen_scratch(a.max_ch, a.max_cw, a.max_rx, a.max_ry).total;
         a.scratch_per_block = per;
@@ -861,7 +887,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         if ((st = f64_enqueue(ctx, a, P, (unsigned char*)ctx->scratch.ptr))) return st;
     } else {
         HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(AR_WG), a.lds_bytes, ctx->stream, a);
+        hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(kwg), a.lds_bytes, ctx->stream, a);
         HIPCHK(ctx, hipGetLastError());
     }
     HIPCHK(ctx, hipEventRecord(ctx->evm, ctx->stream));
@@ -873,7 +899,7 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     ctx->timing_pending = true;
     ctx->pending_box = box_timed;
     ctx->pending_refine = up > 1;
-    ctx->last_path = use_u8 ? 0 : (use_mp ? 2 : (use_f64 ? 3 : 1));
+    ctx->last_path = use_u8 ? 0 : (use_mp ? 2 : (use_fft ? 4 : (use_f64 ? 3 : 1)));
     if (hint) return AR_OK;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ar_resolve_timing(ctx);

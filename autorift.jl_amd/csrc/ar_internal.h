@@ -110,5 +110,15 @@ namespace ar { struct TrackArgs; }
 // mixed-precision pass set-up (track_mp.hip): operand scales, Float64 box images, kernel choice
 ar_status ar_mp_setup(ar_context* ctx, ar::TrackArgs& a, void (**kern)(ar::TrackArgs),
                       size_t* lds);
+// Float64 box images of the pass's chip size (mixed-precision and FFT paths)
+ar_status ar_f64_box_images(ar_context* ctx, ar::TrackArgs& a);
+// FFT path set-up (track_fft.hip): transform sizes, box images, kernel choice; AR_ERR_UNSUPPORTED
+// when the transform does not fit the kernel's LDS
+ar_status ar_fft_setup(ar_context* ctx, ar::TrackArgs& a, void (**kern)(ar::TrackArgs),
+                       size_t* lds);
+// threads per workgroup of the FFT tracker
+#define AR_FFT_WG 512
+// next_fft_size (src/correlate.jl:357-403)
+int64_t ar_next_fft_size(int64_t n);
 ar_status ar_upload_points(ar_context* ctx, const ar_points* pts, int64_t n, ar_points* dp,
                            int64_t csx_fill, int64_t csy_fill);

@@ -70,6 +70,8 @@ struct TrackArgs {
     const double* fbox2;
     float wscale, cscale;
     double numscale;
+    // FFT path: transform extents (rows, columns) of the pass
+    int32_t fft_fy, fft_fx;
     // per-pass maxima the LDS layout is carved for
     int32_t max_ch, max_cw, max_rx, max_ry;
     size_t lds_bytes;
@@ -434,6 +436,25 @@ __host__ __device__ inline void taps(int k, int n, int& m, int& c, int& q, int& 
 
 __device__ inline float tap3(float a, float b, float c) { return AR_WC * b + AR_WS * (a + c); }
 __device__ inline float tap2(float b, float c) { return AR_WH * (b + c); }
+
+// Prepared Float32 pixel at padded 1-based (r, c); 0 in the padding.
+__device__ inline float mp_pix(const TrackArgs& a, const float* img, int64_t r, int64_t c) {
+    const int64_t k = pix(a, r, c);
+    return k < 0 ? 0.0f : img[k];
+}
+
+// The reference's normalisation (src/correlate.jl:591-600 ZNCC, :618-627 NCC) of a numerator.
+__device__ inline float mp_normalise(double num, double s1, double s2, double n, double cnorm,
+                                     int measure) {
+    double den;
+    if (measure == 1) {
+        den = cnorm * sqrt(s2 < 0.0 ? 0.0 : s2);
+    } else {
+        const double wv = s2 - s1 * s1 / n;
+        den = cnorm * sqrt(wv < 0.0 ? 0.0 : wv);
+    }
+    return den > 0 ? (float)(num / den) : 0.0f;
+}
 
 // ---- shared tail of the tracker kernels: outputs, refinement hand-off, patch origin
 

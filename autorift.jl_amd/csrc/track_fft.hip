@@ -1,0 +1,409 @@
+// FFT tracker for Float32 prepared images (precision = :fft): the reference's numerator by
+// transform (_numerators_fft!, src/correlate.jl:816-848) in single precision, one workgroup per
+// point with the whole transform in LDS.
+//
+// Per point: the window goes in the real part and the reversed mean-removed chip
+// Float32(x - mean) in the imaginary part of one zero-padded fy x fx complex array (fy, fx =
+// next_fft_size of the pass's window extents, src/correlate.jl:237-238); one forward 2-D
+// transform gives both spectra (A = (Z[k] + conj Z[-k]) / 2, B = (Z[k] - conj Z[-k]) / 2i), their
+// product P = A B is Hermitian, and the real part of its inverse, times 1 / (fy fx), is the
+// correlation; the valid block starts at (ch - 1, cw - 1) as in the reference's read. The 1-D
+// transforms are mixed-radix Stockham passes (radices 4, 2, 3, 5, 7) over every column, then
+// every row; each wave takes a few lines at a time through all their passes, reading each pass's
+// inputs into registers before writing its outputs in place. Twiddles are Float64 sincospi
+// values rounded once, held in LDS. The normalisation is the reference's Float64 formula on the pass's Float64 box images
+// (src/correlate.jl:575-630), the argmax follows peak_index (src/peak.jl:32-61), and the 5x5
+// patch handed to the subpixel refinement is read from the same surface, as the reference does.
+// Results agree with the reference's FFTW numerators to single-precision rounding, not bit for
+// bit (FFTW's butterfly order is its own).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "ar_internal.h"
+#include "layout.h"
+#include "refine.h"
+#include "track_common.h"
+
+namespace ar {
+
+__device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// Radices of n, largest powers of 4 first, as 4-bit fields of the result (stage s in bits
+// 4s .. 4s + 3; a zero field ends the list); 0 when n has a prime factor above 7.
+__device__ __host__ inline uint64_t fft_radices(int n) {
+    uint64_t r = 0;
+    int k = 0;
+    while (n % 4 == 0) { r |= (uint64_t)4 << (4 * k++); n /= 4; }
+    while (n % 2 == 0) { r |= (uint64_t)2 << (4 * k++); n /= 2; }
+    for (int p = 3; p <= 7; p += 2)
+        while (n % p == 0) { r |= (uint64_t)p << (4 * k++); n /= p; }
+    return n == 1 ? r : 0;
+}
+
+// cos and sin of 2 pi k / R for the odd radices (k < R)
+template <int R> struct Roots;
+template <> struct Roots<3> {
+    static constexpr float c[3] = {1.0f, -0.5f, -0.5f};
+    static constexpr float s[3] = {0.0f, 0.86602540378443864676f, -0.86602540378443864676f};
+};
+template <> struct Roots<5> {
+    static constexpr float c[5] = {1.0f, 0.30901699437494742410f, -0.80901699437494742410f,
+                                   -0.80901699437494742410f, 0.30901699437494742410f};
+    static constexpr float s[5] = {0.0f, 0.95105651629515357212f, 0.58778525229247312917f,
+                                   -0.58778525229247312917f, -0.95105651629515357212f};
+};
+template <> struct Roots<7> {
+    static constexpr float c[7] = {1.0f, 0.62348980185873353053f, -0.22252093395631440429f,
+                                   -0.90096886790241912624f, -0.90096886790241912624f,
+                                   -0.22252093395631440429f, 0.62348980185873353053f};
+    static constexpr float s[7] = {0.0f, 0.78183148246802980871f, 0.97492791218182360702f,
+                                   0.43388373911755812048f, -0.43388373911755812048f,
+                                   -0.97492791218182360702f, -0.78183148246802980871f};
+};
+
+// DFT of odd size R by its symmetric form: with s_r = v_r + v_{R-r}, d_r = v_r - v_{R-r},
+// out[q] = v_0 + sum_r (s_r cos - i d_r sin)(2 pi r q / R) and out[R - q] with +i (forward).
+template <int R>
+__device__ inline void dft_odd(const float2 (&v)[R], float2 (&o)[R], bool inv) {
+    constexpr int H = (R - 1) / 2;
+    float2 sm[H + 1], df[H + 1];
+    float2 o0 = v[0];
+#pragma unroll
+    for (int r = 1; r <= H; r++) {
+        sm[r] = cadd(v[r], v[R - r]);
+        df[r] = csub(v[r], v[R - r]);
+        o0 = cadd(o0, sm[r]);
+    }
+    o[0] = o0;
+#pragma unroll
+    for (int q = 1; q <= H; q++) {
+        float2 av = v[0], bv = make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int r = 1; r <= H; r++) {
+            const float c = Roots<R>::c[(r * q) % R], sn = Roots<R>::s[(r * q) % R];
+            av.x += sm[r].x * c;
+            av.y += sm[r].y * c;
+            bv.x += df[r].x * sn;
+            bv.y += df[r].y * sn;
+        }
+        // forward: out[q] = a - i b, out[R - q] = a + i b; inverse the other way round
+        const float2 mib = make_float2(bv.y, -bv.x), pib = make_float2(-bv.y, bv.x);
+        o[q] = cadd(av, inv ? pib : mib);
+        o[R - q] = cadd(av, inv ? mib : pib);
+    }
+}
+
+// One wave's Stockham pass of radix R over lines l0 .. l0 + S - 1 of the array: element (line l,
+// position t) at Z[l ls + t es]; lines of length n, Ns = product of the radices already
+// applied. T holds exp(-2 pi i m / n), m < n; inv conjugates every factor. S n <= 256, so a lane
+// holds at most U butterflies; every read of the pass precedes every write in the wave.
+template <int R>
+__device__ inline void fft_stage(float2* Z, const float2* T, int n, int Ns, int l0, int S, int es,
+                                 int ls, bool inv) {
+    constexpr int U = (256 / R + 63) / 64;
+    const int lane = threadIdx.x & 63;
+    const int nb = n / R, B = S * nb, step = n / (Ns * R);
+    float2 out[U][R];
+    int dst[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int b = lane + 64 * u;
+        dst[u] = -1;
+        if (b < B) {
+            const int lb = b / nb, jj = b - lb * nb, k = jj % Ns;
+            const int base = (l0 + lb) * ls;
+            float2 v[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                float2 x = Z[base + (jj + r * nb) * es];
+                if (r > 0) {
+                    float2 w = T[r * k * step];
+                    if (inv) w = cconj(w);
+                    x = cmul(x, w);
+                }
+                v[r] = x;
+            }
+            if constexpr (R == 2) {
+                out[u][0] = cadd(v[0], v[1]);
+                out[u][1] = csub(v[0], v[1]);
+            } else if constexpr (R == 4) {
+                const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+                const float2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+                // forward: (-i) b1; inverse: (+i) b1
+                const float2 jb = inv ? make_float2(-b1.y, b1.x) : make_float2(b1.y, -b1.x);
+                out[u][0] = cadd(a0, b0);
+                out[u][2] = csub(a0, b0);
+                out[u][1] = cadd(a1, jb);
+                out[u][3] = csub(a1, jb);
+            } else {
+                dft_odd<R>(v, out[u], inv);
+            }
+            dst[u] = base + ((jj / Ns) * Ns * R + k) * es;
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (dst[u] >= 0)
+#pragma unroll
+            for (int q = 0; q < R; q++) Z[dst[u] + q * Ns * es] = out[u][q];
+    wave_sync();
+}
+
+// Threads of the FFT tracker's workgroup: eight waves (the transform's LDS admits two
+// workgroups per CU; more waves hide more of its latency).
+#define FFT_WG AR_FFT_WG
+#define FFT_NW (FFT_WG / 64)
+
+// The 1-D transform of length n (at most 128) along one axis of the array, nlines lines; each
+// wave takes batches of 256 / n lines. Ends with a workgroup barrier.
+__device__ inline void fft_axis(float2* Z, const float2* T, int n, int nlines, int es, int ls,
+                                bool inv) {
+    const uint64_t rad = fft_radices(n);
+    const int S = 256 / n;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int l0 = wave * S; l0 < nlines; l0 += FFT_NW * S) {
+        const int sl = nlines - l0 < S ? nlines - l0 : S;
+        int Ns = 1;
+        for (uint64_t rr = rad; rr != 0; rr >>= 4) {
+            const int R = (int)(rr & 15);
+            switch (R) {
+                case 4: fft_stage<4>(Z, T, n, Ns, l0, sl, es, ls, inv); break;
+                case 2: fft_stage<2>(Z, T, n, Ns, l0, sl, es, ls, inv); break;
+                case 3: fft_stage<3>(Z, T, n, Ns, l0, sl, es, ls, inv); break;
+                case 5: fft_stage<5>(Z, T, n, Ns, l0, sl, es, ls, inv); break;
+                default: fft_stage<7>(Z, T, n, Ns, l0, sl, es, ls, inv); break;
+            }
+            Ns *= R;
+        }
+    }
+    __syncthreads();
+}
+
+// LDS of the FFT tracker: the complex array (columns of ld = fy | 1 elements: an odd stride keeps
+// the row passes' lanes on distinct banks), the two twiddle tables, and one Float32 plane for the
+// chip and then the surface.
+struct FftLayout {
+    size_t off_z, off_ty, off_tx, off_s, total;
+};
+__host__ __device__ inline FftLayout fft_layout(int fy, int fx, int max_ch, int max_cw,
+                                                int max_rx, int max_ry) {
+    FftLayout L;
+    L.off_z = 0;
+    L.off_ty = (size_t)(fy | 1) * fx * 8;
+    L.off_tx = L.off_ty + (size_t)fy * 8;
+    L.off_s = L.off_tx + (size_t)fx * 8;
+    const size_t surf = (size_t)(2 * max_ry) * (2 * max_rx), chip = (size_t)max_ch * max_cw;
+    L.total = L.off_s + 4 * (surf > chip ? surf : chip);
+    return L;
+}
+
+// Block reductions over the FFT_NW waves (every thread gets the result).
+__device__ inline double fft_block_sum(double v, double* red) {
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = red[0];
+    for (int k = 1; k < FFT_NW; k++) t += red[k];
+    return t;
+}
+__device__ inline Best fft_block_best(Best b, Best* red) {
+    b = wave_best(b);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
+    __syncthreads();
+    Best r = red[0];
+    for (int k = 1; k < FFT_NW; k++) r = best_merge(r, red[k]);
+    return r;
+}
+// _any_valid (src/track.jl:416-421) over the chip rectangle, block-uniform.
+__device__ inline bool fft_any_valid(const TrackArgs& a, const Geom& g) {
+    if (a.ok == nullptr) {
+        const int64_t r0 = g.cr0 - a.pady - 1, c0 = g.cc0 - a.padx - 1;
+        return r0 + g.ch - 1 >= 0 && r0 < a.ny && c0 + g.cw - 1 >= 0 && c0 < a.nx;
+    }
+    int found = 0;
+    for (int k = threadIdx.x; k < g.ch * g.cw; k += FFT_WG) {
+        const int v = k / g.ch, u = k - v * g.ch;
+        const int64_t q = pix(a, g.cr0 + u, g.cc0 + v);
+        if (q >= 0 && a.ok[q]) found = 1;
+    }
+    return __syncthreads_or(found) != 0;
+}
+
+__global__ __launch_bounds__(FFT_WG) __attribute__((amdgpu_waves_per_eu(4))) void track_fft_kernel(
+    TrackArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double red_d[FFT_NW];
+    __shared__ Best red[FFT_NW];
+    const int tid = threadIdx.x;
+    const int fy = a.fft_fy, fx = a.fft_fx;
+    const FftLayout L = fft_layout(fy, fx, a.max_ch, a.max_cw, a.max_rx, a.max_ry);
+    float2* Z = (float2*)(smem + L.off_z);
+    float2* Ty = (float2*)(smem + L.off_ty);
+    float2* Tx = (float2*)(smem + L.off_tx);
+    float* plane = (float*)(smem + L.off_s);
+    const float* ref = (const float*)a.ref;
+    const float* sec = (const float*)a.sec;
+    // twiddles exp(-2 pi i m / n): Float64 sincospi, rounded once
+    for (int m = tid; m < fy + fx; m += FFT_WG) {
+        const bool y = m < fy;
+        const int n = y ? fy : fx, mm = y ? m : m - fy;
+        double sv, cv;
+        sincospi(2.0 * (double)mm / (double)n, &sv, &cv);
+        (y ? Ty : Tx)[mm] = make_float2((float)cv, (float)-sv);
+    }
+    __syncthreads();
+    const int nz = fy * fx, ld = fy | 1;
+    const float inv_nz = 1.0f / (float)nz;
+
+    const PointRange pr = point_range(a.n);
+    for (int64_t p = pr.first; p < pr.end; p += pr.step) {
+        const Geom g = point_geom(a, p);
+        if (!g.status || !fft_any_valid(a, g)) { write_skip(a, p, false); continue; }
+        const int ch = g.ch, cw = g.cw, NR = 2 * g.ry, NC = 2 * g.rx;
+        const int winh = ch + NR - 1, winw = cw + NC - 1;
+
+        // ---- chip: Float32 samples, Float64 mean and energy (block sums in a fixed order)
+        const int nn = ch * cw;
+        for (int k = tid; k < nn; k += FFT_WG) {
+            const int v = k / ch, u = k - v * ch;
+            plane[k] = mp_pix(a, sec, g.cr0 + u, g.cc0 + v);
+        }
+        __syncthreads();
+        double s = 0.0;
+        for (int k = tid; k < nn; k += FFT_WG) s += (double)plane[k];
+        const double mean = fft_block_sum(s, red_d) / (double)nn;
+        double q = 0.0;
+        for (int k = tid; k < nn; k += FFT_WG) {
+            const double d = (double)plane[k] - mean;
+            q += d * d;
+        }
+        const double sq = fft_block_sum(q, red_d);
+        if (!(sq > 2.220446049250313e-16 * (double)ch * (double)cw)) {   // degenerate chip
+            write_skip(a, p, true);
+            __syncthreads();
+            continue;
+        }
+        const double cnorm = sqrt(sq);
+
+        // ---- Z = window + i * reversed Float32(chip - mean), zero-padded to fy x fx
+        {
+            const int64_t orow = g.wr0 - a.pady - 1, ocol = g.wc0 - a.padx - 1;
+            const bool inside = orow >= 0 && ocol >= 0 && orow + winh <= a.ny &&
+                                ocol + winw <= a.nx;
+            const float* wb = ref + (inside ? orow + ocol * a.ny : 0);
+            for (int t = tid; t < nz; t += FFT_WG) {
+                const int j = t / fy, i = t - j * fy;
+                float re = 0.0f, im = 0.0f;
+                if (i < winh && j < winw)
+                    re = inside ? wb[i + (int64_t)j * a.ny] : mp_pix(a, ref, g.wr0 + i, g.wc0 + j);
+                if (i < ch && j < cw)
+                    im = (float)((double)plane[(ch - 1 - i) + (cw - 1 - j) * ch] - mean);
+                Z[i + j * ld] = make_float2(re, im);
+            }
+        }
+        __syncthreads();
+
+        // ---- forward transform: columns (length fy), then rows (length fx)
+        fft_axis(Z, Ty, fy, fx, 1, ld, false);
+        fft_axis(Z, Tx, fx, fy, ld, 1, false);
+
+        // ---- the two spectra and their product, in place: A = (Z[k] + conj Z[-k]) / 2 and
+        // B = (Z[k] - conj Z[-k]) / 2i are Hermitian, so P[-k] = conj P[k]; each thread owns the
+        // pairs {k, -k} whose first index is the smaller
+        for (int t = tid; t < nz; t += FFT_WG) {
+            const int j = t / fy, i = t - j * fy;
+            const int mi = i == 0 ? 0 : fy - i, mj = j == 0 ? 0 : fx - j;
+            if (mi + mj * fy < t) continue;
+            const int k = i + j * ld, mk = mi + mj * ld;
+            const float2 z = Z[k], zm = cconj(Z[mk]);
+            const float2 A = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y + zm.y));
+            const float2 d = csub(z, zm);
+            const float2 Bq = make_float2(0.5f * d.y, -0.5f * d.x);   // d / 2i
+            const float2 P = cmul(A, Bq);
+            Z[k] = P;
+            if (mk != k) Z[mk] = cconj(P);
+        }
+        __syncthreads();
+
+        // ---- inverse transform; the correlation is the real part over fy fx
+        fft_axis(Z, Ty, fy, fx, 1, ld, true);
+        fft_axis(Z, Tx, fx, fy, ld, 1, true);
+
+        // ---- normalisation per shift (Float64, as the reference) and the argmax
+        const double n = (double)(ch * cw);
+        const int64_t bb = (g.wr0 - 1) + (g.wc0 - 1) * a.box_ld;
+        Best b = best_empty();
+        for (int t = tid; t < NR * NC; t += FFT_WG) {
+            const int j = t / NR, i = t - j * NR;
+            const double num = (double)(Z[(i + ch - 1) + (j + cw - 1) * ld].x * inv_nz);
+            const int64_t qb = bb + i + (int64_t)j * a.box_ld;
+            const float v = mp_normalise(num, a.fbox1[qb], a.fbox2[qb], n, cnorm, a.measure);
+            plane[t] = v;
+            const int idx = i * NC + j;
+            if (better(v, idx, b)) { b.v = v; b.idx = idx; }
+        }
+        b = fft_block_best(b, red);   // (its barriers also order the plane's writes)
+        if (b.idx == 0x7fffffff) b.idx = 0;
+        const int pi = b.idx / NC + 1, pj = b.idx % NC + 1;
+        int i0 = 1, j0 = 1;
+        const bool refine = a.up > 1 && patch_origin(pi, pj, NR, NC, i0, j0);
+        const float pval = plane[(pi - 1) + (pj - 1) * NR];
+        float pv = 0.0f;
+        if (refine && tid < 25) pv = plane[(i0 - 1 + tid / 5) + (j0 - 1 + tid % 5) * NR];
+        finish_point(a, p, g, pi, pj, pval, i0, j0, refine, pv);
+        __syncthreads();
+    }
+}
+
+
+}  // namespace ar
+
+using namespace ar;
+
+// next_fft_size (src/correlate.jl:357-403): the 7-smooth m in [n, 1.5 n] of least
+// m^2 log2(m) (1 - 0.35 (twos / log2 m)).
+int64_t ar_next_fft_size(int64_t n) {
+    if (n <= 1) return 1;
+    int64_t best = 0;
+    double best_cost = INFINITY;
+    const int64_t hi = (int64_t)std::ceil(1.5 * (double)n);
+    for (int64_t m = n; m <= hi; m++) {
+        int64_t r = m;
+        int twos = 0;
+        while (r % 2 == 0) { r /= 2; twos++; }
+        for (int64_t p : {3, 5, 7})
+            while (r % p == 0) r /= p;
+        if (r != 1) continue;
+        const double lg = std::log2((double)m);
+        const double cost = (double)m * (double)m * lg * (1.0 - 0.35 * ((double)twos / lg));
+        if (cost < best_cost) {
+            best = m;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
+ar_status ar_fft_setup(ar_context* ctx, TrackArgs& a, void (**kern)(TrackArgs), size_t* lds) {
+    const int64_t fy = ar_next_fft_size(a.max_ch + 2 * a.max_ry - 1);
+    const int64_t fx = ar_next_fft_size(a.max_cw + 2 * a.max_rx - 1);
+    if (fy > 128 || fx > 128 || !fft_radices((int)fy) || !fft_radices((int)fx))
+        return AR_ERR_UNSUPPORTED;
+    const FftLayout L = fft_layout((int)fy, (int)fx, a.max_ch, a.max_cw, a.max_rx, a.max_ry);
+    if (L.total > AR_LDS_LIMIT) return AR_ERR_UNSUPPORTED;
+    a.fft_fy = (int32_t)fy;
+    a.fft_fx = (int32_t)fx;
+    if (ar_status st = ar_f64_box_images(ctx, a)) return st;
+    *kern = track_fft_kernel;
+    *lds = L.total;
+    return AR_OK;
+}

@@ -56,11 +56,6 @@ template <> struct MpType<AR_PREC_FP16> {
     }
 };
 
-// Prepared Float32 pixel at padded 1-based (r, c); 0 in the padding.
-__device__ inline float mp_pix(const TrackArgs& a, const float* img, int64_t r, int64_t c) {
-    const int64_t k = pix(a, r, c);
-    return k < 0 ? 0.0f : img[k];
-}
 
 // sum chip' * W of one batch of 4 row-shift tiles (ti0 .. ti0 + 3) of column-shift tile tj.
 // Output rows are column shifts j, output columns row shifts i; K = 32 is four chip columns
@@ -101,18 +96,6 @@ __device__ inline void mp_batch(const uint16_t* Wl, const uint16_t* Cl, int ldw,
     }
 }
 
-// The reference's normalisation (src/correlate.jl:591-600 ZNCC, :618-627 NCC) of a numerator.
-__device__ inline float mp_normalise(double num, double s1, double s2, double n, double cnorm,
-                                     int measure) {
-    double den;
-    if (measure == 1) {
-        den = cnorm * sqrt(s2 < 0.0 ? 0.0 : s2);
-    } else {
-        const double wv = s2 - s1 * s1 / n;
-        den = cnorm * sqrt(wv < 0.0 ? 0.0 : wv);
-    }
-    return den > 0 ? (float)(num / den) : 0.0f;
-}
 
 template <int PREC, int NRB>
 __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
@@ -446,7 +429,20 @@ ar_status ar_mp_setup(ar_context* ctx, TrackArgs& a, void (**kern)(TrackArgs), s
     a.wscale = wscale;
     a.cscale = cscale;
     a.numscale = numscale;
-    // Float64 box images over the padded image's chip positions
+    if (ar_status st = ar_f64_box_images(ctx, a)) return st;
+#define AR_MP_PICK(P)                                                          \
+    *kern = nrb == 4 ? track_mp_kernel<P, 4> : nrb == 6 ? track_mp_kernel<P, 6> \
+          : nrb == 8 ? track_mp_kernel<P, 8> : nrb == 10 ? track_mp_kernel<P, 10> \
+          : nrb == 14 ? track_mp_kernel<P, 14> : track_mp_kernel<P, 18>;
+    if (prec == AR_PREC_BF16) { AR_MP_PICK(AR_PREC_BF16) }
+    else { AR_MP_PICK(AR_PREC_FP16) }
+#undef AR_MP_PICK
+    return AR_OK;
+}
+
+// Float64 box images of W and W^2 over the padded Float32 reference's chip positions for the
+// pass's chip size (a.max_ch x a.max_cw): a.fbox1, a.fbox2, a.box_ld.
+ar_status ar_f64_box_images(ar_context* ctx, TrackArgs& a) {
     const int64_t NY = ctx->ny + 2 * a.pady, NX = ctx->nx + 2 * a.padx;
     const int64_t HB = NY - a.max_ch + 1, WB = NX - a.max_cw + 1;
     if (HB < 1 || WB < 1) return AR_ERR_UNSUPPORTED;
@@ -465,12 +461,5 @@ ar_status ar_mp_setup(ar_context* ctx, TrackArgs& a, void (**kern)(TrackArgs), s
     a.fbox1 = B1;
     a.fbox2 = B2;
     a.box_ld = HB;
-#define AR_MP_PICK(P)                                                          \
-    *kern = nrb == 4 ? track_mp_kernel<P, 4> : nrb == 6 ? track_mp_kernel<P, 6> \
-          : nrb == 8 ? track_mp_kernel<P, 8> : nrb == 10 ? track_mp_kernel<P, 10> \
-          : nrb == 14 ? track_mp_kernel<P, 14> : track_mp_kernel<P, 18>;
-    if (prec == AR_PREC_BF16) { AR_MP_PICK(AR_PREC_BF16) }
-    else { AR_MP_PICK(AR_PREC_FP16) }
-#undef AR_MP_PICK
     return AR_OK;
 }

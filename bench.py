@@ -288,18 +288,21 @@ def coherence_block(autorift, steps, n=2048):
 
 
 FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 vector (and matrix), TFLOP/s
+FP32_PEAK_TFLOPS = 157.3         # MI355X FP32 (vector and matrix), TFLOP/s
 BF16_PEAK_TFLOPS = 2500.0        # MI355X dense BF16 / FP16 matrix, TFLOP/s
 
 
 def mixed_block(autorift, tex, sec, pts, steps):
     """The Float32 pair (quantize = :none) over the full grid: the exact direct Float64 tracker
-    (the plain Float32 drop-in, bit-exact) and the BF16 / FP16 matrix-core tracker, each with the
-    roofline of its arithmetic; the one-kernel generic Float64 tracker on every 7th grid row and
-    column for comparison; agreement on the 512^2 shifted pair."""
+    (the plain Float32 drop-in, bit-exact), the single-precision FFT tracker (the reference's own
+    numerator), and the BF16 / FP16 matrix-core tracker, each with the roofline of its
+    arithmetic (the FFT's as direct-equivalent multiply-adds against the FP32 peak); the
+    one-kernel generic Float64 tracker on every 7th grid row and column for comparison;
+    agreement on the 512^2 shifted pair."""
     import os as _os
     out = {}
     flop = FLOP_PER_POINT
-    for prec in ("bf16", "fp16", "exact", "generic"):
+    for prec in ("bf16", "fp16", "fft", "exact", "generic"):
         ctx = autorift.Context(autorift.params(quantize="none",
                                                precision="exact" if prec == "generic" else prec))
         ctx.set_images(tex, sec)
@@ -320,18 +323,23 @@ def mixed_block(autorift, tex, sec, pts, steps):
         n_s = int(np.count_nonzero(r.searched))
         out[prec] = {"points": len(use), "ms": ms, "track_ms": ms_t,
                      "points_per_s": n_s / (ms / 1000), "path": ctx.last_kernel()[1]}
-        peak = FP64_PEAK_TFLOPS if prec in ("exact", "generic") else BF16_PEAK_TFLOPS
+        peak = (FP64_PEAK_TFLOPS if prec in ("exact", "generic") else
+                FP32_PEAK_TFLOPS if prec == "fft" else BF16_PEAK_TFLOPS)
         ach = n_s * flop / (ms_t / 1000) / 1e12
         # the tracker phase: HIP events on the pass's stream around the correlation launches
         # (for the direct Float64 tracker: the first chunk's preparation, then every
         # track_f64_kernel launch; later chunks' preparation overlaps on a second stream)
-        out[prec]["roofline"] = {"bound": "fp64" if peak == FP64_PEAK_TFLOPS else "mfma",
+        out[prec]["roofline"] = {"bound": {FP64_PEAK_TFLOPS: "fp64", FP32_PEAK_TFLOPS: "fp32"}
+                                 .get(peak, "mfma"),
                                  "achieved": ach, "peak": peak, "unit": "TFLOP/s",
                                  "frac": ach / peak, "scope": "tracker phase"}
+        if prec == "fft":
+            out[prec]["roofline"]["work"] = "direct-equivalent multiply-adds (5.12 MFLOP a point)"
         ctx.close()
         log(f"mixed {prec}: {out[prec]['points_per_s']:.0f} points/s over {len(use)} points "
             f"({ach:.1f} TFLOP/s)")
     out["speedup_bf16_vs_exact"] = out["bf16"]["points_per_s"] / out["exact"]["points_per_s"]
+    out["speedup_fft_vs_exact"] = out["fft"]["points_per_s"] / out["exact"]["points_per_s"]
     out["speedup_exact_vs_generic"] = (out["exact"]["points_per_s"] /
                                        out["generic"]["points_per_s"])
     # agreement on the 512^2 pair shifted by (3.30, -1.70)
@@ -339,13 +347,13 @@ def mixed_block(autorift, tex, sec, pts, steps):
     xs = np.arange(40.0, 473.0, 16.0)
     g = autorift.gridpoints_xy(xs, xs, chip_size=32, search_radius=25)
     res = {}
-    for prec in ("exact", "bf16", "fp16"):
+    for prec in ("exact", "bf16", "fp16", "fft"):
         with autorift.Context(autorift.params(quantize="none", precision=prec)) as c:
             c.set_images(a, b)
             res[prec] = (c.track(g, upsampling=1), c.track(g, upsampling=64))
     e1, e64 = res["exact"]
     ok = np.isfinite(e64.dx)
-    for prec in ("bf16", "fp16"):
+    for prec in ("bf16", "fp16", "fft"):
         r1, r64 = res[prec]
         same = (r1.dx[ok] == e1.dx[ok]) & (r1.dy[ok] == e1.dy[ok])
         err = np.hypot(r64.dx[ok] - e64.dx[ok], r64.dy[ok] - e64.dy[ok])

@@ -46,9 +46,13 @@ typedef enum { AR_QUANT_NONE = 0, AR_QUANT_UINT8 = 1 } ar_quantize;
 /* Arithmetic of the correlation contraction for Float32 prepared images (quantize = :none).
  * EXACT follows the reference's Float64 direct numerator (src/correlate.jl:799-814); BF16 and
  * FP16 run it on the matrix cores with 16-bit operands and FP32 accumulation, keeping Float64
- * for the box sums, the normalisation and the integer-peak / subpixel-patch values. UInt8
- * prepared images always take the exact integer path. No counterpart in the reference. */
-typedef enum { AR_PREC_EXACT = 0, AR_PREC_BF16 = 1, AR_PREC_FP16 = 2 } ar_precision;
+ * for the box sums, the normalisation and the integer-peak / subpixel-patch values. FFT is the
+ * reference's own dispatch (src/correlate.jl:785-797): above DIRECT_THRESHOLD = 20 000
+ * multiply-adds per point the numerator is a single-precision transform
+ * (_numerators_fft!, 816-848), below it the exact direct form; transforms larger than 128 per
+ * side take the exact direct form too. UInt8 prepared images always take the exact integer
+ * path. */
+typedef enum { AR_PREC_EXACT = 0, AR_PREC_BF16 = 1, AR_PREC_FP16 = 2, AR_PREC_FFT = 3 } ar_precision;
 
 /* Flattened Params (src/types.jl:781-829) with the defaults of params() (src/params.jl:306-392).
  * Built by the host from its resolved parameter set; the library never parses strings. */
