@@ -28,8 +28,17 @@
 namespace ar {
 
 __device__ inline float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
 }
+
+// floor(a / d) for 0 <= a < 256 and 1 <= d <= 128 as (a m) >> 16, m = ceil(2^16 / d): with
+// e = m d - 2^16 < d, a e < 2^16 keeps the product's excess over a / d below 1 / d, so the floor
+// is exact.
+struct SmallDiv {
+    int d, m;
+};
+__device__ inline SmallDiv small_div(int d) { return SmallDiv{d, (65536 + d - 1) / d}; }
+__device__ inline int sdiv(int a, SmallDiv v) { return (a * v.m) >> 16; }
 __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -69,8 +78,8 @@ template <> struct Roots<7> {
 
 // DFT of odd size R by its symmetric form: with s_r = v_r + v_{R-r}, d_r = v_r - v_{R-r},
 // out[q] = v_0 + sum_r (s_r cos - i d_r sin)(2 pi r q / R) and out[R - q] with +i (forward).
-template <int R>
-__device__ inline void dft_odd(const float2 (&v)[R], float2 (&o)[R], bool inv) {
+template <int R, bool INV>
+__device__ inline void dft_odd(const float2 (&v)[R], float2 (&o)[R]) {
     constexpr int H = (R - 1) / 2;
     float2 sm[H + 1], df[H + 1];
     float2 o0 = v[0];
@@ -87,73 +96,36 @@ __device__ inline void dft_odd(const float2 (&v)[R], float2 (&o)[R], bool inv) {
 #pragma unroll
         for (int r = 1; r <= H; r++) {
             const float c = Roots<R>::c[(r * q) % R], sn = Roots<R>::s[(r * q) % R];
-            av.x += sm[r].x * c;
-            av.y += sm[r].y * c;
-            bv.x += df[r].x * sn;
-            bv.y += df[r].y * sn;
+            av.x = fmaf(sm[r].x, c, av.x);
+            av.y = fmaf(sm[r].y, c, av.y);
+            bv.x = fmaf(df[r].x, sn, bv.x);
+            bv.y = fmaf(df[r].y, sn, bv.y);
         }
         // forward: out[q] = a - i b, out[R - q] = a + i b; inverse the other way round
         const float2 mib = make_float2(bv.y, -bv.x), pib = make_float2(-bv.y, bv.x);
-        o[q] = cadd(av, inv ? pib : mib);
-        o[R - q] = cadd(av, inv ? mib : pib);
+        o[q] = cadd(av, INV ? pib : mib);
+        o[R - q] = cadd(av, INV ? mib : pib);
     }
 }
 
-// One wave's Stockham pass of radix R over lines l0 .. l0 + S - 1 of the array: element (line l,
-// position t) at Z[l ls + t es]; lines of length n, Ns = product of the radices already
-// applied. T holds exp(-2 pi i m / n), m < n; inv conjugates every factor. S n <= 256, so a lane
-// holds at most U butterflies; every read of the pass precedes every write in the wave.
-template <int R>
-__device__ inline void fft_stage(float2* Z, const float2* T, int n, int Ns, int l0, int S, int es,
-                                 int ls, bool inv) {
-    constexpr int U = (256 / R + 63) / 64;
-    const int lane = threadIdx.x & 63;
-    const int nb = n / R, B = S * nb, step = n / (Ns * R);
-    float2 out[U][R];
-    int dst[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-        const int b = lane + 64 * u;
-        dst[u] = -1;
-        if (b < B) {
-            const int lb = b / nb, jj = b - lb * nb, k = jj % Ns;
-            const int base = (l0 + lb) * ls;
-            float2 v[R];
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                float2 x = Z[base + (jj + r * nb) * es];
-                if (r > 0) {
-                    float2 w = T[r * k * step];
-                    if (inv) w = cconj(w);
-                    x = cmul(x, w);
-                }
-                v[r] = x;
-            }
-            if constexpr (R == 2) {
-                out[u][0] = cadd(v[0], v[1]);
-                out[u][1] = csub(v[0], v[1]);
-            } else if constexpr (R == 4) {
-                const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-                const float2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
-                // forward: (-i) b1; inverse: (+i) b1
-                const float2 jb = inv ? make_float2(-b1.y, b1.x) : make_float2(b1.y, -b1.x);
-                out[u][0] = cadd(a0, b0);
-                out[u][2] = csub(a0, b0);
-                out[u][1] = cadd(a1, jb);
-                out[u][3] = csub(a1, jb);
-            } else {
-                dft_odd<R>(v, out[u], inv);
-            }
-            dst[u] = base + ((jj / Ns) * Ns * R + k) * es;
-        }
+// DFT of size R (forward: exp(-2 pi i r q / R); INV: the conjugate kernel)
+template <int R, bool INV>
+__device__ inline void dft(const float2 (&v)[R], float2 (&o)[R]) {
+    if constexpr (R == 2) {
+        o[0] = cadd(v[0], v[1]);
+        o[1] = csub(v[0], v[1]);
+    } else if constexpr (R == 4) {
+        const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        const float2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+        // forward: (-i) b1; inverse: (+i) b1
+        const float2 jb = INV ? make_float2(-b1.y, b1.x) : make_float2(b1.y, -b1.x);
+        o[0] = cadd(a0, b0);
+        o[2] = csub(a0, b0);
+        o[1] = cadd(a1, jb);
+        o[3] = csub(a1, jb);
+    } else {
+        dft_odd<R, INV>(v, o);
     }
-    wave_sync();
-#pragma unroll
-    for (int u = 0; u < U; u++)
-        if (dst[u] >= 0)
-#pragma unroll
-            for (int q = 0; q < R; q++) Z[dst[u] + q * Ns * es] = out[u][q];
-    wave_sync();
 }
 
 // Threads of the FFT tracker's workgroup: eight waves (the transform's LDS admits two
@@ -161,36 +133,119 @@ __device__ inline void fft_stage(float2* Z, const float2* T, int n, int Ns, int 
 #define FFT_WG AR_FFT_WG
 #define FFT_NW (FFT_WG / 64)
 
-// The 1-D transform of length n (at most 128) along one axis of the array, nlines lines; each
-// wave takes batches of 256 / n lines. Ends with a workgroup barrier.
-__device__ inline void fft_axis(float2* Z, const float2* T, int n, int nlines, int es, int ls,
-                                bool inv) {
-    const uint64_t rad = fft_radices(n);
-    const int S = 256 / n;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int l0 = wave * S; l0 < nlines; l0 += FFT_NW * S) {
-        const int sl = nlines - l0 < S ? nlines - l0 : S;
-        int Ns = 1;
-        for (uint64_t rr = rad; rr != 0; rr >>= 4) {
-            const int R = (int)(rr & 15);
-            switch (R) {
-                case 4: fft_stage<4>(Z, T, n, Ns, l0, sl, es, ls, inv); break;
-                case 2: fft_stage<2>(Z, T, n, Ns, l0, sl, es, ls, inv); break;
-                case 3: fft_stage<3>(Z, T, n, Ns, l0, sl, es, ls, inv); break;
-                case 5: fft_stage<5>(Z, T, n, Ns, l0, sl, es, ls, inv); break;
-                default: fft_stage<7>(Z, T, n, Ns, l0, sl, es, ls, inv); break;
-            }
-            Ns *= R;
-        }
-    }
-    __syncthreads();
+// The workgroup's items of one pass: item = line + nl * butterfly, thread t takes items
+// t, t + FFT_WG, ...; consecutive lanes take consecutive lines, so every access of a stage
+// moves along the line stride (1, or the odd column stride) and no two lanes of a half-wave meet
+// on an LDS bank.
+struct PassWalk {
+    int line, bf, dl, db;
+};
+__device__ inline PassWalk pass_walk(int nl) {
+    PassWalk w;
+    w.bf = threadIdx.x / nl;
+    w.line = threadIdx.x - w.bf * nl;
+    w.db = FFT_WG / nl;
+    w.dl = FFT_WG - w.db * nl;
+    return w;
 }
 
-// LDS of the FFT tracker: the complex array (columns of ld = fy | 1 elements: an odd stride keeps
-// the row passes' lanes on distinct banks), the two twiddle tables, and one Float32 plane for the
-// chip and then the surface.
+// One in-place radix-R stage over lines l0 .. l0 + nl - 1 (element t of line l at
+// Z[l ls + t es]) of length n: blocks of L = R m elements, butterfly (blk, j) on the elements
+// blk L + j + r m. Forward (decimation in frequency): DFT_R, then output q times
+// exp(-2 pi i j q / L). Inverse (decimation in time, the adjoint): inputs times the conjugate
+// factors, then the conjugate DFT_R. T holds exp(-2 pi i k / n), k < n.
+template <int R, bool INV>
+__device__ inline void fft_stage(float2* Z, const float2* T, int n, int L, int m, int l0, int nl,
+                                 int es, int ls, PassWalk w) {
+    const int nbf = n / R, tw = n / L, st = m * es;
+    const SmallDiv dm = small_div(m);
+    for (; w.bf < nbf;) {
+        const int blk = sdiv(w.bf, dm), j = w.bf - blk * m;
+        const int base = (l0 + w.line) * ls + (blk * L + j) * es;
+        float2 v[R], o[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = Z[base + r * st];
+        if (INV && m > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) v[r] = cmul(v[r], cconj(T[j * r * tw]));
+        }
+        dft<R, INV>(v, o);
+        if (!INV && m > 1) {
+#pragma unroll
+            for (int q = 1; q < R; q++) o[q] = cmul(o[q], T[j * q * tw]);
+        }
+#pragma unroll
+        for (int q = 0; q < R; q++) Z[base + q * st] = o[q];
+        w.line += w.dl;
+        w.bf += w.db;
+        if (w.line >= nl) {
+            w.line -= nl;
+            w.bf++;
+        }
+    }
+}
+
+// The 1-D transform of length n (at most 128, radices rad) of lines l0 .. l0 + nl - 1 along
+// one axis. Forward leaves frequency f at its mixed-radix digit-reversed position (f = q1 + R1
+// (q2 + R2 (...)) at q1 m1 + q2 m2 + ..., m_s = n / (R1 ... R_s)); inverse takes that order back
+// to natural order. A workgroup barrier ends every stage.
+template <bool INV>
+__device__ __attribute__((always_inline)) inline void fft_pass(float2* Z, const float2* T, int n,
+                                                               uint64_t rad, int l0, int nl, int es,
+                                                               int ls) {
+    const PassWalk w = pass_walk(nl);
+    int ns = 0;
+    for (uint64_t rr = rad; rr != 0; rr >>= 4) ns++;
+    int m = INV ? 1 : n;
+    for (int k = 0; k < ns; k++) {
+        const int R = (int)((rad >> (4 * (INV ? ns - 1 - k : k))) & 15);
+        int L;
+        if (INV) {
+            L = m * R;
+        } else {
+            L = m;
+            m = L / R;
+        }
+        switch (R) {
+            case 4: fft_stage<4, INV>(Z, T, n, L, m, l0, nl, es, ls, w); break;
+            case 2: fft_stage<2, INV>(Z, T, n, L, m, l0, nl, es, ls, w); break;
+            case 3: fft_stage<3, INV>(Z, T, n, L, m, l0, nl, es, ls, w); break;
+            case 5: fft_stage<5, INV>(Z, T, n, L, m, l0, nl, es, ls, w); break;
+            default: fft_stage<7, INV>(Z, T, n, L, m, l0, nl, es, ls, w); break;
+        }
+        if (INV) m = L;
+        __syncthreads();
+    }
+}
+
+// Position of -f for the frequency f held at digit-reversed position p (see fft_pass).
+__device__ inline int fft_mirror_pos(int p, int n, uint64_t rad) {
+    int m = n, P = 1, f = 0, rem = p;
+    for (uint64_t rr = rad; rr != 0; rr >>= 4) {
+        const int R = (int)(rr & 15);
+        m /= R;
+        const int q = rem / m;
+        rem -= q * m;
+        f += q * P;
+        P *= R;
+    }
+    int mf = f == 0 ? 0 : n - f, pos = 0;
+    m = n;
+    for (uint64_t rr = rad; rr != 0; rr >>= 4) {
+        const int R = (int)(rr & 15);
+        m /= R;
+        pos += (mf % R) * m;
+        mf /= R;
+    }
+    return pos;
+}
+
+// LDS of the FFT tracker: the complex array (columns of ld = fy | 1 elements: an odd column
+// stride keeps the column passes' lanes on distinct banks), the two twiddle tables, the two mirror
+// tables (position of -f per position of f), and one Float32 plane for the chip and then the
+// surface.
 struct FftLayout {
-    size_t off_z, off_ty, off_tx, off_s, total;
+    size_t off_z, off_ty, off_tx, off_my, off_mx, off_s, total;
 };
 __host__ __device__ inline FftLayout fft_layout(int fy, int fx, int max_ch, int max_cw,
                                                 int max_rx, int max_ry) {
@@ -198,7 +253,9 @@ __host__ __device__ inline FftLayout fft_layout(int fy, int fx, int max_ch, int 
     L.off_z = 0;
     L.off_ty = (size_t)(fy | 1) * fx * 8;
     L.off_tx = L.off_ty + (size_t)fy * 8;
-    L.off_s = L.off_tx + (size_t)fx * 8;
+    L.off_my = L.off_tx + (size_t)fx * 8;
+    L.off_mx = L.off_my + 128;
+    L.off_s = L.off_mx + 128;
     const size_t surf = (size_t)(2 * max_ry) * (2 * max_rx), chip = (size_t)max_ch * max_cw;
     L.total = L.off_s + 4 * (surf > chip ? surf : chip);
     return L;
@@ -243,26 +300,30 @@ __global__ __launch_bounds__(FFT_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double red_d[FFT_NW];
     __shared__ Best red[FFT_NW];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int fy = a.fft_fy, fx = a.fft_fx;
+    const uint64_t rady = fft_radices(fy), radx = fft_radices(fx);
     const FftLayout L = fft_layout(fy, fx, a.max_ch, a.max_cw, a.max_rx, a.max_ry);
     float2* Z = (float2*)(smem + L.off_z);
     float2* Ty = (float2*)(smem + L.off_ty);
     float2* Tx = (float2*)(smem + L.off_tx);
+    unsigned char* mir_y = smem + L.off_my;
+    unsigned char* mir_x = smem + L.off_mx;
     float* plane = (float*)(smem + L.off_s);
     const float* ref = (const float*)a.ref;
     const float* sec = (const float*)a.sec;
-    // twiddles exp(-2 pi i m / n): Float64 sincospi, rounded once
+    // twiddles exp(-2 pi i m / n): Float64 sincospi, rounded once; mirror positions
     for (int m = tid; m < fy + fx; m += FFT_WG) {
         const bool y = m < fy;
         const int n = y ? fy : fx, mm = y ? m : m - fy;
         double sv, cv;
         sincospi(2.0 * (double)mm / (double)n, &sv, &cv);
         (y ? Ty : Tx)[mm] = make_float2((float)cv, (float)-sv);
+        (y ? mir_y : mir_x)[mm] = (unsigned char)fft_mirror_pos(mm, n, y ? rady : radx);
     }
     __syncthreads();
-    const int nz = fy * fx, ld = fy | 1;
-    const float inv_nz = 1.0f / (float)nz;
+    const int ld = fy | 1;
+    const float inv_nz = 1.0f / (float)(fy * fx);
 
     const PointRange pr = point_range(a.n);
     for (int64_t p = pr.first; p < pr.end; p += pr.step) {
@@ -271,11 +332,22 @@ __global__ __launch_bounds__(FFT_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
         const int ch = g.ch, cw = g.cw, NR = 2 * g.ry, NC = 2 * g.rx;
         const int winh = ch + NR - 1, winw = cw + NC - 1;
 
-        // ---- chip: Float32 samples, Float64 mean and energy (block sums in a fixed order)
+        // ---- chip: Float32 samples, Float64 mean and energy (block sums in a fixed order).
+        // Extents are at most 128 (two rows per lane); four columns per wave at a time keep
+        // eight loads in flight.
         const int nn = ch * cw;
-        for (int k = tid; k < nn; k += FFT_WG) {
-            const int v = k / ch, u = k - v * ch;
-            plane[k] = mp_pix(a, sec, g.cr0 + u, g.cc0 + v);
+        for (int v0 = wave; v0 < cw; v0 += 4 * FFT_NW) {
+            float c[4][2];
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int v = v0 + (e >> 1) * FFT_NW, u = lane + 64 * (e & 1);
+                c[e >> 1][e & 1] = v < cw && u < ch ? mp_pix(a, sec, g.cr0 + u, g.cc0 + v) : 0.0f;
+            }
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int v = v0 + (e >> 1) * FFT_NW, u = lane + 64 * (e & 1);
+                if (v < cw && u < ch) plane[u + v * ch] = c[e >> 1][e & 1];
+            }
         }
         __syncthreads();
         double s = 0.0;
@@ -300,56 +372,83 @@ __global__ __launch_bounds__(FFT_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
             const bool inside = orow >= 0 && ocol >= 0 && orow + winh <= a.ny &&
                                 ocol + winw <= a.nx;
             const float* wb = ref + (inside ? orow + ocol * a.ny : 0);
-            for (int t = tid; t < nz; t += FFT_WG) {
-                const int j = t / fy, i = t - j * fy;
-                float re = 0.0f, im = 0.0f;
-                if (i < winh && j < winw)
-                    re = inside ? wb[i + (int64_t)j * a.ny] : mp_pix(a, ref, g.wr0 + i, g.wc0 + j);
-                if (i < ch && j < cw)
-                    im = (float)((double)plane[(ch - 1 - i) + (cw - 1 - j) * ch] - mean);
-                Z[i + j * ld] = make_float2(re, im);
+            for (int j0 = wave; j0 < fx; j0 += 4 * FFT_NW) {
+                float re[4][2];
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int j = j0 + (e >> 1) * FFT_NW, i = lane + 64 * (e & 1);
+                    float x = 0.0f;
+                    if (i < winh && j < winw)
+                        x = inside ? wb[i + (int64_t)j * a.ny] : mp_pix(a, ref, g.wr0 + i, g.wc0 + j);
+                    re[e >> 1][e & 1] = x;
+                }
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int j = j0 + (e >> 1) * FFT_NW, i = lane + 64 * (e & 1);
+                    if (j >= fx || i >= fy) continue;
+                    float im = 0.0f;
+                    if (i < ch && j < cw)
+                        im = (float)((double)plane[(ch - 1 - i) + (cw - 1 - j) * ch] - mean);
+                    Z[i + j * ld] = make_float2(re[e >> 1][e & 1], im);
+                }
             }
         }
         __syncthreads();
 
-        // ---- forward transform: columns (length fy), then rows (length fx)
-        fft_axis(Z, Ty, fy, fx, 1, ld, false);
-        fft_axis(Z, Tx, fx, fy, ld, 1, false);
+        // ---- forward transform: columns (length fy; columns from winw on are zero), then rows
+        fft_pass<false>(Z, Ty, fy, rady, 0, winw, 1, ld);
+        fft_pass<false>(Z, Tx, fx, radx, 0, fy, ld, 1);
 
         // ---- the two spectra and their product, in place: A = (Z[k] + conj Z[-k]) / 2 and
         // B = (Z[k] - conj Z[-k]) / 2i are Hermitian, so P[-k] = conj P[k]; each thread owns the
-        // pairs {k, -k} whose first index is the smaller
-        for (int t = tid; t < nz; t += FFT_WG) {
-            const int j = t / fy, i = t - j * fy;
-            const int mi = i == 0 ? 0 : fy - i, mj = j == 0 ? 0 : fx - j;
-            if (mi + mj * fy < t) continue;
-            const int k = i + j * ld, mk = mi + mj * ld;
-            const float2 z = Z[k], zm = cconj(Z[mk]);
-            const float2 A = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y + zm.y));
-            const float2 d = csub(z, zm);
-            const float2 Bq = make_float2(0.5f * d.y, -0.5f * d.x);   // d / 2i
-            const float2 P = cmul(A, Bq);
-            Z[k] = P;
-            if (mk != k) Z[mk] = cconj(P);
-        }
+        // pairs {k, -k} whose first position is the smaller (positions are digit-reversed)
+        for (int j = wave; j < fx; j += FFT_NW)
+            for (int i = lane; i < fy; i += 64) {
+                const int mi = mir_y[i], mj = mir_x[j];
+                if (mi + mj * fy < i + j * fy) continue;
+                const int k = i + j * ld, mk = mi + mj * ld;
+                const float2 z = Z[k], zm = cconj(Z[mk]);
+                const float2 A = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y + zm.y));
+                const float2 d = csub(z, zm);
+                const float2 Bq = make_float2(0.5f * d.y, -0.5f * d.x);   // d / 2i
+                const float2 P = cmul(A, Bq);
+                Z[k] = P;
+                if (mk != k) Z[mk] = cconj(P);
+            }
         __syncthreads();
 
-        // ---- inverse transform; the correlation is the real part over fy fx
-        fft_axis(Z, Ty, fy, fx, 1, ld, true);
-        fft_axis(Z, Tx, fx, fy, ld, 1, true);
+        // ---- inverse transform (natural order again); only rows ch - 1 .. ch + NR - 2 of the
+        // row pass are read. The correlation is the real part over fy fx.
+        fft_pass<true>(Z, Ty, fy, rady, 0, fx, 1, ld);
+        fft_pass<true>(Z, Tx, fx, radx, ch - 1, NR, ld, 1);
 
         // ---- normalisation per shift (Float64, as the reference) and the argmax
         const double n = (double)(ch * cw);
         const int64_t bb = (g.wr0 - 1) + (g.wc0 - 1) * a.box_ld;
         Best b = best_empty();
-        for (int t = tid; t < NR * NC; t += FFT_WG) {
-            const int j = t / NR, i = t - j * NR;
-            const double num = (double)(Z[(i + ch - 1) + (j + cw - 1) * ld].x * inv_nz);
-            const int64_t qb = bb + i + (int64_t)j * a.box_ld;
-            const float v = mp_normalise(num, a.fbox1[qb], a.fbox2[qb], n, cnorm, a.measure);
-            plane[t] = v;
-            const int idx = i * NC + j;
-            if (better(v, idx, b)) { b.v = v; b.idx = idx; }
+        for (int j0 = wave; j0 < NC; j0 += 2 * FFT_NW) {
+            double s1[2][2], s2[2][2];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int j = j0 + (e >> 1) * FFT_NW, i = lane + 64 * (e & 1);
+                s1[e >> 1][e & 1] = s2[e >> 1][e & 1] = 0.0;
+                if (j < NC && i < NR) {
+                    const int64_t qb = bb + i + (int64_t)j * a.box_ld;
+                    s1[e >> 1][e & 1] = a.fbox1[qb];
+                    s2[e >> 1][e & 1] = a.fbox2[qb];
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int j = j0 + (e >> 1) * FFT_NW, i = lane + 64 * (e & 1);
+                if (j >= NC || i >= NR) continue;
+                const double num = (double)(Z[(i + ch - 1) + (j + cw - 1) * ld].x * inv_nz);
+                const float v = mp_normalise(num, s1[e >> 1][e & 1], s2[e >> 1][e & 1], n, cnorm,
+                                             a.measure);
+                plane[i + j * NR] = v;
+                const int idx = i * NC + j;
+                if (better(v, idx, b)) { b.v = v; b.idx = idx; }
+            }
         }
         b = fft_block_best(b, red);   // (its barriers also order the plane's writes)
         if (b.idx == 0x7fffffff) b.idx = 0;
