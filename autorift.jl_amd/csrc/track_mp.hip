@@ -306,57 +306,82 @@ __device__ inline double boxf_pix(const float* img, int64_t ny, int64_t nx, int6
 
 #define AR_BOXF_BLK 64
 // V1 (r, c) = sum_{u < ch} W(r + u, c), V2 likewise of W^2, r < HB, every padded column c. One
-// thread per output row and column: consecutive lanes take consecutive rows of a column, so
-// every load and store is coalesced, and the ch terms of a sum are added directly (the
-// overlapping loads of neighbouring lanes hit in the caches).
-__global__ __launch_bounds__(256) void box_f64_colpass_kernel(const float* img, int64_t ny,
-                                                              int64_t nx, int64_t pady,
+// thread per output row and column (workgroup = 256 rows of one column, nrb workgroups per
+// column): consecutive lanes take consecutive rows, so every load and store is coalesced, and the
+// ch terms of a sum are added directly in order (the overlapping loads of neighbouring lanes hit
+// in the caches), eight loads in flight at a time.
+__global__ __launch_bounds__(256) void box_f64_colpass_kernel(const float* __restrict__ img,
+                                                              int64_t ny, int64_t nx, int64_t pady,
                                                               int64_t padx, int ch, int64_t HB,
-                                                              int64_t NX, double* V1, double* V2) {
-    const int64_t total = HB * NX;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = t / HB, r = t - c * HB;
-        double s1 = 0.0, s2 = 0.0;
-        const int64_t ocol = c - padx;
-        if (ocol >= 0 && ocol < nx) {
-            const float* col = img + ocol * ny;
+                                                              int nrb, double* __restrict__ V1,
+                                                              double* __restrict__ V2) {
+    const int64_t c = blockIdx.x / (unsigned)nrb;
+    const int64_t r = (int64_t)(blockIdx.x - (unsigned)c * (unsigned)nrb) * 256 + threadIdx.x;
+    if (r >= HB) return;
+    double s1 = 0.0, s2 = 0.0;
+    const int64_t ocol = c - padx, o0 = r - pady;
+    if (ocol >= 0 && ocol < nx) {
+        const float* col = img + ocol * ny;
+        if (o0 >= 0 && o0 + ch <= ny) {
+            const float* q = col + o0;
+            int u = 0;
+            for (; u + 8 <= ch; u += 8) {
+                float w[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) w[k] = q[u + k];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const double d = (double)w[k];
+                    s1 += d;
+                    s2 += d * d;
+                }
+            }
+            for (; u < ch; u++) {
+                const double d = (double)q[u];
+                s1 += d;
+                s2 += d * d;
+            }
+        } else {
             for (int u = 0; u < ch; u++) {
-                const int64_t orow = r + u - pady;
-                const double w = (orow >= 0 && orow < ny) ? (double)col[orow] : 0.0;
-                s1 += w;
-                s2 += w * w;
+                const int64_t orow = o0 + u;
+                const double d = (orow >= 0 && orow < ny) ? (double)col[orow] : 0.0;
+                s1 += d;
+                s2 += d * d;
             }
         }
-        V1[t] = s1;
-        V2[t] = s2;
     }
+    V1[c * HB + r] = s1;
+    V2[c * HB + r] = s2;
 }
 
-// B (r, c) = sum_{v < cw} V(r, c + v), c < WB. One thread per (row, block of 64 columns);
-// consecutive threads take consecutive rows.
-__global__ __launch_bounds__(256) void box_f64_rowpass_kernel(const double* V1, const double* V2,
+// B (r, c) = sum_{v < cw} V(r, c + v), c < WB. One thread per (row, block of 64 columns):
+// the block's first sum in order, then a sliding sum; consecutive lanes take consecutive rows
+// (workgroup = 256 rows of one column block, nrb workgroups per block).
+__global__ __launch_bounds__(256) void box_f64_rowpass_kernel(const double* __restrict__ V1,
+                                                              const double* __restrict__ V2,
                                                               int cw, int64_t HB, int64_t WB,
-                                                              double* B1, double* B2) {
-    const int64_t nblk = (WB + AR_BOXF_BLK - 1) / AR_BOXF_BLK;
-    const int64_t total = nblk * HB;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t blk = t / HB, r = t - blk * HB;
-        const int64_t c0 = blk * AR_BOXF_BLK;
-        const int64_t c1 = c0 + AR_BOXF_BLK < WB ? c0 + AR_BOXF_BLK : WB;
-        double s1 = 0.0, s2 = 0.0;
-        for (int v = 0; v < cw; v++) {
-            s1 += V1[r + (c0 + v) * HB];
-            s2 += V2[r + (c0 + v) * HB];
-        }
-        for (int64_t c = c0; c < c1; c++) {
-            B1[r + c * HB] = s1;
-            B2[r + c * HB] = s2;
-            if (c + 1 < c1) {
-                s1 = (s1 + V1[r + (c + cw) * HB]) - V1[r + c * HB];
-                s2 = (s2 + V2[r + (c + cw) * HB]) - V2[r + c * HB];
-            }
+                                                              int nrb, double* __restrict__ B1,
+                                                              double* __restrict__ B2) {
+    const int64_t blk = blockIdx.x / (unsigned)nrb;
+    const int64_t r = (int64_t)(blockIdx.x - (unsigned)blk * (unsigned)nrb) * 256 + threadIdx.x;
+    if (r >= HB) return;
+    const int64_t c0 = blk * AR_BOXF_BLK;
+    const int64_t c1 = c0 + AR_BOXF_BLK < WB ? c0 + AR_BOXF_BLK : WB;
+    const double* v1 = V1 + r;
+    const double* v2 = V2 + r;
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll 8
+    for (int v = 0; v < cw; v++) {
+        s1 += v1[(c0 + v) * HB];
+        s2 += v2[(c0 + v) * HB];
+    }
+#pragma unroll 8
+    for (int64_t c = c0; c < c1; c++) {
+        B1[r + c * HB] = s1;
+        B2[r + c * HB] = s2;
+        if (c + 1 < c1) {
+            s1 = (s1 + v1[(c + cw) * HB]) - v1[c * HB];
+            s2 = (s2 + v2[(c + cw) * HB]) - v2[c * HB];
         }
     }
 }
@@ -452,11 +477,13 @@ ar_status ar_f64_box_images(ar_context* ctx, TrackArgs& a) {
     double* V2 = (double*)((char*)ctx->box.ptr + vbytes);
     double* B1 = (double*)((char*)ctx->box.ptr + 2 * vbytes);
     double* B2 = (double*)((char*)ctx->box.ptr + 2 * vbytes + bbytes);
-    const int64_t cwork = NX * HB;
-    const int64_t rwork = HB * ((WB + AR_BOXF_BLK - 1) / AR_BOXF_BLK);
-    box_f64_colpass_kernel<<<mp_grid(cwork), 256, 0, ctx->stream>>>(
-        (const float*)ctx->pref.ptr, ctx->ny, ctx->nx, a.pady, a.padx, a.max_ch, HB, NX, V1, V2);
-    box_f64_rowpass_kernel<<<mp_grid(rwork), 256, 0, ctx->stream>>>(V1, V2, a.max_cw, HB, WB, B1, B2);
+    const int64_t nrb = (HB + 255) / 256, nblk = (WB + AR_BOXF_BLK - 1) / AR_BOXF_BLK;
+    if (nrb * NX > 0x7fffffff || nrb * nblk > 0x7fffffff) return AR_ERR_UNSUPPORTED;
+    box_f64_colpass_kernel<<<(unsigned)(nrb * NX), 256, 0, ctx->stream>>>(
+        (const float*)ctx->pref.ptr, ctx->ny, ctx->nx, a.pady, a.padx, a.max_ch, HB, (int)nrb, V1,
+        V2);
+    box_f64_rowpass_kernel<<<(unsigned)(nrb * nblk), 256, 0, ctx->stream>>>(
+        V1, V2, a.max_cw, HB, WB, (int)nrb, B1, B2);
     if (hipGetLastError() != hipSuccess) return AR_ERR_DEVICE;
     a.fbox1 = B1;
     a.fbox2 = B2;
