@@ -346,6 +346,164 @@ __device__ inline Best cascade_tiles(const float* Z, float* ws, Best b3) {
     return best;
 }
 
+// ---- nested blocks: the common case of a single sharp peak in one pass per level
+//
+// A source sample s of a pyrUp axis over n samples feeds exactly the outputs
+// [max(2s - 2, 0), min(2s + 2, 2n - 1)] (taps above, reflection at 0 and clamping at n - 1
+// included). Blocks of 16 x 16 samples are computed at levels 4 .. F, each centred on the
+// previous block's argmax (and placed so that every sample feeding it lies in that block; level
+// 4 is fed by the materialised 8x level). A sample of level L is interior when every sample it
+// feeds lies in the level L + 1 block. Every final-level sample outside the final block is fed only
+// by non-interior samples of the level before, and, level by level, every sample outside a block
+// only by non-interior samples of the level below; so with M the largest non-NaN value among the
+// non-interior samples of the 8x level and of the blocks below F, every final-level sample outside
+// the final block is at most ub_margin(M) (at most six passes, the bound above). When that is
+// strictly below the final block's maximum, the block's argmax is the level's. Otherwise the tile
+// search runs.
+
+// Outputs of level 2n fed by source s: [max(2s - 2, 0), min(2s + 2, 2n - 1)] within [lo, lo + 15].
+__device__ inline bool feeds_inside(int s, int n, int lo) {
+    const int a = 2 * s - 2 > 0 ? 2 * s - 2 : 0;
+    const int b = 2 * s + 2 < 2 * n - 1 ? 2 * s + 2 : 2 * n - 1;
+    return a >= lo && b <= lo + 15;
+}
+
+// Level-L samples (R0 .. R0 + 15) x (C0 .. C0 + 15) of a level of n samples a side, from the
+// level below held at src (its sample (s, t) at src[(s - sr0) sp + t - sc0]) through tmp (16 x 16
+// floats, pitch 16), into dst (pitch 16). Returns the lane's argmax under the peak_index rule, indexed
+// r 16 + c within the block.
+__device__ inline Best block16(const float* src, int sr0, int sc0, int sp, int n, int R0, int C0,
+                               float* tmp, float* dst) {
+    const int lane = threadIdx.x & 63, ns = n >> 1;
+    int clo, chi;
+    feed_range(C0, C0 + 15, ns, clo, chi);
+    const int ncs = chi - clo + 1;   // at most feed_width(16) = 10
+    float xa[4], xb[4], xq[4];
+    bool x3[4];
+#pragma unroll
+    for (int u = 0; u < 3; u++) {   // vertical: 16 rows x ncs source columns
+        int t = lane + 64 * u;
+        int r = t / 10, c = t - r * 10;
+        r = r < 16 ? r : 15;
+        c = c < ncs ? c : ncs - 1;
+        int m, cc, q, nt;
+        taps(R0 + r, ns, m, cc, q, nt);
+        const float* col = src + (clo + c - sc0);
+        xa[u] = col[(m - sr0) * sp];
+        xb[u] = col[(cc - sr0) * sp];
+        xq[u] = col[(q - sr0) * sp];
+        x3[u] = nt == 3;
+    }
+#pragma unroll
+    for (int u = 0; u < 3; u++) {
+        int t = lane + 64 * u;
+        int r = t / 10, c = t - r * 10;
+        if (r < 16 && c < ncs) tmp[r * 16 + c] = x3[u] ? tap3(xa[u], xb[u], xq[u]) : tap2(xb[u], xq[u]);
+    }
+    wave_sync();
+#pragma unroll
+    for (int u = 0; u < 4; u++) {   // horizontal: 16 x 16
+        const int t = lane + 64 * u, r = t >> 4, c = t & 15;
+        int m, cc, q, nt;
+        taps(C0 + c, ns, m, cc, q, nt);
+        const float* row = tmp + r * 16 - clo;
+        xa[u] = row[m];
+        xb[u] = row[cc];
+        xq[u] = row[q];
+        x3[u] = nt == 3;
+    }
+    Best best = best_empty();
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int t = lane + 64 * u;
+        const float v = x3[u] ? tap3(xa[u], xb[u], xq[u]) : tap2(xb[u], xq[u]);
+        if (dst) dst[t] = v;
+        if (better(v, t, best)) { best.v = v; best.idx = t; }
+    }
+    wave_sync();
+    return best;
+}
+
+// Start row of the level-2n block centred on row i of a level-n block at R0: every sample
+// feeding it (feed_range) stays in R0 .. R0 + 15, and the block stays in the level.
+__device__ inline int next_block(int R0, int i, int n) {
+    int r = 2 * i - 7;
+    const int lo = R0 == 0 ? 0 : 2 * R0 + 1;
+    const int hi = R0 + 16 == n ? 2 * n - 16 : 2 * R0 + 14;
+    r = r < lo ? lo : (r > hi ? hi : r);
+    return r < 0 ? 0 : (r > 2 * n - 16 ? 2 * n - 16 : r);
+}
+
+// Largest non-NaN value among the non-interior samples of a 16 x 16 block at (R0, C0) of a level
+// of n samples, against the next level's block at (R1, C1) (wave-uniform).
+__device__ inline float block_ring_max(const float* blk, int R0, int C0, int n, int R1, int C1) {
+    const int lane = threadIdx.x & 63;
+    float mx = -INFINITY;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int t = lane + 64 * u, r = t >> 4, c = t & 15;
+        const float v = blk[t];
+        if (!(feeds_inside(R0 + r, n, R1) && feeds_inside(C0 + c, n, C1)) && v > mx) mx = v;
+    }
+    return wave_max(mx);
+}
+
+// Final levels 16x..64x (F = 4..6) by nested blocks from the materialised 8x level Z (40 x 40)
+// and its argmax b3; ws holds 768 floats. Returns false when the bound does not settle the
+// argmax (then `out` is unset).
+template <int F>
+__device__ inline bool cascade_nested(const float* Z, float* ws, Best b3, Best& out) {
+    if (b3.idx == 0x7fffffff) return false;
+    const int lane = threadIdx.x & 63;
+    float* blk[2] = {ws, ws + 256};
+    float* tmp = ws + 512;
+    const int i3 = b3.idx / 40, j3 = b3.idx - 40 * (b3.idx / 40);
+    int R0 = 2 * i3 - 7, C0 = 2 * j3 - 7;
+    R0 = R0 < 0 ? 0 : (R0 > 64 ? 64 : R0);
+    C0 = C0 < 0 ? 0 : (C0 > 64 ? 64 : C0);
+    // non-interior samples of the 8x level against the level-4 block (rows of 10 float4)
+    float mring = -INFINITY;
+    {
+        const float4* Z4 = (const float4*)Z;
+        for (int k = lane; k < 400; k += 64) {
+            const int r = k / 10, c4 = 4 * (k - 10 * r);
+            const bool rin = feeds_inside(r, 40, R0);
+            const float4 v4 = Z4[k];
+            const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (!(rin && feeds_inside(c4 + e, 40, C0)) && vv[e] > mring) mring = vv[e];
+        }
+        mring = wave_max(mring);
+    }
+    const float* src = Z;
+    int sr0 = 0, sc0 = 0, sp = 40;
+#pragma unroll
+    for (int L = 4; L <= F; L++) {
+        const int n = 5 << L;
+        float* dst = L < F ? blk[L & 1] : nullptr;
+        const Best b = wave_best_dpp(block16(src, sr0, sc0, sp, n, R0, C0, tmp, dst));
+        if (L == F) {
+            if (!(b.idx != 0x7fffffff && ub_margin(mring) < b.v)) return false;
+            out.v = b.v;
+            out.idx = (R0 + (b.idx >> 4)) * n + C0 + (b.idx & 15);
+            return true;
+        }
+        if (b.idx == 0x7fffffff) return false;
+        const int R1 = next_block(R0, R0 + (b.idx >> 4), n);
+        const int C1 = next_block(C0, C0 + (b.idx & 15), n);
+        const float m = block_ring_max(dst, R0, C0, n, R1, C1);
+        mring = m > mring ? m : mring;
+        src = dst;
+        sr0 = R0;
+        sc0 = C0;
+        sp = 16;
+        R0 = R1;
+        C0 = C1;
+    }
+    return false;
+}
+
 // The cascade for upsampling `up` (power of two, 2..64). `reg` is the wave's region
 // (cascade_floats(up) floats, 16-byte aligned) with the 5x5 patch (row-major) at its front.
 // Returns the final-level argmax under the peak_index rule, identical in every lane.
@@ -367,9 +525,10 @@ __device__ inline CascadeOut cascade(float* reg, int up) {
 
     int nf = 5 << LM;
     if (L >= 4) {
-        if (L == 4) b = cascade_tiles<4>(Y, T, b);
-        else if (L == 5) b = cascade_tiles<5>(Y, T, b);
-        else b = cascade_tiles<6>(Y, T, b);
+        Best nb;
+        if (L == 4) b = cascade_nested<4>(Y, T, b, nb) ? nb : cascade_tiles<4>(Y, T, b);
+        else if (L == 5) b = cascade_nested<5>(Y, T, b, nb) ? nb : cascade_tiles<5>(Y, T, b);
+        else b = cascade_nested<6>(Y, T, b, nb) ? nb : cascade_tiles<6>(Y, T, b);
         nf = 5 << L;
     }
     CascadeOut out;
