@@ -350,16 +350,16 @@ __device__ inline Best cascade_tiles(const float* Z, float* ws, Best b3) {
 //
 // A source sample s of a pyrUp axis over n samples feeds exactly the outputs
 // [max(2s - 2, 0), min(2s + 2, 2n - 1)] (taps above, reflection at 0 and clamping at n - 1
-// included). Blocks of 16 x 16 samples are computed at levels 4 .. F, each centred on the
+// included). Blocks of 16 x 16 samples are computed at levels 3 .. F, each centred on the
 // previous block's argmax (and placed so that every sample feeding it lies in that block; level
-// 4 is fed by the materialised 8x level). A sample of level L is interior when every sample it
+// 3 is fed by the materialised 4x level). A sample of level L is interior when every sample it
 // feeds lies in the level L + 1 block. Every final-level sample outside the final block is fed only
 // by non-interior samples of the level before, and, level by level, every sample outside a block
 // only by non-interior samples of the level below; so with M the largest non-NaN value among the
-// non-interior samples of the 8x level and of the blocks below F, every final-level sample outside
-// the final block is at most ub_margin(M) (at most six passes, the bound above). When that is
-// strictly below the final block's maximum, the block's argmax is the level's. Otherwise the tile
-// search runs.
+// non-interior samples of the 4x level and of the blocks below F, every final-level sample outside
+// the final block is at most ub_margin(M) (at most eight passes: 8 * 1.75 u < 1e-6, and the
+// subnormal terms stay below 1e-30). When that is strictly below the final block's maximum, the
+// block's argmax is the level's. Otherwise the 8x level is materialised and the tile search runs.
 
 // Outputs of level 2n fed by source s: [max(2s - 2, 0), min(2s + 2, 2n - 1)] within [lo, lo + 15].
 __device__ inline bool feeds_inside(int s, int n, int lo) {
@@ -448,38 +448,39 @@ __device__ inline float block_ring_max(const float* blk, int R0, int C0, int n, 
     return wave_max(mx);
 }
 
-// Final levels 16x..64x (F = 4..6) by nested blocks from the materialised 8x level Z (40 x 40)
-// and its argmax b3; ws holds 768 floats. Returns false when the bound does not settle the
-// argmax (then `out` is unset).
-template <int F>
-__device__ inline bool cascade_nested(const float* Z, float* ws, Best b3, Best& out) {
-    if (b3.idx == 0x7fffffff) return false;
+// Final levels 16x..64x (F = 4..6) by nested blocks from the materialised level BL (Z, NB x NB
+// with NB = 5 * 2^BL) and its argmax bb; ws holds 768 floats. Returns false when the bound does
+// not settle the argmax (then `out` is unset).
+template <int F, int BL>
+__device__ inline bool cascade_nested(const float* Z, float* ws, Best bb, Best& out) {
+    constexpr int NB = 5 << BL, NQ = NB / 4;
+    if (bb.idx == 0x7fffffff) return false;
     const int lane = threadIdx.x & 63;
     float* blk[2] = {ws, ws + 256};
     float* tmp = ws + 512;
-    const int i3 = b3.idx / 40, j3 = b3.idx - 40 * (b3.idx / 40);
-    int R0 = 2 * i3 - 7, C0 = 2 * j3 - 7;
-    R0 = R0 < 0 ? 0 : (R0 > 64 ? 64 : R0);
-    C0 = C0 < 0 ? 0 : (C0 > 64 ? 64 : C0);
-    // non-interior samples of the 8x level against the level-4 block (rows of 10 float4)
+    const int ib = bb.idx / NB, jb = bb.idx - NB * (bb.idx / NB);
+    int R0 = 2 * ib - 7, C0 = 2 * jb - 7;
+    R0 = R0 < 0 ? 0 : (R0 > 2 * NB - 16 ? 2 * NB - 16 : R0);
+    C0 = C0 < 0 ? 0 : (C0 > 2 * NB - 16 ? 2 * NB - 16 : C0);
+    // non-interior samples of level BL against the first block (rows of NQ float4)
     float mring = -INFINITY;
     {
         const float4* Z4 = (const float4*)Z;
-        for (int k = lane; k < 400; k += 64) {
-            const int r = k / 10, c4 = 4 * (k - 10 * r);
-            const bool rin = feeds_inside(r, 40, R0);
+        for (int k = lane; k < NB * NQ; k += 64) {
+            const int r = k / NQ, c4 = 4 * (k - NQ * r);
+            const bool rin = feeds_inside(r, NB, R0);
             const float4 v4 = Z4[k];
             const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
 #pragma unroll
             for (int e = 0; e < 4; e++)
-                if (!(rin && feeds_inside(c4 + e, 40, C0)) && vv[e] > mring) mring = vv[e];
+                if (!(rin && feeds_inside(c4 + e, NB, C0)) && vv[e] > mring) mring = vv[e];
         }
         mring = wave_max(mring);
     }
     const float* src = Z;
-    int sr0 = 0, sc0 = 0, sp = 40;
+    int sr0 = 0, sc0 = 0, sp = NB;
 #pragma unroll
-    for (int L = 4; L <= F; L++) {
+    for (int L = BL + 1; L <= F; L++) {
         const int n = 5 << L;
         float* dst = L < F ? blk[L & 1] : nullptr;
         const Best b = wave_best_dpp(block16(src, sr0, sc0, sp, n, R0, C0, tmp, dst));
@@ -520,16 +521,26 @@ __device__ inline CascadeOut cascade(float* reg, int up) {
     Best b = best_empty();
     if (LM >= 1) b = wave_pyrup<5>(patch, T, Y);
     if (LM >= 2) b = wave_pyrup<10>(Y, T, Y);
-    if (LM >= 3) b = wave_pyrup<20>(Y, T, Y);
-    b = wave_best_dpp(b);
-
     int nf = 5 << LM;
     if (L >= 4) {
+        // nested blocks from the 4x level; if they do not settle it, the 8x level and the tiles
+        b = wave_best_dpp(b);
         Best nb;
-        if (L == 4) b = cascade_nested<4>(Y, T, b, nb) ? nb : cascade_tiles<4>(Y, T, b);
-        else if (L == 5) b = cascade_nested<5>(Y, T, b, nb) ? nb : cascade_tiles<5>(Y, T, b);
-        else b = cascade_nested<6>(Y, T, b, nb) ? nb : cascade_tiles<6>(Y, T, b);
+        const bool done = L == 4 ? cascade_nested<4, 2>(Y, T, b, nb)
+                        : L == 5 ? cascade_nested<5, 2>(Y, T, b, nb)
+                                 : cascade_nested<6, 2>(Y, T, b, nb);
+        if (done) {
+            b = nb;
+        } else {
+            b = wave_best_dpp(wave_pyrup<20>(Y, T, Y));
+            if (L == 4) b = cascade_tiles<4>(Y, T, b);
+            else if (L == 5) b = cascade_tiles<5>(Y, T, b);
+            else b = cascade_tiles<6>(Y, T, b);
+        }
         nf = 5 << L;
+    } else {
+        if (LM >= 3) b = wave_pyrup<20>(Y, T, Y);
+        b = wave_best_dpp(b);
     }
     CascadeOut out;
     // All samples NaN: the reference's scan leaves (1, 1), whose value is NaN.

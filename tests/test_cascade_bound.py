@@ -93,3 +93,29 @@ def test_cascade_8x_to_64x_within_the_pruning_margin():
                 bound = m + F32(abs(m)) * F32(1e-6) + F32(1e-30)   # ub_margin, in FP32
                 tile = l64[t0:t0 + 16, u0:u0 + 16]
                 assert tile.max() <= bound, (trial, t0, u0, tile.max(), bound)
+
+
+def test_cascade_4x_to_64x_within_the_margin():
+    """The nested-block path bounds 64x samples from the 4x level: eight passes, at most
+    (1 + 1.75 u)^8 - 1 < 8.4e-7 |M| above the largest 4x feeder, inside ub_margin's 1e-6."""
+    rng = np.random.default_rng(13)
+    for trial in range(24):
+        patch = (_stress_inputs(rng, 5)[trial % 17] if trial % 2 else
+                 (rng.random((5, 5)) * 2 - 1).astype(F32))
+        levels = [patch]
+        for _ in range(6):
+            levels.append(oracle.pyrup(levels[-1]))
+        l4, l64 = levels[2].astype(np.float64), levels[6]
+        n4 = l4.shape[0]
+
+        def feed(k0):
+            a = set(range(k0, k0 + 16))
+            for n in (n4 * 8, n4 * 4, n4 * 2, n4):
+                a = {f for j in a for f in _feeders(j, n)}
+            return sorted(a)
+        for t0 in range(0, 320, 32):
+            rows = feed(t0)
+            for u0 in range(0, 320, 32):
+                m = F32(l4[np.ix_(rows, feed(u0))].max())
+                bound = m + F32(abs(m)) * F32(1e-6) + F32(1e-30)
+                assert l64[t0:t0 + 16, u0:u0 + 16].max() <= bound, (trial, t0, u0)
