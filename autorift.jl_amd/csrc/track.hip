@@ -112,9 +112,10 @@ __device__ inline float exact_u8(const U8Point& P, int32_t acc, int32_t s1, int3
 // FP32 screening score (measure M: 0 ZNCC, 1 NCC): (n num) / (cnorm sqrt(n) sqrt(n var)),
 // 0 for a flat window. n num = n acc + (128 n - sumT) s1 + n (128 sumT - 16384 n) and n var are
 // the exact integers of n_times_num / n_times_var (every partial sum of the fused forms is an
-// integer below 2^53). Two conversions to Float32, v_rsq_f32 (1 ulp), the rounded constant and
-// two products: relative error below 6 * 2^-24 < 4e-7. A flat window has n var = 0 and n num = 0,
-// so flooring n var at 1e-30 (every other n var is >= 1) makes its score 0 without a branch.
+// integer below 2^53). Two conversions to Float32, v_rsq_f32 (1 ulp), the constant (sq n
+// converted to Float32, then v_rsq_f32) and two products: relative error below 8 * 2^-24 < 5e-7.
+// A flat window has n var = 0 and n num = 0, so flooring n var at 1e-30 (every other n var is
+// >= 1) makes its score 0 without a branch.
 template <int M>
 __device__ inline float approx_u8(const U8Point& P, int32_t acc, int32_t s1, int32_t s2) {
     const double ds1 = (double)s1;
@@ -191,6 +192,9 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     int32_t* colS = (int32_t*)(smem + L.off_tmp);
     // the wave index in a scalar register: loops over a wave's share are scalar loops
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // floor(t / (cpad / 4)) as a multiply-high by ceil(2^32 / (cpad / 4)): exact for every
+    // t below 2^32 / (cpad / 4)
+    const uint32_t nd_magic = 0xffffffffu / (uint32_t)(L.cpad >> 2) + 1u;
     // chip band rows and trailing columns are zero; the fetched chip path writes only chip rows
     for (int t = tid; t < (L.cols_t * L.cpad) >> 2; t += AR_WG) ((uint32_t*)Tp)[t] = 0;
     __syncthreads();
@@ -320,7 +324,7 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             // only.
             const int nd = L.cpad >> 2, c0 = ch >> 2;   // dwords per column; rows 16 + u: u < ch
             for (int t = tid; t < L.acc_cols_t * nd; t += AR_WG) {
-                const int v = t / nd, d = t - v * nd;
+                const int v = (int)__umulhi((uint32_t)t, nd_magic), d = t - v * nd;
                 if (v >= cw || d < 4 || d >= 4 + c0) ((uint32_t*)Tp)[t] = 0u;
             }
         }
@@ -428,15 +432,16 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         P.sumT = sum1;
         const int64_t sumx2 = sum2;
         const int64_t sumk2 = P.n * P.n * sumx2 - P.n * P.sumT * P.sumT;
-        double sq = (double)sumk2 / ((double)P.n * (double)P.n);
+        // n = 2^lg: 1 / n and the division by n^2 are exact scalings
+        P.invn = ldexp(1.0, -(int)__builtin_ctzll((unsigned long long)P.n));
+        double sq = (double)sumk2 * P.invn * P.invn;
         bool ok = sq > 2.220446049250313e-16 * (double)ch * (double)cw;
         if (!ok) { write_skip(a, p, true); __syncthreads(); continue; }
         P.cnorm = sqrt(sq);
         P.nd = (double)P.n;
-        P.invn = 1.0 / P.nd;
         P.cd = (double)(128 * P.n - P.sumT);
         P.dd = (double)(P.n * (128 * P.sumT - 16384 * P.n));
-        P.k = (float)(1.0 / (P.cnorm * sqrt(P.nd)));
+        P.k = __builtin_amdgcn_rsqf((float)(sq * P.nd));   // 1 / (cnorm sqrt(n)), sq n exact
 
         // ---- integer box sums S1 (W) and S2 (W^2) over every shift (unless read from the box
         // images above): sliding sums, with each window column split into KC row chunks and each

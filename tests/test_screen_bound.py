@@ -4,9 +4,9 @@ The matrix-core path scores every shift in Float32 before evaluating the few shi
 still be the maximum with the reference's Float64 normalisation (src/correlate.jl:591-600); it
 relies on |screening score - exact score| <= AR_SCREEN_EPS (4e-6, csrc/track.hip). This test
 evaluates the kernel's screening arithmetic with numpy Float32 operations -- n num and n var as
-exact integers, two conversions, the per-point constant 1 / (cnorm sqrt(n)), v_rsq_f32 taken as
-the correctly rounded 1/sqrt moved by -1, 0 and +1 ulp (its documented error is 1 ulp), two
-products -- and compares it with the oracle's exact surface (oracle.correlate, the reference's
+exact integers, two conversions, the per-point constant 1 / (cnorm sqrt(n)) as v_rsq_f32 of
+Float32(cnorm^2 n), v_rsq_f32 taken as the correctly rounded 1/sqrt moved by -1, 0 and +1 ulp
+(its documented error is 1 ulp), two products -- and compares it with the oracle's exact surface (oracle.correlate, the reference's
 own order) on windows built to stress it: random, nearly flat (variance tiny against the mean),
 flat, saturated, and the largest chip area the path takes (128 x 128).
 """
@@ -38,12 +38,15 @@ def _screen(search, chip, measure, rsq_ulps):
     nn = n * acc + (128 * n - sum_t) * s1 + n * (128 * sum_t - 16384 * n)
     nv = n * s2 if measure == oracle.NCC else n * s2 - s1 * s1
     x = chip.astype(np.float64)
-    cnorm = np.sqrt(((x * n - sum_t) ** 2).sum() / (n * n))
-    k = np.float32(1.0 / (cnorm * np.sqrt(float(n))))
+    sq = ((x * n - sum_t) ** 2).sum() / (n * n)          # cnorm^2
+    # k = 1 / (cnorm sqrt(n)) = v_rsq_f32(Float32(cnorm^2 n)), off by the same ulps as below
+    k = np.float32(1.0 / np.sqrt(np.float64(np.float32(sq * n))))
     nvf = np.maximum(nv.astype(np.float32), np.float32(1e-30))
     r = (1.0 / np.sqrt(nvf.astype(np.float64))).astype(np.float32)
     if rsq_ulps:
-        r = np.nextafter(r, np.float32(np.inf if rsq_ulps > 0 else 0.0)).astype(np.float32)
+        to = np.float32(np.inf if rsq_ulps > 0 else 0.0)
+        r = np.nextafter(r, to).astype(np.float32)
+        k = np.nextafter(k, to).astype(np.float32)
     return (nn.astype(np.float32) * k) * r
 
 
