@@ -6,7 +6,8 @@
 // B(wr0 - 1 + i, wc0 - 1 + j): the integer box sums of src/correlate.jl:574-600 (summed-area
 // tables, src/integral.jl:141-170) read instead of rebuilt per point, identical as integers.
 //
-// One wave per (band of 64 RPL padded rows, chunk of output columns); lane l holds rows
+// One wave per (band of 64 RPL padded rows, chunk of output columns, its width chosen per launch
+// so that the chunks fill the resident waves in one round); lane l holds rows
 // RPL l .. RPL l + RPL - 1 of the band. The wave walks the chunk's columns keeping each row's
 // horizontal running sums over cw columns in registers (loads of one column are coalesced
 // across lanes), then forms the vertical box of ch rows from an inclusive prefix over the band
@@ -21,8 +22,6 @@
 #include "ar_internal.h"
 
 namespace ar {
-
-#define BX_CHUNK 64    // output columns per wave
 
 // rows per lane for chip height ch: the band of 64 RPL rows yields at least half as many outputs
 __host__ __device__ inline int box_rpl(int ch) { return ch <= 128 ? 4 : 8; }
@@ -43,8 +42,8 @@ __device__ inline uint32_t wave_scan_u32(uint32_t v) {
 template <int RPL>
 __global__ __launch_bounds__(256) void box_u8_kernel(const uint8_t* img, int64_t ny, int64_t nx,
                                                     int64_t pady, int64_t padx, int ch, int cw,
-                                                    int64_t HB, int64_t WB, int32_t* B1,
-                                                    int32_t* B2) {
+                                                    int64_t HB, int64_t WB, int chunk_w,
+                                                    int32_t* B1, int32_t* B2) {
     // columns per group of loads: 16 loaded dwords per lane in flight whatever the band height
     constexpr int BX_G = 32 / RPL;
     __shared__ __align__(16) uint32_t sS[4][2][64 * RPL];
@@ -52,12 +51,12 @@ __global__ __launch_bounds__(256) void box_u8_kernel(const uint8_t* img, int64_t
     uint32_t* s1 = sS[wave][0];
     uint32_t* s2 = sS[wave][1];
     const int OUT = 64 * RPL - ch + 1;   // output rows per band
-    const int64_t nb = (HB + OUT - 1) / OUT, nchunks = (WB + BX_CHUNK - 1) / BX_CHUNK;
+    const int64_t nb = (HB + OUT - 1) / OUT, nchunks = (WB + chunk_w - 1) / chunk_w;
     const int64_t nw = (int64_t)gridDim.x * 4;
     for (int64_t w = (int64_t)blockIdx.x * 4 + wave; w < nb * nchunks; w += nw) {
         const int64_t band = w % nb, chunk = w / nb;
-        const int64_t R0 = band * OUT, C0 = chunk * BX_CHUNK;
-        const int64_t C1 = C0 + BX_CHUNK < WB ? C0 + BX_CHUNK : WB;
+        const int64_t R0 = band * OUT, C0 = chunk * chunk_w;
+        const int64_t C1 = C0 + chunk_w < WB ? C0 + chunk_w : WB;
         // image rows of this lane (padded rows R0 + RPL lane + k); the column walk is wave-uniform
         const int64_t ir0 = R0 + (int64_t)RPL * lane - pady;
         uint32_t rowok = 0;
@@ -199,17 +198,26 @@ ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t p
     int32_t* p2 = (int32_t*)((char*)ctx->box.ptr + each);
     const int rpl = box_rpl(ch);
     const int OUT = 64 * rpl - ch + 1;
-    const int64_t waves = ((HB + OUT - 1) / OUT) * ((WB + BX_CHUNK - 1) / BX_CHUNK);
     auto kern = rpl == 4 ? box_u8_kernel<4> : (rpl == 8 ? box_u8_kernel<8> : box_u8_kernel<16>);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess ||
         per_cu < 1)
         return AR_ERR_DEVICE;
+    // Column chunks sized so that the work items (row band, column chunk) fill the resident
+    // waves in one round: with a fixed width the last round ran mostly idle. Chunks of at least
+    // 64 columns keep the horizontal warm-up (cw columns per chunk) a minor share.
+    const int64_t nb = (HB + OUT - 1) / OUT, resident = (int64_t)ctx->ncu * per_cu * 4;
+    int64_t nchunks = resident / nb;
+    if (nchunks < 1) nchunks = 1;
+    int64_t chunk_w = (WB + nchunks - 1) / nchunks;
+    if (chunk_w < 64) chunk_w = 64;
+    if (chunk_w > 1 << 20) chunk_w = 1 << 20;
+    const int64_t waves = nb * ((WB + chunk_w - 1) / chunk_w);
     int64_t blocks = (int64_t)ctx->ncu * per_cu;
     if (blocks > (waves + 3) / 4) blocks = (waves + 3) / 4;
     const uint8_t* img = (const uint8_t*)ctx->pref.ptr;
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, img, ctx->ny,
-                       ctx->nx, pady, padx, ch, cw, HB, WB, p1, p2);
+                       ctx->nx, pady, padx, ch, cw, HB, WB, (int)chunk_w, p1, p2);
     if (hipGetLastError() != hipSuccess) return AR_ERR_DEVICE;
     *b1 = p1;
     *b2 = p2;
