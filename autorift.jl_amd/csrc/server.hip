@@ -101,8 +101,9 @@ struct ar_server {
 };
 
 // Device bytes a context needs for one pair: raw upload and preparation work images, the
-// prepared pair and masks, box images (two int32 planes), the four byte-shifted int8 copies of
-// each prepared image, per-point buffers and the multi-chip grid planes, the tracker scratch of the
+// prepared pair and masks, box images (the Float64 planes of the 16-bit and transform paths),
+// the four byte-shifted int8 copies of each prepared image, per-point buffers and the
+// multi-chip grid planes, the tracker scratch of the
 // largest chip-size level the pair can need (the direct Float64 tracker's staging for Float32
 // pairs with exact arithmetic or chip areas that are not a power of two, the generic tracker's
 // per-workgroup regions for a rotation search, the Coherence tracker's for complex pairs), plus a
@@ -116,7 +117,12 @@ static size_t pair_footprint(const ar_params& p, int64_t ny, int64_t nx, int32_t
     const size_t work = N * (8 + 4) + N * 4 * (p.preprocess == AR_PRE_WALLIS ? 3 : 1);
     const size_t pe = dtype == AR_C64 ? 8 : 4;      // prepared element: ComplexF32 or Float32
     const size_t prep = 2 * N * pe + 3 * N;         // prepared pair (at most pe bytes), masks
-    const size_t box = 2 * N * 8;                   // box images (Float64 for the 16-bit path)
+    // box images: the 16-bit and transform paths' Float64 column sums and box sums and Float32
+    // window variances; otherwise the UInt8 path's int32 planes (bounded by 2 N 8)
+    const bool fbox = p.quantize == AR_QUANT_NONE &&
+                      (p.precision == AR_PREC_BF16 || p.precision == AR_PREC_FP16 ||
+                       p.precision == AR_PREC_FFT);
+    const size_t box = fbox ? 4 * N * 8 + N * 4 : 2 * N * 8;
     const size_t copies = 8 * N + 4096;             // int8 copies of the UInt8 pair
     const size_t pts = (size_t)npts * 512;          // point fields, outputs, hand-off, grid planes
     // the largest level: chip_size_max (its rows by the aspect), the largest radius

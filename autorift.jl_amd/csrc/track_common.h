@@ -68,6 +68,8 @@ struct TrackArgs {
     // the power-of-two operand scales (window, chip) with the factor that undoes them
     const double* fbox1;
     const double* fbox2;
+    const float* fwv;           // Float32(max(window variance, 0)) per chip position (the
+                                // measure's: s2 - s1^2 / n for ZNCC, s2 for NCC), leading dim box_ld
     float wscale, cscale;
     double numscale;
     // FFT path: transform extents (rows, columns) of the pass

@@ -242,13 +242,11 @@ __device__ inline int fft_mirror_pos(int p, int n, uint64_t rad) {
 
 // LDS of the FFT tracker: the complex array (columns of ld = fy | 1 elements: an odd column
 // stride keeps the column passes' lanes on distinct banks), the two twiddle tables, the two mirror
-// tables (position of -f per position of f), and one Float32 plane for the chip and then the
-// surface.
+// tables (position of -f per position of f), and one Float32 plane for the chip.
 struct FftLayout {
     size_t off_z, off_ty, off_tx, off_my, off_mx, off_s, total;
 };
-__host__ __device__ inline FftLayout fft_layout(int fy, int fx, int max_ch, int max_cw,
-                                                int max_rx, int max_ry) {
+__host__ __device__ inline FftLayout fft_layout(int fy, int fx, int max_ch, int max_cw) {
     FftLayout L;
     L.off_z = 0;
     L.off_ty = (size_t)(fy | 1) * fx * 8;
@@ -256,8 +254,7 @@ __host__ __device__ inline FftLayout fft_layout(int fy, int fx, int max_ch, int 
     L.off_my = L.off_tx + (size_t)fx * 8;
     L.off_mx = L.off_my + 128;
     L.off_s = L.off_mx + 128;
-    const size_t surf = (size_t)(2 * max_ry) * (2 * max_rx), chip = (size_t)max_ch * max_cw;
-    L.total = L.off_s + 4 * (surf > chip ? surf : chip);
+    L.total = L.off_s + 4 * (size_t)max_ch * max_cw;
     return L;
 }
 
@@ -303,7 +300,7 @@ __global__ __launch_bounds__(FFT_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int fy = a.fft_fy, fx = a.fft_fx;
     const uint64_t rady = fft_radices(fy), radx = fft_radices(fx);
-    const FftLayout L = fft_layout(fy, fx, a.max_ch, a.max_cw, a.max_rx, a.max_ry);
+    const FftLayout L = fft_layout(fy, fx, a.max_ch, a.max_cw);
     float2* Z = (float2*)(smem + L.off_z);
     float2* Ty = (float2*)(smem + L.off_ty);
     float2* Tx = (float2*)(smem + L.off_tx);
@@ -422,42 +419,46 @@ __global__ __launch_bounds__(FFT_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
         fft_pass<true>(Z, Ty, fy, rady, 0, fx, 1, ld);
         fft_pass<true>(Z, Tx, fx, radx, ch - 1, NR, ld, 1);
 
-        // ---- normalisation per shift (Float64, as the reference) and the argmax
+        // ---- score per shift and the argmax (peak_index rule): the FFT numerator over the
+        // window deviation from the pass's Float32 window variances (formed in Float64 by the box
+        // pass); the reported peak and 5x5 patch take the reference's Float64 normalisation
+        // (src/correlate.jl:575-630) below
         const double n = (double)(ch * cw);
         const int64_t bb = (g.wr0 - 1) + (g.wc0 - 1) * a.box_ld;
+        const float inv_cn = (float)(1.0 / cnorm);
         Best b = best_empty();
-        for (int j0 = wave; j0 < NC; j0 += 2 * FFT_NW) {
-            double s1[2][2], s2[2][2];
+        for (int j0 = wave; j0 < NC; j0 += 4 * FFT_NW) {
+            float den[4][2];
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
+            for (int e = 0; e < 8; e++) {
                 const int j = j0 + (e >> 1) * FFT_NW, i = lane + 64 * (e & 1);
-                s1[e >> 1][e & 1] = s2[e >> 1][e & 1] = 0.0;
-                if (j < NC && i < NR) {
-                    const int64_t qb = bb + i + (int64_t)j * a.box_ld;
-                    s1[e >> 1][e & 1] = a.fbox1[qb];
-                    s2[e >> 1][e & 1] = a.fbox2[qb];
-                }
+                den[e >> 1][e & 1] = j < NC && i < NR ? a.fwv[bb + i + (int64_t)j * a.box_ld] : 0.0f;
             }
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
+            for (int e = 0; e < 8; e++) {
                 const int j = j0 + (e >> 1) * FFT_NW, i = lane + 64 * (e & 1);
                 if (j >= NC || i >= NR) continue;
-                const double num = (double)(Z[(i + ch - 1) + (j + cw - 1) * ld].x * inv_nz);
-                const float v = mp_normalise(num, s1[e >> 1][e & 1], s2[e >> 1][e & 1], n, cnorm,
-                                             a.measure);
-                plane[i + j * NR] = v;
+                const float d = den[e >> 1][e & 1];
+                const float num = Z[(i + ch - 1) + (j + cw - 1) * ld].x * inv_nz;
+                const float v = d > 0.0f ? num * inv_cn * __builtin_amdgcn_rsqf(d) : 0.0f;
                 const int idx = i * NC + j;
                 if (better(v, idx, b)) { b.v = v; b.idx = idx; }
             }
         }
-        b = fft_block_best(b, red);   // (its barriers also order the plane's writes)
+        b = fft_block_best(b, red);
         if (b.idx == 0x7fffffff) b.idx = 0;
         const int pi = b.idx / NC + 1, pj = b.idx % NC + 1;
         int i0 = 1, j0 = 1;
         const bool refine = a.up > 1 && patch_origin(pi, pj, NR, NC, i0, j0);
-        const float pval = plane[(pi - 1) + (pj - 1) * NR];
+        // exact values: the peak (every thread) and the patch (threads 0..24, row-major)
+        auto exact = [&](int i, int j) {   // 0-based shift
+            const double num = (double)(Z[(i + ch - 1) + (j + cw - 1) * ld].x * inv_nz);
+            const int64_t qb = bb + i + (int64_t)j * a.box_ld;
+            return mp_normalise(num, a.fbox1[qb], a.fbox2[qb], n, cnorm, a.measure);
+        };
+        const float pval = exact(pi - 1, pj - 1);
         float pv = 0.0f;
-        if (refine && tid < 25) pv = plane[(i0 - 1 + tid / 5) + (j0 - 1 + tid % 5) * NR];
+        if (refine && tid < 25) pv = exact(i0 - 1 + tid / 5, j0 - 1 + tid % 5);
         finish_point(a, p, g, pi, pj, pval, i0, j0, refine, pv);
         __syncthreads();
     }
@@ -497,7 +498,7 @@ ar_status ar_fft_setup(ar_context* ctx, TrackArgs& a, void (**kern)(TrackArgs), 
     const int64_t fx = ar_next_fft_size(a.max_cw + 2 * a.max_rx - 1);
     if (fy > 128 || fx > 128 || !fft_radices((int)fy) || !fft_radices((int)fx))
         return AR_ERR_UNSUPPORTED;
-    const FftLayout L = fft_layout((int)fy, (int)fx, a.max_ch, a.max_cw, a.max_rx, a.max_ry);
+    const FftLayout L = fft_layout((int)fy, (int)fx, a.max_ch, a.max_cw);
     if (L.total > AR_LDS_LIMIT) return AR_ERR_UNSUPPORTED;
     a.fft_fy = (int32_t)fy;
     a.fft_fx = (int32_t)fx;

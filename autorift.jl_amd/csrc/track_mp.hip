@@ -212,27 +212,37 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
         __syncthreads();
 
         // ---- normalisation per shift and the argmax (peak_index rule): the window variance
-        // s2 - s1^2 / n in Float64 (where it cancels), the rest of the 16-bit surface's score in
-        // FP32 (the exact values of the chosen peak and patch are Float64, below)
+        // s2 - s1^2 / n formed in Float64 by the box pass (where it cancels) and read as Float32,
+        // the rest of the 16-bit surface's score in FP32 (the exact values of the chosen peak and
+        // patch are Float64, below)
         const double n = (double)(ch * cw);
         const int64_t bb = (g.wr0 - 1) + (g.wc0 - 1) * a.box_ld;
         const float inv_cn = (float)(1.0 / cnorm);
         const float nsc = (float)a.numscale;
-        const double inv_n = 1.0 / n;
         Best b = best_empty();
-        for (int j = wave; j < NC; j += AR_NWAVES) {
-            const double* b1 = a.fbox1 + bb + (int64_t)j * a.box_ld;
-            const double* b2 = a.fbox2 + bb + (int64_t)j * a.box_ld;
-            for (int i = lane; i < NR; i += 64) {
-                const double s2 = b2[i], s1 = b1[i];
-                const double wv = a.measure == 1 ? s2 : s2 - s1 * s1 * inv_n;
-                const float den = (float)(wv < 0.0 ? 0.0 : wv);
-                const float v =
-                    den > 0.0f ? num[j * NR + i] * nsc * inv_cn * __builtin_amdgcn_rsqf(den) : 0.0f;
-                const int idx = i * NC + j;
-                if (better(v, idx, b)) { b.v = v; b.idx = idx; }
+        // the variances of four shift columns (two row blocks of 64 each) loaded before any is
+        // used
+        for (int jb = wave; jb < NC; jb += 4 * AR_NWAVES)
+            for (int ib = 0; ib < NR; ib += 128) {
+                float den[4][2];
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int j = jb + (e >> 1) * AR_NWAVES, i = ib + lane + 64 * (e & 1);
+                    den[e >> 1][e & 1] =
+                        j < NC && i < NR ? a.fwv[bb + i + (int64_t)j * a.box_ld] : 0.0f;
+                }
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int j = jb + (e >> 1) * AR_NWAVES, i = ib + lane + 64 * (e & 1);
+                    if (j >= NC || i >= NR) continue;
+                    const float d = den[e >> 1][e & 1];
+                    const float v = d > 0.0f
+                                        ? num[j * NR + i] * nsc * inv_cn * __builtin_amdgcn_rsqf(d)
+                                        : 0.0f;
+                    const int idx = i * NC + j;
+                    if (better(v, idx, b)) { b.v = v; b.idx = idx; }
+                }
             }
-        }
         b = block_best(b, red);
         if (b.idx == 0x7fffffff) b.idx = 0;
         const int pi = b.idx / NC + 1, pj = b.idx % NC + 1;
@@ -354,14 +364,18 @@ __global__ __launch_bounds__(256) void box_f64_colpass_kernel(const float* __res
     V2[c * HB + r] = s2;
 }
 
-// B (r, c) = sum_{v < cw} V(r, c + v), c < WB. One thread per (row, block of 64 columns):
+// B (r, c) = sum_{v < cw} V(r, c + v), c < WB, and FW (r, c) = Float32(max(wv, 0)) with the
+// window variance wv = s2 - s1 s1 (1 / n) (ZNCC) or s2 (NCC), formed as the trackers' screening
+// forms it. One thread per (row, block of 64 columns):
 // the block's first sum in order, then a sliding sum; consecutive lanes take consecutive rows
 // (workgroup = 256 rows of one column block, nrb workgroups per block).
 __global__ __launch_bounds__(256) void box_f64_rowpass_kernel(const double* __restrict__ V1,
                                                               const double* __restrict__ V2,
                                                               int cw, int64_t HB, int64_t WB,
-                                                              int nrb, double* __restrict__ B1,
-                                                              double* __restrict__ B2) {
+                                                              int nrb, double inv_n, int measure,
+                                                              double* __restrict__ B1,
+                                                              double* __restrict__ B2,
+                                                              float* __restrict__ FW) {
     const int64_t blk = blockIdx.x / (unsigned)nrb;
     const int64_t r = (int64_t)(blockIdx.x - (unsigned)blk * (unsigned)nrb) * 256 + threadIdx.x;
     if (r >= HB) return;
@@ -379,6 +393,8 @@ __global__ __launch_bounds__(256) void box_f64_rowpass_kernel(const double* __re
     for (int64_t c = c0; c < c1; c++) {
         B1[r + c * HB] = s1;
         B2[r + c * HB] = s2;
+        const double wv = measure == 1 ? s2 : s2 - s1 * s1 * inv_n;
+        FW[r + c * HB] = (float)(wv < 0.0 ? 0.0 : wv);
         if (c + 1 < c1) {
             s1 = (s1 + v1[(c + cw) * HB]) - v1[c * HB];
             s2 = (s2 + v2[(c + cw) * HB]) - v2[c * HB];
@@ -466,27 +482,31 @@ ar_status ar_mp_setup(ar_context* ctx, TrackArgs& a, void (**kern)(TrackArgs), s
 }
 
 // Float64 box images of W and W^2 over the padded Float32 reference's chip positions for the
-// pass's chip size (a.max_ch x a.max_cw): a.fbox1, a.fbox2, a.box_ld.
+// pass's chip size (a.max_ch x a.max_cw), and the Float32 window variances the trackers screen
+// with: a.fbox1, a.fbox2, a.fwv, a.box_ld.
 ar_status ar_f64_box_images(ar_context* ctx, TrackArgs& a) {
     const int64_t NY = ctx->ny + 2 * a.pady, NX = ctx->nx + 2 * a.padx;
     const int64_t HB = NY - a.max_ch + 1, WB = NX - a.max_cw + 1;
     if (HB < 1 || WB < 1) return AR_ERR_UNSUPPORTED;
     const size_t vbytes = (size_t)HB * NX * 8, bbytes = (size_t)HB * WB * 8;
-    if (ar_status st = ctx->box.reserve(ctx, 2 * vbytes + 2 * bbytes)) return st;
+    if (ar_status st = ctx->box.reserve(ctx, 2 * vbytes + 2 * bbytes + bbytes / 2)) return st;
     double* V1 = (double*)ctx->box.ptr;
     double* V2 = (double*)((char*)ctx->box.ptr + vbytes);
     double* B1 = (double*)((char*)ctx->box.ptr + 2 * vbytes);
     double* B2 = (double*)((char*)ctx->box.ptr + 2 * vbytes + bbytes);
+    float* FW = (float*)((char*)ctx->box.ptr + 2 * vbytes + 2 * bbytes);
     const int64_t nrb = (HB + 255) / 256, nblk = (WB + AR_BOXF_BLK - 1) / AR_BOXF_BLK;
     if (nrb * NX > 0x7fffffff || nrb * nblk > 0x7fffffff) return AR_ERR_UNSUPPORTED;
     box_f64_colpass_kernel<<<(unsigned)(nrb * NX), 256, 0, ctx->stream>>>(
         (const float*)ctx->pref.ptr, ctx->ny, ctx->nx, a.pady, a.padx, a.max_ch, HB, (int)nrb, V1,
         V2);
+    const double inv_n = 1.0 / (double)((int64_t)a.max_ch * a.max_cw);
     box_f64_rowpass_kernel<<<(unsigned)(nrb * nblk), 256, 0, ctx->stream>>>(
-        V1, V2, a.max_cw, HB, WB, (int)nrb, B1, B2);
+        V1, V2, a.max_cw, HB, WB, (int)nrb, inv_n, a.measure, B1, B2, FW);
     if (hipGetLastError() != hipSuccess) return AR_ERR_DEVICE;
     a.fbox1 = B1;
     a.fbox2 = B2;
+    a.fwv = FW;
     a.box_ld = HB;
     return AR_OK;
 }
