@@ -209,8 +209,11 @@ def served_mixed_vs_plain(autorift, count=8, slots=4, seed=7):
             ctx.set_images(a, b)
             ctx.run_multichip(autorift.api._build_grid(a.shape, ctx.params))
         t = time.perf_counter()
+        set_s = 0.0
         for a, b, _ in pool:
-            ctx.set_images(a, b)
+            t0 = time.perf_counter()
+            ctx.set_images(a, b)                       # upload and preparation of the pair
+            set_s += time.perf_counter() - t0
             plain.append(ctx.run_multichip(autorift.api._build_grid(a.shape, ctx.params)))
         plain_s = time.perf_counter() - t
     with autorift.Server(autorift.params(quantize="none", precision="bf16"), slots=slots) as srv:
@@ -227,10 +230,15 @@ def served_mixed_vs_plain(autorift, count=8, slots=4, seed=7):
         sq.append((r.dx[ok] - e.dx[ok]) ** 2 + (r.dy[ok] - e.dy[ok]) ** 2)
     res = {"pairs": count, "sides": [int(a.shape[0]) for a, _, _ in pool], "slots": slots,
            "plain_fp32_pairs_per_s": count / plain_s, "served_bf16_pairs_per_s": count / served_s,
-           "speedup": plain_s / served_s, "identical_fraction": same / max(tot, 1),
+           "speedup": plain_s / served_s,
+           # share of the plain form's time in set_images (host-to-device upload of the two
+           # Float32 images and their preparation): work the served form repeats for every pair
+           "plain_set_images_share": set_s / plain_s,
+           "identical_fraction": same / max(tot, 1),
            "rms_vs_plain_px": float(np.sqrt(np.mean(np.concatenate(sq)))) if sq else None}
     log(f"served BF16 {res['served_bf16_pairs_per_s']:.1f} pairs/s vs plain Float32 "
-        f"{res['plain_fp32_pairs_per_s']:.2f} pairs/s ({res['speedup']:.1f}x); identical "
+        f"{res['plain_fp32_pairs_per_s']:.2f} pairs/s ({res['speedup']:.1f}x; plain time in "
+        f"upload and preparation {100 * res['plain_set_images_share']:.0f} %); identical "
         f"{res['identical_fraction']:.4f}, RMS {res['rms_vs_plain_px']:.4f} px")
     return res
 
