@@ -148,6 +148,23 @@ class Context:
         self._check(lib().ar_get_pair(self._h, _ptr(r), _ptr(s), _ptr(rv), _ptr(sv), C.byref(dt)))
         return r, s, rv.astype(bool), sv.astype(bool)
 
+    def quantizer_stats(self, walk=False):
+        """The UInt8 quantiser's (n, mean, sd, mode) of the prepared reference and secondary
+        (src/preprocess.jl:546-592), as two tuples; with `walk`, each tuple also carries the
+        device walk's (runs, elements added one by one) for the sum and for the squared
+        deviations."""
+        out = (C.c_double * 16)()
+        self._check(lib().ar_quantizer_stats(self._h, out))
+        v = list(out)
+        res = []
+        for i in range(2):
+            o = v[8 * i:8 * i + 8]
+            r = (int(o[0]), o[1], o[2], int(o[3]))
+            if walk:
+                r += ((int(o[4]), int(o[6])), (int(o[5]), int(o[7])))
+            res.append(r)
+        return tuple(res)
+
     # -- correlation -----------------------------------------------------------------------------
     def track(self, pts, upsampling: int = 1, measure: int | None = None) -> DisplacementField:
         """track (src/track.jl:103-160) over every point of `pts`; `measure` defaults to the

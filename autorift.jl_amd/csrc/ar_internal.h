@@ -54,6 +54,8 @@ struct ar_context {
     DevBuf refx;                // int8 copies (value ^ 0x80) of the UInt8 pair, for the matrix cores
     DevBuf prec;                // per-point geometry records of the current pass (u8 path)
     DevBuf mc;                  // multi-chip driver planes (grid, level, coarse and result planes)
+    DevBuf qsum;                // the quantiser's sequential-sum chunks and segment records
+    DevBuf qstats;              // the quantiser's parameters of the two prepared images
     // page-locked host staging for the multi-chip driver's uploads, results and readbacks
     void* pin = nullptr;
     size_t pin_cap = 0;
@@ -73,7 +75,7 @@ struct ar_context {
     std::vector<DevBuf*> all_bufs() {
         return {&pref, &psec, &rmask, &smask, &okmask, &raw, &rawmask, &work_f32, &work_f64,
                 &flags, &geom, &pts, &out, &scratch, &grid_a, &grid_b, &refine, &box, &refx, &prec,
-                &mc};
+                &mc, &qsum, &qstats};
     }
 };
 

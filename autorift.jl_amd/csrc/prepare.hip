@@ -19,6 +19,7 @@
 
 #include "../../include/autorift_hip.h"
 #include "ar_internal.h"
+#include "seqsum.h"
 
 namespace ar {
 
@@ -420,81 +421,55 @@ __global__ void finite_mask_t_kernel(const T* img, uint8_t* out, int64_t n) {
     for (; k < n; k += (int64_t)gridDim.x * blockDim.x) out[k] = isfinite(as_f64(img, k)) ? 1 : 0;
 }
 
-// Quantiser statistics, deterministic two-level reduction: per-block partials over a fixed
-// block-cyclic assignment, then one block combining them in index order.
-#define QBLOCKS 1024
-// The quantiser's parameters, formed on the device (no host round trip): mean, Bessel standard
-// deviation, lo = mean - 3 sd, scale = 255 / (6 sd), and the mode: 0 nothing valid, 1 sd == 0,
-// 2 map (src/preprocess.jl:546-592).
+// The quantiser's parameters, formed on the device (no host round trip): count, mean, Bessel
+// standard deviation, lo = mean - 3 sd, scale = 255 / (6 sd), and the mode: 0 nothing valid, 1
+// sd == 0, 2 map (src/preprocess.jl:546-592).
 struct QParams {
-    double mean, lo, scale;
+    double mean, lo, scale, sd;
     long long n;
     int mode;
+    long long walk_items[2], walk_in_order[2];   // per sum: items walked, elements summed in order
 };
 
+// The summands of the quantiser's two sequential sums (src/preprocess.jl:551-570): pass 0 the
+// valid finite values, pass 1 their squared deviations from the mean; 0 elsewhere (adding +0.0
+// leaves a running sum unchanged).
 template <typename T>
-__global__ void q_partial_kernel(const T* img, const uint8_t* mask, int64_t n, int pass,
-                                 const QParams* qp, double* part_s, long long* part_n) {
-    const double mean = pass == 1 ? qp->mean : 0.0;
-    __shared__ double ss[256];
-    __shared__ long long sn[256];
-    double s = 0.0;
-    long long c = 0;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-         k += (int64_t)gridDim.x * blockDim.x) {
-        double v = as_f64(img, k);
-        if (mask[k] && isfinite(v)) {
-            if (pass == 0) s += v;
-            else { double d = v - mean; s += d * d; }
-            c += 1;
-        }
+struct QTerm {
+    const T* img;
+    const uint8_t* mask;
+    const QParams* qp;
+    int pass;
+    __device__ inline double operator()(int64_t k, bool& ok) const {
+        const double v = as_f64(img, k);
+        ok = mask[k] && isfinite(v);
+        if (!ok) return 0.0;
+        if (pass == 0) return v;
+        const double d = v - qp->mean;
+        return d * d;
     }
-    ss[threadIdx.x] = s;
-    sn[threadIdx.x] = c;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
-            ss[threadIdx.x] += ss[threadIdx.x + o];
-            sn[threadIdx.x] += sn[threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        part_s[blockIdx.x] = ss[0];
-        part_n[blockIdx.x] = sn[0];
-    }
-}
+};
 
-__global__ void q_final_kernel(const double* part_s, const long long* part_n, int nparts,
-                               int pass, QParams* qp) {
-    __shared__ double ss[256];
-    __shared__ long long sn[256];
-    double s = 0.0;
-    long long c = 0;
-    for (int k = threadIdx.x; k < nparts; k += 256) { s += part_s[k]; c += part_n[k]; }
-    ss[threadIdx.x] = s;
-    sn[threadIdx.x] = c;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
-            ss[threadIdx.x] += ss[threadIdx.x + o];
-            sn[threadIdx.x] += sn[threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
+// After the walk of pass 0: n and mean = s / n; after pass 1: sd = sqrt(ss / (n - 1)) and the map.
+struct QFinish {
+    QParams* qp;
+    int pass;
+    __device__ inline void operator()(double sum, long long count, const SqWalkStats& st) const {
+        qp->walk_items[pass] = st.items;
+        qp->walk_in_order[pass] = st.in_order;
         if (pass == 0) {
-            qp->n = sn[0];
-            qp->mean = ss[0] / (double)sn[0];
-        } else {
-            const long long cnt = qp->n;
-            const double sd = cnt > 1 ? sqrt(ss[0] / (double)(cnt - 1)) : 0.0;
-            qp->mode = cnt == 0 ? 0 : (sd == 0 ? 1 : 2);
-            qp->lo = qp->mean - 3 * sd;
-            qp->scale = 255.0 / (6 * sd);
+            qp->n = count;
+            qp->mean = sum / (double)count;
+            return;
         }
+        const long long cnt = qp->n;
+        const double sd = cnt > 1 ? sqrt(sum / (double)(cnt - 1)) : 0.0;
+        qp->sd = sd;
+        qp->mode = cnt == 0 ? 0 : (sd == 0 ? 1 : 2);
+        qp->lo = qp->mean - 3 * sd;
+        qp->scale = 255.0 / (6 * sd);
     }
-}
+};
 
 // round(UInt8, clamp((v - lo) * scale, 0, 255)), ties to even; invalid -> 0; flat -> 0x80.
 template <typename T>
@@ -551,18 +526,40 @@ static int gsz(int64_t n) {
     return (int)(b < 1 ? 1 : (b > 65535 ? 65535 : b));
 }
 
-// Quantise a float (or raw) image on the device into out (u8).
+// Quantise a float (or raw) image on the device into out (u8); the parameters stay in the
+// context's slot `slot` (0 reference, 1 secondary) for inspection.
 template <typename T>
 static ar_status quantize_u8(ar_context* ctx, const T* img, const uint8_t* mask, int64_t N,
-                             uint8_t* out) {
+                             uint8_t* out, int slot) {
     ar_status st;
-    if ((st = ctx->grid_a.reserve(ctx, QBLOCKS * 16 + 256))) return st;
-    double* ps = (double*)ctx->grid_a.ptr;
-    long long* pn = (long long*)(ps + QBLOCKS);
-    QParams* qp = (QParams*)(pn + QBLOCKS);
+    const int64_t nch = (N + SQ_CHUNK - 1) / SQ_CHUNK;
+    if (nch >= ((int64_t)1 << 31))
+        return pfail(ctx, AR_ERR_UNSUPPORTED, "image too large for the UInt8 quantiser");
+    const size_t recb = (size_t)nch * SQ_SLOTS * sizeof(SqRec);
+    const size_t per = SQ_WG * 8 + 8 + 8 + 8 + 4 + 4 + 4;   // tsum, csum, coff, ccnt, clow, cg, cnt
+    if ((st = ctx->qsum.reserve(ctx, 2 * recb + (size_t)nch * per + 256))) return st;
+    if ((st = ctx->qstats.reserve(ctx, 2 * sizeof(QParams)))) return st;
+    char* b = (char*)ctx->qsum.ptr;
+    SqRec* rec = (SqRec*)b;
+    SqRec* items = (SqRec*)(b + recb);
+    double* tsum = (double*)(b + 2 * recb);
+    double* csum = tsum + (size_t)nch * SQ_WG;
+    double* coff = csum + nch;
+    long long* ccnt = (long long*)(coff + nch);
+    int* clow = (int*)(ccnt + nch);
+    int* cg = clow + nch;
+    int* cnt = cg + nch;
+    QParams* qp = (QParams*)ctx->qstats.ptr + slot;
+    const bool inline_scan = nch <= SQ_SCAN_INLINE;
     for (int pass = 0; pass < 2; pass++) {
-        q_partial_kernel<T><<<QBLOCKS, 256, 0, ctx->stream>>>(img, mask, N, pass, qp, ps, pn);
-        q_final_kernel<<<1, 256, 0, ctx->stream>>>(ps, pn, QBLOCKS, pass, qp);
+        const QTerm<T> term{img, mask, qp, pass};
+        sq_chunk_kernel<<<(unsigned)nch, SQ_WG, 0, ctx->stream>>>(term, N, tsum, csum, clow, ccnt);
+        if (!inline_scan)
+            sq_scan_kernel<<<1, 1024, 0, ctx->stream>>>(csum, clow, ccnt, nch, coff, cg);
+        sq_segment_kernel<<<(unsigned)nch, SQ_WG, 0, ctx->stream>>>(
+            term, N, tsum, csum, clow, inline_scan ? nullptr : coff, cg, cnt, rec);
+        sq_walk_kernel<<<1, SQ_WG, 0, ctx->stream>>>(term, N, nch, ccnt, cnt, rec, items,
+                                                     QFinish{qp, pass});
     }
     q_map_kernel<T><<<gsz(N), 256, 0, ctx->stream>>>(img, mask, N, qp, out);
     PCHK(hipGetLastError());
@@ -573,7 +570,7 @@ static ar_status quantize_u8(ar_context* ctx, const T* img, const uint8_t* mask,
 // slot (dst image, dst mask).
 template <typename T>
 static ar_status prepare_one(ar_context* ctx, const T* raw, const uint8_t* mask, int64_t ny,
-                             int64_t nx, void* dst, uint8_t* dmask) {
+                             int64_t nx, void* dst, uint8_t* dmask, int slot) {
     const ar_params& p = ctx->params;
     int64_t N = ny * nx;
     ar_status st;
@@ -669,8 +666,8 @@ static ar_status prepare_one(ar_context* ctx, const T* raw, const uint8_t* mask,
         curmask = dmask;
     }
     if (p.quantize == AR_QUANT_UINT8) {
-        if (hp) return quantize_u8<float>(ctx, hp, curmask, N, (uint8_t*)dst);
-        return quantize_u8<T>(ctx, raw, curmask, N, (uint8_t*)dst);
+        if (hp) return quantize_u8<float>(ctx, hp, curmask, N, (uint8_t*)dst, slot);
+        return quantize_u8<T>(ctx, raw, curmask, N, (uint8_t*)dst, slot);
     }
     // NoQuantize: Float32 output
     if (hp) {
@@ -695,7 +692,7 @@ static size_t dt_size(int32_t dt) {
 
 template <typename T>
 static ar_status prepare_slot(ar_context* ctx, const void* host_img, const uint8_t* host_mask,
-                              int64_t ny, int64_t nx, void* dst, uint8_t* dmask) {
+                              int64_t ny, int64_t nx, void* dst, uint8_t* dmask, int slot) {
     int64_t N = ny * nx;
     ar_status st;
     if ((st = ctx->raw.reserve(ctx, N * sizeof(T))) || (st = ctx->rawmask.reserve(ctx, N)))
@@ -710,7 +707,7 @@ static ar_status prepare_slot(ar_context* ctx, const void* host_img, const uint8
         PCHK(hipGetLastError());
     }
     return prepare_one<T>(ctx, (const T*)ctx->raw.ptr, (const uint8_t*)ctx->rawmask.ptr, ny, nx,
-                          dst, dmask);
+                          dst, dmask, slot);
 }
 
 extern "C" ar_status ar_set_images(ar_context* ctx, const void* ref, const void* sec,
@@ -779,10 +776,10 @@ extern "C" ar_status ar_set_images(ar_context* ctx, const void* ref, const void*
         }
         switch (dtype) {
             case AR_C64: st = ar_prepare_complex(ctx, img, m, ny, nx, dst, dm); break;
-            case AR_U8: st = prepare_slot<uint8_t>(ctx, img, m, ny, nx, dst, dm); break;
-            case AR_I16: st = prepare_slot<int16_t>(ctx, img, m, ny, nx, dst, dm); break;
-            case AR_F32: st = prepare_slot<float>(ctx, img, m, ny, nx, dst, dm); break;
-            default: st = prepare_slot<double>(ctx, img, m, ny, nx, dst, dm); break;
+            case AR_U8: st = prepare_slot<uint8_t>(ctx, img, m, ny, nx, dst, dm, slot); break;
+            case AR_I16: st = prepare_slot<int16_t>(ctx, img, m, ny, nx, dst, dm, slot); break;
+            case AR_F32: st = prepare_slot<float>(ctx, img, m, ny, nx, dst, dm, slot); break;
+            default: st = prepare_slot<double>(ctx, img, m, ny, nx, dst, dm, slot); break;
         }
         if (st) return st;
         // (no wait: the next slot's upload into the same raw buffer is ordered on the stream)
@@ -793,4 +790,27 @@ extern "C" ar_status ar_set_images(ar_context* ctx, const void* ref, const void*
     ctx->has_pair = true;
     // masks are already in rmask/smask: combine without re-uploading
     return ar_upload_masks(ctx, nullptr, nullptr, N);
+}
+
+extern "C" ar_status ar_quantizer_stats(ar_context* ctx, double* stats) {
+    if (!ctx || !stats) return AR_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_pair || ctx->params.quantize != AR_QUANT_UINT8 || !ctx->qstats.ptr)
+        return pfail(ctx, AR_ERR_INVALID_ARGUMENT,
+                     "no UInt8-quantised pair: the quantiser has no parameters to report");
+    PCHK(hipSetDevice(ctx->device));
+    QParams q[2];
+    PCHK(hipMemcpyAsync(q, ctx->qstats.ptr, sizeof(q), hipMemcpyDeviceToHost, ctx->stream));
+    PCHK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < 2; i++) {
+        double* o = stats + 8 * i;
+        o[0] = (double)q[i].n;
+        o[1] = q[i].mean;
+        o[2] = q[i].sd;
+        o[3] = (double)q[i].mode;
+        for (int k = 0; k < 2; k++) {
+            o[4 + k] = (double)q[i].walk_items[k];
+            o[6 + k] = (double)q[i].walk_in_order[k];
+        }
+    }
+    return AR_OK;
 }

@@ -137,6 +137,15 @@ ar_status ar_set_pair(ar_context* ctx, const void* ref, const void* sec,
 ar_status ar_get_pair(ar_context* ctx, void* ref, void* sec, uint8_t* ref_valid,
                       uint8_t* sec_valid, int32_t* dtype);
 
+/* The UInt8 quantiser's statistics of the prepared pair (src/preprocess.jl:546-592), for
+ * inspection and tests: stats[8 i + 0..3] = valid count n, mean, Bessel standard deviation and
+ * mode (0 nothing valid, 1 sd == 0, 2 mapped) of image i (0 reference, 1 secondary); stats[8 i +
+ * 4..7] = for the sum of values and the sum of squared deviations, the number of runs the device
+ * walked and the number of elements it added one by one. The mean and the sum of squared
+ * deviations are the reference's left-to-right Float64 sums, bit for bit.
+ * AR_ERR_INVALID_ARGUMENT unless the context holds a pair prepared with AR_QUANT_UINT8. */
+ar_status ar_quantizer_stats(ar_context* ctx, double* stats);
+
 /* track (src/track.jl:103-160) over n points with host buffers. upsampling 1 = integer peak
  * (NoRefine), else PyramidRefine(upsampling), a power of two <= 64. Writes dx, dy,
  * correlation (NaN where there is no measurement) and searched (0/1). */

@@ -229,6 +229,16 @@ def to_uint8(img, mask):
     return out
 
 
+def quant_stats(img, mask):
+    """(n, mean, sd) of the UInt8 quantiser (src/preprocess.jl:547-570): left-to-right Float64
+    sums over the valid finite values, in column-major order."""
+    v = _f(img, np.float64)
+    mask = _f(mask, np.uint8)
+    out = np.empty(3, np.float64)
+    lib().orc_quant_stats(_p(v), _p(mask), C.c_int64(v.size), _p(out))
+    return int(out[0]), float(out[1]), float(out[2])
+
+
 def count_agreeing(a, w, tol):
     a = _f(a, np.float32)
     out = np.empty(a.shape, np.float32, order="F")

@@ -1145,36 +1145,48 @@ void orc_wallis_filtered(const void* img, const uint8_t* mask, int dt, int64_t n
     filtered_finish(out, mask, allmask, nr, nc, w, vmask);
 }
 
-/* _to_uint8 -- src/preprocess.jl:546-592 */
-void orc_to_uint8(const float* img, const uint8_t* mask, int64_t nr, int64_t nc, uint8_t* out) {
-    int64_t N = nr * nc, n = 0;
+/* The statistics of _to_uint8 -- src/preprocess.jl:547-570: valid count, mean and Bessel standard
+ * deviation, each a left-to-right Float64 sum over the valid finite values v (given as Float64). */
+void orc_quant_stats(const double* v, const uint8_t* mask, int64_t N, double* out) {
+    int64_t n = 0;
     double s = 0.0;
-    for (int64_t k = 0; k < N; k++) {
-        double v = (double)img[k];
-        if (mask[k] && isfinite(v)) { n += 1; s += v; }
-    }
-    memset(out, 0, (size_t)N);
+    for (int64_t k = 0; k < N; k++)
+        if (mask[k] && isfinite(v[k])) { n += 1; s += v[k]; }
+    out[0] = (double)n;
+    out[1] = out[2] = 0.0;
     if (n == 0) return;
     double mean = s / (double)n;
     double ss = 0.0;
-    for (int64_t k = 0; k < N; k++) {
-        double v = (double)img[k];
-        if (mask[k] && isfinite(v)) { double d = v - mean; ss += d * d; }
-    }
-    double sd = n > 1 ? sqrt(ss / (double)(n - 1)) : 0.0;
+    for (int64_t k = 0; k < N; k++)
+        if (mask[k] && isfinite(v[k])) { double d = v[k] - mean; ss += d * d; }
+    out[1] = mean;
+    out[2] = n > 1 ? sqrt(ss / (double)(n - 1)) : 0.0;
+}
+
+/* _to_uint8 -- src/preprocess.jl:546-592 */
+void orc_to_uint8(const float* img, const uint8_t* mask, int64_t nr, int64_t nc, uint8_t* out) {
+    int64_t N = nr * nc;
+    double* v = (double*)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
+    for (int64_t k = 0; k < N; k++) v[k] = (double)img[k];
+    double st[3];
+    orc_quant_stats(v, mask, N, st);
+    memset(out, 0, (size_t)N);
+    if (st[0] == 0) { free(v); return; }
+    double mean = st[1], sd = st[2];
     if (sd == 0) {
         for (int64_t k = 0; k < N; k++) out[k] = mask[k] ? 0x80 : 0x00;
+        free(v);
         return;
     }
     double lo = mean - 3 * sd;
     double scale = 255.0 / (6 * sd);
     for (int64_t k = 0; k < N; k++) {
-        double v = (double)img[k];
-        if (!mask[k] || !isfinite(v)) { out[k] = 0; continue; }
-        double t = (v - lo) * scale;
+        if (!mask[k] || !isfinite(v[k])) { out[k] = 0; continue; }
+        double t = (v[k] - lo) * scale;
         t = t < 0.0 ? 0.0 : (t > 255.0 ? 255.0 : t);
         out[k] = (uint8_t)nearbyint(t);
     }
+    free(v);
 }
 
 /* quantize(::NoQuantize) for floating images -- src/preprocess.jl:497-507 */

@@ -84,6 +84,7 @@ void orc_highpass_filtered(const void* img, const uint8_t* mask, int dtype, int6
 void orc_wallis_filtered(const void* img, const uint8_t* mask, int dtype, int64_t nr, int64_t nc,
                          int64_t w, double min_std, float* out, uint8_t* vmask);
 void orc_to_uint8(const float* img, const uint8_t* mask, int64_t nr, int64_t nc, uint8_t* out);
+void orc_quant_stats(const double* v, const uint8_t* mask, int64_t N, double* out);
 void orc_noquantize(const float* img, int64_t nr, int64_t nc, float* out);
 
 /* Grid-level reductions and the outlier filter. */

@@ -468,3 +468,32 @@ def test_multichip_resolves_shift():
     assert meas.mean() > 0.9
     assert np.all(cs[meas] == 32)
     assert np.all(np.abs(dx[meas] + 4) <= 3 / 64) and np.all(np.abs(dy[meas] - 3) <= 3 / 64)
+
+
+def test_quant_stats_is_the_sequential_sum():
+    """orc_quant_stats (src/preprocess.jl:547-570) is a left-to-right Float64 sum, not a
+    correctly rounded or pairwise one. Pinned by a case where the orders disagree: 2^53 followed
+    by ones. Left to right every 2^53 + 1 rounds back to 2^53 (ties to even), so the ones vanish;
+    the exact sum keeps them. The squared deviations are pinned the same way against a plain
+    Python loop, whose float additions are sequential."""
+    n = 1001
+    v = np.ones(n)
+    v[0] = 2.0 ** 53
+    mask = np.ones(n, bool)
+    cnt, mean, sd = oracle.quant_stats(v.reshape(-1, 1), mask.reshape(-1, 1))
+    assert cnt == n
+    assert mean == 2.0 ** 53 / n                      # the ones were absorbed
+    assert mean != (2.0 ** 53 + (n - 1)) / n          # what an exact sum gives
+    ss = 0.0
+    for x in v:
+        d = x - mean
+        ss += d * d
+    assert sd == np.sqrt(ss / (n - 1))
+    # closed form on small integers: [1, 2, 3, 4] -> mean 2.5, sd sqrt(5/3); masked and
+    # non-finite elements are skipped
+    w = np.array([[1.0, np.nan], [2.0, 99.0], [3.0, np.inf], [4.0, 7.0]])
+    m = np.array([[True, True], [True, False], [True, True], [True, False]])
+    cnt, mean, sd = oracle.quant_stats(w, m)
+    assert (cnt, mean, sd) == (4, 2.5, np.sqrt(5.0 / 3.0))
+    assert oracle.quant_stats(w, np.zeros_like(m)) == (0, 0.0, 0.0)
+    assert oracle.quant_stats(np.array([[5.0]]), np.array([[True]])) == (1, 5.0, 0.0)
