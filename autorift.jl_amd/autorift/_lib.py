@@ -78,6 +78,8 @@ SIGNATURES = [
     ("ar_get_pair", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                               C.POINTER(C.c_int32)]),
     ("ar_quantizer_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("ar_sequential_sum", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                    C.POINTER(C.c_double), C.c_void_p]),
     ("ar_track", C.c_int, [C.c_void_p, C.POINTER(Points), C.c_int64, C.c_int64, C.c_int32,
                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("ar_track_device", C.c_int, [C.c_void_p, C.POINTER(Points), C.c_int64, C.c_int64,

@@ -165,6 +165,17 @@ class Context:
             res.append(r)
         return tuple(res)
 
+    def sequential_sum(self, x, method=0):
+        """sum(x) added left to right in Float64, computed on the device by the chunk walk
+        (method 0: returns (sum, runs walked, elements added one by one)) or by the one-wave
+        replay (method 1: non-negative finite terms; returns (sum, 0, 0))."""
+        x = np.ascontiguousarray(x, np.float64).ravel()
+        out = C.c_double()
+        walk = np.zeros(2, np.int64)
+        self._check(lib().ar_sequential_sum(self._h, _ptr(x), x.size, method, C.byref(out),
+                                            _ptr(walk)))
+        return out.value, int(walk[0]), int(walk[1])
+
     # -- correlation -----------------------------------------------------------------------------
     def track(self, pts, upsampling: int = 1, measure: int | None = None) -> DisplacementField:
         """track (src/track.jl:103-160) over every point of `pts`; `measure` defaults to the

@@ -450,6 +450,34 @@ struct QTerm {
     }
 };
 
+// The elements of a plain array, every one counted (ar_sequential_sum).
+struct ArrayTerm {
+    const double* x;
+    __device__ inline double operator()(int64_t k, bool& ok) const {
+        ok = true;
+        return x[k];
+    }
+};
+
+struct SumOut {
+    double* out;
+    long long* stats;
+    __device__ inline void operator()(double sum, long long, const SqWalkStats& st) const {
+        *out = sum;
+        stats[0] = st.items;
+        stats[1] = st.in_order;
+    }
+};
+
+// x[0] + x[1] + ... left to right by one wave's whole-wave steps (non-negative finite terms).
+__global__ __launch_bounds__(64) void wave_sum_kernel(const double* x, int64_t n, double* out) {
+    const int lane = threadIdx.x;
+    double s = 0.0;
+    for (int64_t k0 = 0; k0 < n; k0 += 64)
+        s = seq_sum64_ties(s, k0 + lane < n ? x[k0 + lane] : 0.0, lane);
+    if (lane == 0) *out = s;
+}
+
 // After the walk of pass 0: n and mean = s / n; after pass 1: sd = sqrt(ss / (n - 1)) and the map.
 struct QFinish {
     QParams* qp;
@@ -811,6 +839,63 @@ extern "C" ar_status ar_quantizer_stats(ar_context* ctx, double* stats) {
             o[4 + k] = (double)q[i].walk_items[k];
             o[6 + k] = (double)q[i].walk_in_order[k];
         }
+    }
+    return AR_OK;
+}
+
+extern "C" ar_status ar_sequential_sum(ar_context* ctx, const double* x, int64_t n, int32_t method,
+                                       double* out, int64_t* walk) {
+    if (!ctx || !x || !out || n <= 0) return AR_ERR_INVALID_ARGUMENT;
+    if (method != 0 && method != 1)
+        return pfail(ctx, AR_ERR_INVALID_ARGUMENT, "method must be 0 (chunk walk) or 1 (one wave)");
+    if (method == 1)
+        for (int64_t k = 0; k < n; k++)
+            if (!(x[k] >= 0.0) || !std::isfinite(x[k]))
+                return pfail(ctx, AR_ERR_INVALID_ARGUMENT,
+                             "the one-wave sum takes non-negative finite terms");
+    const int64_t nch = (n + SQ_CHUNK - 1) / SQ_CHUNK;
+    if (nch >= ((int64_t)1 << 31)) return pfail(ctx, AR_ERR_UNSUPPORTED, "array too long");
+    PCHK(hipSetDevice(ctx->device));
+    ar_status st;
+    const size_t recb = (size_t)nch * SQ_SLOTS * sizeof(SqRec);
+    const size_t per = SQ_WG * 8 + 8 + 8 + 8 + 4 + 4 + 4;
+    if ((st = ctx->work_f64.reserve(ctx, (size_t)n * 8 + 64))) return st;
+    if ((st = ctx->qsum.reserve(ctx, 2 * recb + (size_t)nch * per + 256))) return st;
+    double* dx = (double*)ctx->work_f64.ptr;
+    double* dout = dx + n;
+    long long* dstats = (long long*)(dout + 1);
+    PCHK(hipMemcpyAsync(dx, x, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
+    if (method == 1) {
+        wave_sum_kernel<<<1, 64, 0, ctx->stream>>>(dx, n, dout);
+    } else {
+        char* b = (char*)ctx->qsum.ptr;
+        SqRec* rec = (SqRec*)b;
+        SqRec* items = (SqRec*)(b + recb);
+        double* tsum = (double*)(b + 2 * recb);
+        double* csum = tsum + (size_t)nch * SQ_WG;
+        double* coff = csum + nch;
+        long long* ccnt = (long long*)(coff + nch);
+        int* clow = (int*)(ccnt + nch);
+        int* cg = clow + nch;
+        int* cnt = cg + nch;
+        const bool inline_scan = nch <= SQ_SCAN_INLINE;
+        const ArrayTerm term{dx};
+        sq_chunk_kernel<<<(unsigned)nch, SQ_WG, 0, ctx->stream>>>(term, n, tsum, csum, clow, ccnt);
+        if (!inline_scan)
+            sq_scan_kernel<<<1, 1024, 0, ctx->stream>>>(csum, clow, ccnt, nch, coff, cg);
+        sq_segment_kernel<<<(unsigned)nch, SQ_WG, 0, ctx->stream>>>(
+            term, n, tsum, csum, clow, inline_scan ? nullptr : coff, cg, cnt, rec);
+        sq_walk_kernel<<<1, SQ_WG, 0, ctx->stream>>>(term, n, nch, ccnt, cnt, rec, items,
+                                                     SumOut{dout, dstats});
+    }
+    PCHK(hipGetLastError());
+    long long ws[2] = {0, 0};
+    PCHK(hipMemcpyAsync(out, dout, 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (method == 0) PCHK(hipMemcpyAsync(ws, dstats, 16, hipMemcpyDeviceToHost, ctx->stream));
+    PCHK(hipStreamSynchronize(ctx->stream));
+    if (walk) {
+        walk[0] = ws[0];
+        walk[1] = ws[1];
     }
     return AR_OK;
 }

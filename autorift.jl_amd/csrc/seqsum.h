@@ -258,7 +258,7 @@ __global__ __launch_bounds__(SQ_WG) void sq_chunk_kernel(TERM term, int64_t n, d
 
 // One workgroup: sq_scan_block over every chunk (images of more than SQ_SCAN_INLINE chunks; the
 // segment kernel forms the prefixes of smaller ones itself).
-__global__ __launch_bounds__(1024) void sq_scan_kernel(const double* csum, const int* clow,
+static __global__ __launch_bounds__(1024) void sq_scan_kernel(const double* csum, const int* clow,
                                                        const long long* ccnt, int64_t nch,
                                                        double* coff, int* cg) {
     sq_scan_block(csum, clow, ccnt, nch, coff, cg);
@@ -781,6 +781,74 @@ __global__ __launch_bounds__(SQ_WG) void sq_walk_kernel(TERM term, int64_t n, in
             fin(sum, count, st);
         }
     }
+}
+
+// ---- the reference's left-to-right Float64 sum of non-negative terms, one wave at a time
+
+__device__ inline double wave_lane_f64(double v, int l) {
+    const uint64_t u = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// s + t_0 + t_1 + ... + t_63 added left to right in Float64 (t_l >= 0 finite, this lane's t) for a
+// wave-uniform s >= 0: the bits of the sequential loop. While the running sum stays strictly
+// inside one binade [2^E, 2^(E+1)) every addition rounds its term to the grid u = 2^(E - 52) that
+// s lies on: s_k = s + u (r_0 + ... + r_k) with r_l = t_l / u rounded to the nearest integer,
+// where a tie (t_l / u halfway between two integers) takes the neighbour that makes s_k / u even.
+// A tie's choice depends on the parity of the running sum, so each lane's step is a map of the
+// entry parity (what it adds and the parity it leaves), and an inclusive scan of those maps over
+// the lanes gives every partial sum for the actual parity of s / u. The first lane whose partial
+// sum would leave the binade is added on its own, the reference's way, and the scan resumes after
+// it. Every step is exact: the r_l and their prefix sums are integers below 2^53.
+__device__ inline double seq_sum64_ties(double s, double t, int lane) {
+    int start = 0;
+    while (start < 64) {
+        if (!(s >= 0x1p-900)) {   // zero or tiny: one term at a time
+            s = s + wave_lane_f64(t, start);
+            start++;
+            continue;
+        }
+        const int E = (int)((__double_as_longlong(s) >> 52) & 0x7ff) - 1023;
+        const double u = __longlong_as_double((long long)(E - 52 + 1023) << 52);
+        const double inv = __longlong_as_double((long long)(52 - E + 1023) << 52);
+        const double q = lane >= start ? t * inv : 0.0;   // exact: scaling by a power of two
+        const double fl = floor(q), fr = q - fl;
+        const bool tie = fr == 0.5;
+        const bool fl_even = fmod(fl, 2.0) == 0.0;
+        // what the lane adds when the running sum is even (a) / odd (b), and the parity it leaves
+        const double a = tie ? (fl_even ? fl : fl + 1.0) : (fr > 0.5 ? fl + 1.0 : fl);
+        const double b = tie ? (fl_even ? fl + 1.0 : fl) : a;
+        double T0 = a, T1 = b;
+        int P0 = (int)(fmod(a, 2.0) != 0.0), P1 = (int)(fmod(1.0 + b, 2.0) != 0.0);
+        for (int d = 1; d < 64; d <<= 1) {   // inclusive scan of the maps (earlier lanes first)
+            const double o0 = __shfl_up(T0, d, 64), o1 = __shfl_up(T1, d, 64);
+            const int q0 = __shfl_up(P0, d, 64), q1 = __shfl_up(P1, d, 64);
+            if (lane >= d) {
+                const double n0 = o0 + (q0 ? T1 : T0), n1 = o1 + (q1 ? T1 : T0);
+                const int m0 = q0 ? P1 : P0, m1 = q1 ? P1 : P0;
+                T0 = n0;
+                T1 = n1;
+                P0 = m0;
+                P1 = m1;
+            }
+        }
+        const int par = (int)((__double_as_longlong(s) >> 0) & 1);   // parity of s / u
+        const double P = par ? T1 : T0;
+        const double lim = __longlong_as_double((long long)(E + 1 + 1023) << 52) - u;
+        const bool bad = lane >= start && !(s + P * u <= lim);
+        const uint64_t bm = __ballot(bad);
+        const int f = bm ? __builtin_ctzll(bm) : 64;
+        if (f > start) s = s + wave_lane_f64(P, f - 1) * u;   // lanes start .. f-1: exact
+        if (f < 64) {
+            s = s + wave_lane_f64(t, f);
+            start = f + 1;
+        } else {
+            start = 64;
+        }
+    }
+    return s;
 }
 
 }  // namespace ar

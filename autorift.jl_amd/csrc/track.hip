@@ -20,6 +20,7 @@
 
 #include "layout.h"
 #include "refine.h"
+#include "seqsum.h"
 #include "track_common.h"
 
 namespace ar {
@@ -168,6 +169,28 @@ __device__ inline void mfma_batch(const int8_t* Wl, const int8_t* Tp, int ldw, i
 }
 
 #define AR_NRB_MAX 9   // band heights with a dedicated MFMA loop (chips up to 130 rows)
+
+// The chip energy sum (x - mean)^2 in the reference's order (src/correlate.jl:440-447: column by
+// column, left to right in Float64) over the staged chip (byte u of column v at Tp[v cpad + 16 + u],
+// value ^ 0x80), by one wave. Kept out of line: only chips whose integer energy reaches 2^53 / n^2
+// take it, and it should cost the tracker's main loop no registers.
+__device__ __noinline__ double u8_energy_in_order(const int8_t* Tp, int cpad, int ch, int cw,
+                                                  int64_t sumT, double invn, int lane) {
+    const double mean = (double)sumT * invn;   // exact: n is a power of two
+    const int n = ch * cw;
+    double acc = 0.0;
+    for (int e0 = 0; e0 < n; e0 += 64) {
+        const int e = e0 + lane;
+        double t = 0.0;
+        if (e < n) {
+            const int v = e / ch, u = e - v * ch;
+            const double d = (double)(uint8_t)(Tp[v * cpad + 16 + u] ^ 0x80) - mean;
+            t = d * d;
+        }
+        acc = seq_sum64_ties(acc, t, lane);
+    }
+    return acc;
+}
 
 // Four workgroups per CU (128 VGPRs) when the LDS layout allows it.
 // BATCH: many pairs in one launch. Point k of the launch is order[k]; its images come from its
@@ -435,6 +458,25 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         // n = 2^lg: 1 / n and the division by n^2 are exact scalings
         P.invn = ldexp(1.0, -(int)__builtin_ctzll((unsigned long long)P.n));
         double sq = (double)sumk2 * P.invn * P.invn;
+        if (sumk2 >= ((int64_t)1 << 53)) {
+            // The reference sums (x - mean)^2 left to right (src/correlate.jl:440-447). Each term is
+            // a multiple of 1/n^2 and the partial sums grow, so the sum is exact, and equal to the
+            // integer form above, while it stays below 2^53 / n^2; past that (128 x 128 chips whose
+            // variance exceeds 2048) its additions round: one wave replays them.
+            __syncthreads();   // every wave has read the chip sums out of red_chip
+            if (wave == 0) {
+                const double acc = u8_energy_in_order(Tp, L.cpad, ch, cw, P.sumT, P.invn, lane);
+                if (lane == 0) {
+                    const uint64_t b = (uint64_t)__double_as_longlong(acc);
+                    red_chip[0] = (uint32_t)b;
+                    red_chip[1] = (uint32_t)(b >> 32);
+                }
+            }
+            __syncthreads();
+            sq = __longlong_as_double(
+                (long long)(((uint64_t)red_chip[1] << 32) | (uint64_t)red_chip[0]));
+            __syncthreads();   // red_chip is the next point's reduction slot
+        }
         bool ok = sq > 2.220446049250313e-16 * (double)ch * (double)cw;
         if (!ok) { write_skip(a, p, true); __syncthreads(); continue; }
         P.cnorm = sqrt(sq);

@@ -508,4 +508,5 @@ __device__ inline bool patch_origin(int pi, int pj, int nr, int nc, int& i0, int
     return !(nr < 5 || nc < 5);
 }
 
+
 }  // namespace ar

@@ -146,6 +146,15 @@ ar_status ar_get_pair(ar_context* ctx, void* ref, void* sec, uint8_t* ref_valid,
  * AR_ERR_INVALID_ARGUMENT unless the context holds a pair prepared with AR_QUANT_UINT8. */
 ar_status ar_quantizer_stats(ar_context* ctx, double* stats);
 
+/* x[0] + x[1] + ... + x[n-1] added left to right in Float64 -- the reference's sequential sums
+ * (e.g. src/preprocess.jl:551-570, src/correlate.jl:435-447) -- computed on the context's device
+ * by the library's parallel emulations, for inspection and tests: method 0 the chunk walk the
+ * quantiser uses (any finite or non-finite terms; walk[0..1] = runs walked, elements added one
+ * by one; walk may be NULL), method 1 the one-wave replay the trackers use for chip energies
+ * (non-negative finite terms). No counterpart in the reference. */
+ar_status ar_sequential_sum(ar_context* ctx, const double* x, int64_t n, int32_t method,
+                            double* out, int64_t* walk);
+
 /* track (src/track.jl:103-160) over n points with host buffers. upsampling 1 = integer peak
  * (NoRefine), else PyramidRefine(upsampling), a power of two <= 64. Writes dx, dy,
  * correlation (NaN where there is no measurement) and searched (0/1). */

@@ -150,3 +150,74 @@ def test_walk_takes_the_fast_path():
         for items, in_order in st[4:]:
             assert items <= 256
             assert in_order <= 4096
+
+
+def _left_to_right(v):
+    s = 0.0
+    for x in v.tolist():   # Python float additions: sequential, round to nearest even
+        s += x
+    return s
+
+
+def _sequences(rng):
+    out = {}
+    v = np.ones(5000)
+    v[0] = 2.0 ** 53
+    out["absorbed_ones"] = v                              # 2^53 + 1 rounds back to 2^53
+    out["squares"] = (rng.standard_normal(300_000) * 30) ** 2      # monotone, rounds every step
+    t = np.full(70_000, 2.0 ** -23)
+    t[rng.random(t.size) < 0.3] = 3 * 2.0 ** -23
+    t[0] = 2.0 ** 30
+    out["ties"] = t                                       # every step a tie on the 2^-22 grid
+    k = rng.integers(-2 ** 21, 2 ** 21, 16384) * 2 + 1
+    out["u8_energy"] = (k.astype(np.float64) ** 2) / 2.0 ** 28     # odd k: ties past 2^25
+    out["tiny_then_large"] = np.concatenate([rng.random(3000) * 1e-300,
+                                             rng.random(3000) * 1e300])
+    return out
+
+
+def test_sequential_sum_one_wave():
+    """The one-wave replay the trackers use for chip energies (non-negative terms)."""
+    rng = np.random.default_rng(5)
+    with autorift.Context() as ctx:
+        for name, v in _sequences(rng).items():
+            print(f"one-wave sum {name} ({v.size})", flush=True)
+            got, _, _ = ctx.sequential_sum(v, method=1)
+            exp = _left_to_right(v)
+            assert np.float64(got).tobytes() == np.float64(exp).tobytes(), (name, got, exp)
+
+
+def test_sequential_sum_chunk_walk():
+    """The chunk walk the quantiser uses, on signed and non-finite sequences as well."""
+    rng = np.random.default_rng(6)
+    seqs = _sequences(rng)
+    seqs["walk_f32"] = (rng.standard_normal(2_000_003) * 30).astype(np.float32).astype(np.float64)
+    w = rng.standard_normal(200_000) * 10.0 ** rng.integers(-30, 30, 200_000)
+    seqs["signed_wide"] = w
+    z = rng.standard_normal(50_000)
+    z[::7] = -0.0
+    z[1000:1100] = 5e-324
+    seqs["zeros_denormals"] = z
+    inf = rng.standard_normal(20_000)
+    inf[15_000] = np.inf
+    seqs["infinite"] = inf
+    nan = rng.standard_normal(20_000)
+    nan[100] = np.inf
+    nan[200] = -np.inf
+    seqs["nan"] = nan
+    with autorift.Context() as ctx:
+        for name, v in seqs.items():
+            print(f"chunk-walk sum {name} ({v.size})", flush=True)
+            got, runs, one_by_one = ctx.sequential_sum(v, method=0)
+            exp = _left_to_right(v)
+            assert np.float64(got).tobytes() == np.float64(exp).tobytes() or (
+                np.isnan(got) and np.isnan(exp)), (name, got, exp)
+            print(f"  runs {runs}, added one by one {one_by_one}", flush=True)
+
+
+def test_sequential_sum_rejects_bad_input():
+    with autorift.Context() as ctx:
+        with pytest.raises(Exception):
+            ctx.sequential_sum(np.array([1.0, -2.0]), method=1)
+        with pytest.raises(Exception):
+            ctx.sequential_sum(np.array([1.0]), method=2)

@@ -149,3 +149,54 @@ def test_track_known_shift_recovery(ctx):
     assert np.all(g.dx == -4) and np.all(g.dy == 3)
     g = ctx.track(pts, upsampling=64)
     assert np.all(np.abs(g.dx + 4) <= 3 / 64) and np.all(np.abs(g.dy - 3) <= 3 / 64)
+
+
+def _energy_in_order(chip):
+    """sum (x - mean)^2 of a chip in the reference's order (src/correlate.jl:435-447), as plain
+    Python float additions, which are sequential."""
+    v = [float(x) for x in chip.ravel(order="F")]
+    s = 0.0
+    for x in v:
+        s += x
+    mean = s / len(v)
+    sq = 0.0
+    for x in v:
+        d = x - mean
+        sq += d * d
+    return sq
+
+
+def _energy_exact(chip):
+    """The same sum without rounding, rounded once."""
+    n = chip.size
+    s = int(chip.sum(dtype=np.int64))
+    s2 = int((chip.astype(np.int64) ** 2).sum())
+    return float((n * n * s2 - n * s * s)) / n / n
+
+
+@pytest.mark.parametrize("odd", [False, True])
+def test_u8_chip_energy_in_reference_order(ctx, odd):
+    """128 x 128 UInt8 chips of high contrast: past 2^53 / n^2 the reference's sequential energy
+    sum rounds (every step ties when the chip sum is odd), so an exact integer energy differs from
+    it in the last bits. The device must take the reference's; the case is checked to have that
+    difference, and the correlation surface's peak and value are compared bit for bit."""
+    rng = np.random.default_rng(17 + odd)
+    r = 8
+    found = 0
+    for k in range(6):
+        base = rng.integers(0, 2, (128 + 2 * r + 15, 128 + 2 * r + 15)) * 255
+        base = np.clip(base + rng.integers(-40, 41, base.shape), 0, 255).astype(np.uint8)
+        search = np.ascontiguousarray(base[5:5 + 128 + 2 * r - 1, 7:7 + 128 + 2 * r - 1])
+        chip = base[5 + r + 2:5 + r + 2 + 128, 7 + r - 3:7 + r - 3 + 128].copy()
+        if (int(chip.sum(dtype=np.int64)) % 2 == 1) != odd:
+            chip[0, 0] = chip[0, 0] + 1 if chip[0, 0] < 255 else chip[0, 0] - 1
+        if _energy_in_order(chip) != _energy_exact(chip):
+            found += 1
+        print(f"u8 energy c128 r{r} odd={odd} case {k}", flush=True)
+        for up in (1, 64):
+            got = ctx.correlate_point(search, chip, r, upsampling=up)
+            assert ctx.last_kernel()[1] == 0   # the int8 matrix-core tracker
+            exp = _oracle_point(search, chip, r, r, up, oracle.ZNCC)
+            for g, e in zip(got, exp):
+                assert _same(g, e), (k, up, got, exp)
+    assert found >= 3   # the cases exercise the rounding sequential sum
