@@ -196,32 +196,44 @@ def serving_sweep(autorift, rates, nreq_max, slots, pool_size, seed=6):
     return res
 
 
-def served_mixed_vs_plain(autorift, count=8, slots=4, seed=7):
+def served_mixed_vs_plain(autorift, count=8, slots=4, seed=7, reps=3):
     """Float32 pairs (quantize = :none), default multi-chip search: the plain form (exact Float64
     contraction, one pair at a time on one context) against the served mixed-precision form
-    (BF16 contraction, pairs submitted together to a Server with `slots` slots): pairs/s of each,
-    and how far the served results are from the plain ones."""
+    (BF16 contraction, pairs submitted together to a Server with `slots` slots): pairs/s of each
+    (the best of `reps` timed passes over the pool), and how far the served results are from the
+    plain ones."""
     pool = serve_pool(count, seed, lo=1024, hi=4096)
     log(f"served-vs-plain pool: sides {[a.shape[0] for a, _, _ in pool]}")
     plain = []
+    # Each form is warmed up on the whole pool first (untimed), so both are timed at steady
+    # state: the plain context's buffers have grown to the largest pair, the server's page-locked
+    # staging pool holds a buffer for every pair in flight.
     with autorift.Context(autorift.params(quantize="none")) as ctx:
-        for a, b, _ in pool[:1]:                       # warm-up
+        for a, b, _ in pool:                           # warm-up
             ctx.set_images(a, b)
             ctx.run_multichip(autorift.api._build_grid(a.shape, ctx.params))
-        t = time.perf_counter()
-        set_s = 0.0
-        for a, b, _ in pool:
-            t0 = time.perf_counter()
-            ctx.set_images(a, b)                       # upload and preparation of the pair
-            set_s += time.perf_counter() - t0
-            plain.append(ctx.run_multichip(autorift.api._build_grid(a.shape, ctx.params)))
-        plain_s = time.perf_counter() - t
+        plain_s = float("inf")
+        for _ in range(reps):
+            t = time.perf_counter()
+            set_k = 0.0
+            plain = []
+            for a, b, _ in pool:
+                t0 = time.perf_counter()
+                ctx.set_images(a, b)                   # upload and preparation of the pair
+                set_k += time.perf_counter() - t0
+                plain.append(ctx.run_multichip(autorift.api._build_grid(a.shape, ctx.params)))
+            el = time.perf_counter() - t
+            if el < plain_s:
+                plain_s, set_s = el, set_k
     with autorift.Server(autorift.params(quantize="none", precision="bf16"), slots=slots) as srv:
-        srv.result(srv.submit(pool[0][0], pool[0][1]))   # warm-up
-        t = time.perf_counter()
-        tickets = [srv.submit(a, b) for a, b, _ in pool]
-        served = [srv.result(k)[0] for k in tickets]
-        served_s = time.perf_counter() - t
+        for k in [srv.submit(a, b) for a, b, _ in pool]:   # warm-up
+            srv.result(k)
+        served_s = float("inf")
+        for _ in range(reps):
+            t = time.perf_counter()
+            tickets = [srv.submit(a, b) for a, b, _ in pool]
+            served = [srv.result(k)[0] for k in tickets]
+            served_s = min(served_s, time.perf_counter() - t)
     same, tot, sq = 0, 0, []
     for r, e in zip(served, plain):
         ok = np.isfinite(e.dx) & np.isfinite(r.dx)
@@ -230,7 +242,7 @@ def served_mixed_vs_plain(autorift, count=8, slots=4, seed=7):
         sq.append((r.dx[ok] - e.dx[ok]) ** 2 + (r.dy[ok] - e.dy[ok]) ** 2)
     res = {"pairs": count, "sides": [int(a.shape[0]) for a, _, _ in pool], "slots": slots,
            "plain_fp32_pairs_per_s": count / plain_s, "served_bf16_pairs_per_s": count / served_s,
-           "speedup": plain_s / served_s,
+           "speedup": plain_s / served_s, "timed_passes": reps,
            # share of the plain form's time in set_images (host-to-device upload of the two
            # Float32 images and their preparation): work the served form repeats for every pair
            "plain_set_images_share": set_s / plain_s,
