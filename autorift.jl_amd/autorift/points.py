@@ -31,6 +31,12 @@ class PointSet:
                 raise DimensionMismatch(
                     f"PointSet field `{f.name}` has shape {np.shape(getattr(self, f.name))}, "
                     f"expected {shp} to match `x`")
+        # column-major storage, as the reference's arrays and the C ABI have it: handing the
+        # fields to the library is then a view, not a transposing copy per call
+        for f in fields(self):
+            v = getattr(self, f.name)
+            if isinstance(v, np.ndarray) and v.ndim > 1 and not v.flags.f_contiguous:
+                setattr(self, f.name, np.asfortranarray(v))
 
     @property
     def shape(self):

@@ -613,7 +613,7 @@ static ar_status f64_enqueue(ar_context* ctx, const TrackArgs& a, const F64Plan&
 // The pass on device-resident points. All pointers device pointers.
 ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t up,
                         int32_t measure, float* dx, float* dy, float* corr, uint8_t* srch,
-                        const ar_pass_hint* hint) {
+                        const ar_pass_hint* hint, int64_t searchable_max) {
     if (!ctx->has_pair) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "no image pair set");
     if (n < 0) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "negative point count");
     if (up < 1 || up > 64 || (up & (up - 1)) != 0)
@@ -810,8 +810,8 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     const int kwg = use_fft ? AR_FFT_WG : AR_WG;   // threads per workgroup of the tracker kernel
     const int bmode = box_images_mode();
     // Box images of W and W^2 replace the per-point box sums when every point has the same chip,
-    // the points' shifts cover at least a quarter of the image's chip positions and there are at
-    // least 2^22 positions (below that the extra launch costs more than it saves).
+    // the searchable points' shifts cover at least a quarter of the image's chip positions and
+    // there are at least 2^22 positions (below that the extra launch costs more than it saves).
     // (The kernel reads a point's S1 / S2 at 32-bit byte offsets 4 (i + j ld) from its first
     // shift, ld = ny + 2 pady - ch + 1.)
     const int64_t box_reach = 4 * (2 * a.max_ry + 2 * a.max_rx * (ctx->ny + 2 * a.pady));
@@ -819,7 +819,10 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         box_reach < ((int64_t)1 << 32)) {
         const double positions = (double)(a.ny + 2 * a.pady - a.max_ch + 1) *
                                  (double)(a.nx + 2 * a.padx - a.max_cw + 1);
-        const double shifts = (double)n * (2.0 * a.max_rx) * (2.0 * a.max_ry);
+        // points that can be searched: all of them, or fewer when the caller knows (a multi-chip
+        // level whose points were mostly finished at a smaller chip)
+        const int64_t nsearch = searchable_max >= 0 && searchable_max < n ? searchable_max : n;
+        const double shifts = (double)nsearch * (2.0 * a.max_rx) * (2.0 * a.max_ry);
         if (positions >= 1 &&
             (bmode == 1 || (positions >= 4194304.0 && 4.0 * shifts >= positions))) {
             int32_t *b1 = nullptr, *b2 = nullptr;

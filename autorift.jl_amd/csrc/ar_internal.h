@@ -98,9 +98,11 @@ struct ar_pass_hint {
 // With a hint the pass also returns without waiting for the stream (the caller's next readback
 // orders it); its kernel times are resolved by ar_resolve_timing.
 void ar_resolve_timing(ar_context* ctx);
+// searchable_max: an upper bound on the pass's searchable points when the caller knows one
+// (-1: every point may be searched); it only informs the choice of box images.
 ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t up,
                         int32_t measure, float* dx, float* dy, float* corr, uint8_t* srch,
-                        const ar_pass_hint* hint = nullptr);
+                        const ar_pass_hint* hint = nullptr, int64_t searchable_max = -1);
 ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t padx,
                         int32_t** b1, int32_t** b2, int64_t* ld);
 // Page-locked host staging of at least n bytes (grow-only; the stream is idle when it grows).

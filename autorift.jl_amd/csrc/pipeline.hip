@@ -875,7 +875,9 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
                                                            crx, cry, ccx, ccy, cs8, cnt + 2);
             MCHK(hipGetLastError());
             ar_points cp = {cxp, cyp, crx, cry, cpdx, cpdy, ccx, ccy};
-            if ((st = ar_track_pass(ctx, cp, M, 1, measure, cdx, cdy, ccr, cs8, hp))) return st;
+            // the level's points bound the searchable coarse points (c2[1], read back above)
+            if ((st = ar_track_pass(ctx, cp, M, 1, measure, cdx, cdy, ccr, cs8, hp, (int64_t)c2[1])))
+                return st;
             uint8_t* keep = nullptr;
             int64_t iters = p.gardner_iterations - 1 > 1 ? p.gardner_iterations - 1 : 1;
             if ((st = reject(ctx, cdx, cdy, crx, cry, cmeas, nrc, ncc, iters, cnx, cny, cka, ckb,
@@ -892,7 +894,8 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
 
         // ---- fine pass, rejection, hole filling, merge
         ar_points lp = {gx, gy, lrx, lry, gpdx, gpdy, lcx, lcy};
-        if ((st = ar_track_pass(ctx, lp, N, p.upsampling, measure, fdx, fdy, fc, fs, hp)))
+        if ((st = ar_track_pass(ctx, lp, N, p.upsampling, measure, fdx, fdy, fc, fs, hp,
+                                (int64_t)c2[1])))
             return st;
         uint8_t* keep = nullptr;
         if ((st = reject(ctx, fdx, fdy, lrx, lry, meas, nr, nc, p.gardner_iterations, nxb, nyb, ka,

@@ -255,6 +255,49 @@ def served_mixed_vs_plain(autorift, count=8, slots=4, seed=7, reps=3):
     return res
 
 
+def cfg2_block(autorift, tex, sec, fx, fy, reps=2):
+    """The production configuration as written: the multi-chip search over chip sizes 16, 32, 64
+    and 128 (search radius 25, grid spacing 24) on the 10000^2 pair, for each quantiser and
+    contraction: preparation (upload of the two Float32 images included) and search times (best
+    of `reps`), grid points per second of the search, points finished per chip-size level, and
+    the error against the known displacement field."""
+    out = {}
+    # held column-major, as the ABI takes them (a Julia caller's layout: no host transposition)
+    tex, sec = np.asfortranarray(tex), np.asfortranarray(sec)
+    forms = (("uint8", dict(quantize="uint8")),
+             ("float32_exact", dict(quantize="none")),
+             ("float32_bf16", dict(quantize="none", precision="bf16")),
+             ("float32_fft", dict(quantize="none", precision="fft")))
+    for name, kw in forms:
+        p = autorift.params(chip_size=16, chip_size_max=128, grid_spacing=24, search_radius=25,
+                            **kw)
+        grid = autorift.api._build_grid(tex.shape, p)
+        prep_s = search_s = float("inf")
+        with autorift.Context(p) as ctx:
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                ctx.set_images(tex, sec)
+                t1 = time.perf_counter()
+                r = ctx.run_multichip(grid)
+                t2 = time.perf_counter()
+                prep_s, search_s = min(prep_s, t1 - t0), min(search_s, t2 - t1)
+        ok = np.isfinite(r.dx)
+        iy = np.clip(np.floor(grid.y).astype(int) - 1, 0, tex.shape[0] - 1)
+        ix = np.clip(np.floor(grid.x).astype(int) - 1, 0, tex.shape[1] - 1)
+        err = np.hypot(r.dx + fx[iy, ix], r.dy + fy[iy, ix])[ok]
+        levels = {str(int(c)): int((r.chip_size == c).sum()) for c in np.unique(r.chip_size)}
+        out[name] = {"grid_points": int(grid.x.size), "prepare_ms": prep_s * 1000,
+                     "search_ms": search_s * 1000,
+                     "grid_points_per_s": grid.x.size / search_s,
+                     "measured": float(ok.mean()), "points_per_chip_size": levels,
+                     "median_error_px": float(np.median(err)),
+                     "p95_error_px": float(np.percentile(err, 95))}
+        log(f"cfg2 {name}: prepare {prep_s * 1000:.1f} ms, search {search_s * 1000:.1f} ms "
+            f"({grid.x.size / search_s / 1e6:.2f} M grid points/s), levels {levels}, median "
+            f"error {np.median(err):.3f} px")
+    return out
+
+
 def speckle_pair(n, shift, seed):
     """Circular Gaussian speckle smoothed by two 2x2 box passes (wrap-around), and a copy rolled
     by `shift` plus a little complex noise: a synthetic single-look-complex pair."""
@@ -645,6 +688,8 @@ def main():
     ap.add_argument("--no-served-compare", action="store_true",
                     help="skip the served BF16 against plain Float32 comparison")
     ap.add_argument("--no-coherence", action="store_true")
+    ap.add_argument("--no-cfg2", action="store_true",
+                    help="skip the full multi-chip search on the production-size pair")
     ap.add_argument("--batch-pairs", type=int, default=256,
                     help="pairs of the batched workload (0: skip it)")
     ap.add_argument("--batch-points", type=int, default=1_000_000)
@@ -665,7 +710,7 @@ def main():
 
     n = args.size
     t0 = time.time()
-    tex, sec = synthetic_pair(n)
+    tex, sec, fx, fy = synthetic_pair(n, fields=True)
     log(f"synthetic pair {n}x{n} in {time.time() - t0:.1f}s")
     grid = autorift.gridpoints((n, n), args.spacing, chip_size=128, search_radius=25)
     pts = grid.rebuild(chip_size_x=np.full(grid.shape, 32, np.int64),
@@ -717,7 +762,10 @@ def main():
     mixed = None
     if not args.no_mixed:
         mixed = mixed_block(autorift, tex, sec, pts, min(args.steps, 3))
-    del tex, sec
+    cfg2 = None
+    if not args.no_cfg2 and n == 10000:
+        cfg2 = cfg2_block(autorift, tex, sec, fx, fy)
+    del tex, sec, fx, fy
 
     coherence = None
     if not args.no_coherence:
@@ -778,6 +826,7 @@ def main():
         "mixed_precision": mixed,
         "coherence": coherence,
         "served_mixed_vs_plain_fp32": served_vs_plain,
+        "multichip_10000sq": cfg2,
         "prepare_s": prep_s,
     }
     print(json.dumps(line), flush=True)
