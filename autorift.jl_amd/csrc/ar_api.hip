@@ -418,6 +418,7 @@ void ar_context_release(ar_context* ctx) {
     for (DevBuf* b : ctx->all_bufs()) b->release();
     ctx->has_pair = false;
     ctx->refx_gen = ~0ULL;
+    ctx->box_key.ptr = nullptr;
     ctx->absmax_gen = ~0ULL;
 }
 

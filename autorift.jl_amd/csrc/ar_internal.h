@@ -51,6 +51,14 @@ struct ar_context {
     DevBuf flags, geom, pts, out, scratch, grid_a, grid_b;
     DevBuf refine;              // per-point refinement hand-off: 5x5 patches and origins
     DevBuf box;                 // box images of W and W^2 for a pass with one chip size
+    // what the integer box images in `box` were made for (chip, padding, pair version, buffer):
+    // a pass with the same key reuses them (a multi-chip level's coarse and fine passes)
+    struct {
+        int ch = 0, cw = 0;
+        int64_t pady = -1, padx = -1;
+        uint64_t gen = ~0ULL;
+        void* ptr = nullptr;
+    } box_key;
     DevBuf refx;                // int8 copies (value ^ 0x80) of the UInt8 pair, for the matrix cores
     DevBuf prec;                // per-point geometry records of the current pass (u8 path)
     DevBuf mc;                  // multi-chip driver planes (grid, level, coarse and result planes)

@@ -196,6 +196,15 @@ ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t p
     if (st) return st;
     int32_t* p1 = (int32_t*)ctx->box.ptr;
     int32_t* p2 = (int32_t*)((char*)ctx->box.ptr + each);
+    auto& k = ctx->box_key;
+    if (k.ptr == ctx->box.ptr && k.ch == ch && k.cw == cw && k.pady == pady && k.padx == padx &&
+        k.gen == ctx->ref_gen) {
+        *b1 = p1;   // made by an earlier pass over the same pair, chip and padding
+        *b2 = p2;
+        *ld = HB;
+        return AR_OK;
+    }
+    k.ptr = nullptr;
     const int rpl = box_rpl(ch);
     const int OUT = 64 * rpl - ch + 1;
     auto kern = rpl == 4 ? box_u8_kernel<4> : (rpl == 8 ? box_u8_kernel<8> : box_u8_kernel<16>);
@@ -219,6 +228,12 @@ ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t p
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, img, ctx->ny,
                        ctx->nx, pady, padx, ch, cw, HB, WB, (int)chunk_w, p1, p2);
     if (hipGetLastError() != hipSuccess) return AR_ERR_DEVICE;
+    k.ch = ch;
+    k.cw = cw;
+    k.pady = pady;
+    k.padx = padx;
+    k.gen = ctx->ref_gen;
+    k.ptr = ctx->box.ptr;
     *b1 = p1;
     *b2 = p2;
     *ld = HB;

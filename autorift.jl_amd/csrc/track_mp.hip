@@ -490,6 +490,7 @@ ar_status ar_f64_box_images(ar_context* ctx, TrackArgs& a) {
     if (HB < 1 || WB < 1) return AR_ERR_UNSUPPORTED;
     const size_t vbytes = (size_t)HB * NX * 8, bbytes = (size_t)HB * WB * 8;
     if (ar_status st = ctx->box.reserve(ctx, 2 * vbytes + 2 * bbytes + bbytes / 2)) return st;
+    ctx->box_key.ptr = nullptr;   // the integer box images are overwritten
     double* V1 = (double*)ctx->box.ptr;
     double* V2 = (double*)((char*)ctx->box.ptr + vbytes);
     double* B1 = (double*)((char*)ctx->box.ptr + 2 * vbytes);
