@@ -219,6 +219,21 @@ static bool f64_direct_enabled() {
     return !(e && std::strcmp(e, "never") == 0);
 }
 
+// With precision BF16 / FP16, passes of fewer multiply-adds per point than this take the exact
+// direct Float64 tracker instead: it measured faster there on MI355X (chip 16 r 8: 29 against 21
+// M points/s; chip 24 r 12: 16.5 against 15.8; chip 16 r 25: 27 against 31 ms for the 10000^2
+// search), and exact results are within any tolerance of the 16-bit form. AR_MP_MIN_MACS
+// (environment, read per pass) sets the threshold; 0 sends every such pass to the 16-bit tracker.
+static double mp_min_macs() {
+    const char* e = std::getenv("AR_MP_MIN_MACS");
+    if (e && *e) {
+        char* end = nullptr;
+        const double v = std::strtod(e, &end);
+        if (end && *end == 0 && v >= 0) return v;
+    }
+    return 8.0e5;
+}
+
 // Staging bytes the direct Float64 tracker uses per chunk of points.
 static const size_t kF64StageBytes = (size_t)2 << 30;
 
@@ -780,8 +795,9 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     }
     // Float32 images with a 16-bit contraction requested: the mixed-precision kernel, which
     // carves its LDS for one chip size per pass
+    const double macs = (double)(2 * a.max_ry) * (2 * a.max_rx) * a.max_ch * a.max_cw;
     const bool use_mp = !use_u8 && nrot == 0 && ctx->pdtype == AR_F32 &&
-                        (prec == AR_PREC_BF16 || prec == AR_PREC_FP16);
+                        (prec == AR_PREC_BF16 || prec == AR_PREC_FP16) && macs >= mp_min_macs();
     if (use_fft) {
         // set up above
     } else if (use_mp) {

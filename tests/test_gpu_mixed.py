@@ -77,14 +77,16 @@ def test_cfg1_pair_agreement(prec):
 
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
 @pytest.mark.parametrize("cs,r", [(16, 6), (32, 10), (64, 25), (128, 25), (48, 17)])
-def test_geometry_vs_oracle(prec, cs, r):
-    """Chip and radius shapes (every band-height instance), against the CPU oracle directly."""
+def test_geometry_vs_oracle(prec, cs, r, monkeypatch):
+    """Chip and radius shapes (every band-height instance), against the CPU oracle directly;
+    the 16-bit tracker on every shape, small ones included (AR_MP_MIN_MACS=0)."""
+    monkeypatch.setenv("AR_MP_MIN_MACS", "0")
     n = 4 * cs + 4 * r + 40
     ref, sec = shifted_pair(n, (2.4, -1.3), seed=cs + r, dtype=np.float32)
     ref = ref * 255
     sec = sec * 255
     h = cs // 2 + r + 3
-    xs = np.linspace(h, n - h, 5)
+    xs = np.linspace(h, n - h, 20)   # 400 points
     pts = autorift.gridpoints_xy(xs, xs, chip_size=cs, search_radius=r)
     print(f"shape c{cs} r{r} {prec}", flush=True)
     p = autorift.params(quantize="none", precision=prec, preprocess="none")
@@ -99,20 +101,24 @@ def test_geometry_vs_oracle(prec, cs, r):
     fin = np.isfinite(exp[0])
     assert np.array_equal(np.isfinite(got.dx), fin)
     agree = (got.dx[fin] == exp[0][fin]) & (got.dy[fin] == exp[1][fin])
-    assert agree.mean() >= 0.96, agree.mean()
+    print(f"agreement c{cs} r{r} {prec}: {agree.mean():.4f} of {fin.sum()}", flush=True)
+    # measured on this texture: 1 of 400 integer peaks differs at chip 16 r 6 in BF16, none
+    # elsewhere (such small passes take the exact tracker by default)
+    assert agree.mean() >= 0.995, agree.mean()
     # the reported correlation is the exact Float64 value of the chosen peak: where the peak
     # agrees it equals the oracle's to the last bit of the normalisation's box sums
     np.testing.assert_allclose(got.correlation[fin][agree], exp[2][fin][agree], rtol=1e-6)
 
 
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
-def test_ncc_and_padding(prec):
+def test_ncc_and_padding(prec, monkeypatch):
     """NCC measure, windows past the image edge (virtual zero padding) and a flat chip."""
     n = 160
     ref, sec = shifted_pair(n, (1.0, 2.0), seed=7, dtype=np.float32)
     ref = ref * 200 + 10
     sec = sec * 200 + 10
     sec[60:100, 60:100] = 42.0    # flat: degenerate chip, NaN result but searched
+    monkeypatch.setenv("AR_MP_MIN_MACS", "0")   # the 16-bit tracker at chip 32, r 12 too
     xs = np.array([12.0, 40.0, 80.0, 120.0, 150.0])
     pts = autorift.gridpoints_xy(xs, xs, chip_size=32, search_radius=12)
     for measure in (0, 1):
@@ -144,9 +150,11 @@ def test_fp16_operand_scaling():
 
 
 def test_mixed_chip_sizes_rejected():
-    t = (synthetic_texture((128, 128), seed=3) * 255).astype(np.float32)
-    xs = np.array([50.0, 70.0])
-    pts = autorift.gridpoints_xy(xs, xs, chip_size=16, search_radius=6)
+    """The 16-bit tracker carves its LDS for one chip size per pass (radius 25: above the
+    crossover where small passes take the exact tracker, which accepts mixed sizes)."""
+    t = (synthetic_texture((160, 160), seed=3) * 255).astype(np.float32)
+    xs = np.array([60.0, 90.0])
+    pts = autorift.gridpoints_xy(xs, xs, chip_size=16, search_radius=25)
     pts.chip_size_x[0, 0] = 32
     pts.chip_size_y[0, 0] = 32
     p = autorift.params(quantize="none", precision="bf16")
@@ -179,3 +187,28 @@ def test_gaps_and_masks(prec):
     fin = np.isfinite(exp.dx)
     err = np.hypot(got.dx[fin] - exp.dx[fin], got.dy[fin] - exp.dy[fin])
     assert np.mean(err == 0) >= 0.98 and (~got.searched).any()
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_small_passes_take_the_exact_tracker(prec):
+    """Below the measured crossover (8e5 multiply-adds per point) the 16-bit precisions take the
+    exact direct tracker, faster there: bit-exact against the oracle; above it, the 16-bit one."""
+    n = 200
+    ref, sec = shifted_pair(n, (2.4, -1.3), seed=3, dtype=np.float32)
+    ref, sec = ref * 255, sec * 255
+    xs = np.linspace(50, 150, 6)
+    for cs, r, path in ((16, 25, 3), (24, 12, 3), (32, 16, 2)):
+        pts = autorift.gridpoints_xy(xs, xs, chip_size=cs, search_radius=r)
+        print(f"dispatch {prec} c{cs} r{r}", flush=True)
+        p = autorift.params(quantize="none", precision=prec, preprocess="none")
+        with autorift.Context(p) as ctx:
+            ctx.set_images(ref, sec)
+            got = ctx.track(pts, upsampling=64)
+            assert ctx.last_kernel()[1] == path
+            pr, ps, _, _ = ctx.get_pair()
+        if path == 3:
+            ok = np.ones(pr.shape, bool)
+            exp = oracle.track(pr, ps, ok, pts.x, pts.y, pts.radius_x, pts.radius_y,
+                               pts.chip_size_x, pts.chip_size_y, up=64)
+            for g, e in zip((got.dx, got.dy, got.correlation), exp[:3]):
+                assert np.array_equal(g, e, equal_nan=True)
