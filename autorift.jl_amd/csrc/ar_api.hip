@@ -632,7 +632,7 @@ static ar_status f64_enqueue(ar_context* ctx, const TrackArgs& a, const F64Plan&
 // The pass on device-resident points. All pointers device pointers.
 ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t up,
                         int32_t measure, float* dx, float* dy, float* corr, uint8_t* srch,
-                        const ar_pass_hint* hint, int64_t searchable_max) {
+                        const ar_pass_hint* hint, int64_t searchable_max, bool reuse_box) {
     if (!ctx->has_pair) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "no image pair set");
     if (n < 0) return fail(ctx, AR_ERR_INVALID_ARGUMENT, "negative point count");
     if (up < 1 || up > 64 || (up & (up - 1)) != 0)
@@ -849,7 +849,8 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
             int64_t ld = 0;
             HIPCHK(ctx, hipEventRecord(ctx->evb, ctx->stream));
             box_timed = true;
-            st = ar_box_images(ctx, a.max_ch, a.max_cw, a.pady, a.padx, &b1, &b2, &ld);
+            st = ar_box_images(ctx, a.max_ch, a.max_cw, a.pady, a.padx, reuse_box, &b1, &b2,
+                               &ld);
             if (st == AR_ERR_OUT_OF_MEMORY) return fail(ctx, st, "out of device memory (box images)");
             if (st == AR_ERR_DEVICE) return fail(ctx, st, "box image kernel failed to launch");
             if (st == AR_OK) {

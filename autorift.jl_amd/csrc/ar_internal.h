@@ -108,10 +108,13 @@ struct ar_pass_hint {
 void ar_resolve_timing(ar_context* ctx);
 // searchable_max: an upper bound on the pass's searchable points when the caller knows one
 // (-1: every point may be searched); it only informs the choice of box images.
+// reuse_box: the pass may read box images an earlier pass made from the same pair, chip and
+// padding (a multi-chip level's coarse and fine passes); otherwise it builds its own.
 ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t up,
                         int32_t measure, float* dx, float* dy, float* corr, uint8_t* srch,
-                        const ar_pass_hint* hint = nullptr, int64_t searchable_max = -1);
-ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t padx,
+                        const ar_pass_hint* hint = nullptr, int64_t searchable_max = -1,
+                        bool reuse_box = false);
+ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t padx, bool reuse,
                         int32_t** b1, int32_t** b2, int64_t* ld);
 // Page-locked host staging of at least n bytes (grow-only; the stream is idle when it grows).
 ar_status ar_pinned(ar_context* ctx, size_t n, void** out);

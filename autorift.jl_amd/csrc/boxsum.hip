@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void box_u8_kernel(const uint8_t* img, int64_t
 
 using namespace ar;
 
-ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t padx,
+ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t padx, bool reuse,
                         int32_t** b1, int32_t** b2, int64_t* ld) {
     const int64_t NY = ctx->ny + 2 * pady, NX = ctx->nx + 2 * padx;
     const int64_t HB = NY - ch + 1, WB = NX - cw + 1;
@@ -197,7 +197,7 @@ ar_status ar_box_images(ar_context* ctx, int ch, int cw, int64_t pady, int64_t p
     int32_t* p1 = (int32_t*)ctx->box.ptr;
     int32_t* p2 = (int32_t*)((char*)ctx->box.ptr + each);
     auto& k = ctx->box_key;
-    if (k.ptr == ctx->box.ptr && k.ch == ch && k.cw == cw && k.pady == pady && k.padx == padx &&
+    if (reuse && k.ptr == ctx->box.ptr && k.ch == ch && k.cw == cw && k.pady == pady && k.padx == padx &&
         k.gen == ctx->ref_gen) {
         *b1 = p1;   // made by an earlier pass over the same pair, chip and padding
         *b2 = p2;

@@ -718,6 +718,7 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
     MCHK(hipSetDevice(ctx->device));
     const int64_t N = nr * nc;
     hipStream_t s = ctx->stream;
+    ctx->box_key.ptr = nullptr;   // box images are shared within this run only
     // planes: the grid's six input fields and the five result planes each contiguous (one copy
     // in, one copy out through page-locked staging), then the level and coarse-grid planes
     int64_t stride = p.coarse_stride;
@@ -876,7 +877,8 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
             MCHK(hipGetLastError());
             ar_points cp = {cxp, cyp, crx, cry, cpdx, cpdy, ccx, ccy};
             // the level's points bound the searchable coarse points (c2[1], read back above)
-            if ((st = ar_track_pass(ctx, cp, M, 1, measure, cdx, cdy, ccr, cs8, hp, (int64_t)c2[1])))
+            if ((st = ar_track_pass(ctx, cp, M, 1, measure, cdx, cdy, ccr, cs8, hp, (int64_t)c2[1],
+                                     true)))
                 return st;
             uint8_t* keep = nullptr;
             int64_t iters = p.gardner_iterations - 1 > 1 ? p.gardner_iterations - 1 : 1;
@@ -895,7 +897,7 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
         // ---- fine pass, rejection, hole filling, merge
         ar_points lp = {gx, gy, lrx, lry, gpdx, gpdy, lcx, lcy};
         if ((st = ar_track_pass(ctx, lp, N, p.upsampling, measure, fdx, fdy, fc, fs, hp,
-                                (int64_t)c2[1])))
+                                (int64_t)c2[1], true)))
             return st;
         uint8_t* keep = nullptr;
         if ((st = reject(ctx, fdx, fdy, lrx, lry, meas, nr, nc, p.gardner_iterations, nxb, nyb, ka,

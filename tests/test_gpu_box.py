@@ -89,3 +89,21 @@ def test_box_row_counts(ctx, box_mode, ny):
     for up in (1, 64):
         g = _compare(ctx, ref, sec, pts, up, box_mode)
         assert g.searched.all()
+
+
+def test_single_passes_rebuild_box_images(ctx, monkeypatch):
+    """Only a multi-chip run shares box images between its passes; two plain passes over the same
+    pair each build their own (and agree)."""
+    monkeypatch.setenv("AR_BOX_IMAGES", "always")
+    ref, sec = shifted_pair(300, (1.5, -2.0), seed=13, dtype=np.uint8)
+    ok = np.ones(ref.shape, bool)
+    ctx.set_pair(ref, sec, ok, ok)
+    xs = np.arange(40, 260, 17, dtype=float)
+    pts = autorift.gridpoints_xy(xs, xs, chip_size=32, search_radius=20)
+    print("box images rebuilt per pass", flush=True)
+    g1 = ctx.track(pts, upsampling=16)
+    assert ctx.last_timing()[0] > 0
+    g2 = ctx.track(pts, upsampling=16)
+    assert ctx.last_timing()[0] > 0
+    np.testing.assert_array_equal(g1.dx.view(np.uint32), g2.dx.view(np.uint32))
+    np.testing.assert_array_equal(g1.dy.view(np.uint32), g2.dy.view(np.uint32))
