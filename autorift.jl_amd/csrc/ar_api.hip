@@ -224,14 +224,26 @@ static bool f64_direct_enabled() {
 // M points/s; chip 24 r 12: 16.5 against 15.8; chip 16 r 25: 27 against 31 ms for the 10000^2
 // search), and exact results are within any tolerance of the 16-bit form. AR_MP_MIN_MACS
 // (environment, read per pass) sets the threshold; 0 sends every such pass to the 16-bit tracker.
-static double mp_min_macs() {
+//
+// The 16-bit pass also builds Float64 box images over every chip position of the padded
+// reference (about 26 ps a position: 2.6 ms at 10000^2), which the exact tracker does without.
+// On references of at least 4 M positions the pass's work must repay them: measured at chip 32,
+// r 25 on the 10000^2 pair, the 16-bit contraction saves about 54 fs per multiply-add, so it
+// needs searched points x multiply-adds >= 512 x positions (a multi-chip level holding a few
+// hundred unfinished points of a 10000^2 search fails this; the full 168 100-point pass clears
+// it eightfold). Setting AR_MP_MIN_MACS turns this second test off. The FFT tracker builds the
+// same images and saves about as much per multiply-add (45 fs), so its passes take the same test.
+static bool box_images_repaid(double macs, int64_t nsearch, double positions) {
+    return positions < 4194304.0 || (double)nsearch * macs >= 512.0 * positions;
+}
+static bool mp_worth_it(double macs, int64_t nsearch, double positions) {
     const char* e = std::getenv("AR_MP_MIN_MACS");
     if (e && *e) {
         char* end = nullptr;
         const double v = std::strtod(e, &end);
-        if (end && *end == 0 && v >= 0) return v;
+        if (end && *end == 0 && v >= 0) return macs >= v;
     }
-    return 8.0e5;
+    return macs >= 8.0e5 && box_images_repaid(macs, nsearch, positions);
 }
 
 // Staging bytes the direct Float64 tracker uses per chunk of points.
@@ -774,12 +786,18 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     const int prec = ctx->params.precision;
     if (prec < AR_PREC_EXACT || prec > AR_PREC_FFT)
         return fail(ctx, AR_ERR_INVALID_ARGUMENT, "precision must be EXACT, BF16, FP16 or FFT");
+    const double macs = (double)(2 * a.max_ry) * (2 * a.max_rx) * a.max_ch * a.max_cw;
+    // points that can be searched: all of them, or fewer when the caller knows (a multi-chip
+    // level whose points were mostly finished at a smaller chip)
+    const int64_t nsearch = searchable_max >= 0 && searchable_max < n ? searchable_max : n;
+    const double box_positions = (double)(a.ny + 2 * a.pady - a.max_ch + 1) *
+                                 (double)(a.nx + 2 * a.padx - a.max_cw + 1);
     // FFT requested: the reference's dispatch (src/correlate.jl:785-797) on the pass's largest
     // point; the transform kernel carves its LDS for one chip size per pass and takes transforms
-    // up to 128 per side, the exact direct tracker everything else
+    // up to 128 per side, the exact direct tracker everything else, and passes too small to
+    // repay the transform pass's box images on a large reference
     bool use_fft = !use_u8 && nrot == 0 && !cplx && ctx->pdtype == AR_F32 && prec == AR_PREC_FFT &&
-                   uniform &&
-                   (double)(2 * a.max_ry) * (2 * a.max_rx) * a.max_ch * a.max_cw > 20000.0;
+                   uniform && macs > 20000.0 && box_images_repaid(macs, nsearch, box_positions);
     bool mp_box_timed = false;
     if (use_fft) {
         size_t lds = 0;
@@ -798,9 +816,9 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
     }
     // Float32 images with a 16-bit contraction requested: the mixed-precision kernel, which
     // carves its LDS for one chip size per pass
-    const double macs = (double)(2 * a.max_ry) * (2 * a.max_rx) * a.max_ch * a.max_cw;
     const bool use_mp = !use_u8 && nrot == 0 && ctx->pdtype == AR_F32 &&
-                        (prec == AR_PREC_BF16 || prec == AR_PREC_FP16) && macs >= mp_min_macs();
+                        (prec == AR_PREC_BF16 || prec == AR_PREC_FP16) &&
+                        mp_worth_it(macs, nsearch, box_positions);
     if (use_fft) {
         // set up above
     } else if (use_mp) {
@@ -839,9 +857,6 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         box_reach < ((int64_t)1 << 32)) {
         const double positions = (double)(a.ny + 2 * a.pady - a.max_ch + 1) *
                                  (double)(a.nx + 2 * a.padx - a.max_cw + 1);
-        // points that can be searched: all of them, or fewer when the caller knows (a multi-chip
-        // level whose points were mostly finished at a smaller chip)
-        const int64_t nsearch = searchable_max >= 0 && searchable_max < n ? searchable_max : n;
         const double shifts = (double)nsearch * (2.0 * a.max_rx) * (2.0 * a.max_ry);
         if (positions >= 1 &&
             (bmode == 1 || (positions >= 4194304.0 && 4.0 * shifts >= positions))) {

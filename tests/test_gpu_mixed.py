@@ -212,3 +212,28 @@ def test_small_passes_take_the_exact_tracker(prec):
                                pts.chip_size_x, pts.chip_size_y, up=64)
             for g, e in zip((got.dx, got.dy, got.correlation), exp[:3]):
                 assert np.array_equal(g, e, equal_nan=True)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fft"])
+def test_few_points_on_a_large_reference_take_the_exact_tracker(prec):
+    """The 16-bit and transform trackers build Float64 box images over the whole padded
+    reference; on references of at least 4 M chip positions a pass must have the work to repay
+    them (searched points x multiply-adds >= 512 x positions), else it takes the exact direct
+    tracker. 2100^2 at chip 32, r 25: 9 points fall short, 1681 clear it."""
+    n = 2100
+    ref, sec = shifted_pair(n, (1.6, -2.2), seed=9, dtype=np.float32)
+    ref, sec = ref * 255, sec * 255
+    p = autorift.params(quantize="none", precision=prec, preprocess="none")
+    with autorift.Context(p) as ctx:
+        ctx.set_images(ref, sec)
+        for xs, path in ((np.linspace(200, 1900, 3), 3),
+                         (np.arange(50.0, 2050.0, 50.0)[:41], 2 if prec == "bf16" else 4)):
+            pts = autorift.gridpoints_xy(xs, xs, chip_size=32, search_radius=25)
+            print(f"large-reference dispatch {prec}: {len(pts)} points", flush=True)
+            got = ctx.track(pts, upsampling=16)
+            assert ctx.last_kernel()[1] == path
+            ok = np.isfinite(got.dx)
+            assert ok.mean() > 0.9
+            # the tracker reports the secondary's offset as seen from the reference: (-1.6, 2.2)
+            assert np.abs(np.median(got.dx[ok]) + 1.6) < 0.1
+            assert np.abs(np.median(got.dy[ok]) - 2.2) < 0.1
