@@ -214,20 +214,29 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(OCC))) vo
     int32_t* S2 = SG ? S1 + a.sg_stride / 2 : (int32_t*)(smem + L.off_s2);
     int32_t* colS = (int32_t*)(smem + L.off_tmp);
     // the wave index in a scalar register: loops over a wave's share are scalar loops
-    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     // floor(t / (cpad / 4)) as a multiply-high by ceil(2^32 / (cpad / 4)): exact for every
     // t below 2^32 / (cpad / 4)
     const uint32_t nd_magic = 0xffffffffu / (uint32_t)(L.cpad >> 2) + 1u;
     // chip band rows and trailing columns are zero; the fetched chip path writes only chip rows
-    for (int t = tid; t < (L.cols_t * L.cpad) >> 2; t += AR_WG) ((uint32_t*)Tp)[t] = 0;
+    for (int t = threadIdx.x; t < (L.cols_t * L.cpad) >> 2; t += AR_WG) ((uint32_t*)Tp)[t] = 0;
     __syncthreads();
     const PointRange pr = point_range(a0.n);
     for (int64_t k = pr.first; k < pr.end; k += pr.step) {
-        int64_t p = k;
-        if (BATCH) {
-            p = a0.order[k];
-            use_pair(a, p);   // every field it sets is set again for each point
-        }
+        // The lane index hidden from the optimiser per point: addresses derived from it are
+        // formed again for each point instead of being held across the loop and spilled (a
+        // reload is a vector memory access, and waiting for it waits for the window DMA too).
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int lane = tid & 63;
+        const int64_t p = BATCH ? a0.order[k] : k;
+        // The argument block (and a batched point's pair) read again at each phase: a field is
+        // then live within its phase only, which keeps the scalar registers from spilling.
+        auto refresh = [&]() {
+            a = reload_args();
+            if (BATCH) use_pair(a, p);
+        };
+        refresh();
         const uint8_t* ref = (const uint8_t*)a.ref;
         const uint8_t* sec = (const uint8_t*)a.sec;
         Geom g = load_geom(a, p);
@@ -527,6 +536,7 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             }
         }
 
+        refresh();
         // ---- numerator on int8 MFMA, stored per shift (column-major o = j NR + i)
         int32_t* accL = (int32_t*)(smem + L.off_acc);   // int32 [NR*NC]
         const int g16 = lane >> 4, l16 = lane & 15;
@@ -631,6 +641,7 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         }
         __syncthreads();
 
+        refresh();
         // ---- integer peak: screen every shift in FP32 (all lanes busy, the matrix-core tiles'
         // padding left out), then exact values for the contenders. The screening best of a lane
         // is the first maximum it meets, indexed by o: ties are settled by the exact values. It also
@@ -692,6 +703,7 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(OCC))) vo
             const int o = (j0 - 1 + tid % 5) * NR + (i0 - 1 + tid / 5);
             pv = exact_u8(P, accL[o], S1[o], S2[o], a.measure);
         }
+        refresh();
         finish_point(a, p, g, pi, pj, pval, i0, j0, refine, pv);
         __syncthreads();
     }

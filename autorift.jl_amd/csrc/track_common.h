@@ -94,6 +94,16 @@ struct TrackArgs {
     const int32_t* order;
 };
 
+// The kernel's argument block read again from the kernarg segment through a pointer the compiler
+// cannot follow: a field used after such a read is loaded there (scalar loads) instead of being
+// held in scalar registers, or spilled to vector lanes, since the previous one.
+__device__ inline TrackArgs reload_args() {
+    typedef const __attribute__((address_space(4))) TrackArgs* KArgs;
+    KArgs p = (KArgs)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return *(const TrackArgs*)p;
+}
+
 // A batched pass reads point p's images from its pair's descriptor (uniform per workgroup).
 __device__ inline void use_pair(TrackArgs& a, int64_t p) {
     const PairDesc d = a.pairs[a.pair_of[p]];
