@@ -254,7 +254,6 @@ __global__ __launch_bounds__(F64_PREP_WG) void f64_prep_kernel(TrackArgs a, F64S
     __shared__ double red_d[F64_PREP_WG / 64];
     __shared__ int red_i[2 * (F64_PREP_WG / 64)];
     __shared__ double stat[2];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t lchip = ((size_t)a.max_ch * a.max_cw * 4 + 15) / 16 * 16;
     const size_t lwin = MODE == 2 ? ((size_t)(a.max_ch + 2 * a.max_ry - 1) *
                                      (a.max_cw + 2 * a.max_rx - 1) * 4 + 15) / 16 * 16 : 0;
@@ -263,8 +262,14 @@ __global__ __launch_bounds__(F64_PREP_WG) void f64_prep_kernel(TrackArgs a, F64S
                           : S.tables + (size_t)blockIdx.x * S.table_doubles;
     for (int64_t k = blockIdx.x; k < S.cnt; k += gridDim.x) {
         const int64_t p = a.order ? (int64_t)a.order[S.p0 + k] : S.p0 + k;
-        TrackArgs b = a;
-        if (a.pairs) use_pair(b, p);
+        // The argument block read again from the kernarg segment per point, and the lane index
+        // hidden from the optimiser: what a point needs is formed for it instead of being held
+        // in (spilled) registers across the loop.
+        TrackArgs b = reload_args();
+        if (b.pairs) use_pair(b, p);
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int lane = tid & 63, wave = tid >> 6;
         unsigned char* slot = S.slots + (size_t)k * S.slot_bytes;
         const Geom g = point_geom(b, p);
         bool go = g.status != 0;

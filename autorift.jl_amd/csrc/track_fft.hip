@@ -293,11 +293,12 @@ __device__ inline bool fft_any_valid(const TrackArgs& a, const Geom& g) {
 }
 
 __global__ __launch_bounds__(FFT_WG) __attribute__((amdgpu_waves_per_eu(4))) void track_fft_kernel(
-    TrackArgs a) {
+    TrackArgs a0) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double red_d[FFT_NW];
     __shared__ Best red[FFT_NW];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    TrackArgs a = a0;
+    const int tid = threadIdx.x;
     const int fy = a.fft_fy, fx = a.fft_fx;
     const uint64_t rady = fft_radices(fy), radx = fft_radices(fx);
     const FftLayout L = fft_layout(fy, fx, a.max_ch, a.max_cw);
@@ -307,8 +308,6 @@ __global__ __launch_bounds__(FFT_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
     unsigned char* mir_y = smem + L.off_my;
     unsigned char* mir_x = smem + L.off_mx;
     float* plane = (float*)(smem + L.off_s);
-    const float* ref = (const float*)a.ref;
-    const float* sec = (const float*)a.sec;
     // twiddles exp(-2 pi i m / n): Float64 sincospi, rounded once; mirror positions
     for (int m = tid; m < fy + fx; m += FFT_WG) {
         const bool y = m < fy;
@@ -324,6 +323,15 @@ __global__ __launch_bounds__(FFT_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
 
     const PointRange pr = point_range(a.n);
     for (int64_t p = pr.first; p < pr.end; p += pr.step) {
+        // The argument block is read again from the kernarg segment at each phase and the lane
+        // index is hidden from the optimiser per point: values the loop does not need across
+        // phases are formed where they are used instead of being held in (spilled) registers.
+        a = reload_args();
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int lane = tid & 63, wave = tid >> 6;
+        const float* ref = (const float*)a.ref;
+        const float* sec = (const float*)a.sec;
         const Geom g = point_geom(a, p);
         if (!g.status || !fft_any_valid(a, g)) { write_skip(a, p, false); continue; }
         const int ch = g.ch, cw = g.cw, NR = 2 * g.ry, NC = 2 * g.rx;
@@ -392,6 +400,7 @@ __global__ __launch_bounds__(FFT_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
         }
         __syncthreads();
 
+        a = reload_args();
         // ---- forward transform: columns (length fy; columns from winw on are zero), then rows
         fft_pass<false>(Z, Ty, fy, rady, 0, winw, 1, ld);
         fft_pass<false>(Z, Tx, fx, radx, 0, fy, ld, 1);
@@ -423,6 +432,7 @@ __global__ __launch_bounds__(FFT_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
         // window deviation from the pass's Float32 window variances (formed in Float64 by the box
         // pass); the reported peak and 5x5 patch take the reference's Float64 normalisation
         // (src/correlate.jl:575-630) below
+        a = reload_args();
         const double n = (double)(ch * cw);
         const int64_t bb = (g.wr0 - 1) + (g.wc0 - 1) * a.box_ld;
         const float inv_cn = (float)(1.0 / cnorm);
@@ -459,6 +469,7 @@ __global__ __launch_bounds__(FFT_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
         const float pval = exact(pi - 1, pj - 1);
         float pv = 0.0f;
         if (refine && tid < 25) pv = exact(i0 - 1 + tid / 5, j0 - 1 + tid % 5);
+        a = reload_args();
         finish_point(a, p, g, pi, pj, pval, i0, j0, refine, pv);
         __syncthreads();
     }

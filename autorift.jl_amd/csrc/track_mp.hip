@@ -98,13 +98,13 @@ __device__ inline void mp_batch(const uint16_t* Wl, const uint16_t* Cl, int ldw,
 
 
 template <int PREC, int NRB>
-__global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
+__global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a0) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double red_d[AR_NWAVES];
     __shared__ float pvals[32];
     __shared__ Best red[AR_NWAVES];
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    TrackArgs a = a0;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const MpLayout L = mp_layout(a.max_ch, a.max_cw, a.max_rx, a.max_ry, NRB);
     uint16_t* Wl = (uint16_t*)(smem + L.off_w);
     float* num = (float*)(smem + L.off_num);
@@ -112,12 +112,18 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
     float* chipf = (float*)(smem + L.off_w);      // raw Float32 chip, before the window
     float* pwin = (float*)(smem + L.off_w);       // after the argmax: window block under the patch
     float* pchip = (float*)(smem + L.off_pc);     // and the mean-removed chip
-    const float* ref = (const float*)a.ref;
-    const float* sec = (const float*)a.sec;
-    const int g16 = lane >> 4, l16 = lane & 15;
 
     const PointRange pr = point_range(a.n);
     for (int64_t p = pr.first; p < pr.end; p += pr.step) {
+        // The argument block is read again from the kernarg segment at each phase and the lane
+        // index is hidden from the optimiser per point: values the loop does not need across
+        // phases are formed where they are used instead of being held in (spilled) registers.
+        a = reload_args();
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int lane = tid & 63, g16 = lane >> 4, l16 = lane & 15;
+        const float* ref = (const float*)a.ref;
+        const float* sec = (const float*)a.sec;
         const Geom g = point_geom(a, p);
         if (!g.status || !any_valid(a, g)) { write_skip(a, p, false); continue; }
         const int ch = g.ch, cw = g.cw, NR = 2 * g.ry, NC = 2 * g.rx;
@@ -175,6 +181,7 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
         }
         __syncthreads();
 
+        a = reload_args();
         // ---- numerators on the matrix cores
         const int nti = (NR + 15) >> 4, ntj = (NC + 15) >> 4, nbt = (nti + 3) >> 2;
         auto store = [&](const v4f (&acc)[4], int tj, int ti0) {
@@ -211,6 +218,7 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
         }
         __syncthreads();
 
+        a = reload_args();
         // ---- normalisation per shift and the argmax (peak_index rule): the window variance
         // s2 - s1^2 / n formed in Float64 by the box pass (where it cancels) and read as Float32,
         // the rest of the 16-bit surface's score in FP32 (the exact values of the chosen peak and
@@ -290,6 +298,7 @@ __global__ __launch_bounds__(AR_WG) void track_mp_kernel(TrackArgs a) {
         }
         __syncthreads();
         const float pv = tid < 25 ? pvals[tid] : 0.0f;
+        a = reload_args();
         finish_point(a, p, g, pi, pj, pvals[25], i0, j0, refine, pv);
         __syncthreads();
     }
