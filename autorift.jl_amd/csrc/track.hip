@@ -649,11 +649,14 @@ __global__ __launch_bounds__(AR_WG) __attribute__((amdgpu_waves_per_eu(OCC))) vo
         const int nsh = NR * NC;
         Best lb = best_empty();
         float lb2 = -INFINITY;
+        // (lb2 <= lb.v throughout, so the median of the two and the new score is the new
+        // second-largest; scores are never NaN)
         auto screen = [&](auto mc) {
+#pragma unroll 2
             for (int o = tid; o < nsh; o += AR_WG) {
                 const float sv = approx_u8<decltype(mc)::value>(P, accL[o], S1[o], S2[o]);
                 const bool up = sv > lb.v;
-                lb2 = up ? lb.v : fmaxf(lb2, sv);
+                lb2 = __builtin_amdgcn_fmed3f(lb.v, lb2, sv);
                 lb.v = up ? sv : lb.v;
                 lb.idx = up ? o : lb.idx;
             }
