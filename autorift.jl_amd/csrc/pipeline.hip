@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/autorift_hip.h"
@@ -334,6 +335,13 @@ __global__ __launch_bounds__(256) void gardner_stage1_w5_kernel(const float* nx,
                                                                 int64_t nr, int64_t nc, Win w,
                                                                 float tol, float needed) {
     const Slot5 q = slot5(nr * nc, nr, nc, w);
+    // A point that is not kept reads NaN at its centre and counts no agreement, so with a
+    // positive threshold it is rejected whatever its window: a wave whose two points are both
+    // such writes them and leaves (levels with few measured points are mostly such waves).
+    if (needed > 0.0f && __ballot(q.k >= 0 && kin[q.k]) == 0) {
+        if (q.k >= 0 && q.s == 0) kout[q.k] = 0;
+        return;
+    }
     const float vx = q.kk >= 0 ? masked(nx, kin, q.kk) : qnan();
     const float vy = q.kk >= 0 ? masked(ny, kin, q.kk) : qnan();
     const float cx = q.k >= 0 ? masked(nx, kin, q.k) : qnan();
@@ -349,6 +357,12 @@ __global__ __launch_bounds__(256) void gardner_stage2_w5_kernel(
     const float* nx, const float* ny, const uint8_t* kin, uint8_t* kout, const int64_t* rxa,
     const int64_t* rya, int64_t nr, int64_t nc, Win w, double scale, int64_t up) {
     const Slot5 q = slot5(nr * nc, nr, nc, w);
+    // A point that is not kept is rejected whatever its window: a wave whose two points are both
+    // such writes them and leaves (levels with few measured points are mostly such waves).
+    if (__ballot(q.k >= 0 && kin[q.k]) == 0) {
+        if (q.k >= 0 && q.s == 0) kout[q.k] = 0;
+        return;
+    }
     const float vx = q.kk >= 0 ? masked(nx, kin, q.kk) : qnan();
     const float vy = q.kk >= 0 ? masked(ny, kin, q.kk) : qnan();
     float medx, madx, medy, mady;
@@ -433,6 +447,38 @@ __global__ void fraction_kernel(const int64_t* rx, const uint8_t* meas, uint8_t*
         if (cd) atomicAdd(&out[0], cd);
         if (cu) atomicAdd(&out[1], cu);
     }
+}
+
+// dilate_within (src/resample.jl:211-219) directly: a cell is set when a kept cell lies within
+// squared distance r2 of it. The squared distances are exact integers in Float64, so this is the
+// distance-transform form below cell for cell (no kept cell at all: every distance infinite,
+// nothing set). A workgroup takes a 16 x 16 tile of cells, stages the tile and its margin of
+// R = floor(sqrt(r2)) cells in LDS, and each thread ORs the kept flags of its disc without
+// branches; the pipeline takes it for radii up to kDiscMax, where it is many times faster than
+// the two serial transform passes (one thread per row or column).
+constexpr int kDiscMax = 24, kDiscTile = 16;
+__global__ __launch_bounds__(kDiscTile * kDiscTile) void dilate_disc_kernel(
+    const uint8_t* m, int64_t nr, int64_t nc, int R, double r2, uint8_t* out) {
+    constexpr int S = kDiscTile + 2 * kDiscMax;
+    __shared__ uint8_t tile[S * S];
+    const int w = kDiscTile + 2 * R;
+    const int64_t i0 = (int64_t)blockIdx.x * kDiscTile - R, j0 = (int64_t)blockIdx.y * kDiscTile - R;
+    for (int e = threadIdx.x; e < w * w; e += kDiscTile * kDiscTile) {
+        const int a = e % w, b = e / w;
+        const int64_t ii = i0 + a, jj = j0 + b;
+        tile[e] = (ii >= 0 && ii < nr && jj >= 0 && jj < nc) ? m[ii + jj * nr] : 0;
+    }
+    __syncthreads();
+    const int li = threadIdx.x % kDiscTile, lj = threadIdx.x / kDiscTile;
+    const int64_t i = i0 + R + li, j = j0 + R + lj;
+    if (i >= nr || j >= nc) return;
+    uint32_t hit = 0;
+    for (int dj = -R; dj <= R; dj++) {
+        const uint8_t* col = tile + (lj + R + dj) * w + li + R;
+        for (int di = -R; di <= R; di++)
+            if ((double)(di * di + dj * dj) <= r2) hit |= col[di];
+    }
+    out[i + j * nr] = hit ? 1 : 0;
 }
 
 // Exact squared Euclidean distance transform (src/resample.jl:225-325): column pass.
@@ -701,6 +747,70 @@ static ar_status reject(ar_context* ctx, const float* dx, const float* dy, const
     return AR_OK;
 }
 
+// The grid's fields copied into page-locked staging (x, y, dx_prior, dy_prior, radius_x,
+// radius_y, each a contiguous plane of N 8-byte values) in one pass that also gathers what the
+// level planning needs: whether every coordinate and prior is finite and below 1e15 in
+// magnitude, the coordinate extents, the largest prior and the largest radii of the searchable
+// points. Large grids are split over up to 8 threads.
+struct GridScan {
+    bool ok = true;
+    double xlo = INFINITY, xhi = -INFINITY, ylo = INFINITY, yhi = -INFINITY, pmax = 0.0;
+    int64_t rx = 0, ry = 0;
+};
+
+static void grid_scan_range(const ar_points* g, char* pin, int64_t N, int64_t a, int64_t b,
+                            GridScan& r) {
+    const void* src[6] = {g->x, g->y, g->dx_prior, g->dy_prior, g->radius_x, g->radius_y};
+    for (int f = 0; f < 6; f++)
+        std::memcpy(pin + (size_t)f * 8 * N + 8 * (size_t)a, (const char*)src[f] + 8 * (size_t)a,
+                    8 * (size_t)(b - a));
+    bool ok = true;
+    double xlo = r.xlo, xhi = r.xhi, ylo = r.ylo, yhi = r.yhi, pmax = r.pmax;
+    int64_t rx = r.rx, ry = r.ry;
+    for (int64_t k = a; k < b; k++) {
+        const double x = g->x[k], y = g->y[k];
+        const double ax = std::fabs(g->dx_prior[k]), ay = std::fabs(g->dy_prior[k]);
+        ok = ok && std::fabs(x) <= 1e15 && std::fabs(y) <= 1e15 && ax <= 1e15 && ay <= 1e15;
+        xlo = x < xlo ? x : xlo;
+        xhi = x > xhi ? x : xhi;
+        ylo = y < ylo ? y : ylo;
+        yhi = y > yhi ? y : yhi;
+        const double m = ax > ay ? ax : ay;
+        pmax = m > pmax ? m : pmax;
+        const int64_t gx = g->radius_x[k], gy = g->radius_y[k];
+        if (gx > 0 && gy > 0) {
+            rx = gx > rx ? gx : rx;
+            ry = gy > ry ? gy : ry;
+        }
+    }
+    r.ok = r.ok && ok;
+    r.xlo = xlo; r.xhi = xhi; r.ylo = ylo; r.yhi = yhi; r.pmax = pmax; r.rx = rx; r.ry = ry;
+}
+
+static GridScan grid_scan(const ar_points* g, char* pin, int64_t N) {
+    int nt = (int)std::min<int64_t>(8, N / 32768);
+    if (nt < 1) nt = 1;
+    std::vector<GridScan> part(nt);
+    std::vector<std::thread> ts;
+    const int64_t per = (N + nt - 1) / nt;
+    for (int t = 1; t < nt; t++) {
+        const int64_t a = t * per, b = std::min(N, a + per);
+        if (a < b) ts.emplace_back([=, &part] { grid_scan_range(g, pin, N, a, b, part[t]); });
+    }
+    grid_scan_range(g, pin, N, 0, std::min(N, per), part[0]);
+    for (std::thread& t : ts) t.join();
+    GridScan r = part[0];
+    for (int t = 1; t < nt; t++) {
+        const GridScan& q = part[t];
+        r.ok = r.ok && q.ok;
+        r.xlo = std::min(r.xlo, q.xlo); r.xhi = std::max(r.xhi, q.xhi);
+        r.ylo = std::min(r.ylo, q.ylo); r.yhi = std::max(r.yhi, q.yhi);
+        r.pmax = std::max(r.pmax, q.pmax);
+        r.rx = std::max(r.rx, q.rx); r.ry = std::max(r.ry, q.ry);
+    }
+    return r;
+}
+
 extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, int64_t nr,
                                       int64_t nc, float* dx, float* dy, float* corr,
                                       uint16_t* chip_size, uint8_t* interpolated) {
@@ -777,12 +887,8 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
     if ((st = ar_pinned(ctx, rb_off + 256, &pinv))) return st;
     char* pin = (char*)pinv;
     unsigned long long* rb = (unsigned long long*)(pin + rb_off);
-    {
-        const void* src[6] = {grid->x, grid->y, grid->dx_prior, grid->dy_prior, grid->radius_x,
-                              grid->radius_y};
-        for (int f = 0; f < 6; f++) std::memcpy(pin + (size_t)f * 8 * N, src[f], 8 * (size_t)N);
-        MCHK(hipMemcpyAsync(gin, pin, 48 * (size_t)N, hipMemcpyHostToDevice, s));
-    }
+    const GridScan gs = grid_scan(grid, pin, N);
+    MCHK(hipMemcpyAsync(gin, pin, 48 * (size_t)N, hipMemcpyHostToDevice, s));
     fill_result_kernel<<<gblocks(N), GK, 0, s>>>(rdx, rdy, rc, rcs, rit, N);
     MCHK(hipGetLastError());
 
@@ -790,27 +896,14 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
     // extent, its largest radius and prior. A level's points (coarse and fine) sit at grid
     // positions with at most that radius, so when every grid position's window fits the image at
     // that radius the passes need no padding and their geometry is known here.
-    bool grid_known = true;
-    double fx_lo = 0, fx_hi = 0, fy_lo = 0, fy_hi = 0, prior_max = 0;
-    int64_t r0x = 0, r0y = 0;
-    for (int64_t k = 0; k < N && grid_known; k++) {
-        const double x = grid->x[k], y = grid->y[k];
-        const double ax = std::fabs(grid->dx_prior[k]), ay = std::fabs(grid->dy_prior[k]);
-        if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(ax) || !std::isfinite(ay) ||
-            std::fabs(x) > 1e15 || std::fabs(y) > 1e15 || ax > 1e15 || ay > 1e15) {
-            grid_known = false;
-            break;
-        }
-        const double fx = std::floor(x), fy = std::floor(y);
-        if (k == 0) { fx_lo = fx_hi = fx; fy_lo = fy_hi = fy; }
-        fx_lo = std::min(fx_lo, fx); fx_hi = std::max(fx_hi, fx);
-        fy_lo = std::min(fy_lo, fy); fy_hi = std::max(fy_hi, fy);
-        prior_max = std::max(prior_max, std::max(ax, ay));
-        if (grid->radius_x[k] > 0 && grid->radius_y[k] > 0) {
-            r0x = std::max(r0x, grid->radius_x[k]);
-            r0y = std::max(r0y, grid->radius_y[k]);
-        }
-    }
+    // (floor is monotone: the floors of the extremes are the extremes of the floors)
+    const bool grid_known = gs.ok;
+    const double fx_lo = grid_known ? std::floor(gs.xlo) : 0;
+    const double fx_hi = grid_known ? std::floor(gs.xhi) : 0;
+    const double fy_lo = grid_known ? std::floor(gs.ylo) : 0;
+    const double fy_hi = grid_known ? std::floor(gs.yhi) : 0;
+    const double prior_max = grid_known ? gs.pmax : 0;
+    const int64_t r0x = gs.rx, r0y = gs.ry;
 
     if (p.level_measure_count < 0 || p.level_measure_count > 8)
         return mfail(ctx, AR_ERR_INVALID_ARGUMENT, "level_measure_count must be in 0..8");
@@ -887,8 +980,17 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
                 return st;
             fraction_kernel<<<gblocks(M), GK, 0, s>>>(crx, cmeas, keep, M, cnt + 2);
             double r2 = (double)p.coarse_buffer * (double)p.coarse_buffer;
-            edt_col_kernel<<<gblocks(ncc), GK, 0, s>>>(keep, nrc, ncc, d2);
-            edt_row_dilate_kernel<<<gblocks(nrc), GK, 0, s>>>(d2, nrc, ncc, r2, ev, ez, edd, grown);
+            const int64_t rdisc = (int64_t)std::floor(std::sqrt(r2));
+            if (rdisc <= kDiscMax && ncc <= (int64_t)65535 * kDiscTile) {
+                const dim3 tiles((unsigned)((nrc + kDiscTile - 1) / kDiscTile),
+                                 (unsigned)((ncc + kDiscTile - 1) / kDiscTile));
+                dilate_disc_kernel<<<tiles, kDiscTile * kDiscTile, 0, s>>>(keep, nrc, ncc,
+                                                                           (int)rdisc, r2, grown);
+            } else {
+                edt_col_kernel<<<gblocks(ncc), GK, 0, s>>>(keep, nrc, ncc, d2);
+                edt_row_dilate_kernel<<<gblocks(nrc), GK, 0, s>>>(d2, nrc, ncc, r2, ev, ez, edd,
+                                                                   grown);
+            }
             nearest_gate_kernel<<<gblocks(N), GK, 0, s>>>(grown, nrc, ncc, nr, nc, lrx, lry, flags,
                                                           cnt + 2, p.min_coarse_valid_fraction);
             MCHK(hipGetLastError());

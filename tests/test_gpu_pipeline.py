@@ -238,3 +238,36 @@ def test_multichip_refuses_more_measures_than_levels():
         ctx.set_pair(ref, sec)
         with pytest.raises(autorift.ArgumentError, match="names 3 measures but there are only 2"):
             ctx.run_multichip(grid)
+
+
+@pytest.mark.parametrize("buffer", [1, 2, 30])
+def test_multichip_coarse_dilation_radii(buffer):
+    """The coarse mask's dilation (dilate_within, src/resample.jl:211-219) at radii the direct
+    disc form takes (1, 2) and one the distance-transform form takes (30), on a grid whose coarse
+    mask has holes (a low-texture half): bit-exact against the oracle's transform."""
+    ref, sec = shifted_pair(360, (1.5, 2.5), seed=7, dtype=np.float32)
+    ref, sec = ref * 255, sec * 255
+    ref[:, 180:] = ref[:, 180:] * 0.02 + 100
+    sec[:, 180:] = sec[:, 180:] * 0.02 + 100
+    rng = np.random.default_rng(2)
+    sec = (sec + rng.normal(0, 1, sec.shape)).astype(np.float32)
+    ref = np.clip(np.rint(ref), 0, 255).astype(np.uint8)
+    sec = np.clip(np.rint(sec), 0, 255).astype(np.uint8)
+    ok = np.ones(ref.shape, bool)
+    grid = _grid(360, 5, 55)
+    p = _params(preprocess=0, quantize=0, chip_size_base=16, chip_size_min=16, chip_size_max=64,
+                coarse_buffer=buffer)
+    op = oracle.params(chip_size=16, chip_size_max=64, coarse_buffer=buffer)
+    print(f"multichip coarse_buffer {buffer} grid {grid.shape}", flush=True)
+    with autorift.Context(p) as ctx:
+        ctx.set_pair(ref, sec, ok, ok)
+        g = ctx.run_multichip(grid)
+    e = oracle.multichip(ref, sec, ok, grid.x, grid.y, grid.radius_x, grid.radius_y,
+                         grid.dx_prior, grid.dy_prior, p=op, nthreads=8)
+    for a, b in zip((g.dx, g.dy, g.correlation), e[:3]):
+        np.testing.assert_array_equal(np.isnan(a), np.isnan(b))
+        np.testing.assert_array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
+    np.testing.assert_array_equal(g.chip_size, e[3])
+    np.testing.assert_array_equal(g.interpolated, e[4])
+    print(f"  measured {g.nmeasured()} of {g.dx.size}, levels "
+          f"{dict(zip(*np.unique(g.chip_size, return_counts=True)))}", flush=True)
