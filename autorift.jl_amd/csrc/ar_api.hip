@@ -568,6 +568,14 @@ static void f64_plan_shape(const TrackArgs& a, bool u8, F64Plan* P) {
     P->tbytes = mode >= 1 ? 0 : td8;   // one block's table; scaled by the launch plan
 }
 
+ar_status ar_stream2(ar_context* ctx) {
+    if (ctx->stream2) return AR_OK;
+    HIPCHK(ctx, hipStreamCreate(&ctx->stream2));
+    for (int k = 0; k < 5; k++)
+        HIPCHK(ctx, hipEventCreateWithFlags(&ctx->evf[k], hipEventDisableTiming));
+    return AR_OK;
+}
+
 static ar_status f64_plan(ar_context* ctx, const TrackArgs& a, bool u8, bool allow_overlap,
                           F64Plan* P) {
     ar_status st;
@@ -592,11 +600,7 @@ static ar_status f64_plan(ar_context* ctx, const TrackArgs& a, bool u8, bool all
     P->half = (size_t)chunk * P->sl.bytes;
     P->tbytes = (size_t)P->nbuf * P->nprep * table1;
     if ((st = ctx->scratch.reserve(ctx, P->nbuf * P->half + P->tbytes))) return st;
-    if (P->overlap && !ctx->stream2) {
-        HIPCHK(ctx, hipStreamCreate(&ctx->stream2));
-        for (int k = 0; k < 5; k++)
-            HIPCHK(ctx, hipEventCreateWithFlags(&ctx->evf[k], hipEventDisableTiming));
-    }
+    if (P->overlap) return ar_stream2(ctx);
     return AR_OK;
 }
 

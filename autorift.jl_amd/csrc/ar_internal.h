@@ -46,6 +46,9 @@ struct ar_context {
 
     // raw images of the current pair (preparation inputs)
     DevBuf raw, rawmask, work_f32, work_f64;
+    // the secondary's own preparation scratch when the two images are prepared side by side
+    // (pairs of at most kSidePrepMax elements, on `stream2`)
+    DevBuf raw2, rawmask2, work_f32_2, work_f64_2, flags2, qsum2;
 
     // scratch
     DevBuf flags, geom, pts, out, scratch, grid_a, grid_b;
@@ -83,9 +86,18 @@ struct ar_context {
     std::vector<DevBuf*> all_bufs() {
         return {&pref, &psec, &rmask, &smask, &okmask, &raw, &rawmask, &work_f32, &work_f64,
                 &flags, &geom, &pts, &out, &scratch, &grid_a, &grid_b, &refine, &box, &refx, &prec,
-                &mc, &qsum, &qstats};
+                &mc, &qsum, &qstats, &raw2, &rawmask2, &work_f32_2, &work_f64_2, &flags2,
+                &qsum2};
     }
 };
+
+// Pairs of at most this many elements per image prepare their two images side by side (the
+// secondary on the context's second stream, with its own scratch): at such sizes each image's
+// preparation is a chain of latency-bound launches.
+constexpr int64_t kSidePrepMax = (int64_t)1 << 24;
+
+// The context's second stream and its ordering events (evf), made on first use.
+ar_status ar_stream2(ar_context* ctx);
 
 // internal entry points shared between translation units
 // Complex pairs: preparation of one image (isfinite / given mask, Deramp, NoQuantize) and the

@@ -554,20 +554,35 @@ static int gsz(int64_t n) {
     return (int)(b < 1 ? 1 : (b > 65535 ? 65535 : b));
 }
 
+// The stream and scratch one image's preparation runs on: the context's own, or, for the
+// secondary of a pair prepared side by side, the second stream with its own scratch.
+struct PrepLane {
+    hipStream_t s;
+    DevBuf *raw, *rawmask, *work_f32, *work_f64, *flags, *qsum;
+};
+
+static PrepLane prep_lane(ar_context* ctx, bool side) {
+    if (!side)
+        return {ctx->stream, &ctx->raw, &ctx->rawmask, &ctx->work_f32, &ctx->work_f64,
+                &ctx->flags, &ctx->qsum};
+    return {ctx->stream2, &ctx->raw2, &ctx->rawmask2, &ctx->work_f32_2, &ctx->work_f64_2,
+            &ctx->flags2, &ctx->qsum2};
+}
+
 // Quantise a float (or raw) image on the device into out (u8); the parameters stay in the
 // context's slot `slot` (0 reference, 1 secondary) for inspection.
 template <typename T>
 static ar_status quantize_u8(ar_context* ctx, const T* img, const uint8_t* mask, int64_t N,
-                             uint8_t* out, int slot) {
+                             uint8_t* out, int slot, const PrepLane& L) {
     ar_status st;
     const int64_t nch = (N + SQ_CHUNK - 1) / SQ_CHUNK;
     if (nch >= ((int64_t)1 << 31))
         return pfail(ctx, AR_ERR_UNSUPPORTED, "image too large for the UInt8 quantiser");
     const size_t recb = (size_t)nch * SQ_SLOTS * sizeof(SqRec);
     const size_t per = SQ_WG * 8 + 8 + 8 + 8 + 4 + 4 + 4;   // tsum, csum, coff, ccnt, clow, cg, cnt
-    if ((st = ctx->qsum.reserve(ctx, 2 * recb + (size_t)nch * per + 256))) return st;
+    if ((st = L.qsum->reserve(ctx, 2 * recb + (size_t)nch * per + 256))) return st;
     if ((st = ctx->qstats.reserve(ctx, 2 * sizeof(QParams)))) return st;
-    char* b = (char*)ctx->qsum.ptr;
+    char* b = (char*)L.qsum->ptr;
     SqRec* rec = (SqRec*)b;
     SqRec* items = (SqRec*)(b + recb);
     double* tsum = (double*)(b + 2 * recb);
@@ -581,15 +596,15 @@ static ar_status quantize_u8(ar_context* ctx, const T* img, const uint8_t* mask,
     const bool inline_scan = nch <= SQ_SCAN_INLINE;
     for (int pass = 0; pass < 2; pass++) {
         const QTerm<T> term{img, mask, qp, pass};
-        sq_chunk_kernel<<<(unsigned)nch, SQ_WG, 0, ctx->stream>>>(term, N, tsum, csum, clow, ccnt);
+        sq_chunk_kernel<<<(unsigned)nch, SQ_WG, 0, L.s>>>(term, N, tsum, csum, clow, ccnt);
         if (!inline_scan)
-            sq_scan_kernel<<<1, 1024, 0, ctx->stream>>>(csum, clow, ccnt, nch, coff, cg);
-        sq_segment_kernel<<<(unsigned)nch, SQ_WG, 0, ctx->stream>>>(
+            sq_scan_kernel<<<1, 1024, 0, L.s>>>(csum, clow, ccnt, nch, coff, cg);
+        sq_segment_kernel<<<(unsigned)nch, SQ_WG, 0, L.s>>>(
             term, N, tsum, csum, clow, inline_scan ? nullptr : coff, cg, cnt, rec);
-        sq_walk_kernel<<<1, SQ_WG, 0, ctx->stream>>>(term, N, nch, ccnt, cnt, rec, items,
+        sq_walk_kernel<<<1, SQ_WG, 0, L.s>>>(term, N, nch, ccnt, cnt, rec, items,
                                                      QFinish{qp, pass});
     }
-    q_map_kernel<T><<<gsz(N), 256, 0, ctx->stream>>>(img, mask, N, qp, out);
+    q_map_kernel<T><<<gsz(N), 256, 0, L.s>>>(img, mask, N, qp, out);
     PCHK(hipGetLastError());
     return AR_OK;
 }
@@ -598,7 +613,7 @@ static ar_status quantize_u8(ar_context* ctx, const T* img, const uint8_t* mask,
 // slot (dst image, dst mask).
 template <typename T>
 static ar_status prepare_one(ar_context* ctx, const T* raw, const uint8_t* mask, int64_t ny,
-                             int64_t nx, void* dst, uint8_t* dmask, int slot) {
+                             int64_t nx, void* dst, uint8_t* dmask, int slot, const PrepLane& L) {
     const ar_params& p = ctx->params;
     int64_t N = ny * nx;
     ar_status st;
@@ -610,7 +625,7 @@ static ar_status prepare_one(ar_context* ctx, const T* raw, const uint8_t* mask,
         if (w < 1) return pfail(ctx, AR_ERR_INVALID_ARGUMENT, "filter width must be >= 1");
         if (wallis && !(p.min_std >= 0.0))
             return pfail(ctx, AR_ERR_INVALID_ARGUMENT, "Wallis `min_std` must be >= 0");
-        if ((st = ctx->flags.reserve(ctx, 64))) return st;
+        if ((st = L.flags->reserve(ctx, 64))) return st;
         // Highpass of a non-Float64 image needs no host decision: Float32(x) == x for every such
         // element, so the masked NaN-aware box mean equals the reference's dense and unmasked
         // forms wherever those apply, and the erosion of an all-valid mask is the mask itself.
@@ -619,23 +634,23 @@ static ar_status prepare_one(ar_context* ctx, const T* raw, const uint8_t* mask,
         bool allmask = false, anynan = true;
         if (!certified) {
             int f0[2] = {1, 0};
-            PCHK(hipMemcpyAsync(ctx->flags.ptr, f0, 8, hipMemcpyHostToDevice, ctx->stream));
-            mask_stats_kernel<T><<<gsz(N) > 4096 ? 4096 : gsz(N), 256, 0, ctx->stream>>>(
-                raw, mask, N, (int*)ctx->flags.ptr);
+            PCHK(hipMemcpyAsync(L.flags->ptr, f0, 8, hipMemcpyHostToDevice, L.s));
+            mask_stats_kernel<T><<<gsz(N) > 4096 ? 4096 : gsz(N), 256, 0, L.s>>>(
+                raw, mask, N, (int*)L.flags->ptr);
             int f[2];
             PCHK(hipGetLastError());
-            PCHK(hipMemcpyAsync(f, ctx->flags.ptr, 8, hipMemcpyDeviceToHost, ctx->stream));
-            PCHK(hipStreamSynchronize(ctx->stream));
+            PCHK(hipMemcpyAsync(f, L.flags->ptr, 8, hipMemcpyDeviceToHost, L.s));
+            PCHK(hipStreamSynchronize(L.s));
             allmask = f[0] != 0;
             anynan = f[1] != 0;
         }
         int wx = (int)(w < nx ? w : nx), wy = (int)(w < ny ? w : ny);
         int lx = wx / 2, ly = wy / 2, rx = wx - 1 - lx, ry = wy - 1 - ly;
-        if ((st = ctx->work_f64.reserve(ctx, N * 8 + N * 4))) return st;
-        if ((st = ctx->work_f32.reserve(ctx, N * 4 * (wallis ? 3 : 1)))) return st;
-        double* colsum = (double*)ctx->work_f64.ptr;
+        if ((st = L.work_f64->reserve(ctx, N * 8 + N * 4))) return st;
+        if ((st = L.work_f32->reserve(ctx, N * 4 * (wallis ? 3 : 1)))) return st;
+        double* colsum = (double*)L.work_f64->ptr;
         int32_t* colcnt = (int32_t*)(colsum + N);
-        hp = (float*)ctx->work_f32.ptr;
+        hp = (float*)L.work_f32->ptr;
         int64_t nb = ny < 16 ? ny : 16;
         int64_t nthr = ((ny + nb - 1) / nb) * nx;
         int gb = (int)((nthr + 255) / 256);
@@ -645,63 +660,63 @@ static ar_status prepare_one(ar_context* ctx, const T* raw, const uint8_t* mask,
             float* mean = hp + N;
             float* dev2 = hp + 2 * N;
             if (allmask && !anynan) {
-                box_colpass_kernel<T, false, false><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
-                box_rowpass_wallis_dev_kernel<T, false><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, mean, dev2);
+                box_colpass_kernel<T, false, false><<<gb, 256, 0, L.s>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
+                box_rowpass_wallis_dev_kernel<T, false><<<gr, 64, 0, L.s>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, mean, dev2);
             } else if (allmask) {
-                box_colpass_kernel<T, false, true><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
-                box_rowpass_wallis_dev_kernel<T, true><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, mean, dev2);
+                box_colpass_kernel<T, false, true><<<gb, 256, 0, L.s>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
+                box_rowpass_wallis_dev_kernel<T, true><<<gr, 64, 0, L.s>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, mean, dev2);
             } else {
-                box_colpass_kernel<T, true, true><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
-                box_rowpass_wallis_dev_kernel<T, true><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, mean, dev2);
+                box_colpass_kernel<T, true, true><<<gb, 256, 0, L.s>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
+                box_rowpass_wallis_dev_kernel<T, true><<<gr, 64, 0, L.s>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, mean, dev2);
             }
-            box_colpass_kernel<float, false, true><<<gb, 256, 0, ctx->stream>>>(dev2, mask, ny, nx, ly, ry, colsum, colcnt);
-            box_rowpass_wallis_out_kernel<T><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, colsum, colcnt, mean, (float)p.min_std, hp);
+            box_colpass_kernel<float, false, true><<<gb, 256, 0, L.s>>>(dev2, mask, ny, nx, ly, ry, colsum, colcnt);
+            box_rowpass_wallis_out_kernel<T><<<gr, 64, 0, L.s>>>(raw, mask, ny, nx, lx, rx, colsum, colcnt, mean, (float)p.min_std, hp);
         } else if (certified) {
             // exact-sum certificate on the device; the direct kernels run when it holds, the
             // reference's sequential running sums otherwise (each kernel checks it and exits)
-            int* stats = (int*)ctx->flags.ptr;
-            stats_init_kernel<<<1, 64, 0, ctx->stream>>>(stats);
+            int* stats = (int*)L.flags->ptr;
+            stats_init_kernel<<<1, 64, 0, L.s>>>(stats);
             // about 16 values per thread, at most 1024 workgroups
             const int64_t sb = (N + 4095) / 4096;
             exact_stats_kernel<T><<<(unsigned)(sb < 1 ? 1 : (sb > 1024 ? 1024 : sb)), 256, 0,
-                                    ctx->stream>>>(raw, mask, N, stats);
+                                    L.s>>>(raw, mask, N, stats);
             const Cert cert{stats, wx, wy};
             const unsigned gp = (unsigned)((N + 255) / 256);
-            box_col_direct_kernel<T><<<gp, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, cert, colsum, colcnt);
-            box_colpass_kernel<T, true, true><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt, cert);
-            box_row_direct_highpass_kernel<T><<<gp, 256, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, cert, colsum, colcnt, hp);
-            box_rowpass_highpass_kernel<T, true><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, hp, cert);
+            box_col_direct_kernel<T><<<gp, 256, 0, L.s>>>(raw, mask, ny, nx, ly, ry, cert, colsum, colcnt);
+            box_colpass_kernel<T, true, true><<<gb, 256, 0, L.s>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt, cert);
+            box_row_direct_highpass_kernel<T><<<gp, 256, 0, L.s>>>(raw, mask, ny, nx, lx, rx, cert, colsum, colcnt, hp);
+            box_rowpass_highpass_kernel<T, true><<<gr, 64, 0, L.s>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, hp, cert);
         } else if (allmask && !anynan) {
-            box_colpass_kernel<T, false, false><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
-            box_rowpass_highpass_kernel<T, false><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, hp);
+            box_colpass_kernel<T, false, false><<<gb, 256, 0, L.s>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
+            box_rowpass_highpass_kernel<T, false><<<gr, 64, 0, L.s>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, hp);
         } else if (allmask) {
-            box_colpass_kernel<T, false, true><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
-            box_rowpass_highpass_kernel<T, true><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, hp);
+            box_colpass_kernel<T, false, true><<<gb, 256, 0, L.s>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
+            box_rowpass_highpass_kernel<T, true><<<gr, 64, 0, L.s>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, hp);
         } else {
-            box_colpass_kernel<T, true, true><<<gb, 256, 0, ctx->stream>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
-            box_rowpass_highpass_kernel<T, true><<<gr, 64, 0, ctx->stream>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, hp);
+            box_colpass_kernel<T, true, true><<<gb, 256, 0, L.s>>>(raw, mask, ny, nx, ly, ry, colsum, colcnt);
+            box_rowpass_highpass_kernel<T, true><<<gr, 64, 0, L.s>>>(raw, mask, ny, nx, lx, rx, ly, ry, colsum, colcnt, hp);
         }
         PCHK(hipGetLastError());
         // erosion uses the full filter width w (clamped to the image) like windowmin(f, w)
-        erode_finalize_kernel<<<(unsigned)((N + 255) / 256), 256, 0, ctx->stream>>>(
+        erode_finalize_kernel<<<(unsigned)((N + 255) / 256), 256, 0, L.s>>>(
             hp, mask, dmask, ny, nx, lx, rx, ly, ry, allmask ? 1 : 0,
-            certified ? (const int*)ctx->flags.ptr : nullptr);
+            certified ? (const int*)L.flags->ptr : nullptr);
         PCHK(hipGetLastError());
         curmask = dmask;
     } else {
-        copy_u8_kernel<<<gsz(N), 256, 0, ctx->stream>>>(mask, dmask, N);
+        copy_u8_kernel<<<gsz(N), 256, 0, L.s>>>(mask, dmask, N);
         PCHK(hipGetLastError());
         curmask = dmask;
     }
     if (p.quantize == AR_QUANT_UINT8) {
-        if (hp) return quantize_u8<float>(ctx, hp, curmask, N, (uint8_t*)dst, slot);
-        return quantize_u8<T>(ctx, raw, curmask, N, (uint8_t*)dst, slot);
+        if (hp) return quantize_u8<float>(ctx, hp, curmask, N, (uint8_t*)dst, slot, L);
+        return quantize_u8<T>(ctx, raw, curmask, N, (uint8_t*)dst, slot, L);
     }
     // NoQuantize: Float32 output
     if (hp) {
-        PCHK(hipMemcpyAsync(dst, hp, N * 4, hipMemcpyDeviceToDevice, ctx->stream));
+        PCHK(hipMemcpyAsync(dst, hp, N * 4, hipMemcpyDeviceToDevice, L.s));
     } else {
-        noquant_kernel<T><<<gsz(N), 256, 0, ctx->stream>>>(raw, N, (float*)dst);
+        noquant_kernel<T><<<gsz(N), 256, 0, L.s>>>(raw, N, (float*)dst);
         PCHK(hipGetLastError());
     }
     return AR_OK;
@@ -720,22 +735,23 @@ static size_t dt_size(int32_t dt) {
 
 template <typename T>
 static ar_status prepare_slot(ar_context* ctx, const void* host_img, const uint8_t* host_mask,
-                              int64_t ny, int64_t nx, void* dst, uint8_t* dmask, int slot) {
+                              int64_t ny, int64_t nx, void* dst, uint8_t* dmask, int slot,
+                              const PrepLane& L) {
     int64_t N = ny * nx;
     ar_status st;
-    if ((st = ctx->raw.reserve(ctx, N * sizeof(T))) || (st = ctx->rawmask.reserve(ctx, N)))
+    if ((st = L.raw->reserve(ctx, N * sizeof(T))) || (st = L.rawmask->reserve(ctx, N)))
         return st;
     // the caller's image may be host or device memory (unified addressing picks the direction)
-    PCHK(hipMemcpyAsync(ctx->raw.ptr, host_img, N * sizeof(T), hipMemcpyDefault, ctx->stream));
+    PCHK(hipMemcpyAsync(L.raw->ptr, host_img, N * sizeof(T), hipMemcpyDefault, L.s));
     if (host_mask) {
-        PCHK(hipMemcpyAsync(ctx->rawmask.ptr, host_mask, N, hipMemcpyDefault, ctx->stream));
+        PCHK(hipMemcpyAsync(L.rawmask->ptr, host_mask, N, hipMemcpyDefault, L.s));
     } else {
-        finite_mask_t_kernel<T><<<gsz(N), 256, 0, ctx->stream>>>((const T*)ctx->raw.ptr,
-                                                                 (uint8_t*)ctx->rawmask.ptr, N);
+        finite_mask_t_kernel<T><<<gsz(N), 256, 0, L.s>>>((const T*)L.raw->ptr,
+                                                                 (uint8_t*)L.rawmask->ptr, N);
         PCHK(hipGetLastError());
     }
-    return prepare_one<T>(ctx, (const T*)ctx->raw.ptr, (const uint8_t*)ctx->rawmask.ptr, ny, nx,
-                          dst, dmask, slot);
+    return prepare_one<T>(ctx, (const T*)L.raw->ptr, (const uint8_t*)L.rawmask->ptr, ny, nx,
+                          dst, dmask, slot, L);
 }
 
 extern "C" ar_status ar_set_images(ar_context* ctx, const void* ref, const void* sec,
@@ -783,6 +799,18 @@ extern "C" ar_status ar_set_images(ar_context* ctx, const void* ref, const void*
             (st = ctx->rmask.reserve(ctx, N)) || (st = ctx->smask.reserve(ctx, N)))
             return st;
     }
+    // Both images of a pair of at most kSidePrepMax elements are prepared side by side: the
+    // secondary on the second stream, which first waits for everything already queued on the
+    // context's stream (a pass may still read the prepared images), and which the context's
+    // stream then waits for.
+    const bool side = ref && sec && N <= kSidePrepMax && dtype != AR_C64 &&
+                      !(pdt == AR_U8 && p.preprocess == AR_PRE_NONE &&
+                        p.quantize == AR_QUANT_NONE);
+    if (side) {
+        if ((st = ar_stream2(ctx))) return st;
+        PCHK(hipEventRecord(ctx->evf[0], ctx->stream));
+        PCHK(hipStreamWaitEvent(ctx->stream2, ctx->evf[0], 0));
+    }
     for (int slot = 0; slot < 2; slot++) {
         const void* img = slot == 0 ? ref : sec;
         const uint8_t* m = slot == 0 ? ref_valid : sec_valid;
@@ -802,15 +830,24 @@ extern "C" ar_status ar_set_images(ar_context* ctx, const void* ref, const void*
             }
             continue;
         }
+        const PrepLane L = prep_lane(ctx, side && slot == 1);
         switch (dtype) {
             case AR_C64: st = ar_prepare_complex(ctx, img, m, ny, nx, dst, dm); break;
-            case AR_U8: st = prepare_slot<uint8_t>(ctx, img, m, ny, nx, dst, dm, slot); break;
-            case AR_I16: st = prepare_slot<int16_t>(ctx, img, m, ny, nx, dst, dm, slot); break;
-            case AR_F32: st = prepare_slot<float>(ctx, img, m, ny, nx, dst, dm, slot); break;
-            default: st = prepare_slot<double>(ctx, img, m, ny, nx, dst, dm, slot); break;
+            case AR_U8: st = prepare_slot<uint8_t>(ctx, img, m, ny, nx, dst, dm, slot, L); break;
+            case AR_I16: st = prepare_slot<int16_t>(ctx, img, m, ny, nx, dst, dm, slot, L); break;
+            case AR_F32: st = prepare_slot<float>(ctx, img, m, ny, nx, dst, dm, slot, L); break;
+            default: st = prepare_slot<double>(ctx, img, m, ny, nx, dst, dm, slot, L); break;
         }
-        if (st) return st;
-        // (no wait: the next slot's upload into the same raw buffer is ordered on the stream)
+        if (st) {
+            if (side) hipStreamSynchronize(ctx->stream2);
+            return st;
+        }
+        // (otherwise no wait: the next slot's upload into the same raw buffer is ordered on the
+        // stream)
+    }
+    if (side) {
+        PCHK(hipEventRecord(ctx->evf[1], ctx->stream2));
+        PCHK(hipStreamWaitEvent(ctx->stream, ctx->evf[1], 0));
     }
     ctx->ny = ny;
     ctx->nx = nx;

@@ -143,7 +143,12 @@ static size_t pair_footprint(const ar_params& p, int64_t ny, int64_t nx, int32_t
         const size_t want = (size_t)npts * slot;
         tracker = want <= half ? want : 2 * half;
     }
-    return raw + work + prep + box + copies + pts + tracker + ((size_t)64 << 20);
+    // the quantiser's sequential-sum records (under 3 bytes an element) and, for a pair whose
+    // images are prepared side by side, the secondary's own work planes and records (`raw`
+    // already holds both uploads)
+    const size_t qs = p.quantize == AR_QUANT_UINT8 ? 3 * N + ((size_t)1 << 20) : 0;
+    const size_t side = (int64_t)N <= kSidePrepMax ? work + qs : 0;
+    return raw + work + prep + box + copies + pts + tracker + qs + side + ((size_t)64 << 20);
 }
 
 // Host copy split over up to 8 threads for large buffers: one thread's memcpy reaches only a
