@@ -287,27 +287,25 @@ __device__ inline Slot5 slot5(int64_t n, int64_t nr, int64_t nc, Win w) {
     return q;
 }
 
-// value of slot t of this lane's half-wave
-__device__ inline float slot_val(float v, int t, int half) {
-    const float a = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), t));
-    const float b = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 32 + t));
-    return half ? b : a;
-}
-
-// stable-sort rank of this lane's value among the non-NaN values of its half-wave
-__device__ inline int rank5(float v, const Slot5& q) {
+// stable-sort rank of this lane's value among the non-NaN values of its half-wave: the
+// half-wave's 32 values go to its slots in LDS (sh) and are read back as broadcasts (a wave's
+// LDS operations run in order, so no barrier is needed between its own write and reads)
+__device__ inline int rank5(float v, const Slot5& q, float* sh) {
+    sh[q.s] = v;
+    __builtin_amdgcn_wave_barrier();
     int r = 0;
 #pragma unroll
     for (int t = 0; t < 25; t++) {
-        const float x = slot_val(v, t, q.half);
+        const float x = sh[t];
         r += (t < q.s ? x <= v : x < v) ? 1 : 0;
     }
+    __builtin_amdgcn_wave_barrier();
     return r;
 }
 
 // the value at sorted positions k1 and k2 of the half-wave (every lane gets them)
-__device__ inline float median5(float v, const Slot5& q, int n) {
-    const int r = rank5(v, q);
+__device__ inline float median5(float v, const Slot5& q, int n, float* sh) {
+    const int r = rank5(v, q, sh);
     const bool live = !isnan(v);
     const int k1 = (n - 1) / 2, k2 = n / 2;
     const uint64_t b1 = __ballot(live && r == k1), b2 = __ballot(live && r == k2);
@@ -318,15 +316,15 @@ __device__ inline float median5(float v, const Slot5& q, int n) {
 }
 
 // median and MAD of one plane's window; n = 0 leaves both NaN
-__device__ inline void medmad5(float v, const Slot5& q, float& med, float& mad) {
+__device__ inline void medmad5(float v, const Slot5& q, float& med, float& mad, float* sh) {
     const uint64_t bl = __ballot(!isnan(v));
     const int n = __popc((uint32_t)(bl >> (32 * q.half)));
     med = qnan();
     mad = qnan();
     if (n == 0) return;
-    const float m = median5(v, q, n);
+    const float m = median5(v, q, n, sh);
     med = m;
-    mad = median5(fabsf(v - m), q, n);
+    mad = median5(fabsf(v - m), q, n, sh);
 }
 
 // Stage 1 on 5 x 5 windows: the agreement count is a ballot.
@@ -365,9 +363,11 @@ __global__ __launch_bounds__(256) void gardner_stage2_w5_kernel(
     }
     const float vx = q.kk >= 0 ? masked(nx, kin, q.kk) : qnan();
     const float vy = q.kk >= 0 ? masked(ny, kin, q.kk) : qnan();
+    __shared__ float sh[256];   // each half-wave's window values, for the ranks
+    float* hs = sh + (threadIdx.x & ~31u);
     float medx, madx, medy, mady;
-    medmad5(vx, q, medx, madx);
-    medmad5(vy, q, medy, mady);
+    medmad5(vx, q, medx, madx, hs);
+    medmad5(vy, q, medy, mady, hs);
     if (q.k < 0 || q.s != 0) return;
     const int64_t k = q.k;
     if (!kin[k]) { kout[k] = 0; return; }

@@ -271,3 +271,28 @@ def test_multichip_coarse_dilation_radii(buffer):
     np.testing.assert_array_equal(g.interpolated, e[4])
     print(f"  measured {g.nmeasured()} of {g.dx.size}, levels "
           f"{dict(zip(*np.unique(g.chip_size, return_counts=True)))}", flush=True)
+
+
+def test_multichip_far_points_match_oracle():
+    """Grid points far outside the image: one beyond 1e15 (the host then plans no level geometry
+    up front and every pass takes its padding from the device) and one thousands of pixels off
+    (unsearched; hole filling may give it its neighbours' value): bit-exact against the oracle."""
+    ref, sec = shifted_pair(300, (2.0, -1.5), seed=8, dtype=np.uint8)
+    ok = np.ones(ref.shape, bool)
+    grid = _grid(300, 12, 60)
+    grid.x[0, 0] = 2e15
+    grid.y[1, 1] = -5000.0
+    p = _params(preprocess=0, quantize=0)
+    op = oracle.params(chip_size=32)
+    print(f"multichip far points grid {grid.shape}", flush=True)
+    with autorift.Context(p) as ctx:
+        ctx.set_pair(ref, sec, ok, ok)
+        g = ctx.run_multichip(grid)
+    e = oracle.multichip(ref, sec, ok, grid.x, grid.y, grid.radius_x, grid.radius_y,
+                         grid.dx_prior, grid.dy_prior, p=op, nthreads=8)
+    for a, b in zip((g.dx, g.dy, g.correlation), e[:3]):
+        np.testing.assert_array_equal(np.isnan(a), np.isnan(b))
+        np.testing.assert_array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
+    np.testing.assert_array_equal(g.chip_size, e[3])
+    np.testing.assert_array_equal(g.interpolated, e[4])
+    assert g.nmeasured() > 0.5 * g.dx.size
