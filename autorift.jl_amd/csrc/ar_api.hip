@@ -267,12 +267,44 @@ static bool f64_shape(const TrackArgs& a, size_t* lds, int* pitch, int* threads)
 // Four int8 copies of a UInt8 image (value ^ 0x80), copy k shifted k bytes later
 // (copy_k[y] = ref[y - k] ^ 0x80, int8 zero outside), so any image offset starts a dword in one of
 // them.
+// Copy k (at dst + k stride; stride a multiple of 16) holds src[y - k] ^ 0x80 at position y, and
+// 0x80 where y - k falls outside [0, n). A thread forms 16 bytes of every copy from one 16-byte
+// load and the dword before it (copy k's dword d is bytes 4 - k .. 7 - k of dwords d - 1, d of
+// the source block, one alignbyte); the first block and blocks reaching past n take their bytes
+// one at a time.
 __global__ void refx_kernel(const uint8_t* src, uint8_t* dst, int64_t n, int64_t stride) {
-    int64_t y = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; y < stride; y += (int64_t)gridDim.x * blockDim.x)
+    const int64_t nv = stride >> 4;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t y0 = v << 4;
+        uint32_t o[4][4];
+        if (y0 >= 16 && y0 + 16 <= n) {
+            const uint4 q = *(const uint4*)(src + y0);
+            const uint32_t c[5] = {*(const uint32_t*)(src + y0 - 4), q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                o[0][d] = c[d + 1] ^ 0x80808080u;
+#pragma unroll
+                for (int k = 1; k < 4; k++)
+                    o[k][d] = __builtin_amdgcn_alignbyte(c[d + 1], c[d], 4 - k) ^ 0x80808080u;
+            }
+        } else {
+            for (int k = 0; k < 4; k++)
+                for (int d = 0; d < 4; d++) {
+                    uint32_t w = 0;
+                    for (int i = 0; i < 4; i++) {
+                        const int64_t y = y0 + 4 * d + i;
+                        const uint32_t b = (y >= k && y - k < n) ? (uint32_t)(src[y - k] ^ 0x80)
+                                                                 : 0x80u;
+                        w |= b << (8 * i);
+                    }
+                    o[k][d] = w;
+                }
+        }
 #pragma unroll
         for (int k = 0; k < 4; k++)
-            dst[k * stride + y] = (y >= k && y - k < n) ? (uint8_t)(src[y - k] ^ 0x80) : 0x80;
+            *(uint4*)(dst + k * stride + y0) = make_uint4(o[k][0], o[k][1], o[k][2], o[k][3]);
+    }
 }
 
 // Every kernel that takes dynamic LDS gets the largest allowance once per process (the
@@ -773,10 +805,10 @@ ar_status ar_track_pass(ar_context* ctx, const ar_points& dp, int64_t n, int64_t
         const bool copies = reach < ((int64_t)1 << 32);
         if (copies && ctx->refx_gen != ctx->ref_gen) {
             if ((st = ctx->refx.reserve(ctx, 8 * stride))) return st;
-            refx_kernel<<<grid_for(stride), 256, 0, ctx->stream>>>(
+            refx_kernel<<<grid_for(stride >> 4), 256, 0, ctx->stream>>>(
                 (const uint8_t*)ctx->pref.ptr, (uint8_t*)ctx->refx.ptr, N, stride);
             HIPCHK(ctx, hipGetLastError());
-            refx_kernel<<<grid_for(stride), 256, 0, ctx->stream>>>(
+            refx_kernel<<<grid_for(stride >> 4), 256, 0, ctx->stream>>>(
                 (const uint8_t*)ctx->psec.ptr, (uint8_t*)ctx->refx.ptr + 4 * stride, N, stride);
             HIPCHK(ctx, hipGetLastError());
             ctx->refx_gen = ctx->ref_gen;
@@ -1788,9 +1820,9 @@ ar_status ar_batch_add_pair(ar_batch* B, const void* ref, const void* sec,
         HIPCHK(ctx, hipMemcpyAsync(bp->ok.ptr, ctx->okmask.ptr, N, hipMemcpyDeviceToDevice,
                                    ctx->stream));
     if (bp->copies) {
-        refx_kernel<<<grid_for(bp->stride), 256, 0, ctx->stream>>>(
+        refx_kernel<<<grid_for(bp->stride >> 4), 256, 0, ctx->stream>>>(
             (const uint8_t*)ctx->pref.ptr, (uint8_t*)bp->refx.ptr, N, bp->stride);
-        refx_kernel<<<grid_for(bp->stride), 256, 0, ctx->stream>>>(
+        refx_kernel<<<grid_for(bp->stride >> 4), 256, 0, ctx->stream>>>(
             (const uint8_t*)ctx->psec.ptr, (uint8_t*)bp->refx.ptr + 4 * bp->stride, N, bp->stride);
         HIPCHK(ctx, hipGetLastError());
     }
