@@ -47,7 +47,11 @@ struct ar_context {
     // raw images of the current pair (preparation inputs)
     DevBuf raw, rawmask, work_f32, work_f64;
     // the secondary's own preparation scratch when the two images are prepared side by side
-    // (pairs of at most kSidePrepMax elements, on `stream2`)
+    // (pairs of at most kSidePrepMax elements, on `stream2`). A server's slot contexts clear
+    // it: they prepare one image after the other and stage a grid on one host thread, since
+    // the slots already run side by side (more streams than the process's hardware queues would
+    // share queues, and more host threads than cores would contend).
+    bool side_prep = true;
     DevBuf raw2, rawmask2, work_f32_2, work_f64_2, flags2, qsum2;
 
     // scratch

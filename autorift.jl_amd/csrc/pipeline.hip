@@ -751,7 +751,7 @@ static ar_status reject(ar_context* ctx, const float* dx, const float* dy, const
 // radius_y, each a contiguous plane of N 8-byte values) in one pass that also gathers what the
 // level planning needs: whether every coordinate and prior is finite and below 1e15 in
 // magnitude, the coordinate extents, the largest prior and the largest radii of the searchable
-// points. Large grids are split over up to 8 threads.
+// points. Large grids are split over up to 8 threads when `threads` is set.
 struct GridScan {
     bool ok = true;
     double xlo = INFINITY, xhi = -INFINITY, ylo = INFINITY, yhi = -INFINITY, pmax = 0.0;
@@ -787,8 +787,8 @@ static void grid_scan_range(const ar_points* g, char* pin, int64_t N, int64_t a,
     r.xlo = xlo; r.xhi = xhi; r.ylo = ylo; r.yhi = yhi; r.pmax = pmax; r.rx = rx; r.ry = ry;
 }
 
-static GridScan grid_scan(const ar_points* g, char* pin, int64_t N) {
-    int nt = (int)std::min<int64_t>(8, N / 32768);
+static GridScan grid_scan(const ar_points* g, char* pin, int64_t N, bool threads) {
+    int nt = threads ? (int)std::min<int64_t>(8, N / 32768) : 1;
     if (nt < 1) nt = 1;
     std::vector<GridScan> part(nt);
     std::vector<std::thread> ts;
@@ -887,7 +887,8 @@ extern "C" ar_status ar_run_multichip(ar_context* ctx, const ar_points* grid, in
     if ((st = ar_pinned(ctx, rb_off + 256, &pinv))) return st;
     char* pin = (char*)pinv;
     unsigned long long* rb = (unsigned long long*)(pin + rb_off);
-    const GridScan gs = grid_scan(grid, pin, N);
+    // (a server's slot contexts scan on their worker's thread alone: the slots run side by side)
+    const GridScan gs = grid_scan(grid, pin, N, ctx->side_prep);
     MCHK(hipMemcpyAsync(gin, pin, 48 * (size_t)N, hipMemcpyHostToDevice, s));
     fill_result_kernel<<<gblocks(N), GK, 0, s>>>(rdx, rdy, rc, rcs, rit, N);
     MCHK(hipGetLastError());

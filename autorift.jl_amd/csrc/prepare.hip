@@ -803,7 +803,7 @@ extern "C" ar_status ar_set_images(ar_context* ctx, const void* ref, const void*
     // secondary on the second stream, which first waits for everything already queued on the
     // context's stream (a pass may still read the prepared images), and which the context's
     // stream then waits for.
-    const bool side = ref && sec && N <= kSidePrepMax && dtype != AR_C64 &&
+    const bool side = ctx->side_prep && ref && sec && N <= kSidePrepMax && dtype != AR_C64 &&
                       !(pdt == AR_U8 && p.preprocess == AR_PRE_NONE &&
                         p.quantize == AR_QUANT_NONE);
     if (side) {

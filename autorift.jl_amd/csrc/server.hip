@@ -143,12 +143,10 @@ static size_t pair_footprint(const ar_params& p, int64_t ny, int64_t nx, int32_t
         const size_t want = (size_t)npts * slot;
         tracker = want <= half ? want : 2 * half;
     }
-    // the quantiser's sequential-sum records (under 3 bytes an element) and, for a pair whose
-    // images are prepared side by side, the secondary's own work planes and records (`raw`
-    // already holds both uploads)
+    // the quantiser's sequential-sum records (under 3 bytes an element); slot contexts prepare
+    // their two images one after the other, so one set of preparation scratch
     const size_t qs = p.quantize == AR_QUANT_UINT8 ? 3 * N + ((size_t)1 << 20) : 0;
-    const size_t side = (int64_t)N <= kSidePrepMax ? work + qs : 0;
-    return raw + work + prep + box + copies + pts + tracker + qs + side + ((size_t)64 << 20);
+    return raw + work + prep + box + copies + pts + tracker + qs + ((size_t)64 << 20);
 }
 
 // Host copy split over up to 8 threads for large buffers: one thread's memcpy reaches only a
@@ -297,6 +295,7 @@ ar_status ar_server_create(const ar_params* p, int32_t device, int32_t slots, in
             delete S;
             return st;
         }
+        c->side_prep = false;
         S->ctx.push_back(c);
     }
     S->held.assign(slots, 0);

@@ -64,9 +64,9 @@ def test_admission_by_device_memory():
     p = autorift.params()
     pairs = _pairs()[:4]
     alone = _alone(p, pairs)
-    # footprint estimates of these pairs are 78-131 MB (the smallest two 165 MB together): a
-    # 150 MB budget admits one at a time
-    budget = 150 << 20
+    # footprint estimates of these pairs are 74-116 MB (the smallest two 155 MB together): a
+    # 140 MB budget admits one at a time
+    budget = 140 << 20
     print("serve admission", flush=True)
     with autorift.Server(p, slots=4, hbm_budget=budget) as srv:
         tickets = [srv.submit(a, b) for a, b in pairs]
@@ -119,8 +119,8 @@ def test_one_slot_gives_back_its_own_buffers():
         ctx.set_images(*small)
         exp = ctx.run_multichip(autorift.api._build_grid(small[0].shape, p))
     print("serve one slot, own buffers", flush=True)
-    # a budget just above the large pair's footprint estimate (about 225 MiB)
-    with autorift.Server(p, slots=1, hbm_budget=240 << 20) as srv:
+    # a budget just above the large pair's footprint estimate (about 189 MiB)
+    with autorift.Server(p, slots=1, hbm_budget=200 << 20) as srv:
         r0, _ = srv.result(srv.submit(*big))
         r1, _ = srv.result(srv.submit(*small))
         assert srv.stats()["completed"] == 2
