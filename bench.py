@@ -701,7 +701,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+                                   "(sharding across GPUs is designed, not run)"}))
         sys.exit(2)
 
     import torch
